@@ -1,0 +1,42 @@
+/*
+ * visar_gpu.h -- device controls of libvisar_amd.so (no counterpart in the
+ * reference, whose hot path is CPU-only).
+ *
+ * The imgproc entry points of visar.h accept host or device pointers. These
+ * functions choose the HIP stream the library launches on, whether calls on
+ * device buffers wait for their kernels, and offer plain device buffers to
+ * callers that do not link a HIP runtime of their own.
+ */
+#ifndef VISAR_GPU_H
+#define VISAR_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Stream used by the calling thread's subsequent calls (a hipStream_t); NULL
+ * selects the library's own non-blocking stream. */
+int visar_gpu_set_stream(void *hip_stream);
+void *visar_gpu_get_stream(void);
+
+/* 1 (default): a call whose images are all device buffers returns after its
+ * kernels finish. 0: such calls only enqueue work on the stream. Calls that
+ * stage host memory always finish before returning. Per calling thread. */
+int visar_gpu_set_sync(int sync);
+int visar_gpu_synchronize(void);
+
+/* Device memory for callers without a HIP runtime of their own. */
+int visar_gpu_malloc(size_t bytes, void **out);
+int visar_gpu_free(void *p);
+int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D, 2 D2H, 3 D2D */);
+
+/* Name and compute units of the current device, e.g. "gfx950 256 CUs". */
+int visar_gpu_device_info(char *buf, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

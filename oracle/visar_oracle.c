@@ -1,0 +1,1402 @@
+/*
+ * visar_oracle.c -- CPU restatement of VISAR's imgproc hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path. Only tests,
+ * the smoke entry point and bench.py's CPU-baseline leg use it.
+ *
+ * Scalar C99, built with -ffp-contract=off and without -ffast-math, so every
+ * float expression is evaluated as written (one IEEE rounding per operation,
+ * fma() only where the reference uses @mulAdd). Each function names the
+ * reference lines it follows (paths relative to plyght/VISAR 0.1.0, src/).
+ *
+ * Parity pins (tests/test_oracle_golden.py): the known-answer vectors of the
+ * reference's own unit tests, the closed-form Background model of
+ * tests/py/test_motion_vs_opencv.py:191-197, brute-force integer
+ * convolution, and invariants (constant images, identity warps, etc.).
+ */
+#include "visar_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------ helpers */
+
+static size_t dsize(uint32_t depth) {
+    switch (depth) {
+    case 0: return 1;
+    case 2: return 2;
+    case 3: return 2;
+    case 4: return 4;
+    case 5: return 4;
+    default: return 0;
+    }
+}
+
+/* capi.zig:123-132 + image.zig:71-77, 172-176 */
+static int check(const orc_image *im, uint32_t depth) {
+    if (!im || !im->data) return ORC_INVALID_ARGUMENT;
+    if (im->depth != depth) return ORC_UNSUPPORTED;
+    if (im->channels == 0 || im->channels > 512) return ORC_INVALID_ARGUMENT;
+    if (im->stride % dsize(depth) != 0) return ORC_INVALID_ARGUMENT;
+    if (im->width == 0 || im->height == 0) return ORC_INVALID_ARGUMENT;
+    if (im->stride < (size_t)im->width * im->channels * dsize(depth)) return ORC_INVALID_ARGUMENT;
+    return ORC_OK;
+}
+
+static inline void *rowp(const orc_image *im, long y) { return (char *)im->data + (size_t)y * im->stride; }
+
+/* Loads sample i of row y as a double (exact for every supported depth). */
+static inline double ldd(const orc_image *im, long y, long i) {
+    const void *r = rowp(im, y);
+    switch (im->depth) {
+    case 0: return ((const uint8_t *)r)[i];
+    case 2: return ((const uint16_t *)r)[i];
+    case 3: return ((const int16_t *)r)[i];
+    case 4: return ((const int32_t *)r)[i];
+    default: return ((const float *)r)[i];
+    }
+}
+static inline int64_t ldi(const orc_image *im, long y, long i) { return (int64_t)ldd(im, y, i); }
+static inline float ldf(const orc_image *im, long y, long i) {
+    if (im->depth == 5) return ((const float *)rowp(im, y))[i];
+    return (float)ldi(im, y, i);
+}
+
+static inline void st_i(const orc_image *im, long y, long i, int64_t v) {
+    void *r = rowp(im, y);
+    switch (im->depth) {
+    case 0: ((uint8_t *)r)[i] = (uint8_t)v; break;
+    case 2: ((uint16_t *)r)[i] = (uint16_t)v; break;
+    case 3: ((int16_t *)r)[i] = (int16_t)v; break;
+    case 4: ((int32_t *)r)[i] = (int32_t)v; break;
+    default: break;
+    }
+}
+static inline void st_f(const orc_image *im, long y, long i, float v) { ((float *)rowp(im, y))[i] = v; }
+
+static void depth_range(uint32_t depth, double *lo, double *hi) {
+    switch (depth) {
+    case 0: *lo = 0; *hi = 255; break;
+    case 2: *lo = 0; *hi = 65535; break;
+    case 3: *lo = -32768; *hi = 32767; break;
+    default: *lo = -2147483648.0; *hi = 2147483647.0; break;
+    }
+}
+
+/* border.zig:22-39 */
+long orc_border_map(long p, long n, int border) {
+    if (p >= 0 && p < n) return p;
+    switch (border) {
+    case 0: return -1;
+    case 1: return p < 0 ? 0 : n - 1;
+    case 2:
+    case 4: {
+        if (n == 1) return 0;
+        long d = border == 4 ? 1 : 0;
+        long q = p;
+        while (q < 0 || q >= n) {
+            if (q < 0) q = -q - 1 + d;
+            else q = n - 1 - (q - n) - d;
+        }
+        return q;
+    }
+    case 3: {
+        long m = p % n;
+        if (m < 0) m += n;
+        return m;
+    }
+    default: return -1;
+    }
+}
+
+/* saturate.zig:8-18: magic-number round half to even. */
+static inline double rint64(double x) { return (x + 6755399441055744.0) - 6755399441055744.0; }
+static inline float rint32(float x) { return (x + 12582912.0f) - 12582912.0f; }
+
+/* IEEE minNum/maxNum as Zig's @min/@max: a NaN operand yields the other. */
+static inline float fmaxn(float a, float b) { return isnan(a) ? b : isnan(b) ? a : (a > b ? a : b); }
+static inline float fminn(float a, float b) { return isnan(a) ? b : isnan(b) ? a : (a < b ? a : b); }
+
+/* saturate.zig:21-49 sat.cast(D, float): NaN -> 0, rint in f64, clamp. */
+static int64_t sat_cast_f(double x, uint32_t depth) {
+    double lo, hi;
+    if (isnan(x)) return 0;
+    depth_range(depth, &lo, &hi);
+    double r = rint64(x);
+    if (r <= lo) return (int64_t)lo;
+    if (r >= hi) return (int64_t)hi;
+    return (int64_t)r;
+}
+int orc_sat_u8_f32(float x) { return (int)sat_cast_f((double)x, 0); }
+int orc_sat_i16_f32(float x) { return (int)sat_cast_f((double)x, 3); }
+
+/* simd.zig:90-126 satF32: clamp (NaN -> low bound, then 0 for signed), rint. */
+static int64_t satf32(float v, uint32_t depth) {
+    double lo, hi;
+    depth_range(depth, &lo, &hi);
+    float c = fminn(fmaxn(v, (float)lo), (float)hi);
+    if (lo != 0 && isnan(v)) c = 0;
+    return (int64_t)rint64((double)c);
+}
+int orc_satf32_u8(float x) { return (int)satf32(x, 0); }
+int orc_satf32_i16(float x) { return (int)satf32(x, 3); }
+
+/* filter.zig:44-58 satV(D, f32 vector): rint in f32, clamp with maxNum/minNum
+ * (NaN -> low bound), convert. */
+static int64_t satv_f(float v, uint32_t depth) {
+    double lo, hi;
+    depth_range(depth, &lo, &hi);
+    float r = rint32(v);
+    float c = fminn(fmaxn(r, (float)lo), (float)hi);
+    return (int64_t)c;
+}
+
+/* ------------------------------------------------------ Gaussian kernels */
+
+/* filter.zig:350-399: OpenCV's softfloat exp as IEEE double operations. */
+static double from_bits(uint64_t b) {
+    double d;
+    memcpy(&d, &b, 8);
+    return d;
+}
+double orc_cv_exp(double x) {
+    static const uint64_t raw[64] = {
+        0x3ff0000000000000ull, 0x3ff02c9a3e778061ull, 0x3ff059b0d3158574ull, 0x3ff0874518759bc8ull,
+        0x3ff0b5586cf9890full, 0x3ff0e3ec32d3d1a2ull, 0x3ff11301d0125b51ull, 0x3ff1429aaea92de0ull,
+        0x3ff172b83c7d517bull, 0x3ff1a35beb6fcb75ull, 0x3ff1d4873168b9aaull, 0x3ff2063b88628cd6ull,
+        0x3ff2387a6e756238ull, 0x3ff26b4565e27cddull, 0x3ff29e9df51fdee1ull, 0x3ff2d285a6e4030bull,
+        0x3ff306fe0a31b715ull, 0x3ff33c08b26416ffull, 0x3ff371a7373aa9cbull, 0x3ff3a7db34e59ff7ull,
+        0x3ff3dea64c123422ull, 0x3ff4160a21f72e2aull, 0x3ff44e086061892dull, 0x3ff486a2b5c13cd0ull,
+        0x3ff4bfdad5362a27ull, 0x3ff4f9b2769d2ca7ull, 0x3ff5342b569d4f82ull, 0x3ff56f4736b527daull,
+        0x3ff5ab07dd485429ull, 0x3ff5e76f15ad2148ull, 0x3ff6247eb03a5585ull, 0x3ff6623882552225ull,
+        0x3ff6a09e667f3bcdull, 0x3ff6dfb23c651a2full, 0x3ff71f75e8ec5f74ull, 0x3ff75feb564267c9ull,
+        0x3ff7a11473eb0187ull, 0x3ff7e2f336cf4e62ull, 0x3ff82589994cce13ull, 0x3ff868d99b4492edull,
+        0x3ff8ace5422aa0dbull, 0x3ff8f1ae99157736ull, 0x3ff93737b0cdc5e5ull, 0x3ff97d829fde4e50ull,
+        0x3ff9c49182a3f090ull, 0x3ffa0c667b5de565ull, 0x3ffa5503b23e255dull, 0x3ffa9e6b5579fdbfull,
+        0x3ffae89f995ad3adull, 0x3ffb33a2b84f15fbull, 0x3ffb7f76f2fb5e47ull, 0x3ffbcc1e904bc1d2ull,
+        0x3ffc199bdd85529cull, 0x3ffc67f12e57d14bull, 0x3ffcb720dcef9069ull, 0x3ffd072d4a07897cull,
+        0x3ffd5818dcfba487ull, 0x3ffda9e603db3285ull, 0x3ffdfc97337b9b5full, 0x3ffe502ee78b3ff6ull,
+        0x3ffea4afa2a490daull, 0x3ffefa1bee615a27ull, 0x3fff50765b6e4540ull, 0x3fffa7c1819e90d8ull,
+    };
+    if (isnan(x)) return x;
+    if (isinf(x)) return x > 0 ? x : 0;
+    const double a0 = from_bits(0x3f83ce0f3e46f431ull);
+    const double p5 = 1.0 / a0;
+    const double p4 = from_bits(0x3fe62e42fefa39f1ull) / a0;
+    const double p3 = from_bits(0x3fcebfbdff82a45aull) / a0;
+    const double p2 = from_bits(0x3fac6b08d81fec75ull) / a0;
+    const double p1 = from_bits(0x3f83b2a72b4f3cd3ull) / a0;
+    const double p0 = from_bits(0x3f55e7aa1566c2a4ull) / a0;
+    const double prescale = from_bits(0x3ff71547652b82feull) * 64.0;
+    const double max_val = 3000.0 * 64.0;
+    uint64_t bits;
+    memcpy(&bits, &x, 8);
+    double x0 = (((bits >> 52) & 0x7ff) > 1023 + 10) ? (x < 0 ? -max_val : max_val) : x * prescale;
+    double r = rint64(x0);
+    int32_t v = (int32_t)r;
+    long t = (long)(v >> 6) + 1023;
+    if (t < 0) t = 0;
+    if (t > 2047) t = 2047;
+    double buf = from_bits((uint64_t)t << 52);
+    x0 = (x0 - r) * (1.0 / 64.0);
+    double poly = ((((p0 * x0 + p1) * x0 + p2) * x0 + p3) * x0 + p4) * x0 + p5;
+    return buf * a0 * from_bits(raw[v & 63]) * poly;
+}
+
+/* filter.zig:407-443 getGaussianKernelBitExact */
+void orc_gaussian_kernel(double *out, int n, double sigma) {
+    static const double s1[] = {1.0};
+    static const double s3[] = {0.25, 0.5, 0.25};
+    static const double s5[] = {0.0625, 0.25, 0.375, 0.25, 0.0625};
+    static const double s7[] = {0.03125, 0.109375, 0.21875, 0.28125, 0.21875, 0.109375, 0.03125};
+    static const double s9[] = {4.0 / 256.0, 13.0 / 256.0, 30.0 / 256.0, 51.0 / 256.0, 60.0 / 256.0,
+                                51.0 / 256.0, 30.0 / 256.0, 13.0 / 256.0, 4.0 / 256.0};
+    if (n % 2 == 1 && n <= 9 && sigma <= 0) {
+        const double *t = n == 1 ? s1 : n == 3 ? s3 : n == 5 ? s5 : n == 7 ? s7 : s9;
+        memcpy(out, t, sizeof(double) * (size_t)n);
+        return;
+    }
+    double nf = (double)n;
+    double sx = sigma > 0 ? sigma : fma(nf, 0.15, 0.35);
+    double scale2x = -0.125 / (sx * sx);
+    int half = (n - 1) / 2;
+    double sum = 0;
+    long x = 1 - (long)n;
+    for (int i = 0; i < half; i++) {
+        double t = orc_cv_exp((double)(x * x) * scale2x);
+        out[i] = t;
+        sum += t;
+        x += 2;
+    }
+    sum = sum * 2 + 1;
+    if (n % 2 == 0) sum += 1;
+    double mul1 = 1.0 / sum;
+    for (int i = 0; i < half; i++) {
+        out[i] *= mul1;
+        out[n - 1 - i] = out[i];
+    }
+    out[half] = mul1;
+    if (n % 2 == 0) out[half + 1] = mul1;
+}
+
+/* filter.zig:458-476 getGaussianKernelFixedPoint_ED */
+void orc_gaussian_kernel_fixed(uint32_t *out, int n, double sigma, int bits) {
+    double k[128];
+    orc_gaussian_kernel(k, n, sigma);
+    double m = (double)((uint64_t)1 << bits);
+    double err = 0;
+    uint64_t sum = 0;
+    for (int i = 0; i < n / 2; i++) {
+        double a = k[i] * m + err;
+        double v = rint64(a);
+        err = a - v;
+        uint64_t q = (uint64_t)(v > 0 ? v : 0);
+        out[i] = (uint32_t)q;
+        out[n - 1 - i] = (uint32_t)q;
+        sum += q;
+    }
+    out[n / 2] = (uint32_t)(((uint64_t)1 << bits) - 2 * sum);
+}
+
+/* filter.zig:447-451 */
+int orc_gaussian_ksize(double sigma, int depth_is_u8) {
+    double k = depth_is_u8 ? 3 : 4;
+    double r = round(sigma * k * 2 + 1);
+    if (!(r >= 0) || r > 1e6) return -1;
+    return ((int)r) | 1;
+}
+
+/* filter.zig:817-854 getDerivKernels */
+int orc_deriv_kernel(int32_t *out, int n_out, int order, int ksize) {
+    if (ksize == -1) {
+        if (n_out != 3 || order > 1) return ORC_INVALID_ARGUMENT;
+        if (order == 0) { out[0] = 3; out[1] = 10; out[2] = 3; }
+        else { out[0] = -1; out[1] = 0; out[2] = 1; }
+        return ORC_OK;
+    }
+    int n = ksize == 1 ? 3 : ksize;
+    if (n_out != n || n > 31) return ORC_INVALID_ARGUMENT;
+    if (ksize == 1 && order == 0) { out[0] = 0; out[1] = 1; out[2] = 0; return ORC_OK; }
+    if (order >= n) return ORC_INVALID_ARGUMENT;
+    int32_t a[33] = {0};
+    a[0] = 1;
+    int len = 1;
+    for (int s = 0; s < n - order - 1; s++) {
+        int32_t b[33] = {0};
+        for (int j = 0; j < len; j++) { b[j] += a[j]; b[j + 1] += a[j]; }
+        memcpy(a, b, sizeof a);
+        len++;
+    }
+    for (int s = 0; s < order; s++) {
+        int32_t b[33] = {0};
+        for (int j = 0; j < len; j++) { b[j] -= a[j]; b[j + 1] += a[j]; }
+        memcpy(a, b, sizeof a);
+        len++;
+    }
+    memcpy(out, a, sizeof(int32_t) * (size_t)n);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------ separable engine */
+
+/* Finish kinds of filter.zig:61-116. */
+enum { FIN_FIX8, FIN_FIX16, FIN_CAST, FIN_SCALED };
+
+/*
+ * Integer separable filter (filter.zig:120-205 with exact integer
+ * accumulators). The reference's accumulators (u16/u32/u64/i16/i32) never
+ * overflow or saturate within its own kernel bounds, so exact int64 sums
+ * reproduce it; the pass order does not matter (filter.zig:163-167).
+ */
+static int sep_int(const orc_image *s, const orc_image *d, const int64_t *kx, int nkx, const int64_t *ky, int nky,
+                   int border, int fin, double scale) {
+    long W = s->width, H = s->height, cn = s->channels, n = W * cn;
+    int rx = nkx / 2, ry = nky / 2;
+    int64_t *V = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    if (!V) return ORC_OUT_OF_MEMORY;
+    for (long y = 0; y < H; y++) {
+        for (long c = 0; c < n; c++) {
+            int64_t a = 0;
+            for (int j = 0; j < nky; j++) {
+                long sy = orc_border_map(y + j - ry, H, border);
+                a += ky[j] * (sy < 0 ? 0 : ldi(s, sy, c));
+            }
+            V[c] = a;
+        }
+        for (long e = 0; e < n; e++) {
+            long x = e / cn, ch = e % cn;
+            int64_t a = 0;
+            for (int i = 0; i < nkx; i++) {
+                long sx = orc_border_map(x + i - rx, W, border);
+                a += kx[i] * (sx < 0 ? 0 : V[sx * cn + ch]);
+            }
+            int64_t o;
+            switch (fin) {
+            case FIN_FIX8: o = (a + (1 << 15)) >> 16; if (o > 255) o = 255; break;
+            case FIN_FIX16: o = (int64_t)(((uint64_t)a + (1ull << 31)) >> 32); if (o > 65535) o = 65535; break;
+            case FIN_CAST: {
+                double lo, hi;
+                if (d->depth == 5) { st_f(d, y, e, (float)a); continue; }
+                depth_range(d->depth, &lo, &hi);
+                o = a < (int64_t)lo ? (int64_t)lo : a > (int64_t)hi ? (int64_t)hi : a;
+                break;
+            }
+            default: { /* FIN_SCALED: f32(v) * f32(scale) then satV */
+                float f = (float)a * (float)scale;
+                if (d->depth == 5) { st_f(d, y, e, f); continue; }
+                o = satv_f(f, d->depth);
+                break;
+            }
+            }
+            st_i(d, y, e, o);
+        }
+    }
+    free(V);
+    return ORC_OK;
+}
+
+/* filter.zig:213-229 */
+enum { SYM_NONE, SYM_EVEN, SYM_ODD };
+static int symmetry_f(const float *k, int n) {
+    int even = 1, odd = 1;
+    for (int i = 0; i < n / 2; i++) {
+        float a = k[i], b = k[n - 1 - i];
+        if (a != b) even = 0;
+        if (a != -b) odd = 0;
+    }
+    if (n % 2 == 1 && k[n / 2] != 0) odd = 0;
+    return even ? SYM_EVEN : odd ? SYM_ODD : SYM_NONE;
+}
+
+/* filter.zig:232-252 tapsV for one lane: folded by symmetry. */
+static float taps_fold(const float *v, const float *k, int n, int sym) {
+    int r = n / 2;
+    if (sym == SYM_NONE) {
+        float a = 0.0f;
+        for (int j = 0; j < n; j++) a = a + v[j] * k[j];
+        return a;
+    }
+    float a = sym == SYM_EVEN ? v[r] * k[r] : 0.0f;
+    for (int i = 0; i < r; i++) {
+        float t = v[i], b = v[n - 1 - i];
+        float pair = sym == SYM_EVEN ? t + b : b - t;
+        a = a + pair * (sym == SYM_EVEN ? k[i] : k[n - 1 - i]);
+    }
+    return a;
+}
+
+/* filter.zig:255-259 tapsS (scalar tail); `from_zero` is the compile-time
+ * kernel tail of vfirstConst/hsecondConst (filter.zig:297-301, 310-314). */
+static float taps_seq(const float *v, const float *k, int n, int from_zero) {
+    float a;
+    int j0;
+    if (from_zero) { a = 0.0f; j0 = 0; }
+    else { a = k[0] * v[0]; j0 = 1; }
+    for (int j = j0; j < n; j++) a = a + k[j] * v[j];
+    return a;
+}
+
+/*
+ * f32 separable filter (filter.zig:168-205, 262-272 with H = V = f32). The
+ * vector body (lanes of 16 f32) folds symmetric taps; the scalar tail of
+ * each row (elements past the last full 16-lane vector) sums taps in order.
+ */
+static int sep_f32(const orc_image *s, const orc_image *d, const float *kx, int nkx, const float *ky, int nky,
+                   int border, int fin, double scale, int const_kernel) {
+    long W = s->width, H = s->height, cn = s->channels, n = W * cn;
+    int rx = nkx / 2, ry = nky / 2;
+    long body = (n / 16) * 16;
+    int vsym = symmetry_f(ky, nky), hsym = symmetry_f(kx, nkx);
+    if (const_kernel) vsym = hsym = SYM_EVEN;
+    float *V = (float *)malloc(sizeof(float) * (size_t)(n + 2 * rx * cn));
+    if (!V) return ORC_OUT_OF_MEMORY;
+    float *mid = V + rx * cn;
+    float tv[128];
+    /* const_v: value of a constant-border pixel after the vertical pass */
+    float const_v = 0.0f;
+    for (int j = 0; j < nky; j++) const_v = const_v + ky[j] * 0.0f;
+    long rows[128];
+    for (long y = 0; y < H; y++) {
+        for (int j = 0; j < nky; j++) rows[j] = orc_border_map(y + j - ry, H, border);
+        for (long c = 0; c < n; c++) {
+            for (int j = 0; j < nky; j++) tv[j] = rows[j] < 0 ? 0.0f : ldf(s, rows[j], c);
+            mid[c] = c < body ? taps_fold(tv, ky, nky, vsym) : taps_seq(tv, ky, nky, const_kernel);
+        }
+        for (int i = 0; i < rx; i++) {
+            long lx = orc_border_map((long)i - rx, W, border);
+            long rxm = orc_border_map(W + i, W, border);
+            for (long ch = 0; ch < cn; ch++) {
+                V[i * cn + ch] = lx < 0 ? const_v : mid[lx * cn + ch];
+                V[(rx + W + i) * cn + ch] = rxm < 0 ? const_v : mid[rxm * cn + ch];
+            }
+        }
+        for (long e = 0; e < n; e++) {
+            for (int i = 0; i < nkx; i++) tv[i] = V[e + i * cn];
+            float a = e < body ? taps_fold(tv, kx, nkx, hsym) : taps_seq(tv, kx, nkx, const_kernel);
+            if (fin == FIN_SCALED) a = a * (float)scale;
+            if (d->depth == 5) st_f(d, y, e, a);
+            else st_i(d, y, e, satv_f(a, d->depth));
+        }
+    }
+    free(V);
+    return ORC_OK;
+}
+
+static int same_shape(const orc_image *a, const orc_image *b) {
+    return a->width == b->width && a->height == b->height && a->channels == b->channels;
+}
+
+/* filter.zig:498-607 (Gaussian), dispatch per capi.zig:197-206 */
+int orc_gaussian_blur(const orc_image *src, const orc_image *dst, int kw, int kh, double sx, double sy, int border) {
+    if (border < 0 || border > 4) return ORC_UNSUPPORTED;
+    if (kw < 0 || kh < 0) return ORC_INVALID_ARGUMENT;
+    if (!src) return ORC_INVALID_ARGUMENT;
+    uint32_t T = src->depth;
+    if (T != 0 && T != 2 && T != 3 && T != 5) return ORC_UNSUPPORTED;
+    int rc = check(src, T);
+    if (rc) return rc;
+    if ((rc = check(dst, T))) return rc;
+    double sy_in = sy > 0 ? sy : sx;
+    if (kw == 0) kw = orc_gaussian_ksize(sx, T == 0);
+    if (kh == 0) kh = orc_gaussian_ksize(sy_in, T == 0);
+    if (kw <= 0 || kh <= 0 || kw % 2 == 0 || kh % 2 == 0 || kw > 127 || kh > 127) return ORC_INVALID_ARGUMENT;
+    if (!same_shape(src, dst)) return ORC_SIZE_MISMATCH;
+    if (border != 0) {
+        if (src->width == 1) kw = 1;
+        if (src->height == 1) kh = 1;
+    }
+    if (T == 0 || T == 2) {
+        int bits = T == 0 ? 8 : 16;
+        uint32_t fx[128], fy[128];
+        int64_t kx[128], ky[128];
+        orc_gaussian_kernel_fixed(fx, kw, sx, bits);
+        orc_gaussian_kernel_fixed(fy, kh, sy_in, bits);
+        for (int i = 0; i < kw; i++) kx[i] = fx[i];
+        for (int i = 0; i < kh; i++) ky[i] = fy[i];
+        return sep_int(src, dst, kx, kw, ky, kh, border, T == 0 ? FIN_FIX8 : FIN_FIX16, 1);
+    }
+    double kxd[128], kyd[128];
+    float kx[128], ky[128];
+    orc_gaussian_kernel(kxd, kw, sx);
+    orc_gaussian_kernel(kyd, kh, sy_in);
+    for (int i = 0; i < kw; i++) kx[i] = (float)kxd[i];
+    for (int i = 0; i < kh; i++) ky[i] = (float)kyd[i];
+    int const_kernel = 0;
+    if (T == 5 && kw == kh && memcmp(kx, ky, sizeof(float) * (size_t)kw) == 0) {
+        static const float t3[3] = {0.25f, 0.5f, 0.25f};
+        static const float t5[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+        if ((kw == 3 && memcmp(kx, t3, sizeof t3) == 0) || (kw == 5 && memcmp(kx, t5, sizeof t5) == 0))
+            const_kernel = 1;
+    }
+    return sep_f32(src, dst, kx, kw, ky, kh, border, FIN_CAST, 1, const_kernel);
+}
+
+/* filter.zig:867-906 (Sobel), dispatch per capi.zig:218-232 */
+int orc_sobel(const orc_image *src, const orc_image *dst, int dx, int dy, int ksize, double scale, int border) {
+    if (border < 0 || border > 4) return ORC_UNSUPPORTED;
+    if (dx < 0 || dy < 0) return ORC_INVALID_ARGUMENT;
+    if (!src || !dst) return ORC_INVALID_ARGUMENT;
+    uint32_t S = src->depth;
+    if (S != 0 && S != 2 && S != 3 && S != 5) return ORC_UNSUPPORTED;
+    int rc = check(src, S);
+    if (rc) return rc;
+    if (dst->depth == 3) { if (S != 0) return ORC_UNSUPPORTED; }
+    else if (dst->depth != 5) return ORC_UNSUPPORTED;
+    if ((rc = check(dst, dst->depth))) return rc;
+    if (ksize != -1 && ksize != 1 && ksize != 3 && ksize != 5 && ksize != 7) return ORC_INVALID_ARGUMENT;
+    int n = ksize <= 1 ? 3 : ksize;
+    int nx = (ksize == 1 && dx > 0) ? 3 : n;
+    int ny = (ksize == 1 && dy > 0) ? 3 : n;
+    int32_t kx[32], ky[32];
+    if (orc_deriv_kernel(kx, nx, dx, (ksize == 1 && dx == 0) ? 1 : (ksize == 1 ? 3 : ksize))) return ORC_INVALID_ARGUMENT;
+    if (orc_deriv_kernel(ky, ny, dy, (ksize == 1 && dy == 0) ? 1 : (ksize == 1 ? 3 : ksize))) return ORC_INVALID_ARGUMENT;
+    if (!same_shape(src, dst)) return ORC_SIZE_MISMATCH;
+    if (dst->depth == 5 || S == 5) {
+        float fx[32], fy[32];
+        for (int i = 0; i < nx; i++) fx[i] = (float)kx[i];
+        for (int i = 0; i < ny; i++) fy[i] = (float)ky[i];
+        return sep_f32(src, dst, fx, nx, fy, ny, border, FIN_SCALED, scale, 0);
+    }
+    int64_t bx = 0, by = 0, lx[32], ly[32];
+    for (int i = 0; i < nx; i++) { bx += labs((long)kx[i]); lx[i] = kx[i]; }
+    for (int i = 0; i < ny; i++) { by += labs((long)ky[i]); ly[i] = ky[i]; }
+    (void)bx; (void)by;
+    /* i16 and i32 accumulators are both exact here; scale != 1 takes the
+     * f32-scaled finish (filter.zig:897-905). */
+    return sep_int(src, dst, lx, nx, ly, ny, border, scale == 1 ? FIN_CAST : FIN_SCALED, scale);
+}
+
+/* filter.zig:909-914 (sepFilter2D: f32 kernels, f32 accumulation, FinishCast) */
+int orc_sep_filter_2d(const orc_image *src, const orc_image *dst, const float *kx, size_t nkx, const float *ky,
+                      size_t nky, int border) {
+    if (border < 0 || border > 4) return ORC_UNSUPPORTED;
+    if (!src || !dst) return ORC_INVALID_ARGUMENT;
+    uint32_t S = src->depth, D = dst->depth;
+    if ((S != 0 && S != 2 && S != 3 && S != 5) || (D != 0 && D != 2 && D != 3 && D != 5)) return ORC_UNSUPPORTED;
+    int rc = check(src, S);
+    if (rc) return rc;
+    if ((rc = check(dst, D))) return rc;
+    if (!same_shape(src, dst)) return ORC_SIZE_MISMATCH;
+    if (!kx || !ky || nkx == 0 || nky == 0 || nkx % 2 == 0 || nky % 2 == 0 || nkx > 127 || nky > 127)
+        return ORC_INVALID_ARGUMENT;
+    return sep_f32(src, dst, kx, (int)nkx, ky, (int)nky, border, FIN_CAST, 1, 0);
+}
+
+/* ------------------------------------------------------ color, threshold */
+
+/* color.zig:88-105, 120-131, 324-403 (gray codes 6, 7, 10, 11) */
+int orc_cvt_color(const orc_image *src, const orc_image *dst, int code) {
+    int scn, sb;
+    switch (code) {
+    case 6: scn = 3; sb = 0; break;
+    case 7: scn = 3; sb = 2; break;
+    case 10: scn = 4; sb = 0; break;
+    case 11: scn = 4; sb = 2; break;
+    default: return ORC_UNSUPPORTED;
+    }
+    if (!src || !dst) return ORC_INVALID_ARGUMENT;
+    uint32_t T = src->depth;
+    if (T != 0 && T != 2 && T != 3 && T != 5) return ORC_UNSUPPORTED;
+    int rc = check(src, T);
+    if (rc) return rc;
+    if ((rc = check(dst, T))) return rc;
+    if (src->width != dst->width || src->height != dst->height) return ORC_SIZE_MISMATCH;
+    if ((int)src->channels != scn || dst->channels != 1) return ORC_UNSUPPORTED;
+    if (T == 3) return ORC_UNSUPPORTED;
+    int bi = sb, ri = 2 - sb;
+    for (long y = 0; y < src->height; y++)
+        for (long x = 0; x < src->width; x++) {
+            if (T == 5) {
+                float b = ldf(src, y, x * scn + bi), g = ldf(src, y, x * scn + 1), r = ldf(src, y, x * scn + ri);
+                st_f(dst, y, x, b * 0.114f + g * 0.587f + r * 0.299f);
+            } else {
+                uint32_t b = (uint32_t)ldi(src, y, x * scn + bi), g = (uint32_t)ldi(src, y, x * scn + 1),
+                         r = (uint32_t)ldi(src, y, x * scn + ri);
+                st_i(dst, y, x, (b * 3735u + g * 19235u + r * 9798u + (1u << 14)) >> 15);
+            }
+        }
+    return ORC_OK;
+}
+
+/* threshold.zig:184-207 getThreshVal_Otsu_8u */
+static double otsu(const uint32_t *hist, double total) {
+    double scale = 1.0 / total, mu = 0, mu1 = 0, q1 = 0, max_sigma = 0, max_val = 0;
+    const double eps = 2.220446049250313e-16;
+    for (int i = 0; i < 256; i++) mu += (double)i * (double)hist[i];
+    mu *= scale;
+    for (int i = 0; i < 256; i++) {
+        double p_i = (double)hist[i] * scale;
+        mu1 *= q1;
+        q1 += p_i;
+        double q2 = 1.0 - q1;
+        if ((q1 < q2 ? q1 : q2) < eps || (q1 > q2 ? q1 : q2) > 1.0 - eps) continue;
+        mu1 = (mu1 + (double)i * p_i) / q1;
+        double mu2 = (mu - q1 * mu1) / q2;
+        double sigma = q1 * q2 * (mu1 - mu2) * (mu1 - mu2);
+        if (sigma > max_sigma) { max_sigma = sigma; max_val = (double)i; }
+    }
+    return max_val;
+}
+
+/* threshold.zig:210-253 getThreshVal_Triangle_8u */
+static double triangle(const uint32_t *hist) {
+    int left = 0, right = 0, max_ind = 0, flip = 0;
+    uint32_t mx = 0, h[256];
+    for (int i = 0; i < 256; i++) if (hist[i] > 0) { left = i; break; }
+    if (left > 0) left -= 1;
+    for (int i = 255; i > 0; i--) if (hist[i] > 0) { right = i; break; }
+    if (right < 255) right += 1;
+    for (int i = 0; i < 256; i++) if (hist[i] > mx) { mx = hist[i]; max_ind = i; }
+    for (int i = 0; i < 256; i++) h[i] = hist[i];
+    if (max_ind - left < right - max_ind) {
+        flip = 1;
+        for (int i = 0; i < 256; i++) h[i] = hist[255 - i];
+        left = 255 - right;
+        max_ind = 255 - max_ind;
+    }
+    int thresh = left;
+    double a = (double)mx, b = (double)(left - max_ind), dist = 0;
+    for (int j = left + 1; j <= max_ind; j++) {
+        double t = a * (double)j + b * (double)h[j];
+        if (t > dist) { dist = t; thresh = j; }
+    }
+    thresh -= 1;
+    if (flip) thresh = 255 - thresh;
+    return (double)thresh;
+}
+
+/* threshold.zig:25-119; THRESH_OTSU (8) / THRESH_TRIANGLE (16) on 8U one-channel images */
+int orc_threshold(const orc_image *src, const orc_image *dst, double thresh, double maxval, int type, double *used) {
+    int kind = type & 7;
+    if (kind > 4) return ORC_UNSUPPORTED;
+    int autom = (type & 8) ? 1 : (type & 16) ? 2 : 0;
+    if (!src || !dst) return ORC_INVALID_ARGUMENT;
+    uint32_t T = src->depth;
+    if (T != 0 && T != 2 && T != 3 && T != 5) return ORC_UNSUPPORTED;
+    int rc = check(src, T);
+    if (rc) return rc;
+    if ((rc = check(dst, T))) return rc;
+    if (!same_shape(src, dst)) return ORC_SIZE_MISMATCH;
+    if (autom) {
+        if (T != 0 || src->channels != 1) return ORC_UNSUPPORTED;
+        uint32_t hist[256] = {0};
+        for (long y = 0; y < src->height; y++)
+            for (long x = 0; x < src->width; x++) hist[ldi(src, y, x)]++;
+        thresh = autom == 1 ? otsu(hist, (double)src->width * src->height) : triangle(hist);
+    }
+    if (used) *used = thresh;
+    long n = (long)src->width * src->channels;
+    if (T == 0) {
+        double fl = floor(thresh);
+        long ti = fl < -2147483648.0 ? -2147483647L - 1 : fl > 2147483647.0 ? 2147483647L : (long)fl;
+        int m = (int)sat_cast_f(maxval, 0);
+        int t = ti < 0 ? 0 : ti > 255 ? 255 : (int)ti;
+        uint8_t lut[256];
+        for (int i = 0; i < 256; i++) {
+            int gt = (ti < 0 || ti >= 255) ? (i > ti) : (i > t);
+            int v = i;
+            switch (kind) {
+            case 0: v = gt ? m : 0; break;
+            case 1: v = gt ? 0 : m; break;
+            case 2: v = gt ? ((ti < 0 || ti >= 255) ? (int)(ti < 0 ? 0 : 255) : t) : i; break;
+            case 3: v = gt ? i : 0; break;
+            default: v = gt ? 0 : i; break;
+            }
+            lut[i] = (uint8_t)v;
+        }
+        for (long y = 0; y < src->height; y++)
+            for (long i = 0; i < n; i++) st_i(dst, y, i, lut[ldi(src, y, i)]);
+        return ORC_OK;
+    }
+    if (T == 5) {
+        float t = (float)thresh;
+        float m = (float)maxval;
+        for (long y = 0; y < src->height; y++)
+            for (long i = 0; i < n; i++) {
+                float v = ldf(src, y, i), o;
+                int gt = v > t;
+                switch (kind) {
+                case 0: o = gt ? m : 0.0f; break;
+                case 1: o = gt ? 0.0f : m; break;
+                case 2: o = gt ? t : v; break;
+                case 3: o = gt ? v : 0.0f; break;
+                default: o = gt ? 0.0f : v; break;
+                }
+                st_f(dst, y, i, o);
+            }
+        return ORC_OK;
+    }
+    double lo, hi;
+    depth_range(T, &lo, &hi);
+    int64_t m = sat_cast_f(maxval, T);
+    if (floor(thresh) < lo) {
+        for (long y = 0; y < src->height; y++)
+            for (long i = 0; i < n; i++) {
+                int64_t v = ldi(src, y, i);
+                st_i(dst, y, i, kind == 3 ? v : kind == 0 ? m : 0);
+            }
+        return ORC_OK;
+    }
+    int64_t t = sat_cast_f(floor(thresh), T);
+    for (long y = 0; y < src->height; y++)
+        for (long i = 0; i < n; i++) {
+            int64_t v = ldi(src, y, i), o;
+            int gt = v > t;
+            switch (kind) {
+            case 0: o = gt ? m : 0; break;
+            case 1: o = gt ? 0 : m; break;
+            case 2: o = gt ? t : v; break;
+            case 3: o = gt ? v : 0; break;
+            default: o = gt ? 0 : v; break;
+            }
+            st_i(dst, y, i, o);
+        }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------- resize */
+
+/* resize.zig:186-214 linearTapU8 */
+static void linear_tap(long d, uint32_t ssize, uint32_t dsize_, int area, int pin, long o[2], int w[2]) {
+    double inv = (double)dsize_ / (double)ssize;
+    double scale = 1.0 / inv;
+    double df = (double)d;
+    long sx;
+    float fx;
+    if (area) {
+        sx = (long)floor(df * scale);
+        fx = (float)((df + 1) - (double)(sx + 1) * inv);
+        fx = fx <= 0 ? 0.0f : fx - floorf(fx);
+    } else {
+        fx = (float)((df + 0.5) * scale - 0.5);
+        float fl = floorf(fx);
+        sx = (long)fl;
+        fx -= fl;
+    }
+    long last = (long)ssize - 1;
+    if (pin && sx < 0) { sx = 0; fx = 0; }
+    if (pin && sx >= last) { sx = last; fx = 0; }
+    w[0] = (int)sat_cast_f((double)((1.0f - fx) * 2048.0f), 3);
+    w[1] = (int)sat_cast_f((double)(fx * 2048.0f), 3);
+    o[0] = sx < 0 ? 0 : sx > last ? last : sx;
+    o[1] = sx + 1 < 0 ? 0 : sx + 1 > last ? last : sx + 1;
+}
+
+/* resize.zig:216-242 + resize_u8.zig:641-711: fixed-point bilinear */
+static void linear_u8(const orc_image *s, const orc_image *d, int area) {
+    long cn = s->channels;
+    for (long y = 0; y < d->height; y++) {
+        long yo[2];
+        int b[2];
+        linear_tap(y, s->height, d->height, area, 0, yo, b);
+        for (long x = 0; x < d->width; x++) {
+            long xo[2];
+            int w[2];
+            linear_tap(x, s->width, d->width, area, 1, xo, w);
+            for (long ch = 0; ch < cn; ch++) {
+                int32_t h[2];
+                for (int k = 0; k < 2; k++) {
+                    int32_t acc = (int32_t)ldi(s, yo[k], xo[0] * cn + ch) * w[0] + (int32_t)ldi(s, yo[k], xo[1] * cn + ch) * w[1];
+                    h[k] = (int16_t)(acc >> 4);
+                }
+                int32_t t = ((h[0] * b[0]) >> 16) + ((h[1] * b[1]) >> 16);
+                int32_t o = (t + 2) >> 2;
+                st_i(d, y, x * cn + ch, o < 0 ? 0 : o > 255 ? 255 : o);
+            }
+        }
+    }
+}
+
+/* resize_u8.zig:381-447 exact half */
+static void half_u8(const orc_image *s, const orc_image *d) {
+    long cn = s->channels;
+    for (long y = 0; y < d->height; y++)
+        for (long x = 0; x < d->width; x++)
+            for (long ch = 0; ch < cn; ch++) {
+                long i = 2 * x * cn + ch;
+                uint32_t v = (uint32_t)(ldi(s, 2 * y, i) + ldi(s, 2 * y, i + cn) + ldi(s, 2 * y + 1, i) + ldi(s, 2 * y + 1, i + cn));
+                st_i(d, y, x * cn + ch, (v + 2) >> 2);
+            }
+}
+
+/* resize_u8.zig:1213-1277 integer-factor area */
+static void area_int_u8(const orc_image *s, const orc_image *d, long ix, long iy) {
+    long cn = s->channels;
+    float scale = 1.0f / (float)(ix * iy);
+    for (long y = 0; y < d->height; y++)
+        for (long x = 0; x < d->width; x++)
+            for (long ch = 0; ch < cn; ch++) {
+                uint32_t sum = 0;
+                for (long dy = 0; dy < iy; dy++)
+                    for (long dx = 0; dx < ix; dx++) sum += (uint32_t)ldi(s, y * iy + dy, (x * ix + dx) * cn + ch);
+                st_i(d, y, x * cn + ch, satf32((float)sum * scale, 0));
+            }
+}
+
+typedef struct { uint32_t di, si; float alpha; } area_tab;
+
+/* resize.zig:470-493 computeResizeAreaTab */
+static long area_tab_build(area_tab *out, uint32_t ssize, uint32_t dsz, double scale) {
+    long k = 0;
+    for (long dx = 0; dx < dsz; dx++) {
+        double fsx1 = (double)dx * scale;
+        double fsx2 = fsx1 + scale;
+        double cellw = scale < (double)ssize - fsx1 ? scale : (double)ssize - fsx1;
+        long sx1 = (long)ceil(fsx1);
+        long sx2 = (long)floor(fsx2);
+        long sx2c = sx2 < (long)ssize - 1 ? sx2 : (long)ssize - 1;
+        if (sx1 > (long)ssize - 1) sx1 = (long)ssize - 1;
+        if ((double)sx1 - fsx1 > 1e-3) {
+            out[k].di = (uint32_t)dx; out[k].si = (uint32_t)(sx1 - 1); out[k].alpha = (float)(((double)sx1 - fsx1) / cellw); k++;
+        }
+        for (long sx = sx1; sx < sx2; sx++) {
+            out[k].di = (uint32_t)dx; out[k].si = (uint32_t)sx; out[k].alpha = (float)(1.0 / cellw); k++;
+        }
+        if (fsx2 - (double)sx2 > 1e-3) {
+            double a = fsx2 - (double)sx2;
+            if (a > 1.0) a = 1.0;
+            if (a > cellw) a = cellw;
+            out[k].di = (uint32_t)dx; out[k].si = (uint32_t)sx2c; out[k].alpha = (float)(a / cellw); k++;
+        }
+    }
+    return k;
+}
+
+/* resize.zig:527-557 areaGeneric (f32 table-order sums, no FMA), which the
+ * u8 kernels of resize_u8.zig:1058-1208 reproduce bit for bit. */
+static int area_frac_u8(const orc_image *s, const orc_image *d) {
+    long cn = s->channels;
+    double sxs = (double)s->width / (double)d->width, sys = (double)s->height / (double)d->height;
+    area_tab *xt = (area_tab *)malloc(sizeof(area_tab) * (2 * (size_t)s->width + 2 * (size_t)d->width + 4));
+    area_tab *yt = (area_tab *)malloc(sizeof(area_tab) * (2 * (size_t)s->height + 2 * (size_t)d->height + 4));
+    long nd = (long)d->width * cn;
+    float *hbuf = (float *)malloc(sizeof(float) * (size_t)nd), *sum = (float *)malloc(sizeof(float) * (size_t)nd);
+    if (!xt || !yt || !hbuf || !sum) { free(xt); free(yt); free(hbuf); free(sum); return ORC_OUT_OF_MEMORY; }
+    long nx = area_tab_build(xt, s->width, d->width, sxs);
+    long ny = area_tab_build(yt, s->height, d->height, sys);
+    long k = 0;
+    for (long dy = 0; dy < d->height; dy++) {
+        for (long i = 0; i < nd; i++) sum[i] = 0.0f;
+        for (; k < ny && yt[k].di == (uint32_t)dy; k++) {
+            for (long i = 0; i < nd; i++) hbuf[i] = 0.0f;
+            for (long e = 0; e < nx; e++)
+                for (long ch = 0; ch < cn; ch++)
+                    hbuf[xt[e].di * cn + ch] = hbuf[xt[e].di * cn + ch] + ldf(s, yt[k].si, xt[e].si * cn + ch) * xt[e].alpha;
+            for (long i = 0; i < nd; i++) sum[i] = sum[i] + hbuf[i] * yt[k].alpha;
+        }
+        for (long i = 0; i < nd; i++) st_i(d, dy, i, sat_cast_f((double)sum[i], 0));
+    }
+    free(xt); free(yt); free(hbuf); free(sum);
+    return ORC_OK;
+}
+
+/* resize.zig:62-106 nearest (any depth) */
+static void nearest_any(const orc_image *s, const orc_image *d) {
+    long cn = s->channels;
+    double ifx = (double)s->width / (double)d->width, ify = (double)s->height / (double)d->height;
+    size_t es = dsize(s->depth);
+    for (long y = 0; y < d->height; y++) {
+        long sy = (long)floor((double)y * ify);
+        if (sy > (long)s->height - 1) sy = (long)s->height - 1;
+        for (long x = 0; x < d->width; x++) {
+            long sx = (long)floor((double)x * ifx);
+            if (sx > (long)s->width - 1) sx = (long)s->width - 1;
+            memcpy((char *)rowp(d, y) + (size_t)x * cn * es, (char *)rowp(s, sy) + (size_t)sx * cn * es, (size_t)cn * es);
+        }
+    }
+}
+
+/* resize.zig:28-62 dispatch (u8 with 1-4 channels for LINEAR/AREA; NEAREST any depth) */
+int orc_resize(const orc_image *src, const orc_image *dst, int interp) {
+    if (interp < 0 || interp > 4) return ORC_UNSUPPORTED;
+    if (!src || !dst) return ORC_INVALID_ARGUMENT;
+    uint32_t T = src->depth;
+    if (T != 0 && T != 2 && T != 3 && T != 5) return ORC_UNSUPPORTED;
+    int rc = check(src, T);
+    if (rc) return rc;
+    if ((rc = check(dst, T))) return rc;
+    if (src->channels != dst->channels) return ORC_SIZE_MISMATCH;
+    if (src->width == dst->width && src->height == dst->height) {
+        size_t rl = (size_t)src->width * src->channels * dsize(T);
+        for (long y = 0; y < src->height; y++) memcpy(rowp(dst, y), rowp(src, y), rl);
+        return ORC_OK;
+    }
+    if (interp == 0) { nearest_any(src, dst); return ORC_OK; }
+    if (T != 0 || src->channels > 4 || (interp != 1 && interp != 3)) return ORC_UNSUPPORTED;
+    int exact_half = src->width == 2 * dst->width && src->height == 2 * dst->height;
+    if (interp == 1) {
+        if (exact_half) half_u8(src, dst);
+        else linear_u8(src, dst, 0);
+        return ORC_OK;
+    }
+    if (dst->width > src->width || dst->height > src->height) { linear_u8(src, dst, 1); return ORC_OK; }
+    if (exact_half) { half_u8(src, dst); return ORC_OK; }
+    long ix = src->width / dst->width, iy = src->height / dst->height;
+    if (ix * dst->width == src->width && iy * dst->height == src->height) { area_int_u8(src, dst, ix, iy); return ORC_OK; }
+    return area_frac_u8(src, dst);
+}
+
+/* -------------------------------------------------------------- warps */
+
+static float toF_any(const orc_image *s, long y, long i) { return ldf(s, y, i); }
+
+static void store_round(const orc_image *d, long y, long i, float v) {
+    if (d->depth == 5) st_f(d, y, i, v);
+    else st_i(d, y, i, sat_cast_f((double)v, d->depth));
+}
+
+/* geometry.zig:591-602 */
+static int invert_affine(const double m[6], double o[6]) {
+    double det = m[0] * m[4] - m[1] * m[3];
+    if (det == 0) return 0;
+    double d = 1.0 / det;
+    double a11 = m[4] * d, a22 = m[0] * d, a12 = -m[1] * d, a21 = -m[3] * d;
+    double b1 = -a11 * m[2] - a12 * m[5];
+    double b2 = -a21 * m[2] - a22 * m[5];
+    o[0] = a11; o[1] = a12; o[2] = b1; o[3] = a21; o[4] = a22; o[5] = b2;
+    return 1;
+}
+
+/* geometry.zig:604-613 */
+static int invert3(const double m[9], double o[9]) {
+    double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
+    if (det == 0) return 0;
+    double d = 1.0 / det;
+    o[0] = (m[4] * m[8] - m[5] * m[7]) * d; o[1] = (m[2] * m[7] - m[1] * m[8]) * d; o[2] = (m[1] * m[5] - m[2] * m[4]) * d;
+    o[3] = (m[5] * m[6] - m[3] * m[8]) * d; o[4] = (m[0] * m[8] - m[2] * m[6]) * d; o[5] = (m[2] * m[3] - m[0] * m[5]) * d;
+    o[6] = (m[3] * m[7] - m[4] * m[6]) * d; o[7] = (m[1] * m[6] - m[0] * m[7]) * d; o[8] = (m[0] * m[4] - m[1] * m[3]) * d;
+    return 1;
+}
+
+/* geometry.zig:492-494 */
+static long to_idx(float f) { return (long)fminn(fmaxn(f, -1.0e9f), 1.0e9f); }
+
+static int warp_common_check(const orc_image *src, const orc_image *dst, int border) {
+    if (border < 0 || border > 4) return ORC_UNSUPPORTED;
+    if (!src || !dst) return ORC_INVALID_ARGUMENT;
+    uint32_t T = src->depth;
+    if (T != 0 && T != 2 && T != 3 && T != 5) return ORC_UNSUPPORTED;
+    int rc = check(src, T);
+    if (rc) return rc;
+    if ((rc = check(dst, T))) return rc;
+    return ORC_OK;
+}
+
+/* geometry.zig:410-533 warpAffine (warpRows / linearPixel / nearestPixel;
+ * the 16-pixel fast blocks compute the same values). */
+int orc_warp_affine(const orc_image *src, const orc_image *dst, const double m[6], int interp, int border, double value) {
+    int rc = warp_common_check(src, dst, border);
+    if (rc) return rc;
+    double inv[6];
+    if (!invert_affine(m, inv)) return ORC_UNSUPPORTED;
+    if (src->channels != dst->channels) return ORC_SIZE_MISMATCH;
+    float mf[6];
+    for (int i = 0; i < 6; i++) mf[i] = (float)inv[i];
+    long W = src->width, H = src->height, cn = src->channels;
+    int T = (int)src->depth;
+    double cvd = T == 5 ? (double)(float)value : (double)sat_cast_f(value, (uint32_t)T);
+    float cv = (float)cvd;
+    for (long y = 0; y < dst->height; y++) {
+        float yf = (float)y;
+        float bx = mf[1] * yf + mf[2];
+        float by = mf[4] * yf + mf[5];
+        for (long x = 0; x < dst->width; x++) {
+            float xf = (float)x;
+            float sx = fmaf(mf[0], xf, bx);
+            float sy = fmaf(mf[3], xf, by);
+            if (interp == 0) {
+                long ix = to_idx(rint32(fminn(fmaxn(sx, -4.0e6f), 4.0e6f)));
+                long iy = to_idx(rint32(fminn(fmaxn(sy, -4.0e6f), 4.0e6f)));
+                long mx = orc_border_map(ix, W, border), my = orc_border_map(iy, H, border);
+                for (long ch = 0; ch < cn; ch++) {
+                    if (mx >= 0 && my >= 0) {
+                        if (T == 5) st_f(dst, y, x * cn + ch, ldf(src, my, mx * cn + ch));
+                        else st_i(dst, y, x * cn + ch, ldi(src, my, mx * cn + ch));
+                    } else store_round(dst, y, x * cn + ch, cv);
+                }
+                continue;
+            }
+            float fx = floorf(sx), fy = floorf(sy);
+            float ax = sx - fx, ay = sy - fy;
+            long ix = to_idx(fx), iy = to_idx(fy);
+            if (border == 0 && (ix < -1 || iy < -1 || ix >= W || iy >= H)) {
+                for (long ch = 0; ch < cn; ch++) store_round(dst, y, x * cn + ch, cv);
+                continue;
+            }
+            long xs[2] = {orc_border_map(ix, W, border), orc_border_map(ix + 1, W, border)};
+            long ys[2] = {orc_border_map(iy, H, border), orc_border_map(iy + 1, H, border)};
+            for (long ch = 0; ch < cn; ch++) {
+                float p[4];
+                for (int k = 0; k < 4; k++) {
+                    long px = xs[k & 1], py = ys[k >> 1];
+                    p[k] = (px >= 0 && py >= 0) ? toF_any(src, py, px * cn + ch) : cv;
+                }
+                float v0 = fmaf(ax, p[1] - p[0], p[0]);
+                float v1 = fmaf(ax, p[3] - p[2], p[2]);
+                store_round(dst, y, x * cn + ch, fmaf(ay, v1 - v0, v0));
+            }
+        }
+    }
+    return ORC_OK;
+}
+
+/* std.math.clamp(v, lo, hi) = @max(lo, @min(v, hi)) with NaN-ignoring min/max */
+static double clampd(double v, double lo, double hi) {
+    double a = isnan(v) ? hi : (v < hi ? v : hi);
+    return isnan(a) ? lo : (a > lo ? a : lo);
+}
+
+/* geometry.zig:586-589, 615-672 warpPerspective (f64 coordinates) */
+int orc_warp_perspective(const orc_image *src, const orc_image *dst, const double m[9], int interp, int border,
+                         double value) {
+    int rc = warp_common_check(src, dst, border);
+    if (rc) return rc;
+    double inv[9];
+    if (!invert3(m, inv)) return ORC_UNSUPPORTED;
+    if (src->channels != dst->channels) return ORC_SIZE_MISMATCH;
+    long W = src->width, H = src->height, cn = src->channels;
+    int T = (int)src->depth;
+    double cvd = T == 5 ? (double)(float)value : (double)sat_cast_f(value, (uint32_t)T);
+    for (long y = 0; y < dst->height; y++) {
+        double fy = (double)y;
+        for (long x = 0; x < dst->width; x++) {
+            double fx = (double)x;
+            double sx = inv[0] * fx + inv[1] * fy + inv[2];
+            double sy = inv[3] * fx + inv[4] * fy + inv[5];
+            double w = inv[6] * fx + inv[7] * fy + inv[8];
+            double iw = w != 0 ? 1.0 / w : 0;
+            sx *= iw;
+            sy *= iw;
+            if (interp == 0) {
+                long ix = (long)round(clampd(sx, -1e9, 1e9));
+                long iy = (long)round(clampd(sy, -1e9, 1e9));
+                long mx = orc_border_map(ix, W, border), my = orc_border_map(iy, H, border);
+                for (long ch = 0; ch < cn; ch++) {
+                    if (mx >= 0 && my >= 0) {
+                        if (T == 5) st_f(dst, y, x * cn + ch, ldf(src, my, mx * cn + ch));
+                        else st_i(dst, y, x * cn + ch, ldi(src, my, mx * cn + ch));
+                    } else {
+                        if (T == 5) st_f(dst, y, x * cn + ch, (float)cvd);
+                        else st_i(dst, y, x * cn + ch, (int64_t)cvd);
+                    }
+                }
+                continue;
+            }
+            double x0f = floor(clampd(sx, -1e9, 1e9));
+            double y0f = floor(clampd(sy, -1e9, 1e9));
+            float ax = (float)(sx - x0f), ay = (float)(sy - y0f);
+            long ix = (long)x0f, iy = (long)y0f;
+            if (border == 0 && (ix < -1 || iy < -1 || ix >= W || iy >= H)) {
+                for (long ch = 0; ch < cn; ch++) {
+                    if (T == 5) st_f(dst, y, x * cn + ch, (float)cvd);
+                    else st_i(dst, y, x * cn + ch, (int64_t)cvd);
+                }
+                continue;
+            }
+            long xs[2] = {orc_border_map(ix, W, border), orc_border_map(ix + 1, W, border)};
+            long ys[2] = {orc_border_map(iy, H, border), orc_border_map(iy + 1, H, border)};
+            float wt[4] = {(1 - ax) * (1 - ay), ax * (1 - ay), (1 - ax) * ay, ax * ay};
+            for (long ch = 0; ch < cn; ch++) {
+                float acc = 0;
+                for (int k = 0; k < 4; k++) {
+                    long px = xs[k & 1], py = ys[k >> 1];
+                    float v = (px >= 0 && py >= 0) ? toF_any(src, py, px * cn + ch) : (float)cvd;
+                    acc += v * wt[k];
+                }
+                store_round(dst, y, x * cn + ch, acc);
+            }
+        }
+    }
+    return ORC_OK;
+}
+
+/* undistort.zig:280-394 remap (u8, f32 maps, constant border) and
+ * capi_calib3d.zig:205-213 (border value u8(clamp(round(value)))) */
+int orc_remap(const orc_image *src, const orc_image *dst, const orc_image *mapx, const orc_image *mapy, double value) {
+    int rc;
+    if ((rc = check(src, 0))) return rc;
+    if ((rc = check(dst, 0))) return rc;
+    if ((rc = check(mapx, 5))) return rc;
+    if ((rc = check(mapy, 5))) return rc;
+    if (dst->width != mapx->width || dst->height != mapx->height || mapx->width != mapy->width ||
+        mapx->height != mapy->height || mapx->channels != mapy->channels || mapx->channels != 1)
+        return ORC_SIZE_MISMATCH;
+    if (src->channels != dst->channels || src->channels < 1 || src->channels > 4) return ORC_UNSUPPORTED;
+    double rv = round(clampd(value, 0, 255));
+    int bv = (int)rv;
+    long W = src->width, H = src->height, cn = src->channels;
+    for (long y = 0; y < dst->height; y++)
+        for (long x = 0; x < dst->width; x++) {
+            float sx = ldf(mapx, y, x), sy = ldf(mapy, y, x);
+            float fx = floorf(sx), fy = floorf(sy);
+            float ax = sx - fx, ay = sy - fy;
+            if (!(fx >= -1 && fy >= -1 && fx < (float)W && fy < (float)H)) {
+                for (long ch = 0; ch < cn; ch++) st_i(dst, y, x * cn + ch, bv);
+                continue;
+            }
+            long ix = (long)fx, iy = (long)fy;
+            for (long ch = 0; ch < cn; ch++) {
+                float p[4];
+                for (int q = 0; q < 4; q++) {
+                    long px = ix + (q & 1), py = iy + (q >> 1);
+                    p[q] = (px >= 0 && px < W && py >= 0 && py < H) ? (float)ldi(src, py, px * cn + ch) : (float)bv;
+                }
+                float v0 = fmaf(ax, p[1] - p[0], p[0]);
+                float v1 = fmaf(ax, p[3] - p[2], p[2]);
+                float v = fmaf(ay, v1 - v0, v0);
+                float r = (v + 12582912.0f) - 12582912.0f;
+                st_i(dst, y, x * cn + ch, (int64_t)fminn(fmaxn(r, 0.0f), 255.0f));
+            }
+        }
+    return ORC_OK;
+}
+
+/* --------------------------------------------------------- morphology */
+
+/* morph.zig:32-59 getStructuringElement */
+int orc_structuring_element(int shape, int w, int h, uint8_t *buf) {
+    int ax = w / 2, ay = h / 2, r = ay, c = ax;
+    double inv_r2 = r > 0 ? 1.0 / (double)(r * r) : 0;
+    memset(buf, 0, (size_t)w * (size_t)h);
+    for (int i = 0; i < h; i++) {
+        int j1 = 0, j2 = 0;
+        if (shape == 0 || (shape == 1 && i == ay)) j2 = w;
+        else if (shape == 1) { j1 = ax; j2 = j1 + 1; }
+        else {
+            long dy = (long)i - r;
+            if (labs(dy) <= r) {
+                double dxf = (double)c * sqrt((double)((long)r * r - dy * dy) * inv_r2);
+                long dx = (long)round(dxf);
+                j1 = (int)((long)c - dx > 0 ? (long)c - dx : 0);
+                j2 = (int)((long)c + dx + 1 < (long)w ? (long)c + dx + 1 : (long)w);
+            }
+        }
+        for (int j = j1; j < j2; j++) buf[i * w + j] = 1;
+    }
+    return ORC_OK;
+}
+
+/* morph.zig:489-519 generalPass: the per-tap definition (out-of-image taps
+ * are ignored; identity = max for erode, min for dilate). */
+int orc_morph_pass(const orc_image *s, const orc_image *d, int op_max, const uint8_t *mask, int kw, int kh, int ax,
+                   int ay) {
+    long W = s->width, H = s->height, cn = s->channels;
+    int T = (int)s->depth;
+    double lo, hi;
+    depth_range((uint32_t)T, &lo, &hi);
+    double init = T == 5 ? (op_max ? -INFINITY : INFINITY) : (op_max ? lo : hi);
+    for (long y = 0; y < H; y++)
+        for (long x = 0; x < W; x++)
+            for (long ch = 0; ch < cn; ch++) {
+                double acc = init;
+                for (int ky = 0; ky < kh; ky++) {
+                    long sy = y + ky - ay;
+                    if (sy < 0 || sy >= H) continue;
+                    for (int kx = 0; kx < kw; kx++) {
+                        if (!mask[ky * kw + kx]) continue;
+                        long sx = x + kx - ax;
+                        if (sx < 0 || sx >= W) continue;
+                        double v = ldd(s, sy, sx * cn + ch);
+                        acc = op_max ? (v > acc ? v : acc) : (v < acc ? v : acc);
+                    }
+                }
+                if (T == 5) st_f(d, y, x * cn + ch, (float)acc);
+                else st_i(d, y, x * cn + ch, (int64_t)acc);
+            }
+    return ORC_OK;
+}
+
+typedef struct { orc_image im; void *buf; } tmp_image;
+
+static int tmp_like(const orc_image *s, tmp_image *t) {
+    size_t rl = (size_t)s->width * s->channels * dsize(s->depth);
+    t->buf = malloc(rl * s->height);
+    if (!t->buf) return ORC_OUT_OF_MEMORY;
+    t->im = *s;
+    t->im.stride = rl;
+    t->im.data = t->buf;
+    return ORC_OK;
+}
+
+static void copy_img(const orc_image *s, const orc_image *d) {
+    size_t rl = (size_t)s->width * s->channels * dsize(s->depth);
+    for (long y = 0; y < s->height; y++) memmove(rowp(d, y), rowp(s, y), rl);
+}
+
+/* morph.zig:63-96 erode/dilate with iterations (via the per-tap pass). */
+static int morph_iter(const orc_image *s, const orc_image *d, int op_max, const uint8_t *mask, int kw, int kh,
+                      int iterations) {
+    if (iterations == 0) { copy_img(s, d); return ORC_OK; }
+    tmp_image a, b;
+    if (tmp_like(s, &a)) return ORC_OUT_OF_MEMORY;
+    if (tmp_like(s, &b)) { free(a.buf); return ORC_OUT_OF_MEMORY; }
+    copy_img(s, &a.im);
+    orc_image *cur = &a.im, *nxt = &b.im;
+    for (int it = 0; it < iterations; it++) {
+        orc_morph_pass(cur, nxt, op_max, mask, kw, kh, kw / 2, kh / 2);
+        orc_image *t = cur; cur = nxt; nxt = t;
+    }
+    copy_img(cur, d);
+    free(a.buf); free(b.buf);
+    return ORC_OK;
+}
+
+/* morph.zig:574-593 d = saturate(a - b) */
+static void sub_sat(const orc_image *a, const orc_image *b, const orc_image *d) {
+    long n = (long)a->width * a->channels;
+    for (long y = 0; y < a->height; y++)
+        for (long i = 0; i < n; i++) {
+            if (a->depth == 5) { st_f(d, y, i, ldf(a, y, i) - ldf(b, y, i)); continue; }
+            int64_t x = ldi(a, y, i), z = ldi(b, y, i);
+            st_i(d, y, i, x > z ? x - z : 0);
+        }
+}
+
+/* morph.zig:532-571 morphologyEx, C ABI checks of capi.zig:339-350 */
+int orc_morphology(const orc_image *src, const orc_image *dst, int op, int shape, int kw, int kh, int iterations) {
+    if (op < 0 || op > 6 || shape < 0 || shape > 2) return ORC_UNSUPPORTED;
+    if (kw <= 0 || kh <= 0 || kw > 255 || kh > 255 || iterations < 0) return ORC_INVALID_ARGUMENT;
+    if (!src || !dst) return ORC_INVALID_ARGUMENT;
+    uint32_t T = src->depth;
+    if (T != 0 && T != 2 && T != 3 && T != 5) return ORC_UNSUPPORTED;
+    int rc = check(src, T);
+    if (rc) return rc;
+    if ((rc = check(dst, T))) return rc;
+    if (!same_shape(src, dst)) return ORC_SIZE_MISMATCH;
+    uint8_t *mask = (uint8_t *)malloc((size_t)kw * (size_t)kh);
+    if (!mask) return ORC_OUT_OF_MEMORY;
+    orc_structuring_element(shape, kw, kh, mask);
+    tmp_image t1, t2, s0;
+    if (tmp_like(src, &t1) || tmp_like(src, &t2) || tmp_like(src, &s0)) { free(mask); return ORC_OUT_OF_MEMORY; }
+    copy_img(src, &s0.im);
+    const orc_image *S = &s0.im;
+    switch (op) {
+    case 0: morph_iter(S, dst, 0, mask, kw, kh, iterations); break;
+    case 1: morph_iter(S, dst, 1, mask, kw, kh, iterations); break;
+    case 2: morph_iter(S, &t1.im, 0, mask, kw, kh, iterations); morph_iter(&t1.im, dst, 1, mask, kw, kh, iterations); break;
+    case 3: morph_iter(S, &t1.im, 1, mask, kw, kh, iterations); morph_iter(&t1.im, dst, 0, mask, kw, kh, iterations); break;
+    case 4:
+        morph_iter(S, &t1.im, 0, mask, kw, kh, iterations);
+        morph_iter(S, &t2.im, 1, mask, kw, kh, iterations);
+        sub_sat(&t2.im, &t1.im, dst);
+        break;
+    case 5:
+        morph_iter(S, &t1.im, 0, mask, kw, kh, iterations);
+        morph_iter(&t1.im, &t2.im, 1, mask, kw, kh, iterations);
+        sub_sat(S, &t2.im, dst);
+        break;
+    default:
+        morph_iter(S, &t1.im, 1, mask, kw, kh, iterations);
+        morph_iter(&t1.im, &t2.im, 0, mask, kw, kh, iterations);
+        sub_sat(&t2.im, S, dst);
+        break;
+    }
+    free(mask); free(t1.buf); free(t2.buf); free(s0.buf);
+    return ORC_OK;
+}
+
+/* ----------------------------------------------------------- background */
+
+/* video/track.zig:33-81 Background.apply; capi.zig:446-469 */
+int orc_background_apply(uint16_t *model, int *primed, int shift, const orc_image *frame, const orc_image *mask,
+                         int threshold) {
+    int rc;
+    if ((rc = check(frame, 0))) return rc;
+    if ((rc = check(mask, 0))) return rc;
+    if (frame->channels != 1 || !same_shape(frame, mask)) return ORC_SIZE_MISMATCH;
+    int t = threshold < 0 ? 0 : threshold > 255 ? 255 : threshold;
+    long W = frame->width, H = frame->height;
+    if (!*primed) {
+        for (long y = 0; y < H; y++)
+            for (long x = 0; x < W; x++) {
+                model[y * W + x] = (uint16_t)(ldi(frame, y, x) << 8);
+                st_i(mask, y, x, 0);
+            }
+        *primed = 1;
+        return ORC_OK;
+    }
+    for (long y = 0; y < H; y++)
+        for (long x = 0; x < W; x++) {
+            int32_t fv = (int32_t)ldi(frame, y, x), bv = model[y * W + x];
+            int32_t diff = fv - (bv >> 8);
+            int fg = (diff < 0 ? -diff : diff) > t;
+            st_i(mask, y, x, fg ? 255 : 0);
+            int sh = shift + (fg ? 2 : 0);
+            model[y * W + x] = (uint16_t)(bv + ((((int32_t)fv << 8) - bv) >> sh));
+        }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------- connected components */
+
+static long uf_find(long *p, long x) {
+    while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
+    return x;
+}
+static void uf_union(long *p, long a, long b) {
+    a = uf_find(p, a); b = uf_find(p, b);
+    if (a == b) return;
+    if (a < b) p[b] = a; else p[a] = b;
+}
+
+/*
+ * imgproc/components.zig:408-535 and capi.zig:533-550, restated per pixel:
+ * 4-connected components are numbered in raster order of their first
+ * pixel; 8-connected ones by (first 2x2 block row, smallest block column
+ * of the component inside that block row). Statistics are integer sums
+ * (twice the x sum), centroids sx2 / 2 / area and sy / area.
+ */
+int orc_connected_components(const orc_image *src, const orc_image *labels, int connectivity, int32_t *stats,
+                             double *centroids, size_t max, size_t *count) {
+    int rc;
+    if ((rc = check(src, 0))) return rc;
+    if ((rc = check(labels, 4))) return rc;
+    if (src->channels != 1 || labels->channels != 1 || src->width != labels->width || src->height != labels->height)
+        return ORC_SIZE_MISMATCH;
+    long W = src->width, H = src->height, N = W * H;
+    int eight = connectivity != 4;
+    long *p = (long *)malloc(sizeof(long) * (size_t)N);
+    long *lab = (long *)malloc(sizeof(long) * (size_t)N);
+    if (!p || !lab) { free(p); free(lab); return ORC_OUT_OF_MEMORY; }
+    for (long i = 0; i < N; i++) p[i] = i;
+    for (long y = 0; y < H; y++)
+        for (long x = 0; x < W; x++) {
+            if (!ldi(src, y, x)) continue;
+            long i = y * W + x;
+            if (x > 0 && ldi(src, y, x - 1)) uf_union(p, i, i - 1);
+            if (y > 0) {
+                if (ldi(src, y - 1, x)) uf_union(p, i, i - W);
+                if (eight && x > 0 && ldi(src, y - 1, x - 1)) uf_union(p, i, i - W - 1);
+                if (eight && x + 1 < W && ldi(src, y - 1, x + 1)) uf_union(p, i, i - W + 1);
+            }
+        }
+    /* ordering key per root */
+    long *key = (long *)malloc(sizeof(long) * (size_t)N);
+    long *order = (long *)malloc(sizeof(long) * (size_t)N);
+    if (!key || !order) { free(p); free(lab); free(key); free(order); return ORC_OUT_OF_MEMORY; }
+    for (long i = 0; i < N; i++) key[i] = -1;
+    for (long y = 0; y < H; y++)
+        for (long x = 0; x < W; x++) {
+            if (!ldi(src, y, x)) continue;
+            long r = uf_find(p, y * W + x);
+            long k;
+            if (!eight) k = y * W + x;
+            else {
+                long ry = r / W;
+                if ((y >> 1) != (ry >> 1)) continue; /* only the block row of the first pixel */
+                k = (y >> 1) * (W + 2) + (x >> 1);
+            }
+            if (key[r] < 0 || k < key[r]) key[r] = k;
+        }
+    long nroots = 0;
+    for (long i = 0; i < N; i++)
+        if (key[i] >= 0) order[nroots++] = i;
+    /* sort roots by key (insertion into a keyed array: keys are unique) */
+    long maxkey = eight ? ((H + 1) / 2 + 1) * (W + 2) : N;
+    long *byk = (long *)malloc(sizeof(long) * (size_t)(maxkey + 1));
+    if (!byk) { free(p); free(lab); free(key); free(order); return ORC_OUT_OF_MEMORY; }
+    for (long i = 0; i <= maxkey; i++) byk[i] = -1;
+    for (long i = 0; i < nroots; i++) byk[key[order[i]]] = order[i];
+    long next = 1;
+    for (long i = 0; i < N; i++) lab[i] = 0;
+    for (long k = 0; k <= maxkey; k++)
+        if (byk[k] >= 0) lab[byk[k]] = next++;
+    size_t ncomp = (size_t)next;
+    uint64_t *acc = (uint64_t *)calloc(ncomp * 7, sizeof(uint64_t));
+    if (!acc) { free(p); free(lab); free(key); free(order); free(byk); return ORC_OUT_OF_MEMORY; }
+    for (size_t i = 0; i < ncomp; i++) { acc[i * 7 + 0] = UINT64_MAX; acc[i * 7 + 1] = UINT64_MAX; }
+    for (long y = 0; y < H; y++)
+        for (long x = 0; x < W; x++) {
+            long l = ldi(src, y, x) ? lab[uf_find(p, y * W + x)] : 0;
+            ((int32_t *)rowp(labels, y))[x] = (int32_t)l;
+            uint64_t *a = acc + (size_t)l * 7;
+            if ((uint64_t)x < a[0]) a[0] = (uint64_t)x;
+            if ((uint64_t)y < a[1]) a[1] = (uint64_t)y;
+            if ((uint64_t)x > a[2]) a[2] = (uint64_t)x;
+            if ((uint64_t)y > a[3]) a[3] = (uint64_t)y;
+            a[4] += 1;
+            a[5] += 2 * (uint64_t)x;
+            a[6] += (uint64_t)y;
+        }
+    *count = ncomp;
+    for (size_t i = 0; i < ncomp && i < max; i++) {
+        uint64_t *a = acc + i * 7;
+        if (stats) {
+            if (a[4] == 0) {
+                stats[i * 5 + 0] = -1; stats[i * 5 + 1] = 2147483647; stats[i * 5 + 2] = 0; stats[i * 5 + 3] = 0; stats[i * 5 + 4] = 0;
+            } else {
+                stats[i * 5 + 0] = (int32_t)a[0]; stats[i * 5 + 1] = (int32_t)a[1];
+                stats[i * 5 + 2] = (int32_t)(a[2] - a[0] + 1); stats[i * 5 + 3] = (int32_t)(a[3] - a[1] + 1);
+                stats[i * 5 + 4] = (int32_t)a[4];
+            }
+        }
+        if (centroids) {
+            double ar = (double)(a[4] > 0 ? a[4] : 1);
+            if (a[4] == 0) { centroids[i * 2] = NAN; centroids[i * 2 + 1] = NAN; }
+            else { centroids[i * 2] = (double)a[5] / 2.0 / ar; centroids[i * 2 + 1] = (double)a[6] / ar; }
+        }
+    }
+    free(p); free(lab); free(key); free(order); free(byk); free(acc);
+    return ORC_OK;
+}

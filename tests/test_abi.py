@@ -1,0 +1,112 @@
+"""The C ABI boundary on the CPU: the library loads, exports every symbol the
+headers declare, the headers compile as C99 and C++17 (reference build.zig:91-103),
+and argument validation follows the reference's order and status codes
+(capi.zig:123-132 and the per-call checks) before any device work."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def header_symbols():
+    names = set()
+    for h in os.listdir(INCLUDE):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(visar_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_headers_declare_hot_path():
+    names = header_symbols()
+    for n in ["visar_gaussian_blur", "visar_sobel", "visar_resize", "visar_warp_affine", "visar_remap",
+              "visar_morphology", "visar_cvt_color", "visar_threshold", "visar_background_create",
+              "visar_background_apply", "visar_background_free", "visar_warp_perspective", "visar_sep_filter_2d",
+              "visar_last_error", "visar_version"]:
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    import visar_amd
+    out = subprocess.run(["nm", "-D", "--defined-only", visar_amd.LIB_PATH], capture_output=True, text=True, check=True)
+    exported = {l.split()[-1] for l in out.stdout.splitlines() if " T " in l}
+    missing = [n for n in header_symbols() if n not in exported]
+    assert not missing, missing
+    for n in header_symbols():
+        assert hasattr(visar_amd.lib, n)
+
+
+@pytest.mark.parametrize("std,compiler,src", [("-std=c99", "gcc", "c"), ("-std=c++17", "g++", "cpp")])
+def test_headers_compile(tmp_path, std, compiler, src):
+    f = tmp_path / f"check.{src}"
+    incs = "".join(f'#include "{h}"\n' for h in sorted(os.listdir(INCLUDE)) if h.endswith(".h"))
+    f.write_text(incs + "int main(void) { return visar_version() == 0; }\n")
+    extra = ["-pedantic"] if src == "c" else []
+    subprocess.run([compiler, std, "-Wall", "-Wextra", "-Werror", *extra, "-I", INCLUDE, "-fsyntax-only", str(f)],
+                   check=True)
+
+
+def test_version_and_errors_without_device():
+    import visar_amd as va
+    assert va.version() == "0.1.0"
+    a = np.zeros((4, 4), np.uint8)
+    # border out of range -> UNSUPPORTED before any descriptor or device work
+    with pytest.raises(va.VisarError) as e:
+        va.gaussian_blur(a, (3, 3), 0, border=9)
+    assert e.value.rc == -3 and e.value.name == "Unsupported"
+    with pytest.raises(va.VisarError) as e:
+        va.gaussian_blur(a, (-1, 3), 0)
+    assert e.value.rc == -1
+    # even kernel -> InvalidKernel (INVALID_ARGUMENT)
+    with pytest.raises(va.VisarError) as e:
+        va.gaussian_blur(a, (4, 3), 0)
+    assert e.value.rc == -1 and e.value.name == "InvalidKernel"
+    # shape mismatch
+    with pytest.raises(va.VisarError) as e:
+        va.gaussian_blur(a, (3, 3), 0, dst=np.zeros((4, 5), np.uint8))
+    assert e.value.rc == -2
+    # depth mismatch between src and dst
+    with pytest.raises(va.VisarError) as e:
+        va.gaussian_blur(a, (3, 3), 0, dst=np.zeros((4, 4), np.uint16))
+    assert e.value.rc == -3
+    # morphology arguments
+    with pytest.raises(va.VisarError) as e:
+        va.morphology(a, 0, 0, (256, 3))
+    assert e.value.rc == -1
+    with pytest.raises(va.VisarError) as e:
+        va.morphology(a, 9, 0, (3, 3))
+    assert e.value.rc == -3
+    # Sobel i16 destination needs a u8 source
+    with pytest.raises(va.VisarError) as e:
+        va.sobel(np.zeros((4, 4), np.float32), np.int16, 1, 0)
+    assert e.value.rc == -3
+    # background shift range
+    h = ctypes.c_void_p()
+    assert va.lib.visar_background_create(4, 4, 16, ctypes.byref(h)) == -1
+    # a null image
+    assert va.lib.visar_cvt_color(None, None, 6) == -1
+    assert va.lib.visar_last_error().decode() == "InvalidArgument"
+
+
+def test_last_error_is_thread_local():
+    import threading
+
+    import visar_amd as va
+    a = np.zeros((4, 4), np.uint8)
+    with pytest.raises(va.VisarError):
+        va.gaussian_blur(a, (4, 3), 0)
+    seen = []
+    t = threading.Thread(target=lambda: seen.append(va.lib.visar_last_error().decode()))
+    t.start()
+    t.join()
+    assert seen == [""]
+    assert va.lib.visar_last_error().decode() == "InvalidKernel"

@@ -1,0 +1,370 @@
+"""HIP path vs the CPU oracle, bit for bit (GPU).
+
+Every case runs the product through the C ABI twice: with host NumPy arrays
+(staged by the library) and with device tensors (used in place), and must
+equal the oracle exactly, integer and float paths alike (the float paths
+reproduce the reference's operation order, so no tolerance is needed).
+Case lists follow the reference's difftests (tests/py/test_core_vs_opencv.py,
+test_calib3d_vs_opencv.py, test_motion_vs_opencv.py): every border, 1/3/4
+channels, strided views and degenerate sizes.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def va():
+    import visar_amd
+    assert torch.cuda.is_available()
+    return visar_amd
+
+
+def rnd(shape, seed, dtype=np.uint8):
+    rng = np.random.default_rng(seed)
+    if dtype == np.float32:
+        return rng.uniform(-50, 300, shape).astype(np.float32)
+    info = np.iinfo(dtype)
+    return rng.integers(info.min, info.max + 1, shape, dtype=np.int64).astype(dtype)
+
+
+def natural(h, w, cn, seed, dtype=np.uint8):
+    """Smooth content with noise (like the reference's natural() inputs)."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = 128 + 60 * np.sin(xx / 7.0) * np.cos(yy / 11.0)
+    img = base[:, :, None] + rng.normal(0, 20, (h, w, cn))
+    if dtype == np.float32:
+        out = (img / 255.0).astype(np.float32)
+    elif dtype == np.uint16:
+        out = np.clip(img * 257, 0, 65535).astype(np.uint16)
+    elif dtype == np.int16:
+        out = np.clip((img - 128) * 200, -32768, 32767).astype(np.int16)
+    else:
+        out = np.clip(img, 0, 255).astype(np.uint8)
+    return out[:, :, 0] if cn == 1 else out
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def both(fn_va, fn_or, *arrays):
+    """Runs the product on host and device copies; returns (host, device, oracle)."""
+    host = fn_va(*arrays)
+    dres = fn_va(*[dev(a) if isinstance(a, np.ndarray) else a for a in arrays])
+    dres = dres.cpu().numpy() if hasattr(dres, "cpu") else dres
+    return host, dres, fn_or(*arrays)
+
+
+def assert_same(got, want, msg=""):
+    assert got.shape == want.shape and got.dtype == want.dtype, msg
+    if got.dtype.kind == "f":
+        ok = np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        if not ok:
+            d = np.abs(got.astype(np.float64) - want)
+            raise AssertionError(f"{msg}: f32 bits differ, max |d| {d.max()} rel {np.nanmax(d / (np.abs(want) + 1e-30))}")
+    else:
+        if not np.array_equal(got, want):
+            idx = np.argwhere(got != want)
+            raise AssertionError(f"{msg}: {len(idx)} differ, first {idx[:3].tolist()}")
+
+
+def check(va, orc, name, *arrays, **kw):
+    f_va = getattr(va, name)
+    f_or = getattr(orc, name)
+    host, dres, want = both(lambda *a: f_va(*a, **kw), lambda *a: f_or(*a, **kw), *arrays)
+    assert_same(host, want, f"{name} host {kw}")
+    assert_same(dres, want, f"{name} device {kw}")
+
+
+# ------------------------------------------------------------ Gaussian
+
+GAUSS_SIZES = [(37, 53), (101, 67), (1, 1), (7, 1), (1, 9), (2, 40), (31, 3)]
+
+
+@pytest.mark.parametrize("cn", [1, 3, 4])
+def test_gaussian_u8_sweep(va, orc, cn):
+    for (h, w) in GAUSS_SIZES:
+        a = natural(h, w, cn, h * 31 + w)
+        for (kw, kh), sx, sy in [((3, 15), 0, 0), ((21, 5), 0, 0), ((1, 9), 0, 0), ((13, 1), 0, 0), ((15, 15), 0, 0),
+                                 ((31, 31), 0, 0), ((11, 7), 1.4, 3.2), ((9, 9), 2.0, 0.7), ((0, 0), 1.2, 2.9),
+                                 ((5, 5), 0.4, 0), ((3, 3), 0, 0), ((5, 5), 0, 0), ((7, 7), 0, 0)]:
+            for border in range(5):
+                check(va, orc, "gaussian_blur", a, (kw, kh), sx, sigma_y=sy, border=border)
+
+
+@pytest.mark.parametrize("dt", [np.uint16, np.int16, np.float32])
+def test_gaussian_other_depths(va, orc, dt):
+    for cn in (1, 3):
+        a = natural(47, 61, cn, 5, dt)
+        for k, s in [(3, 0), (5, 0), (7, 0), (9, 2.0), (0, 1.5), (13, 0), (31, 0)]:
+            for border in (0, 1, 2, 4):
+                check(va, orc, "gaussian_blur", a, (k, k), s, border=border)
+
+
+def test_gaussian_1080p_config(va, orc):
+    from visar_amd import synth
+    a = synth.uniform_bytes(0, (1080, 1920))
+    check(va, orc, "gaussian_blur", a, (5, 5), 0, border=4)
+
+
+def test_gaussian_strided_views(va, orc):
+    for cn in (1, 3):
+        big = natural(120, 160, cn, 77)
+        for y0, x0, h, w in [(3, 5, 37, 53), (1, 1, 97, 131), (10, 7, 9, 150)]:
+            v = big[y0:y0 + h, x0:x0 + w]
+            for k, s in [(3, 0), (7, 0), (13, 0), (19, 2.3)]:
+                want = orc.gaussian_blur(np.ascontiguousarray(v), (k, k), s)
+                got = va.gaussian_blur(v, (k, k), s)
+                assert_same(got, want, f"view {y0},{x0} k{k}")
+                dbig = dev(big)
+                dv = dbig[y0:y0 + h, x0:x0 + w]
+                assert_same(va.gaussian_blur(dv, (k, k), s).cpu().numpy(), want, f"device view k{k}")
+
+
+# --------------------------------------------------------------- Sobel
+
+@pytest.mark.parametrize("dx,dy,ks", [(1, 0, 3), (0, 1, 3), (1, 1, 3), (1, 0, 5), (2, 0, 5), (1, 0, -1), (0, 1, 1),
+                                      (1, 0, 7), (2, 1, 7)])
+def test_sobel(va, orc, dx, dy, ks):
+    for cn in (1, 3):
+        a = natural(67, 83, cn, 9)
+        for border in range(5):
+            check(va, orc, "sobel", a, np.int16, dx, dy, ksize=ks, border=border)
+            check(va, orc, "sobel", a, np.float32, dx, dy, ksize=ks, border=border)
+        check(va, orc, "sobel", a, np.int16, dx, dy, ksize=ks, scale=0.37)
+        check(va, orc, "sobel", a, np.float32, dx, dy, ksize=ks, scale=0.37)
+    f = natural(35, 41, 1, 10, np.float32)
+    check(va, orc, "sobel", f, np.float32, dx, dy, ksize=ks)
+    u = natural(35, 41, 1, 11, np.uint16)
+    check(va, orc, "sobel", u, np.float32, dx, dy, ksize=ks)
+
+
+def test_sobel_1080p_config(va, orc):
+    from visar_amd import synth
+    a = synth.uniform_bytes(3, (1080, 1920))
+    check(va, orc, "sobel", a, np.int16, 1, 0, ksize=3, border=4)
+
+
+# --------------------------------------------------------- sepFilter2D
+
+def test_sep_filter_2d(va, orc):
+    kx = np.array([0.1, -0.7, 0.3, 1.25, 0.05], np.float32)
+    ky = np.array([0.2, 0.5, -0.15], np.float32)
+    sym = np.array([0.1, 0.2, 0.4, 0.2, 0.1], np.float32)
+    odd = np.array([-0.5, -0.25, 0, 0.25, 0.5], np.float32)
+    for sdt in (np.uint8, np.uint16, np.int16, np.float32):
+        a = natural(41, 57, 3, 12, sdt)
+        for ddt in (np.uint8, np.int16, np.float32, np.uint16):
+            for kxx, kyy in [(kx, ky), (sym, odd), (odd, sym), (sym, sym)]:
+                for border in (0, 1, 3, 4):
+                    check(va, orc, "sep_filter_2d", a, ddt, kxx, kyy, border=border)
+
+
+# --------------------------------------------------- color / threshold
+
+def test_cvt_gray(va, orc):
+    for code, scn in [(6, 3), (7, 3), (10, 4), (11, 4)]:
+        for dt in (np.uint8, np.uint16, np.float32):
+            for (h, w) in [(2160, 3840), (37, 53), (1, 1), (5, 17)]:
+                a = natural(h, w, scn, code + h, dt)
+                check(va, orc, "cvt_color", a, code)
+
+
+def test_cvt_gray_4k_config(va, orc):
+    from visar_amd import synth
+    a = synth.uniform_bytes(0, (2160, 3840, 3))
+    check(va, orc, "cvt_color", a, 6)
+
+
+def test_threshold(va, orc):
+    for dt in (np.uint8, np.uint16, np.int16, np.float32):
+        a = natural(53, 71, 3, 13, dt)
+        ths = {np.uint8: [127.5, -3, 300, 0, 254.9], np.uint16: [30000.2, -5, 70000], np.int16: [0.5, -40000, 1000],
+               np.float32: [0.5, -1, 2]}[dt]
+        for t in ths:
+            for kind in range(5):
+                mv = {np.uint8: 200, np.uint16: 60000, np.int16: 1234, np.float32: 1.0}[dt]
+                host = va.threshold(a, t, mv, kind)[1]
+                want = orc.threshold(a, t, mv, kind)[1]
+                assert_same(host, want, f"threshold {dt} {t} {kind}")
+                assert_same(va.threshold(dev(a), t, mv, kind)[1].cpu().numpy(), want, "threshold device")
+
+
+def test_threshold_auto(va, orc):
+    a = natural(120, 160, 1, 14)
+    for auto in (8, 16):
+        t_va, d_va = va.threshold(a, 0, 255, auto)
+        t_or = orc.threshold_auto(a, auto)
+        assert t_va == t_or
+        assert_same(d_va, orc.threshold(a, t_or, 255, 0)[1])
+
+
+# -------------------------------------------------------------- resize
+
+def test_resize_cases(va, orc):
+    for (h, w) in [(480, 640), (37, 53), (61, 97)]:
+        for cn in (1, 3, 4):
+            a = natural(h, w, cn, h + cn)
+            for (dw, dh) in [(max(1, w // 2), max(1, h // 2)), (w * 2, h * 2), (max(1, w * 3 // 4), max(1, h * 2 // 3)),
+                             (113, 71), (int(round(w * 1.5)), int(round(h * 1.5))), (w, h)]:
+                for interp in (0, 1, 3):
+                    check(va, orc, "resize", a, (dw, dh), interp)
+
+
+def test_resize_area_cases(va, orc):
+    for (sw, sh), (dw, dh) in [((640, 480), (416, 312)), ((1920, 1080), (640, 360)), ((1920, 1080), (1280, 720)),
+                               ((1280, 720), (320, 180)), ((97, 61), (53, 37)), ((640, 480), (60, 45)),
+                               ((640, 480), (7, 5)), ((640, 480), (640, 300)), ((640, 480), (320, 480)),
+                               ((640, 480), (639, 479)), ((97, 61), (130, 80)), ((97, 61), (50, 80)),
+                               ((1001, 777), (500, 388)), ((1001, 777), (501, 389))]:
+        for cn in (1, 3, 4):
+            a = natural(sh, sw, cn, sw + cn)
+            check(va, orc, "resize", a, (dw, dh), 3)
+
+
+def test_resize_linear_4k_half_config(va, orc):
+    from visar_amd import synth
+    a = synth.uniform_bytes(5, (2160, 3840))
+    check(va, orc, "resize", a, (1920, 1080), 1)
+
+
+# --------------------------------------------------------------- warps
+
+def test_warp_affine(va, orc):
+    from visar_amd import synth
+    for cn in (1, 3, 4):
+        for dt in (np.uint8, np.uint16, np.int16, np.float32):
+            a = natural(90, 120, cn, cn, dt)
+            ms = [synth.rotation_matrix(60, 45, 30, 0.9), synth.affine_matrix(1, 120, 90), synth.affine_matrix(2, 120, 90),
+                  np.array([[1, 0, 0.5], [0, 1, -0.25]]), np.array([[2.5, 0.1, -30], [-0.2, 1.7, 4]])]
+            for m in ms:
+                for interp in (0, 1):
+                    for border in range(5):
+                        for size in [(120, 90), (77, 131)]:
+                            check(va, orc, "warp_affine", a, m, size, interpolation=interp, border=border, value=37.6)
+
+
+def test_warp_affine_1080p_config(va, orc):
+    from visar_amd import synth
+    a = synth.uniform_bytes(7, (1080, 1920))
+    for seed in range(4):
+        check(va, orc, "warp_affine", a, synth.affine_matrix(seed, 1920, 1080), (1920, 1080), interpolation=1, border=0)
+
+
+def test_warp_perspective(va, orc):
+    from visar_amd import synth
+    for cn in (1, 3):
+        for dt in (np.uint8, np.uint16, np.float32):
+            a = natural(90, 120, cn, cn + 7, dt)
+            ms = [synth.perspective_matrix(s, 120, 90) for s in range(3)] + [np.eye(3)]
+            for m in ms:
+                for interp in (0, 1):
+                    for border in range(5):
+                        check(va, orc, "warp_perspective", a, m, (120, 90), interpolation=interp, border=border, value=9)
+
+
+def test_warp_perspective_1080p_config(va, orc):
+    from visar_amd import synth
+    a = synth.uniform_bytes(8, (1080, 1920))
+    for seed in range(3):
+        check(va, orc, "warp_perspective", a, synth.perspective_matrix(seed, 1920, 1080), (1920, 1080), interpolation=1)
+
+
+def undistort_maps(w, h, k1=-0.25, k2=0.08):
+    fx = fy = 0.9 * w
+    cx, cy = w / 2.0, h / 2.0
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    x = (xx - cx) / fx
+    y = (yy - cy) / fy
+    r2 = x * x + y * y
+    d = 1 + k1 * r2 + k2 * r2 * r2
+    return (x * d * fx + cx).astype(np.float32), (y * d * fy + cy).astype(np.float32)
+
+
+def test_remap(va, orc):
+    for cn in (1, 2, 3, 4):
+        a = natural(72, 128, cn, 20 + cn)
+        mx, my = undistort_maps(128, 72)
+        for v in (0, 77.4, 300):
+            check(va, orc, "remap", a, mx, my, value=v)
+        rng = np.random.default_rng(cn)
+        rx = rng.uniform(-3, 131, (50, 60)).astype(np.float32)
+        ry = rng.uniform(-3, 75, (50, 60)).astype(np.float32)
+        rx[3, 4] = np.nan
+        check(va, orc, "remap", a, rx, ry, value=5)
+    a = natural(720, 1280, 3, 33)
+    mx, my = undistort_maps(1280, 720)
+    check(va, orc, "remap", a, mx, my)
+
+
+# ---------------------------------------------------------- morphology
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_morphology_cases(va, orc, shape):
+    for cn in (1, 3):
+        a = natural(61, 83, cn, 40 + shape)
+        for op in range(7):
+            for ks in [(3, 3), (5, 7), (1, 1), (9, 3)]:
+                check(va, orc, "morphology", a, op, shape, ks)
+    g = natural(53, 41, 1, 45)
+    check(va, orc, "morphology", g, 0, 0, (3, 3), iterations=3)
+    check(va, orc, "morphology", g, 2, 2, (5, 5), iterations=2)
+    check(va, orc, "morphology", g, 1, 1, (7, 7), iterations=0)
+
+
+def test_morphology_depths(va, orc):
+    for dt in (np.uint16, np.int16, np.float32):
+        a = natural(37, 45, 3, 50, dt)
+        for op in range(7):
+            check(va, orc, "morphology", a, op, 2, (5, 5), iterations=2)
+
+
+def test_morphology_mask_config(va, orc):
+    from visar_amd import synth
+    m = synth.morph_mask(0, 1024)
+    for op in range(7):
+        for shape, ks in [(0, (3, 3)), (2, (5, 5)), (1, (7, 7))]:
+            for it in (1, 3):
+                check(va, orc, "morphology", m, op, shape, ks, iterations=it)
+
+
+# ---------------------------------------------------------- background
+
+def test_background_closed_form(va, orc):
+    rng = np.random.default_rng(21)
+    for (h, w) in [(240, 320), (37, 53), (1, 1), (7, 1)]:
+        for shift, thr in ((4, 28), (2, 5), (8, 0), (15, 255)):
+            b_host = va.Background(w, h, shift)
+            b_dev = va.Background(w, h, shift)
+            ref = orc.Background(w, h, shift)
+            for i in range(6):
+                frame = natural(h, w, 1, i)
+                frame = np.clip(frame.astype(np.int32) + rng.integers(-60, 60, frame.shape), 0, 255).astype(np.uint8)
+                m_host = np.empty_like(frame)
+                b_host.apply(frame, m_host, thr)
+                m_dev = torch.empty((h, w), dtype=torch.uint8, device="cuda")
+                b_dev.apply(dev(frame), m_dev, thr)
+                want = np.empty_like(frame)
+                ref.apply(frame, want, thr)
+                assert_same(m_host, want, "background host")
+                assert_same(m_dev.cpu().numpy(), want, "background device")
+
+
+# ------------------------------------------------ connected components
+
+def test_connected_components(va, orc):
+    rng = np.random.default_rng(99)
+    for (h, w) in [(720, 1280), (37, 29), (1, 777), (777, 1)]:
+        m = ((rng.random((h, w)) > 0.6) * 255).astype(np.uint8)
+        for conn in (4, 8):
+            n, lab, st, ce = va.connected_components_with_stats(m, conn)
+            rn, rlab, rst, rce = orc.connected_components_with_stats(m, conn)
+            assert n == rn
+            assert_same(lab, rlab)
+            assert_same(st, rst)
+            np.testing.assert_array_equal(np.nan_to_num(ce, nan=-7), np.nan_to_num(rce, nan=-7))

@@ -1,0 +1,528 @@
+"""Pins the CPU oracle (CPU only).
+
+Each check is independent of the oracle's own code: the reference's unit-test
+known answers (tests/golden/known_answers.json), brute-force integer
+convolution and per-tap morphology in NumPy, closed forms (the Background
+model of the reference's tests/py/test_motion_vs_opencv.py:191-197), exact
+geometric identities and float64 references within the reference's own
+tolerances.
+"""
+import json
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KA = json.load(open(os.path.join(HERE, "golden", "known_answers.json")))
+
+PAD = {0: "constant", 1: "edge", 2: "symmetric", 3: "wrap", 4: "reflect"}
+
+
+def rnd(shape, seed, dtype=np.uint8):
+    rng = np.random.default_rng(seed)
+    if dtype == np.float32:
+        return rng.uniform(-100, 300, shape).astype(np.float32)
+    info = np.iinfo(dtype)
+    return rng.integers(info.min, info.max + 1, shape, dtype=np.int64).astype(dtype)
+
+
+# ----------------------------------------------------------- core pins
+
+def test_border_map(orc):
+    for p, n, b, want in KA["border_map"]["cases"]:
+        assert orc.border_map(p, n, b) == want
+
+
+def test_border_map_matches_numpy_pad(orc):
+    # numpy's pad modes restate the same extrapolations, including pads
+    # wider than the image.
+    for b, mode in PAD.items():
+        if b == 0:
+            continue
+        for n in (1, 2, 3, 5, 8):
+            if n == 1 and b in (2, 4):
+                continue
+            idx = np.pad(np.arange(n), 11, mode=mode)
+            got = [orc.border_map(p, n, b) for p in range(-11, n + 11)]
+            assert got == idx.tolist(), (b, n)
+
+
+def test_saturate(orc):
+    lib = orc.lib()
+    for x, want in KA["sat_cast"]["u8_from_f32"]:
+        assert lib.orc_sat_u8_f32(x) == want
+    for x, want in KA["sat_cast"]["i16_from_f32"]:
+        assert lib.orc_sat_i16_f32(x) == want
+    vals = [float("nan") if v == "nan" else v for v in KA["satF32"]["in"]]
+    assert [lib.orc_satf32_u8(v) for v in vals] == KA["satF32"]["u8"]
+    s = KA["satF32"]["i16_scale"]
+    assert [lib.orc_satf32_i16(np.float32(v) * np.float32(s)) for v in vals] == KA["satF32"]["i16"]
+
+
+def test_gaussian_kernels_known_answers(orc):
+    g = KA["gaussian_kernel"]
+    k = orc.gaussian_kernel(3, 0.9)
+    bits = [hex(struct.unpack("<Q", struct.pack("<d", v))[0]) for v in k[:2]]
+    assert bits == g["k3_s0p9_bits"]
+    assert orc.gaussian_kernel_fixed(13, 0, 8).tolist() == g["fixed8_k13"]
+    assert orc.gaussian_kernel_fixed(13, 0, 16).tolist() == g["fixed16_k13"]
+    assert orc.gaussian_kernel_fixed(9, 0, 8).tolist() == g["fixed8_k9"]
+    for n in range(1, 64, 2):
+        for s in (0, 0.6, 1.3, 4.1):
+            assert orc.gaussian_kernel_fixed(n, s, 8).sum() == 256
+            assert orc.gaussian_kernel_fixed(n, s, 16).sum() == 65536
+
+
+def test_cv_exp_close_to_exp(orc):
+    for x in np.linspace(-40, 5, 997):
+        want = math.exp(x)
+        assert abs(orc.cv_exp(x) - want) <= 1e-14 * want
+    assert orc.cv_exp(0.0) == 1.0
+
+
+def test_deriv_kernels(orc):
+    for order, ksize, want in KA["deriv_kernel"]["cases"]:
+        assert orc.deriv_kernel(order, ksize).tolist() == want
+    assert orc.deriv_kernel(0, -1).tolist() == [3, 10, 3]
+
+
+# ---------------------------------------------------------- filters
+
+def test_gaussian_impulse(orc):
+    g = KA["gaussian_impulse_3x3"]
+    a = np.zeros(g["size"], np.uint8)
+    a[g["impulse"][1], g["impulse"][0]] = g["impulse"][2]
+    d = orc.gaussian_blur(a, (3, 3), 0)
+    for x, y, v in g["expect"]:
+        assert d[y, x] == v
+
+
+def conv2d_int(a, kx, ky, border):
+    """Exact 2-D correlation with border extrapolation (channels last)."""
+    rx, ry = len(kx) // 2, len(ky) // 2
+    a3 = a if a.ndim == 3 else a[:, :, None]
+    p = np.pad(a3.astype(np.int64), ((ry, ry), (rx, rx), (0, 0)), mode=PAD[border])
+    h, w = a3.shape[:2]
+    out = np.zeros(a3.shape, np.int64)
+    for j, kyv in enumerate(ky):
+        for i, kxv in enumerate(kx):
+            out += int(kyv) * int(kxv) * p[j:j + h, i:i + w]
+    return out if a.ndim == 3 else out[:, :, 0]
+
+
+@pytest.mark.parametrize("border", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("cn", [1, 3])
+def test_gaussian_u8_is_exact_convolution(orc, border, cn):
+    a = rnd((37, 53, cn) if cn > 1 else (37, 53), 1 + border)
+    for k, s in [(3, 0), (5, 0), (7, 0), (9, 0), (13, 0), (11, 1.3), (21, 2.5)]:
+        taps = orc.gaussian_kernel_fixed(k, s, 8)
+        want = np.minimum((conv2d_int(a, taps, taps, border) + (1 << 15)) >> 16, 255)
+        got = orc.gaussian_blur(a, (k, k), s, border=border)
+        np.testing.assert_array_equal(got, want, err_msg=f"k{k} s{s}")
+
+
+def test_gaussian_u16_is_exact_convolution(orc):
+    a = rnd((29, 31), 5, np.uint16)
+    for k, s in [(3, 0), (7, 0), (15, 2.0)]:
+        taps = orc.gaussian_kernel_fixed(k, s, 16)
+        want = np.minimum((conv2d_int(a, taps, taps, 4) + (1 << 31)) >> 32, 65535)
+        np.testing.assert_array_equal(orc.gaussian_blur(a, (k, k), s), want)
+
+
+def test_gaussian_asymmetric_sizes(orc):
+    a = rnd((23, 41), 9)
+    for kw, kh, sx, sy in [(3, 15, 0, 0), (21, 5, 0, 0), (11, 7, 1.4, 3.2), (1, 9, 0, 0)]:
+        kx = orc.gaussian_kernel_fixed(kw, sx, 8)
+        ky = orc.gaussian_kernel_fixed(kh, sy if sy > 0 else sx, 8)
+        want = np.minimum((conv2d_int(a, kx, ky, 4) + (1 << 15)) >> 16, 255)
+        np.testing.assert_array_equal(orc.gaussian_blur(a, (kw, kh), sx, sy), want)
+
+
+def test_gaussian_f32_close_to_float64(orc):
+    a = rnd((33, 47), 3, np.float32)
+    for k, s in [(3, 0), (5, 0), (9, 2.0), (15, 0)]:
+        kd = orc.gaussian_kernel(k, s)
+        want = conv2d_float(a.astype(np.float64), kd, kd, 4)
+        got = orc.gaussian_blur(a, (k, k), s)
+        np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-3)
+
+
+def conv2d_float(a, kx, ky, border):
+    rx, ry = len(kx) // 2, len(ky) // 2
+    p = np.pad(a, ((ry, ry), (rx, rx)), mode=PAD[border])
+    h, w = a.shape
+    out = np.zeros(a.shape)
+    for j in range(len(ky)):
+        for i in range(len(kx)):
+            out += float(ky[j]) * float(kx[i]) * p[j:j + h, i:i + w]
+    return out
+
+
+@pytest.mark.parametrize("dx,dy,ks", [(1, 0, 3), (0, 1, 3), (1, 1, 3), (1, 0, 5), (2, 0, 5), (1, 0, -1), (1, 0, 7)])
+def test_sobel_is_exact_convolution(orc, dx, dy, ks):
+    a = rnd((31, 45), 11)
+    kx = orc.deriv_kernel(dx, ks)
+    ky = orc.deriv_kernel(dy, ks)
+    full = conv2d_int(a, kx, ky, 4)
+    np.testing.assert_array_equal(orc.sobel(a, np.int16, dx, dy, ks), np.clip(full, -32768, 32767))
+    np.testing.assert_array_equal(orc.sobel(a, np.float32, dx, dy, ks), full.astype(np.float32))
+
+
+def test_sep_filter_f32_close_to_float64(orc):
+    a = rnd((21, 35), 4).astype(np.float32)
+    kx = np.array([0.1, -0.7, 0.3, 1.25, 0.05], np.float32)
+    ky = np.array([0.2, 0.5, -0.15], np.float32)
+    want = conv2d_float(a.astype(np.float64), kx, ky, 1)
+    got = orc.sep_filter_2d(a, np.float32, kx, ky, border=1)
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-4)
+    # a transposed pair must differ (pins kx horizontal, ky vertical)
+    assert not np.allclose(orc.sep_filter_2d(a, np.float32, ky, kx, border=1), want, atol=1e-3)
+
+
+# ------------------------------------------------------- color, threshold
+
+def test_gray_bt601_primaries(orc):
+    px = np.array([[[255, 0, 0], [0, 255, 0], [0, 0, 255], [255, 255, 255], [0, 0, 0]]], np.uint8)
+    # BT.601: 0.114 B + 0.587 G + 0.299 R
+    assert orc.cvt_color(px, 6).ravel().tolist() == [29, 150, 76, 255, 0]
+    assert orc.cvt_color(px, 7).ravel().tolist() == [76, 150, 29, 255, 0]
+
+
+def test_gray_lattice(orc):
+    v = np.minimum(np.arange(52) * 5, 255)
+    b, g, r = np.meshgrid(v, v, v, indexing="ij")
+    img = np.stack([b.ravel(), g.ravel(), r.ravel()], -1).astype(np.uint8)[None]
+    want = (b.ravel() * 3735 + g.ravel() * 19235 + r.ravel() * 9798 + 16384) >> 15
+    assert np.array_equal(orc.cvt_color(img, 6)[0], want)
+    exact = 0.114 * b.ravel() + 0.587 * g.ravel() + 0.299 * r.ravel()
+    assert np.abs(orc.cvt_color(img, 6)[0] - exact).max() <= 0.5 + 1e-9
+
+
+def test_threshold_known(orc):
+    t = KA["threshold"]
+    a = np.array([t["in"]], np.uint8)
+    _, d = orc.threshold(a, t["thresh"], t["maxval"], 0)
+    assert d.ravel().tolist() == t["binary"]
+
+
+def test_threshold_kinds_match_numpy(orc):
+    a = rnd((17, 19, 3), 2)
+    for kind in range(5):
+        _, d = orc.threshold(a, 127.5, 200, kind)
+        gt = a > 127
+        want = [np.where(gt, 200, 0), np.where(gt, 0, 200), np.where(gt, 127, a), np.where(gt, a, 0),
+                np.where(gt, 0, a)][kind]
+        np.testing.assert_array_equal(d, want)
+
+
+# ------------------------------------------------------------- resize
+
+def test_resize_constant(orc):
+    c = KA["resize_constant"]
+    a = np.full((c["src"][1], c["src"][0], c["src"][2]), c["src"][3], np.uint8)
+    for interp in (0, 1, 3):
+        for (w, h) in [(7, 13), (5, 5), (20, 3), (3, 20)]:
+            assert (orc.resize(a, (w, h), interp) == c["src"][3]).all()
+
+
+def test_resize_half_and_area_int(orc):
+    a = rnd((40, 62, 3), 7)
+    want = (a[0::2, 0::2].astype(np.int32) + a[0::2, 1::2] + a[1::2, 0::2] + a[1::2, 1::2] + 2) >> 2
+    np.testing.assert_array_equal(orc.resize(a, (31, 20), 1), want)
+    np.testing.assert_array_equal(orc.resize(a, (31, 20), 3), want)
+    b = rnd((60, 96, 1), 8)[:, :, 0]
+    s = b.reshape(20, 3, 24, 4).astype(np.int64).sum(axis=(1, 3))
+    want = np.rint(s.astype(np.float32) * np.float32(1.0 / 12.0))
+    np.testing.assert_array_equal(orc.resize(b, (24, 20), 3), want)
+
+
+def bilinear64(a, dw, dh):
+    h, w = a.shape[:2]
+    def coords(d, s):
+        f = (np.arange(d) + 0.5) * (s / d) - 0.5
+        i0 = np.floor(f).astype(int)
+        t = f - i0
+        return np.clip(i0, 0, s - 1), np.clip(i0 + 1, 0, s - 1), t
+    x0, x1, tx = coords(dw, w)
+    tx = np.where((np.floor((np.arange(dw) + 0.5) * (w / dw) - 0.5) < 0) |
+                  (np.floor((np.arange(dw) + 0.5) * (w / dw) - 0.5) >= w - 1), 0, tx)
+    y0, y1, ty = coords(dh, h)
+    a = a.astype(np.float64)
+    top = a[y0][:, x0] * (1 - tx)[None, :, None] + a[y0][:, x1] * tx[None, :, None]
+    bot = a[y1][:, x0] * (1 - tx)[None, :, None] + a[y1][:, x1] * tx[None, :, None]
+    return top * (1 - ty)[:, None, None] + bot * ty[:, None, None]
+
+
+def test_resize_linear_within_one_of_bilinear(orc):
+    a = rnd((61, 97, 3), 12)
+    for dw, dh in [(53, 37), (150, 90), (113, 71), (97, 30), (40, 61)]:
+        got = orc.resize(a, (dw, dh), 1).astype(np.float64)
+        want = bilinear64(a, dw, dh)
+        assert np.abs(got - want).max() <= 1.0, (dw, dh)
+
+
+def area64(a, dw, dh):
+    h, w = a.shape[:2]
+    out = np.zeros((dh, dw, a.shape[2]))
+    sx, sy = w / dw, h / dh
+    for y in range(dh):
+        for x in range(dw):
+            x0, x1, y0, y1 = x * sx, (x + 1) * sx, y * sy, (y + 1) * sy
+            acc = 0
+            for yy in range(int(np.floor(y0)), int(np.ceil(y1))):
+                wy = min(y1, yy + 1) - max(y0, yy)
+                for xx in range(int(np.floor(x0)), int(np.ceil(x1))):
+                    wx = min(x1, xx + 1) - max(x0, xx)
+                    acc = acc + wx * wy * a[min(yy, h - 1), min(xx, w - 1)].astype(np.float64)
+            out[y, x] = acc / (sx * sy)
+    return out
+
+
+def test_resize_area_fractional_within_one(orc):
+    a = rnd((61, 97, 3), 13)
+    for dw, dh in [(53, 37), (30, 20), (13, 9), (97, 29), (7, 3)]:
+        got = orc.resize(a, (dw, dh), 3).astype(np.float64)
+        assert np.abs(got - area64(a, dw, dh)).max() <= 1.0, (dw, dh)
+
+
+# ------------------------------------------------------------- warps
+
+def test_warp_affine_identities(orc):
+    a = rnd((40, 40, 3), 21)
+    eye = np.array([[1, 0, 0], [0, 1, 0]], np.float64)
+    for interp in (0, 1):
+        for border in range(5):
+            np.testing.assert_array_equal(orc.warp_affine(a, eye, (40, 40), interp, border), a)
+    # translation by (3, -2) with a constant border of 7
+    t = np.array([[1, 0, 3], [0, 1, -2]], np.float64)
+    d = orc.warp_affine(a, t, (40, 40), 1, 0, 7)
+    np.testing.assert_array_equal(d[:38, 3:], a[2:, :37])
+    assert (d[:, :3] == 7).all() and (d[38:] == 7).all()
+    # rotation by 90 degrees: dst(x, y) = src(y, 39 - x), i.e. np.rot90
+    r = np.array([[0, 1, 0], [-1, 0, 39]], np.float64)
+    np.testing.assert_array_equal(orc.warp_affine(a, r, (40, 40), 1, 4), np.rot90(a))
+
+
+def test_warp_affine_half_pixel_rounds_half_even(orc):
+    a = rnd((9, 30), 22)
+    m = np.array([[1, 0, 0.5], [0, 1, 0]], np.float64)  # dst(x) = src(x - 0.5)
+    d = orc.warp_affine(a, m, (30, 9), 1, 1)
+    mid = (a[:, :-1].astype(np.float64) + a[:, 1:]) / 2
+    np.testing.assert_array_equal(d[:, 1:], np.rint(mid))  # np.rint is half to even
+    dn = orc.warp_affine(a, m, (30, 9), 0, 1)  # nearest, rint(x - 0.5): ties to even
+    xs = np.rint(np.arange(30) - 0.5).astype(int).clip(0, 29)
+    np.testing.assert_array_equal(dn, a[:, xs])
+
+
+def test_warp_perspective_identities(orc):
+    a = rnd((33, 45), 23)
+    np.testing.assert_array_equal(orc.warp_perspective(a, np.eye(3), (45, 33), 1, 4), a)
+    t = np.array([[1, 0, -4], [0, 1, 5], [0, 0, 1]], np.float64)
+    d = orc.warp_perspective(a, t, (45, 33), 1, 0, 9)
+    np.testing.assert_array_equal(d[5:, :41], a[:28, 4:])
+    # nearest uses round-half-away: +0.5 shift maps x to x (x >= 1), x = 0 to -1 (border)
+    h = np.array([[1, 0, 0.5], [0, 1, 0], [0, 0, 1]], np.float64)
+    dn = orc.warp_perspective(a, h, (45, 33), 0, 0, 3)
+    np.testing.assert_array_equal(dn[:, 1:], a[:, 1:])
+    assert (dn[:, 0] == 3).all()
+    # a pure scaling by 2 (w = 1) equals the affine scaling on integer grids
+    s = np.array([[2, 0, 0], [0, 2, 0], [0, 0, 1]], np.float64)
+    np.testing.assert_array_equal(orc.warp_perspective(a, s, (45, 33), 1, 0),
+                                  orc.warp_affine(a, s[:2], (45, 33), 1, 0))
+
+
+def test_remap_identity_and_shift(orc):
+    a = rnd((20, 30, 3), 24)
+    yy, xx = np.mgrid[0:20, 0:30].astype(np.float32)
+    np.testing.assert_array_equal(orc.remap(a, xx, yy), a)
+    d = orc.remap(a, xx + np.float32(0.5), yy, value=11)
+    mid = (a[:, :-1].astype(np.float64) + a[:, 1:]) / 2
+    np.testing.assert_array_equal(d[:, :-1], np.rint(mid))
+    out = orc.remap(a, xx + 100, yy, value=11.6)
+    assert (out == 12).all()
+
+
+# --------------------------------------------------------- morphology
+
+def test_structuring_elements(orc):
+    e = orc.structuring_element(2, 5, 5)
+    assert e.tolist() == [[0, 0, 1, 0, 0], [1, 1, 1, 1, 1], [1, 1, 1, 1, 1], [1, 1, 1, 1, 1], [0, 0, 1, 0, 0]]
+    c = orc.structuring_element(1, 7, 7)
+    assert c.sum() == 13 and c[3].all() and c[:, 3].all()
+    assert orc.structuring_element(0, 3, 5).all()
+
+
+def morph_np(a, mask, op_max):
+    kh, kw = mask.shape
+    ay, ax = kh // 2, kw // 2
+    a3 = a if a.ndim == 3 else a[:, :, None]
+    ident = 0 if op_max else 255
+    p = np.pad(a3, ((ay, kh - 1 - ay), (ax, kw - 1 - ax), (0, 0)), constant_values=ident)
+    h, w = a3.shape[:2]
+    out = np.full(a3.shape, ident, np.uint8)
+    for j in range(kh):
+        for i in range(kw):
+            if mask[j, i]:
+                win = p[j:j + h, i:i + w]
+                out = np.maximum(out, win) if op_max else np.minimum(out, win)
+    return out if a.ndim == 3 else out[:, :, 0]
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("ks", [(3, 3), (5, 5), (5, 7), (7, 3)])
+def test_morph_matches_per_tap_numpy(orc, shape, ks):
+    a = rnd((29, 37, 3), 31 + shape)
+    m = orc.structuring_element(shape, ks[0], ks[1])
+    er = morph_np(a, m, False)
+    di = morph_np(a, m, True)
+    np.testing.assert_array_equal(orc.morphology(a, 0, shape, ks), er)
+    np.testing.assert_array_equal(orc.morphology(a, 1, shape, ks), di)
+    op = morph_np(er, m, True)
+    cl = morph_np(di, m, False)
+    np.testing.assert_array_equal(orc.morphology(a, 2, shape, ks), op)
+    np.testing.assert_array_equal(orc.morphology(a, 3, shape, ks), cl)
+    np.testing.assert_array_equal(orc.morphology(a, 4, shape, ks), di - er)
+    np.testing.assert_array_equal(orc.morphology(a, 5, shape, ks), a - op)
+    np.testing.assert_array_equal(orc.morphology(a, 6, shape, ks), cl - a)
+    e2 = morph_np(morph_np(a, m, False), m, False)
+    np.testing.assert_array_equal(orc.morphology(a, 0, shape, ks, iterations=2), e2)
+    np.testing.assert_array_equal(orc.morphology(a, 1, shape, ks, iterations=0), a)
+
+
+def test_morph_off_centre_general_mask(orc):
+    a = rnd((15, 21), 41)
+    rng = np.random.default_rng(3)
+    mask = (rng.random((4, 6)) > 0.5).astype(np.uint8)
+    for anchor in [(0, 0), (5, 3), (2, 1)]:
+        d = orc.morph_pass(a, 0, mask, anchor)
+        ax, ay = anchor
+        want = np.full(a.shape, 255, np.uint8)
+        for j in range(4):
+            for i in range(6):
+                if not mask[j, i]:
+                    continue
+                for y in range(15):
+                    for x in range(21):
+                        sy, sx = y + j - ay, x + i - ax
+                        if 0 <= sy < 15 and 0 <= sx < 21:
+                            want[y, x] = min(want[y, x], a[sy, sx])
+        np.testing.assert_array_equal(d, want)
+
+
+# ---------------------------------------------------------- background
+
+def test_background_closed_form(orc):
+    rng = np.random.default_rng(21)
+    for (h, w) in [(24, 32), (37, 53), (1, 1), (7, 1)]:
+        for shift, thr in ((4, 28), (2, 5), (8, 0)):
+            bg = orc.Background(w, h, shift)
+            ref = None
+            for _ in range(6):
+                frame = rng.integers(0, 256, (h, w)).astype(np.uint8)
+                mask = np.empty_like(frame)
+                bg.apply(frame, mask, thr)
+                fi = frame.astype(np.int32)
+                if ref is None:
+                    ref, want = fi << 8, np.zeros_like(frame)
+                else:
+                    fg = np.abs(fi - (ref >> 8)) > thr
+                    want = np.where(fg, 255, 0).astype(np.uint8)
+                    ref = ref + (((fi << 8) - ref) >> np.where(fg, shift + 2, shift))
+                np.testing.assert_array_equal(mask, want)
+                np.testing.assert_array_equal(bg.model, ref.astype(np.uint16))
+
+
+# --------------------------------------------------- connected components
+
+def test_components_two(orc):
+    c = KA["components_two"]
+    a = np.zeros((c["size"][1], c["size"][0]), np.uint8)
+    for x, y in c["on"]:
+        a[y, x] = 255
+    n, lab, st, ce = orc.connected_components_with_stats(a, 8)
+    assert n == c["count"]
+    for x, y, l in c["label_at"]:
+        assert lab[y, x] == l
+    assert st[1, 4] == c["area1"] and st[1, 2] == c["width1"]
+
+
+def flood_labels(a, conn):
+    """components.zig:563-590 referenceLabels: flood fill visited in raster
+    order (4) or 2x2 block-raster order (8)."""
+    h, w = a.shape
+    out = np.zeros((h, w), np.int32)
+    nxt = 1
+    if conn == 8:
+        order = [(by * 2 + dy, bx * 2 + dx) for by in range((h + 1) // 2) for bx in range((w + 1) // 2)
+                 for dy in range(2) for dx in range(2)]
+    else:
+        order = [(y, x) for y in range(h) for x in range(w)]
+    for y, x in order:
+        if y >= h or x >= w or a[y, x] == 0 or out[y, x]:
+            continue
+        out[y, x] = nxt
+        st = [(y, x)]
+        while st:
+            py, px = st.pop()
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    if (dy == 0 and dx == 0) or (conn == 4 and dx and dy):
+                        continue
+                    ny, nx = py + dy, px + dx
+                    if 0 <= ny < h and 0 <= nx < w and a[ny, nx] and not out[ny, nx]:
+                        out[ny, nx] = nxt
+                        st.append((ny, nx))
+        nxt += 1
+    return nxt, out
+
+
+@pytest.mark.parametrize("pattern", ["random", "sparse", "checker", "stripes", "diag", "spiral", "ones", "zeros"])
+def test_components_match_flood_fill(orc, pattern):
+    rng = np.random.default_rng(42)
+    for (w, h) in [(1, 1), (1, 30), (30, 1), (37, 29), (64, 64)]:
+        yy, xx = np.mgrid[0:h, 0:w]
+        d = np.minimum(np.minimum(xx, yy), np.minimum(w - 1 - xx, h - 1 - yy))
+        m = {"random": rng.random((h, w)) < 0.5, "sparse": rng.random((h, w)) < 0.1, "checker": (xx + yy) % 2 == 0,
+             "stripes": (xx % 3 == 0) | (yy % 50 == 0), "diag": ((xx + yy) % 7 == 0) | ((xx + 2 * h - yy) % 11 == 0),
+             "spiral": (d % 2 == 0) & ~((xx == d + 1) & (yy == d) & (d + 2 < w)), "ones": np.ones((h, w), bool),
+             "zeros": np.zeros((h, w), bool)}[pattern]
+        a = (m * 255).astype(np.uint8)
+        for conn in (4, 8):
+            n, lab, st, ce = orc.connected_components_with_stats(a, conn)
+            rn, rlab = flood_labels(a, conn)
+            assert n == rn
+            np.testing.assert_array_equal(lab, rlab)
+            for l in range(n):
+                ys, xs = np.nonzero(lab == l)
+                if len(xs) == 0:
+                    assert st[l].tolist() == [-1, 2147483647, 0, 0, 0]
+                    continue
+                assert st[l].tolist() == [xs.min(), ys.min(), xs.max() - xs.min() + 1, ys.max() - ys.min() + 1, len(xs)]
+                assert ce[l, 0] == pytest.approx(xs.mean(), rel=1e-12)
+                assert ce[l, 1] == pytest.approx(ys.mean(), rel=1e-12)
+
+
+def test_otsu_maximises_between_class_variance(orc):
+    t = KA["threshold"]
+    a = np.array([t["in"]], np.uint8)
+    th = orc.threshold_auto(a, 8)
+    assert 30 <= th < 200  # threshold.zig:297-298
+    img = (np.clip(np.random.default_rng(5).normal(90, 30, (64, 64)), 0, 255)).astype(np.uint8)
+    img[:, 32:] = np.clip(img[:, 32:].astype(int) + 90, 0, 255)
+    hist = np.bincount(img.ravel(), minlength=256).astype(np.float64)
+    p = hist / hist.sum()
+    best, arg = -1.0, 0
+    for k in range(256):
+        q1 = p[:k + 1].sum()
+        q2 = 1 - q1
+        if min(q1, q2) < 1e-12:
+            continue
+        m1 = (np.arange(k + 1) * p[:k + 1]).sum() / q1
+        m2 = (np.arange(k + 1, 256) * p[k + 1:]).sum() / q2
+        s = q1 * q2 * (m1 - m2) ** 2
+        if s > best * (1 + 1e-12):
+            best, arg = s, k
+    assert orc.threshold_auto(img, 8) == arg

@@ -1,0 +1,276 @@
+"""visar_amd: VISAR's imgproc hot path on MI355X (gfx950) through a C ABI.
+
+This module is the host-side driver of libvisar_amd.so, shaped like the
+reference's ctypes driver (tests/py/visar_ctypes.py): the same function
+names, argument meaning and error behaviour (a VisarError carrying the
+status and visar_last_error()). Arrays may be NumPy arrays (host memory,
+staged by the library) or torch tensors on the GPU (used in place).
+Importing fails loudly when the HIP library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("VISAR_AMD_LIB", os.path.join(HERE, "libvisar_amd.so"))
+
+_DEPTH = {np.uint8: 0, np.int8: 1, np.uint16: 2, np.int16: 3, np.int32: 4, np.float32: 5, np.float64: 6, np.float16: 7}
+
+
+class Image(ctypes.Structure):
+    """visar_image (include/visar.h)."""
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("channels", ctypes.c_uint32),
+        ("depth", ctypes.c_uint32),
+        ("stride", ctypes.c_size_t),
+        ("data", ctypes.c_void_p),
+    ]
+
+
+class VisarError(RuntimeError):
+    def __init__(self, rc, name):
+        super().__init__(f"{rc}: {name}")
+        self.rc = rc
+        self.name = name
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+
+lib = ctypes.CDLL(LIB_PATH)
+lib.visar_version.restype = ctypes.c_char_p
+lib.visar_build_info.restype = ctypes.c_char_p
+lib.visar_last_error.restype = ctypes.c_char_p
+lib.visar_gpu_get_stream.restype = ctypes.c_void_p
+lib.visar_gpu_set_stream.argtypes = [ctypes.c_void_p]
+
+
+def _check(rc):
+    if rc != 0:
+        raise VisarError(rc, lib.visar_last_error().decode())
+
+
+# ------------------------------------------------------------ descriptors
+
+def _torch():
+    import torch
+    return torch
+
+
+def _is_tensor(a):
+    return type(a).__module__.startswith("torch")
+
+
+_TORCH_DEPTH = None
+
+
+def _tdepth(t):
+    global _TORCH_DEPTH
+    if _TORCH_DEPTH is None:
+        torch = _torch()
+        _TORCH_DEPTH = {torch.uint8: 0, torch.int8: 1, torch.uint16: 2, torch.int16: 3, torch.int32: 4,
+                        torch.float32: 5, torch.float64: 6, torch.float16: 7}
+    return _TORCH_DEPTH[t.dtype]
+
+
+def desc(a):
+    """visar_image describing a NumPy array or torch tensor (rows may be strided; pixels must be packed)."""
+    if a.ndim == 2:
+        h, w = a.shape
+        c = 1
+    elif a.ndim == 3:
+        h, w, c = a.shape
+    else:
+        raise ValueError("expected 2-D or 3-D array")
+    if _is_tensor(a):
+        es = a.element_size()
+        st = a.stride()
+        if st[1] != c or (a.ndim == 3 and st[2] != 1):
+            raise ValueError("pixels must be packed")
+        stride = st[0] * es if h > 1 else w * c * es
+        return Image(w, h, c, _tdepth(a), stride, a.data_ptr())
+    t = a.dtype.type
+    if t not in _DEPTH:
+        raise ValueError(f"unsupported dtype {a.dtype}")
+    if a.strides[-1] != a.itemsize or (a.ndim == 3 and a.strides[1] != a.itemsize * c):
+        raise ValueError("pixels must be packed")
+    # NumPy reports stride 0 for a single row; the row length is the stride then.
+    stride = a.strides[0] if h > 1 else w * c * a.itemsize
+    return Image(w, h, c, _DEPTH[t], stride, a.ctypes.data)
+
+
+def _ref(a):
+    return ctypes.byref(desc(a))
+
+
+def _empty(like, shape, dtype=None):
+    if _is_tensor(like):
+        torch = _torch()
+        dt = like.dtype if dtype is None else dtype
+        if not isinstance(dt, torch.dtype):
+            dt = {np.uint8: torch.uint8, np.int16: torch.int16, np.uint16: torch.uint16, np.float32: torch.float32,
+                  np.int32: torch.int32}[np.dtype(dt).type]
+        return torch.empty(shape, dtype=dt, device=like.device)
+    return np.empty(shape, dtype=like.dtype if dtype is None else dtype)
+
+
+def _shape(a, w=None, h=None, c=None):
+    H, W = a.shape[:2]
+    C = a.shape[2] if a.ndim == 3 else 1
+    w = W if w is None else w
+    h = H if h is None else h
+    c = C if c is None else c
+    return (h, w) if c == 1 else (h, w, c)
+
+
+# ------------------------------------------------------------ library
+
+def version():
+    return lib.visar_version().decode()
+
+
+def build_info():
+    return lib.visar_build_info().decode()
+
+
+def set_num_threads(n):
+    lib.visar_set_num_threads(n)
+
+
+def get_num_threads():
+    return lib.visar_get_num_threads()
+
+
+def set_stream(stream):
+    """Stream for this thread's calls: a torch.cuda.Stream, a raw hipStream_t int, or None (library stream)."""
+    if stream is not None and hasattr(stream, "cuda_stream"):
+        stream = stream.cuda_stream
+    _check(lib.visar_gpu_set_stream(stream))
+
+
+def get_stream():
+    return lib.visar_gpu_get_stream()
+
+
+def set_sync(sync):
+    _check(lib.visar_gpu_set_sync(int(bool(sync))))
+
+
+def synchronize():
+    _check(lib.visar_gpu_synchronize())
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(128)
+    _check(lib.visar_gpu_device_info(buf, 128))
+    return buf.value.decode()
+
+
+# ------------------------------------------------------------ imgproc
+
+_COLOR_CN = {6: (3, 1), 7: (3, 1), 10: (4, 1), 11: (4, 1)}
+
+
+def cvt_color(a, code, dst=None):
+    dst = _empty(a, _shape(a, c=_COLOR_CN[code][1])) if dst is None else dst
+    _check(lib.visar_cvt_color(_ref(a), _ref(dst), code))
+    return dst
+
+
+def resize(a, size, interpolation=1, dst=None):
+    dst = _empty(a, _shape(a, w=size[0], h=size[1])) if dst is None else dst
+    _check(lib.visar_resize(_ref(a), _ref(dst), interpolation))
+    return dst
+
+
+def gaussian_blur(a, ksize, sigma_x, sigma_y=0, border=4, dst=None):
+    dst = _empty(a, tuple(a.shape)) if dst is None else dst
+    _check(lib.visar_gaussian_blur(_ref(a), _ref(dst), ksize[0], ksize[1], ctypes.c_double(sigma_x),
+                                   ctypes.c_double(sigma_y), border))
+    return dst
+
+
+def sobel(a, ddepth, dx, dy, ksize=3, scale=1.0, border=4, dst=None):
+    dst = _empty(a, tuple(a.shape), ddepth) if dst is None else dst
+    _check(lib.visar_sobel(_ref(a), _ref(dst), dx, dy, ksize, ctypes.c_double(scale), border))
+    return dst
+
+
+def sep_filter_2d(a, ddepth, kx, ky, border=4, dst=None):
+    dst = _empty(a, tuple(a.shape), ddepth) if dst is None else dst
+    kx = np.ascontiguousarray(kx, np.float32)
+    ky = np.ascontiguousarray(ky, np.float32)
+    _check(lib.visar_sep_filter_2d(_ref(a), _ref(dst), kx.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(len(kx)),
+                                   ky.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(len(ky)), border))
+    return dst
+
+
+def threshold(a, thresh, maxval, kind, dst=None):
+    dst = _empty(a, tuple(a.shape)) if dst is None else dst
+    used = ctypes.c_double(0)
+    _check(lib.visar_threshold(_ref(a), _ref(dst), ctypes.c_double(thresh), ctypes.c_double(maxval), kind,
+                               ctypes.byref(used)))
+    return used.value, dst
+
+
+def morphology(a, op, shape, ksize, iterations=1, dst=None):
+    dst = _empty(a, tuple(a.shape)) if dst is None else dst
+    _check(lib.visar_morphology(_ref(a), _ref(dst), op, shape, ksize[0], ksize[1], iterations))
+    return dst
+
+
+def warp_affine(a, m, size, interpolation=1, border=0, value=0.0, dst=None):
+    dst = _empty(a, _shape(a, w=size[0], h=size[1])) if dst is None else dst
+    arr = (ctypes.c_double * 6)(*np.asarray(m, np.float64).ravel()[:6])
+    _check(lib.visar_warp_affine(_ref(a), _ref(dst), arr, interpolation, border, ctypes.c_double(value)))
+    return dst
+
+
+def warp_perspective(a, m, size, interpolation=1, border=0, value=0.0, dst=None):
+    dst = _empty(a, _shape(a, w=size[0], h=size[1])) if dst is None else dst
+    arr = (ctypes.c_double * 9)(*np.asarray(m, np.float64).ravel()[:9])
+    _check(lib.visar_warp_perspective(_ref(a), _ref(dst), arr, interpolation, border, ctypes.c_double(value)))
+    return dst
+
+
+def remap(a, mapx, mapy, value=0.0, dst=None):
+    dst = _empty(a, _shape(a, w=mapx.shape[1], h=mapx.shape[0])) if dst is None else dst
+    _check(lib.visar_remap(_ref(a), _ref(dst), _ref(mapx), _ref(mapy), ctypes.c_double(value)))
+    return dst
+
+
+class Background:
+    """Running-average background model with its state in device memory."""
+
+    def __init__(self, width, height, shift=4):
+        self._h = ctypes.c_void_p()
+        _check(lib.visar_background_create(ctypes.c_uint32(width), ctypes.c_uint32(height), shift,
+                                           ctypes.byref(self._h)))
+
+    def apply(self, frame, mask, threshold=28):
+        _check(lib.visar_background_apply(self._h, _ref(frame), _ref(mask), threshold))
+        return mask
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib.visar_background_free(self._h)
+            self._h = None
+
+
+def connected_components_with_stats(a, connectivity=8):
+    """(count, labels i32, stats (count, 5) i32, centroids (count, 2) f64) like the reference driver."""
+    h, w = a.shape[:2]
+    labels = np.empty((h, w), np.int32)
+    cap = min(h * w + 1, 1 << 22)
+    stats = np.empty((cap, 5), np.int32)
+    cent = np.empty((cap, 2), np.float64)
+    count = ctypes.c_size_t(0)
+    src = a.cpu().numpy() if _is_tensor(a) else a
+    _check(lib.visar_connected_components(_ref(src), _ref(labels), connectivity, stats.ctypes.data_as(ctypes.c_void_p),
+                                          cent.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(cap),
+                                          ctypes.byref(count)))
+    n = count.value
+    return n, labels, stats[:n].copy(), cent[:n].copy()

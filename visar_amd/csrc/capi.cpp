@@ -1,0 +1,951 @@
+// capi.cpp -- the C ABI of include/visar.h and include/visar_gpu.h.
+//
+// Mirrors the reference's C-ABI layer (src/capi.zig, src/capi_calib3d.zig):
+// descriptor validation (capi.zig:123-132), error names copied into a
+// thread-local buffer (capi.zig:59-75), depth dispatch (capi.zig:136-144)
+// and per-call argument checks. Underneath, every call resolves its images
+// to device views: device buffers are used in place, host memory is staged
+// through pooled device buffers with 2-D copies on the call's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/visar.h"
+#include "../../include/visar_gpu.h"
+#include "kernels.hpp"
+#include "runtime.hpp"
+#include "tables.hpp"
+
+namespace visar {
+
+// ------------------------------------------------------------- errors
+
+static thread_local char g_last_error[256] = "";
+
+int fail(int status, const char *name) {
+    std::snprintf(g_last_error, sizeof g_last_error, "%s", name);
+    return status;
+}
+
+// ------------------------------------------------------------ runtime
+
+static std::once_flag g_init_once;
+static int g_init_status = INTERNAL;
+static hipStream_t g_own_stream = nullptr;
+static thread_local hipStream_t t_stream = nullptr;
+static thread_local bool t_stream_set = false;
+static thread_local int t_sync = 1;
+
+int ensure_runtime() {
+    std::call_once(g_init_once, [] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+            (void)hipGetLastError();
+            g_init_status = INTERNAL;
+            return;
+        }
+        if (hipStreamCreateWithFlags(&g_own_stream, hipStreamNonBlocking) != hipSuccess) {
+            g_init_status = INTERNAL;
+            return;
+        }
+        g_init_status = OK;
+    });
+    if (g_init_status != OK) return fail(INTERNAL, "NoDevice");
+    return OK;
+}
+
+hipStream_t current_stream() { return t_stream_set ? t_stream : g_own_stream; }
+bool sync_mode() { return t_sync != 0; }
+
+// Device buffers are reused by size class: a staged call takes the smallest
+// cached buffer that fits, and returns it once its work has finished.
+class BufferPool {
+  public:
+    void *get(size_t bytes) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto it = free_.lower_bound(bytes);
+            if (it != free_.end() && it->first <= bytes * 2 + (1 << 20)) {
+                void *p = it->second;
+                sizes_[p] = it->first;
+                free_.erase(it);
+                return p;
+            }
+        }
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            trim();
+            if (hipMalloc(&p, bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                return nullptr;
+            }
+        }
+        std::lock_guard<std::mutex> g(mu_);
+        sizes_[p] = bytes;
+        return p;
+    }
+    void put(void *p) {
+        if (!p) return;
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = sizes_.find(p);
+        if (it == sizes_.end()) return;
+        free_.emplace(it->second, p);
+        sizes_.erase(it);
+    }
+    void trim() {
+        std::lock_guard<std::mutex> g(mu_);
+        for (auto &kv : free_) (void)hipFree(kv.second);
+        free_.clear();
+    }
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void *> free_;
+    std::map<void *, size_t> sizes_;
+};
+
+static BufferPool &pool() {
+    static BufferPool *p = new BufferPool();  // never destroyed: outlives static HIP teardown
+    return *p;
+}
+
+void *pool_get(size_t bytes) { return pool().get(bytes); }
+void pool_put(void *p) { pool().put(p); }
+
+bool is_device_ptr(const void *p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// ------------------------------------------------------------- images
+
+static inline size_t row_bytes(const visar_image *c) { return (size_t)c->width * c->channels * depth_size(c->depth); }
+static inline size_t span_bytes(const visar_image *c) { return (c->height - 1) * c->stride + row_bytes(c); }
+
+// capi.zig:123-132 typed() + image.zig:71-77, 172-176.
+int check_image(const visar_image *c, uint32_t depth) {
+    if (!c || !c->data) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (c->depth != depth) return fail(UNSUPPORTED, "Unsupported");
+    if (c->channels == 0 || c->channels > 512) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    size_t es = depth_size(depth);
+    if (es == 0 || c->stride % es != 0) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (c->width == 0 || c->height == 0) return fail(INVALID_ARGUMENT, "InvalidSize");
+    if (c->stride < row_bytes(c)) return fail(INVALID_ARGUMENT, "InvalidSize");
+    if ((uint64_t)c->width * c->channels > (1u << 30) || c->height > (1u << 30)) return fail(INVALID_ARGUMENT, "InvalidSize");
+    return OK;
+}
+
+Call::~Call() {
+    for (void *p : owned_) pool_put(p);
+}
+
+int Call::begin() {
+    int rc = ensure_runtime();
+    if (rc) return rc;
+    st_ = current_stream();
+    return OK;
+}
+
+DImg Call::view(const visar_image *c, void *p, size_t stride) {
+    DImg d;
+    d.p = static_cast<uint8_t *>(p);
+    d.w = (int)c->width;
+    d.h = (int)c->height;
+    d.cn = (int)c->channels;
+    d.stride = (long)stride;
+    return d;
+}
+
+void *Call::scratch(size_t bytes) {
+    void *p = pool_get(bytes ? bytes : 1);
+    if (p) owned_.push_back(p);
+    return p;
+}
+
+int Call::input(const visar_image *c, DImg &out) {
+    if (is_device_ptr(c->data)) {
+        out = view(c, c->data, c->stride);
+        dev_ranges_.push_back({(const uint8_t *)c->data, span_bytes(c)});
+        return OK;
+    }
+    size_t rb = row_bytes(c);
+    size_t pitch = (rb + 255) & ~(size_t)255;
+    void *buf = scratch(pitch * c->height);
+    if (!buf) return fail(OUT_OF_MEMORY, "OutOfMemory");
+    if (hipMemcpy2DAsync(buf, pitch, c->data, c->stride, rb, c->height, hipMemcpyHostToDevice, st_) != hipSuccess)
+        return fail(INTERNAL, "CopyFailed");
+    staged_ = true;
+    out = view(c, buf, pitch);
+    return OK;
+}
+
+int Call::output(const visar_image *c, DImg &out) {
+    if (is_device_ptr(c->data)) {
+        out = view(c, c->data, c->stride);
+        // An output that overlaps a device input is computed into a
+        // temporary and copied over afterwards (kernels read neighbours).
+        const uint8_t *p = (const uint8_t *)c->data;
+        size_t n = span_bytes(c);
+        bool overlap = false;
+        for (auto &r : dev_ranges_)
+            if (p < r.first + r.second && r.first < p + n) overlap = true;
+        if (!overlap) return OK;
+        size_t rb = row_bytes(c);
+        size_t pitch = (rb + 255) & ~(size_t)255;
+        void *buf = scratch(pitch * c->height);
+        if (!buf) return fail(OUT_OF_MEMORY, "OutOfMemory");
+        pending_.push_back({c, buf, pitch, true});
+        out = view(c, buf, pitch);
+        return OK;
+    }
+    size_t rb = row_bytes(c);
+    size_t pitch = (rb + 255) & ~(size_t)255;
+    void *buf = scratch(pitch * c->height);
+    if (!buf) return fail(OUT_OF_MEMORY, "OutOfMemory");
+    staged_ = true;
+    pending_.push_back({c, buf, pitch, false});
+    out = view(c, buf, pitch);
+    return OK;
+}
+
+int Call::temp_like(const DImg &like, size_t elem, DImg &out) {
+    size_t rb = (size_t)like.w * like.cn * elem;
+    size_t pitch = (rb + 255) & ~(size_t)255;
+    void *buf = scratch(pitch * like.h);
+    if (!buf) return fail(OUT_OF_MEMORY, "OutOfMemory");
+    out = like;
+    out.p = (uint8_t *)buf;
+    out.stride = (long)pitch;
+    return OK;
+}
+
+int Call::finish(int launch_status) {
+    if (launch_status != OK) {
+        (void)hipStreamSynchronize(st_);
+        return launch_status == UNSUPPORTED ? fail(UNSUPPORTED, "Unsupported") : fail(INTERNAL, "LaunchFailed");
+    }
+    for (auto &pd : pending_) {
+        const visar_image *c = pd.desc;
+        hipMemcpyKind kind = pd.device_target ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        if (hipMemcpy2DAsync(c->data, c->stride, pd.buf, pd.pitch, row_bytes(c), c->height, kind, st_) != hipSuccess)
+            return fail(INTERNAL, "CopyFailed");
+    }
+    if (staged_ || sync_mode() || !owned_.empty()) {
+        if (hipStreamSynchronize(st_) != hipSuccess) return fail(INTERNAL, "KernelFailed");
+    }
+    return OK;
+}
+
+}  // namespace visar
+
+using namespace visar;
+
+// ================================================================ C ABI
+
+extern "C" {
+
+const char *visar_version(void) { return "0.1.0"; }
+
+const char *visar_build_info(void) { return "visar_amd gfx950 (HIP)"; }
+
+const char *visar_last_error(void) { return g_last_error; }
+
+static std::atomic<int> g_threads{0};
+
+void visar_set_num_threads(int n) { g_threads.store(n <= 0 ? 0 : n); }
+
+int visar_get_num_threads(void) {
+    int n = g_threads.load();
+    if (n > 0) return n;
+    unsigned h = std::thread::hardware_concurrency();
+    return h ? (int)h : 1;
+}
+
+int visar_gpu_set_stream(void *s) {
+    t_stream = (hipStream_t)s;
+    t_stream_set = s != nullptr;
+    return VISAR_OK;
+}
+
+void *visar_gpu_get_stream(void) {
+    if (ensure_runtime()) return nullptr;
+    return (void *)current_stream();
+}
+
+int visar_gpu_set_sync(int sync) {
+    t_sync = sync != 0;
+    return VISAR_OK;
+}
+
+int visar_gpu_synchronize(void) {
+    int rc = ensure_runtime();
+    if (rc) return rc;
+    return hipStreamSynchronize(current_stream()) == hipSuccess ? VISAR_OK : fail(INTERNAL, "KernelFailed");
+}
+
+int visar_gpu_malloc(size_t bytes, void **out) {
+    int rc = ensure_runtime();
+    if (rc) return rc;
+    if (!out) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(OUT_OF_MEMORY, "OutOfMemory");
+    }
+    return VISAR_OK;
+}
+
+int visar_gpu_free(void *p) {
+    if (!p) return VISAR_OK;
+    return hipFree(p) == hipSuccess ? VISAR_OK : fail(INVALID_ARGUMENT, "InvalidArgument");
+}
+
+int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind) {
+    int rc = ensure_runtime();
+    if (rc) return rc;
+    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    if (kind < 1 || kind > 3 || !dst || !src) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (hipMemcpyAsync(dst, src, bytes, k, current_stream()) != hipSuccess) return fail(INTERNAL, "CopyFailed");
+    return hipStreamSynchronize(current_stream()) == hipSuccess ? VISAR_OK : fail(INTERNAL, "CopyFailed");
+}
+
+int visar_gpu_device_info(char *buf, size_t n) {
+    int rc = ensure_runtime();
+    if (rc) return rc;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return fail(INTERNAL, "NoDevice");
+    if (buf && n) std::snprintf(buf, n, "%s %d CUs", prop.gcnArchName, prop.multiProcessorCount);
+    return VISAR_OK;
+}
+
+// ------------------------------------------------------------ filters
+
+static bool valid_depth(uint32_t d) { return d == 0 || d == 2 || d == 3 || d == 5; }
+
+static bool same_shape(const visar_image *a, const visar_image *b) {
+    return a->width == b->width && a->height == b->height && a->channels == b->channels;
+}
+
+// capi.zig:197-206 + filter.zig:498-607
+int visar_gaussian_blur(const visar_image *src, const visar_image *dst, int kw, int kh, double sx, double sy,
+                        int border) {
+    if (border < 0 || border > 4) return fail(UNSUPPORTED, "Unsupported");
+    if (kw < 0 || kh < 0) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (!src) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    uint32_t T = src->depth;
+    if (!valid_depth(T)) return fail(UNSUPPORTED, "Unsupported");
+    int rc;
+    if ((rc = check_image(src, T)) || (rc = check_image(dst, T))) return rc;
+    double sy_in = sy > 0 ? sy : sx;
+    if (kw == 0) kw = gaussian_ksize(sx, T == 0);
+    if (kh == 0) kh = gaussian_ksize(sy_in, T == 0);
+    if (kw <= 0 || kh <= 0 || kw % 2 == 0 || kh % 2 == 0 || kw > 127 || kh > 127)
+        return fail(INVALID_ARGUMENT, "InvalidKernel");
+    if (!same_shape(src, dst)) return fail(SIZE_MISMATCH, "SizeMismatch");
+    if (border != BORDER_CONSTANT) {
+        if (src->width == 1) kw = 1;
+        if (src->height == 1) kh = 1;
+    }
+    SepParams p;
+    std::memset(&p, 0, sizeof p);
+    p.sdepth = p.ddepth = (int)T;
+    p.nkx = kw;
+    p.nky = kh;
+    p.border = border;
+    p.scale = 1.0f;
+    if (T == 0 || T == 2) {
+        uint32_t fx[kMaxTaps], fy[kMaxTaps];
+        gaussian_kernel_fixed(fx, kw, sx, T == 0 ? 8 : 16);
+        gaussian_kernel_fixed(fy, kh, sy_in, T == 0 ? 8 : 16);
+        for (int i = 0; i < kw; i++) p.kxi[i] = (int32_t)fx[i];
+        for (int i = 0; i < kh; i++) p.kyi[i] = (int32_t)fy[i];
+        p.fin = T == 0 ? FIN_FIX8 : FIN_FIX16;
+    } else {
+        double kxd[kMaxTaps], kyd[kMaxTaps];
+        gaussian_kernel(kxd, kw, sx);
+        gaussian_kernel(kyd, kh, sy_in);
+        for (int i = 0; i < kw; i++) p.kxf[i] = (float)kxd[i];
+        for (int i = 0; i < kh; i++) p.kyf[i] = (float)kyd[i];
+        p.f32mode = 1;
+        p.fin = FIN_CAST;
+        p.vsym = symmetry(p.kyf, kh);
+        p.hsym = symmetry(p.kxf, kw);
+        // filter.zig:586-600: f32 images with OpenCV's 3x3 / 5x5 tables take
+        // the compile-time kernel path (symmetric body, tail summed from 0).
+        static const float t3[3] = {0.25f, 0.5f, 0.25f};
+        static const float t5[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+        if (T == 5 && kw == kh && std::equal(p.kxf, p.kxf + kw, p.kyf) &&
+            ((kw == 3 && std::equal(p.kxf, p.kxf + 3, t3)) || (kw == 5 && std::equal(p.kxf, p.kxf + 5, t5)))) {
+            p.const_kernel = 1;
+            p.vsym = p.hsym = SYM_EVEN;
+        }
+        p.body = ((long)src->width * src->channels / 16) * 16;
+    }
+    Call call;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, p.src)) || (rc = call.output(dst, p.dst))) return rc;
+    return call.finish(launch_sep(p, call.stream()));
+}
+
+// capi.zig:218-232 + filter.zig:856-906
+int visar_sobel(const visar_image *src, const visar_image *dst, int dx, int dy, int ksize, double scale, int border) {
+    if (border < 0 || border > 4) return fail(UNSUPPORTED, "Unsupported");
+    if (dx < 0 || dy < 0) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (!src || !dst) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    uint32_t S = src->depth;
+    if (!valid_depth(S)) return fail(UNSUPPORTED, "Unsupported");
+    int rc;
+    if ((rc = check_image(src, S))) return rc;
+    if (dst->depth == 3) {
+        if (S != 0) return fail(UNSUPPORTED, "Unsupported");
+    } else if (dst->depth != 5) {
+        return fail(UNSUPPORTED, "Unsupported");
+    }
+    if ((rc = check_image(dst, dst->depth))) return rc;
+    // OpenCV's apertures; larger Pascal kernels overflow the reference's i32 sums.
+    if (ksize != -1 && ksize != 1 && ksize != 3 && ksize != 5 && ksize != 7) return fail(INVALID_ARGUMENT, "InvalidKernel");
+    int n = ksize <= 1 ? 3 : ksize;
+    int nx = (ksize == 1 && dx > 0) ? 3 : n, ny = (ksize == 1 && dy > 0) ? 3 : n;
+    int32_t kx[32], ky[32];
+    if (!deriv_kernel(kx, nx, dx, (ksize == 1 && dx == 0) ? 1 : (ksize == 1 ? 3 : ksize)) ||
+        !deriv_kernel(ky, ny, dy, (ksize == 1 && dy == 0) ? 1 : (ksize == 1 ? 3 : ksize)))
+        return fail(INVALID_ARGUMENT, "InvalidKernel");
+    if (!same_shape(src, dst)) return fail(SIZE_MISMATCH, "SizeMismatch");
+    SepParams p;
+    std::memset(&p, 0, sizeof p);
+    p.sdepth = (int)S;
+    p.ddepth = (int)dst->depth;
+    p.nkx = nx;
+    p.nky = ny;
+    p.border = border;
+    p.scale = (float)scale;
+    if (dst->depth == 5 || S == 5) {
+        for (int i = 0; i < nx; i++) p.kxf[i] = (float)kx[i];
+        for (int i = 0; i < ny; i++) p.kyf[i] = (float)ky[i];
+        p.f32mode = 1;
+        p.fin = FIN_SCALED;
+        p.vsym = symmetry(p.kyf, ny);
+        p.hsym = symmetry(p.kxf, nx);
+        p.body = ((long)src->width * src->channels / 16) * 16;
+    } else {
+        for (int i = 0; i < nx; i++) p.kxi[i] = kx[i];
+        for (int i = 0; i < ny; i++) p.kyi[i] = ky[i];
+        p.fin = scale == 1 ? FIN_CAST : FIN_SCALED;
+    }
+    Call call;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, p.src)) || (rc = call.output(dst, p.dst))) return rc;
+    return call.finish(launch_sep(p, call.stream()));
+}
+
+// filter.zig:909-914 (sepFilter2D: f32 taps and accumulation, FinishCast)
+int visar_sep_filter_2d(const visar_image *src, const visar_image *dst, const float *kx, size_t nkx, const float *ky,
+                        size_t nky, int border) {
+    if (border < 0 || border > 4) return fail(UNSUPPORTED, "Unsupported");
+    if (!src || !dst) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (!valid_depth(src->depth) || !valid_depth(dst->depth)) return fail(UNSUPPORTED, "Unsupported");
+    int rc;
+    if ((rc = check_image(src, src->depth)) || (rc = check_image(dst, dst->depth))) return rc;
+    if (!same_shape(src, dst)) return fail(SIZE_MISMATCH, "SizeMismatch");
+    if (!kx || !ky || nkx == 0 || nky == 0 || nkx % 2 == 0 || nky % 2 == 0 || nkx > 127 || nky > 127)
+        return fail(INVALID_ARGUMENT, "InvalidKernel");
+    SepParams p;
+    std::memset(&p, 0, sizeof p);
+    p.sdepth = (int)src->depth;
+    p.ddepth = (int)dst->depth;
+    p.nkx = (int)nkx;
+    p.nky = (int)nky;
+    p.border = border;
+    p.scale = 1.0f;
+    std::memcpy(p.kxf, kx, sizeof(float) * nkx);
+    std::memcpy(p.kyf, ky, sizeof(float) * nky);
+    p.f32mode = 1;
+    p.fin = FIN_CAST;
+    p.vsym = symmetry(p.kyf, (int)nky);
+    p.hsym = symmetry(p.kxf, (int)nkx);
+    p.body = ((long)src->width * src->channels / 16) * 16;
+    Call call;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, p.src)) || (rc = call.output(dst, p.dst))) return rc;
+    return call.finish(launch_sep(p, call.stream()));
+}
+
+// ----------------------------------------------------- color, threshold
+
+// capi.zig:149-161 + color.zig:88-105 (gray codes)
+int visar_cvt_color(const visar_image *src, const visar_image *dst, int code) {
+    int scn, sb;
+    switch (code) {
+    case 6: scn = 3, sb = 0; break;
+    case 7: scn = 3, sb = 2; break;
+    case 10: scn = 4, sb = 0; break;
+    case 11: scn = 4, sb = 2; break;
+    default: return fail(UNSUPPORTED, "Unsupported");
+    }
+    if (!src) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    uint32_t T = src->depth;
+    if (!valid_depth(T)) return fail(UNSUPPORTED, "Unsupported");
+    int rc;
+    if ((rc = check_image(src, T)) || (rc = check_image(dst, T))) return rc;
+    if (src->width != dst->width || src->height != dst->height) return fail(SIZE_MISMATCH, "SizeMismatch");
+    if ((int)src->channels != scn || dst->channels != 1) return fail(UNSUPPORTED, "UnsupportedChannels");
+    if (T == 3) return fail(UNSUPPORTED, "Unsupported");
+    Call call;
+    DImg s, d;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, s)) || (rc = call.output(dst, d))) return rc;
+    return call.finish(launch_gray(s, d, (int)T, scn, sb, call.stream()));
+}
+
+// threshold.zig:184-207 getThreshVal_Otsu_8u
+static double otsu(const uint32_t *hist, double total) {
+    const double scale = 1.0 / total;
+    const double eps = 2.220446049250313e-16;
+    double mu = 0;
+    for (int i = 0; i < 256; i++) mu += (double)i * (double)hist[i];
+    mu *= scale;
+    double mu1 = 0, q1 = 0, max_sigma = 0, max_val = 0;
+    for (int i = 0; i < 256; i++) {
+        double p_i = (double)hist[i] * scale;
+        mu1 *= q1;
+        q1 += p_i;
+        double q2 = 1.0 - q1;
+        if (std::min(q1, q2) < eps || std::max(q1, q2) > 1.0 - eps) continue;
+        mu1 = (mu1 + (double)i * p_i) / q1;
+        double mu2 = (mu - q1 * mu1) / q2;
+        double sigma = q1 * q2 * (mu1 - mu2) * (mu1 - mu2);
+        if (sigma > max_sigma) max_sigma = sigma, max_val = (double)i;
+    }
+    return max_val;
+}
+
+// threshold.zig:210-253 getThreshVal_Triangle_8u
+static double triangle(const uint32_t *hist_in) {
+    int left = 0, right = 0, max_ind = 0;
+    uint32_t mx = 0;
+    for (int i = 0; i < 256; i++)
+        if (hist_in[i] > 0) {
+            left = i;
+            break;
+        }
+    if (left > 0) left -= 1;
+    for (int i = 255; i > 0; i--)
+        if (hist_in[i] > 0) {
+            right = i;
+            break;
+        }
+    if (right < 255) right += 1;
+    for (int i = 0; i < 256; i++)
+        if (hist_in[i] > mx) mx = hist_in[i], max_ind = i;
+    uint32_t h[256];
+    std::memcpy(h, hist_in, sizeof h);
+    bool flip = false;
+    if (max_ind - left < right - max_ind) {
+        flip = true;
+        std::reverse(h, h + 256);
+        left = 255 - right;
+        max_ind = 255 - max_ind;
+    }
+    int thresh = left;
+    double a = (double)mx, b = (double)(left - max_ind), dist = 0;
+    for (int j = left + 1; j <= max_ind; j++) {
+        double t = a * (double)j + b * (double)h[j];
+        if (t > dist) dist = t, thresh = j;
+    }
+    thresh -= 1;
+    if (flip) thresh = 255 - thresh;
+    return (double)thresh;
+}
+
+// capi.zig:310-322 + threshold.zig:25-82
+int visar_threshold(const visar_image *src, const visar_image *dst, double thresh, double maxval, int type,
+                    double *used) {
+    int kind = type & 7;
+    if (kind > 4) return fail(UNSUPPORTED, "Unsupported");
+    int autom = (type & 8) ? 1 : (type & 16) ? 2 : 0;
+    if (!src) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    uint32_t T = src->depth;
+    if (!valid_depth(T)) return fail(UNSUPPORTED, "Unsupported");
+    int rc;
+    if ((rc = check_image(src, T)) || (rc = check_image(dst, T))) return rc;
+    if (!same_shape(src, dst)) return fail(SIZE_MISMATCH, "SizeMismatch");
+    Call call;
+    ThreshParams p;
+    std::memset(&p, 0, sizeof p);
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, p.s))) return rc;
+    if (autom) {
+        if (T != 0 || src->channels != 1) return fail(UNSUPPORTED, "Unsupported");
+        uint32_t *hd = (uint32_t *)call.scratch(256 * sizeof(uint32_t));
+        if (!hd) return fail(OUT_OF_MEMORY, "OutOfMemory");
+        uint32_t hist[256];
+        if (hipMemsetAsync(hd, 0, sizeof hist, call.stream()) != hipSuccess ||
+            launch_hist_u8(p.s, hd, call.stream()) != OK ||
+            hipMemcpyAsync(hist, hd, sizeof hist, hipMemcpyDeviceToHost, call.stream()) != hipSuccess ||
+            hipStreamSynchronize(call.stream()) != hipSuccess)
+            return fail(INTERNAL, "KernelFailed");
+        thresh = autom == 1 ? otsu(hist, (double)src->width * src->height) : triangle(hist);
+    }
+    if (used) *used = thresh;
+    p.depth = (int)T;
+    p.kind = kind;
+    if (T == 0) {
+        // threshold.zig:40-60: floored threshold; the LUT covers every case.
+        double fl = std::floor(thresh);
+        long ti = fl < -2147483648.0 ? -2147483648L : fl > 2147483647.0 ? 2147483647L : (long)fl;
+        int m = sat_cast<uint8_t>(maxval);
+        int tc = ti < 0 ? 0 : ti > 255 ? 255 : (int)ti;
+        for (int i = 0; i < 256; i++) {
+            bool gt = i > ti;
+            int v;
+            switch (kind) {
+            case 0: v = gt ? m : 0; break;
+            case 1: v = gt ? 0 : m; break;
+            case 2: v = gt ? tc : i; break;
+            case 3: v = gt ? i : 0; break;
+            default: v = gt ? 0 : i; break;
+            }
+            p.lut[i] = (uint8_t)v;
+        }
+    } else if (T == 5) {
+        p.t = (double)(float)thresh;
+        p.m = (double)(float)maxval;
+    } else {
+        // threshold.zig:62-81: thresholds below the range make every value
+        // "above" (tozero copies, binary fills maxval, the rest give 0).
+        double lo = T == 2 ? 0 : -32768.0;
+        p.below = std::floor(thresh) < lo;
+        p.t = T == 2 ? (double)sat_cast<uint16_t>(std::floor(thresh)) : (double)sat_cast<int16_t>(std::floor(thresh));
+        p.m = T == 2 ? (double)sat_cast<uint16_t>(maxval) : (double)sat_cast<int16_t>(maxval);
+    }
+    DImg d;
+    if ((rc = call.output(dst, d))) return rc;
+    p.d = d;
+    return call.finish(launch_threshold(p, call.stream()));
+}
+
+// ------------------------------------------------------------- resize
+
+// capi.zig:184-191 + resize.zig:28-62
+int visar_resize(const visar_image *src, const visar_image *dst, int interp) {
+    if (interp < 0 || interp > 4) return fail(UNSUPPORTED, "Unsupported");
+    if (!src) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    uint32_t T = src->depth;
+    if (!valid_depth(T)) return fail(UNSUPPORTED, "Unsupported");
+    int rc;
+    if ((rc = check_image(src, T)) || (rc = check_image(dst, T))) return rc;
+    if (src->channels != dst->channels) return fail(SIZE_MISMATCH, "SizeMismatch");
+    const bool same = src->width == dst->width && src->height == dst->height;
+    const bool u8fast = T == 0 && src->channels <= 4;
+    if (!same && interp != 0 && !(u8fast && (interp == 1 || interp == 3)))
+        return fail(UNSUPPORTED, "Unsupported");
+    Call call;
+    DImg s, d;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, s)) || (rc = call.output(dst, d))) return rc;
+    hipStream_t st = call.stream();
+    if (same) return call.finish(launch_copy(s, d, depth_size(T), st));
+    if (interp == 0) return call.finish(launch_nearest(s, d, depth_size(T), st));
+    const bool exact_half = src->width == 2 * dst->width && src->height == 2 * dst->height;
+    bool area_up = interp == 3 && (dst->width > src->width || dst->height > src->height);
+    if (interp == 1 || area_up) {
+        if (exact_half && interp == 1) return call.finish(launch_half_u8(s, d, st));
+        // resize.zig:216-242: x taps pinned to the edges, y taps not.
+        std::vector<LinTap> tabs(dst->width + dst->height);
+        for (uint32_t x = 0; x < dst->width; x++) {
+            LinTap t = linear_tap_u8(x, src->width, dst->width, area_up, true);
+            t.o0 *= (int32_t)src->channels;
+            t.o1 *= (int32_t)src->channels;
+            tabs[x] = t;
+        }
+        for (uint32_t y = 0; y < dst->height; y++) tabs[dst->width + y] = linear_tap_u8(y, src->height, dst->height, area_up, false);
+        LinTap *dev = (LinTap *)call.scratch(tabs.size() * sizeof(LinTap));
+        if (!dev) return fail(OUT_OF_MEMORY, "OutOfMemory");
+        if (hipMemcpyAsync(dev, tabs.data(), tabs.size() * sizeof(LinTap), hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(INTERNAL, "CopyFailed");
+        return call.finish(launch_linear_u8(s, d, dev, dev + dst->width, st));
+    }
+    // INTER_AREA shrinking
+    if (exact_half) return call.finish(launch_half_u8(s, d, st));
+    uint32_t ix = src->width / dst->width, iy = src->height / dst->height;
+    if (ix * dst->width == src->width && iy * dst->height == src->height)
+        return call.finish(launch_area_int_u8(s, d, (int)ix, (int)iy, st));
+    std::vector<AreaEntry> xe, ye;
+    std::vector<int32_t> xs, ys;
+    area_tab(src->width, dst->width, (double)src->width / (double)dst->width, xe, xs);
+    area_tab(src->height, dst->height, (double)src->height / (double)dst->height, ye, ys);
+    size_t bytes = (xe.size() + ye.size()) * sizeof(AreaEntry) + (xs.size() + ys.size()) * sizeof(int32_t);
+    std::vector<uint8_t> blob(bytes);
+    size_t o = 0;
+    auto put = [&](const void *p, size_t n) {
+        std::memcpy(blob.data() + o, p, n);
+        o += n;
+    };
+    put(xe.data(), xe.size() * sizeof(AreaEntry));
+    put(ye.data(), ye.size() * sizeof(AreaEntry));
+    put(xs.data(), xs.size() * sizeof(int32_t));
+    put(ys.data(), ys.size() * sizeof(int32_t));
+    uint8_t *dev = (uint8_t *)call.scratch(bytes);
+    if (!dev) return fail(OUT_OF_MEMORY, "OutOfMemory");
+    if (hipMemcpyAsync(dev, blob.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess) return fail(INTERNAL, "CopyFailed");
+    const AreaEntry *dxe = (const AreaEntry *)dev;
+    const AreaEntry *dye = dxe + xe.size();
+    const int32_t *dxs = (const int32_t *)(dye + ye.size());
+    const int32_t *dys = dxs + xs.size();
+    return call.finish(launch_area_frac_u8(s, d, dxe, dxs, dye, dys, st));
+}
+
+// ------------------------------------------------------------------ warps
+
+// capi.zig:372-380 + geometry.zig:410-426
+int visar_warp_affine(const visar_image *src, const visar_image *dst, const double m[6], int interp, int border,
+                      double value) {
+    if (border < 0 || border > 4) return fail(UNSUPPORTED, "Unsupported");
+    if (!src) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    uint32_t T = src->depth;
+    if (!valid_depth(T)) return fail(UNSUPPORTED, "Unsupported");
+    int rc;
+    if ((rc = check_image(src, T)) || (rc = check_image(dst, T))) return rc;
+    if (!m) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    double inv[6];
+    if (!invert_affine(m, inv)) return fail(UNSUPPORTED, "Unsupported");
+    if (src->channels != dst->channels) return fail(SIZE_MISMATCH, "SizeMismatch");
+    AffineParams p;
+    std::memset(&p, 0, sizeof p);
+    p.depth = (int)T;
+    p.interp = interp == 0 ? 0 : 1;
+    p.border = border;
+    for (int i = 0; i < 6; i++) p.m[i] = (float)inv[i];
+    switch (T) {
+    case 0: p.cvd = sat_cast<uint8_t>(value); break;
+    case 2: p.cvd = sat_cast<uint16_t>(value); break;
+    case 3: p.cvd = sat_cast<int16_t>(value); break;
+    default: p.cvd = (double)(float)value; break;
+    }
+    p.cv = (float)p.cvd;
+    Call call;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, p.s)) || (rc = call.output(dst, p.d))) return rc;
+    return call.finish(launch_warp_affine(p, call.stream()));
+}
+
+// geometry.zig:586-589, 615-672
+int visar_warp_perspective(const visar_image *src, const visar_image *dst, const double m[9], int interp, int border,
+                           double value) {
+    if (border < 0 || border > 4) return fail(UNSUPPORTED, "Unsupported");
+    if (!src) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    uint32_t T = src->depth;
+    if (!valid_depth(T)) return fail(UNSUPPORTED, "Unsupported");
+    int rc;
+    if ((rc = check_image(src, T)) || (rc = check_image(dst, T))) return rc;
+    if (!m) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    PerspParams p;
+    std::memset(&p, 0, sizeof p);
+    if (!invert3(m, p.m)) return fail(UNSUPPORTED, "Unsupported");
+    if (src->channels != dst->channels) return fail(SIZE_MISMATCH, "SizeMismatch");
+    p.depth = (int)T;
+    p.interp = interp == 0 ? 0 : 1;
+    p.border = border;
+    switch (T) {
+    case 0: p.cvd = sat_cast<uint8_t>(value); break;
+    case 2: p.cvd = sat_cast<uint16_t>(value); break;
+    case 3: p.cvd = sat_cast<int16_t>(value); break;
+    default: p.cvd = (double)(float)value; break;
+    }
+    Call call;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, p.s)) || (rc = call.output(dst, p.d))) return rc;
+    return call.finish(launch_warp_perspective(p, call.stream()));
+}
+
+// capi_calib3d.zig:205-213 + undistort.zig:280-323
+int visar_remap(const visar_image *src, const visar_image *dst, const visar_image *mapx, const visar_image *mapy,
+                double value) {
+    int rc;
+    if ((rc = check_image(src, 0)) || (rc = check_image(dst, 0)) || (rc = check_image(mapx, 5)) ||
+        (rc = check_image(mapy, 5)))
+        return rc;
+    if (dst->width != mapx->width || dst->height != mapx->height || !same_shape(mapx, mapy) || mapx->channels != 1)
+        return fail(SIZE_MISMATCH, "SizeMismatch");
+    if (src->channels != dst->channels || src->channels < 1 || src->channels > 4) return fail(UNSUPPORTED, "Unsupported");
+    double v = std::round(max_num(0.0, min_num(value, 255.0)));
+    Call call;
+    DImg s, d, mx, my;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, s)) || (rc = call.input(mapx, mx)) || (rc = call.input(mapy, my)) ||
+        (rc = call.output(dst, d)))
+        return rc;
+    return call.finish(launch_remap(s, d, mx, my, (int)v, call.stream()));
+}
+
+// ------------------------------------------------------------- morphology
+
+// morph.zig:111-154: the element as horizontal runs of "on" cells.
+static int element_runs(const std::vector<uint8_t> &mask, int w, int h, MorphParams &p) {
+    p.nruns = 0;
+    for (int ky = 0; ky < h; ky++) {
+        int x = 0;
+        while (x < w) {
+            if (!mask[(size_t)ky * w + x]) {
+                x++;
+                continue;
+            }
+            int x0 = x;
+            while (x < w && mask[(size_t)ky * w + x]) x++;
+            if (p.nruns == kMaxRuns) return UNSUPPORTED;
+            p.runs[p.nruns++] = MorphRun{(int16_t)ky, (int16_t)x0, (int16_t)(x - x0), 0};
+        }
+    }
+    return OK;
+}
+
+// erode^n or dilate^n from `s` into `d` (morph.zig:63-96); n = 0 copies.
+static int morph_iter(Call &call, const DImg &s, const DImg &d, MorphParams p, int op_max, int iterations,
+                      size_t elem) {
+    hipStream_t st = call.stream();
+    if (iterations == 0) return launch_copy(s, d, elem, st);
+    DImg t1, t2;
+    if (iterations > 1) {
+        if (call.temp_like(s, elem, t1) || call.temp_like(s, elem, t2)) return OUT_OF_MEMORY;
+    }
+    p.op_max = op_max;
+    DImg cur = s;
+    for (int it = 0; it < iterations; it++) {
+        DImg out = it == iterations - 1 ? d : (it % 2 == 0 ? t1 : t2);
+        p.s = cur;
+        p.d = out;
+        int rc = launch_morph_pass(p, st);
+        if (rc) return rc;
+        cur = out;
+    }
+    return OK;
+}
+
+// capi.zig:339-350 + morph.zig:532-571
+int visar_morphology(const visar_image *src, const visar_image *dst, int op, int shape, int kw, int kh,
+                     int iterations) {
+    if (op < 0 || op > 6 || shape < 0 || shape > 2) return fail(UNSUPPORTED, "Unsupported");
+    if (kw <= 0 || kh <= 0 || kw > 255 || kh > 255 || iterations < 0) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (!src) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    uint32_t T = src->depth;
+    if (!valid_depth(T)) return fail(UNSUPPORTED, "Unsupported");
+    int rc;
+    if ((rc = check_image(src, T)) || (rc = check_image(dst, T))) return rc;
+    if (!same_shape(src, dst)) return fail(SIZE_MISMATCH, "SizeMismatch");
+    std::vector<uint8_t> mask;
+    structuring_element(shape, kw, kh, mask);
+    MorphParams p;
+    std::memset(&p, 0, sizeof p);
+    p.depth = (int)T;
+    p.kw = kw;
+    p.kh = kh;
+    p.ax = kw / 2;
+    p.ay = kh / 2;
+    if (element_runs(mask, kw, kh, p)) return fail(UNSUPPORTED, "Unsupported");
+    const size_t elem = depth_size(T);
+    Call call;
+    DImg s, d;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, s)) || (rc = call.output(dst, d))) return rc;
+    int st = OK;
+    DImg t1, t2;
+    if (op >= 2 && (call.temp_like(s, elem, t1) || call.temp_like(s, elem, t2))) return fail(OUT_OF_MEMORY, "OutOfMemory");
+    switch (op) {
+    case 0: st = morph_iter(call, s, d, p, 0, iterations, elem); break;
+    case 1: st = morph_iter(call, s, d, p, 1, iterations, elem); break;
+    case 2:
+        if (!(st = morph_iter(call, s, t1, p, 0, iterations, elem))) st = morph_iter(call, t1, d, p, 1, iterations, elem);
+        break;
+    case 3:
+        if (!(st = morph_iter(call, s, t1, p, 1, iterations, elem))) st = morph_iter(call, t1, d, p, 0, iterations, elem);
+        break;
+    case 4:
+        if (!(st = morph_iter(call, s, t1, p, 0, iterations, elem)) && !(st = morph_iter(call, s, t2, p, 1, iterations, elem)))
+            st = launch_sub_sat(t2, t1, d, (int)T, call.stream());
+        break;
+    case 5:
+        if (!(st = morph_iter(call, s, t1, p, 0, iterations, elem)) && !(st = morph_iter(call, t1, t2, p, 1, iterations, elem)))
+            st = launch_sub_sat(s, t2, d, (int)T, call.stream());
+        break;
+    default:
+        if (!(st = morph_iter(call, s, t1, p, 1, iterations, elem)) && !(st = morph_iter(call, t1, t2, p, 0, iterations, elem)))
+            st = launch_sub_sat(t2, s, d, (int)T, call.stream());
+        break;
+    }
+    if (st == OUT_OF_MEMORY) return fail(OUT_OF_MEMORY, "OutOfMemory");
+    return call.finish(st);
+}
+
+// ------------------------------------------------------------ background
+
+}  // extern "C"
+
+struct visar_background {
+    uint32_t w, h;
+    int shift;
+    bool primed;
+    uint16_t *model;  // device, w*h contiguous
+};
+
+extern "C" {
+
+// capi.zig:446-456 + track.zig:22-24
+int visar_background_create(uint32_t width, uint32_t height, int shift, visar_background **out) {
+    if (shift < 0 || shift > 15) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (!out) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (width == 0 || height == 0) return fail(INVALID_ARGUMENT, "InvalidSize");
+    int rc = ensure_runtime();
+    if (rc) return rc;
+    visar_background *b = new (std::nothrow) visar_background();
+    if (!b) return fail(OUT_OF_MEMORY, "OutOfMemory");
+    b->w = width;
+    b->h = height;
+    b->shift = shift;
+    b->primed = false;
+    if (hipMalloc(&b->model, (size_t)width * height * sizeof(uint16_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        delete b;
+        return fail(OUT_OF_MEMORY, "OutOfMemory");
+    }
+    *out = b;
+    return VISAR_OK;
+}
+
+// capi.zig:458-464 + track.zig:33-81
+int visar_background_apply(visar_background *b, const visar_image *frame, const visar_image *mask, int threshold) {
+    if (!b) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    int rc;
+    if ((rc = check_image(frame, 0)) || (rc = check_image(mask, 0))) return rc;
+    if (frame->channels != 1 || !same_shape(frame, mask) || frame->width != b->w || frame->height != b->h)
+        return fail(SIZE_MISMATCH, "SizeMismatch");
+    int t = threshold < 0 ? 0 : threshold > 255 ? 255 : threshold;
+    Call call;
+    DImg f, m;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(frame, f)) || (rc = call.output(mask, m))) return rc;
+    rc = launch_background(f, m, b->model, (long)b->w * 2, b->shift, t, b->primed ? 0 : 1, call.stream());
+    if (rc == OK) b->primed = true;
+    return call.finish(rc);
+}
+
+void visar_background_free(visar_background *b) {
+    if (!b) return;
+    (void)hipFree(b->model);
+    delete b;
+}
+
+}  // extern "C"

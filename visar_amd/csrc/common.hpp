@@ -1,0 +1,125 @@
+// common.hpp -- shared types and exact-arithmetic helpers of libvisar_amd.
+//
+// Every translation unit is built with -ffp-contract=off (no implicit FMA):
+// float expressions evaluate exactly as written, and fmaf()/fma() appear only
+// where VISAR's reference kernels use @mulAdd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+
+namespace visar {
+
+// Status codes of visar_status (reference include/visar.h:22-30).
+enum Status : int {
+    OK = 0,
+    INVALID_ARGUMENT = -1,
+    SIZE_MISMATCH = -2,
+    UNSUPPORTED = -3,
+    OUT_OF_MEMORY = -4,
+    INVALID_DATA = -5,
+    INTERNAL = -6,
+};
+
+// Border kinds of reference src/imgproc/border.zig:5-18 (OpenCV numbering).
+enum Border : int { BORDER_CONSTANT = 0, BORDER_REPLICATE = 1, BORDER_REFLECT = 2, BORDER_WRAP = 3, BORDER_REFLECT101 = 4 };
+
+// A device view of one image: interleaved HWC samples, stride in bytes.
+struct DImg {
+    uint8_t *p;
+    int w, h, cn;
+    long stride;
+};
+
+template <typename T>
+__host__ __device__ inline T *row_ptr(const DImg &im, long y) {
+    return reinterpret_cast<T *>(im.p + y * im.stride);
+}
+
+// border.zig:22-39: maps a coordinate into [0, n), or -1 for a constant border.
+__host__ __device__ inline long border_map(long p, long n, int border) {
+    if (p >= 0 && p < n) return p;
+    switch (border) {
+    case BORDER_CONSTANT: return -1;
+    case BORDER_REPLICATE: return p < 0 ? 0 : n - 1;
+    case BORDER_REFLECT:
+    case BORDER_REFLECT101: {
+        if (n == 1) return 0;
+        long d = border == BORDER_REFLECT101 ? 1 : 0;
+        long q = p;
+        while (q < 0 || q >= n) q = q < 0 ? -q - 1 + d : n - 1 - (q - n) - d;
+        return q;
+    }
+    case BORDER_WRAP: {
+        long m = p % n;
+        return m < 0 ? m + n : m;
+    }
+    default: return -1;
+    }
+}
+
+// saturate.zig:8-18: round half to even with the 1.5*2^k magic constant.
+__host__ __device__ inline float rint_magic(float x) { return (x + 12582912.0f) - 12582912.0f; }
+__host__ __device__ inline double rint_magic(double x) { return (x + 6755399441055744.0) - 6755399441055744.0; }
+
+// Zig's @min/@max on floats: a NaN operand yields the other operand.
+__host__ __device__ inline float max_num(float a, float b) { return a != a ? b : (b != b ? a : (a > b ? a : b)); }
+__host__ __device__ inline float min_num(float a, float b) { return a != a ? b : (b != b ? a : (a < b ? a : b)); }
+__host__ __device__ inline double max_num(double a, double b) { return a != a ? b : (b != b ? a : (a > b ? a : b)); }
+__host__ __device__ inline double min_num(double a, double b) { return a != a ? b : (b != b ? a : (a < b ? a : b)); }
+
+template <typename T> struct Lim;
+template <> struct Lim<uint8_t> { static constexpr double lo = 0, hi = 255; };
+template <> struct Lim<uint16_t> { static constexpr double lo = 0, hi = 65535; };
+template <> struct Lim<int16_t> { static constexpr double lo = -32768, hi = 32767; };
+template <> struct Lim<int32_t> { static constexpr double lo = -2147483648.0, hi = 2147483647.0; };
+
+// saturate.zig:21-49 sat.cast(D, float): NaN -> 0, round half to even in f64, clamp.
+template <typename D>
+__host__ __device__ inline D sat_cast(double x) {
+    if constexpr (std::is_same<D, float>::value) {
+        return (float)x;
+    } else {
+        if (x != x) return 0;
+        double r = rint_magic(x);
+        if (r <= Lim<D>::lo) return (D)Lim<D>::lo;
+        if (r >= Lim<D>::hi) return (D)Lim<D>::hi;
+        return (D)r;
+    }
+}
+
+// filter.zig:44-58 satV(D, f32): round in f32, clamp with NaN -> low bound.
+template <typename D>
+__host__ __device__ inline D sat_v(float v) {
+    if constexpr (std::is_same<D, float>::value) {
+        return v;
+    } else {
+        float r = rint_magic(v);
+        float c = min_num(max_num(r, (float)Lim<D>::lo), (float)Lim<D>::hi);
+        return (D)c;
+    }
+}
+
+// simd.zig:90-126 satF32(D, v): clamp (NaN -> low bound, 0 for signed), round.
+template <typename D>
+__host__ __device__ inline D sat_f32(float v) {
+    float c = min_num(max_num(v, (float)Lim<D>::lo), (float)Lim<D>::hi);
+    if (Lim<D>::lo != 0 && v != v) c = 0;
+    return (D)rint_magic((double)c);
+}
+
+__host__ __device__ inline size_t depth_size(uint32_t depth) {
+    switch (depth) {
+    case 0: return 1;
+    case 2: return 2;
+    case 3: return 2;
+    case 4: return 4;
+    case 5: return 4;
+    default: return 0;
+    }
+}
+
+}  // namespace visar

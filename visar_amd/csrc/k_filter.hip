@@ -1,0 +1,231 @@
+// k_filter.hip -- the separable filter engine (GaussianBlur, Sobel,
+// sepFilter2D) for gfx950.
+//
+// Reference semantics: src/imgproc/filter.zig:120-205 (separable/chunk) and
+// :232-328 (taps). A workgroup owns a tile of TH output rows x TWPX pixels.
+// Pass 1 writes the vertically filtered values of the tile's rows, including
+// the horizontal halo (border-mapped columns: the reference replicates
+// vertically filtered columns, filter.zig:193-199), into LDS. Pass 2 runs the
+// horizontal taps out of LDS and finishes. Integer modes are exact, so pass
+// order is free; the f32 mode keeps the reference's per-lane order.
+#include "kernels.hpp"
+
+namespace visar {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+template <typename S>
+__device__ inline float to_f(S v) {
+    return (float)v;
+}
+
+// Integer finish (filter.zig:61-116) of an exact accumulator.
+template <typename D, typename A>
+__device__ inline D finish_int(A a, int fin, float scale) {
+    if (fin == FIN_FIX8) {
+        uint32_t r = ((uint32_t)a + (1u << 15)) >> 16;
+        return (D)(r > 255u ? 255u : r);
+    }
+    if (fin == FIN_FIX16) {
+        uint64_t r = ((uint64_t)a + (1ull << 31)) >> 32;
+        return (D)(r > 65535ull ? 65535ull : r);
+    }
+    if (fin == FIN_SCALED) return sat_v<D>((float)a * scale);
+    if constexpr (std::is_same<D, float>::value) {
+        return (float)a;
+    } else {
+        double lo = Lim<D>::lo, hi = Lim<D>::hi;
+        long long v = (long long)a;
+        return (D)(v < (long long)lo ? (long long)lo : v > (long long)hi ? (long long)hi : v);
+    }
+}
+
+template <typename D>
+__device__ inline D finish_f(float a, int fin, float scale) {
+    if (fin == FIN_SCALED) a = a * scale;
+    return sat_v<D>(a);
+}
+
+// tapsV (filter.zig:232-252) over values v(j), folded by symmetry.
+template <typename F>
+__device__ inline float taps_fold(const float *k, int n, int sym, F v) {
+    int r = n / 2;
+    if (sym == SYM_NONE) {
+        float a = 0.0f;
+        for (int j = 0; j < n; j++) a = a + v(j) * k[j];
+        return a;
+    }
+    float a = sym == SYM_EVEN ? v(r) * k[r] : 0.0f;
+    for (int i = 0; i < r; i++) {
+        float t = v(i), b = v(n - 1 - i);
+        float pair = sym == SYM_EVEN ? t + b : b - t;
+        a = a + pair * (sym == SYM_EVEN ? k[i] : k[n - 1 - i]);
+    }
+    return a;
+}
+
+// tapsS (filter.zig:255-259) and the compile-time-kernel tail (:297-301).
+template <typename F>
+__device__ inline float taps_seq(const float *k, int n, int from_zero, F v) {
+    float a;
+    int j0;
+    if (from_zero) {
+        a = 0.0f;
+        j0 = 0;
+    } else {
+        a = k[0] * v(0);
+        j0 = 1;
+    }
+    for (int j = j0; j < n; j++) a = a + k[j] * v(j);
+    return a;
+}
+
+// Vertical value of column `col` (flat sample index) for output row y, with
+// the tile's mapped source rows in `rmap` (-1 = constant border row).
+template <typename S, typename VT, bool F32>
+__device__ inline VT vert(const SepParams &p, const int *rmap, long col) {
+    if constexpr (F32) {
+        auto v = [&](int j) -> float {
+            int sy = rmap[j];
+            return sy < 0 ? 0.0f : to_f(row_ptr<const S>(p.src, sy)[col]);
+        };
+        return col < p.body ? taps_fold(p.kyf, p.nky, p.vsym, v) : taps_seq(p.kyf, p.nky, p.const_kernel, v);
+    } else {
+        VT a = 0;
+        for (int j = 0; j < p.nky; j++) {
+            int sy = rmap[j];
+            if (sy >= 0) a += (VT)p.kyi[j] * (VT)row_ptr<const S>(p.src, sy)[col];
+        }
+        return a;
+    }
+}
+
+template <typename S, typename D, typename VT, typename AT, bool F32>
+__global__ __launch_bounds__(kThreads) void sep_tile(SepParams p, int twpx, int th) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int cn = p.src.cn, W = p.src.w, H = p.src.h;
+    const int rx = p.nkx / 2, ry = p.nky / 2;
+    const int ldw = (twpx + 2 * rx) * cn;
+    const long x0 = (long)blockIdx.x * twpx;
+    const long y0 = (long)blockIdx.y * th;
+    const int rows = (int)min((long)th, (long)H - y0);
+    int *rmap = reinterpret_cast<int *>(smem);
+    VT *V = reinterpret_cast<VT *>(smem + (((th + 2 * ry) * sizeof(int) + 15) & ~(size_t)15));
+    for (int i = threadIdx.x; i < rows + 2 * ry; i += kThreads) rmap[i] = (int)border_map(y0 + i - ry, H, p.border);
+    __syncthreads();
+    // Pass 1: vertical taps into LDS (halo columns border-mapped).
+    for (int idx = threadIdx.x; idx < rows * ldw; idx += kThreads) {
+        int r = idx / ldw, j = idx - r * ldw;
+        int q = j / cn, ch = j - q * cn;
+        long mx = border_map(x0 - rx + q, W, p.border);
+        VT v = 0;
+        if (mx >= 0) v = vert<S, VT, F32>(p, rmap + r, mx * cn + ch);
+        V[r * ldw + j] = v;
+    }
+    __syncthreads();
+    // Pass 2: horizontal taps and finish.
+    const int outw = (int)min((long)twpx, (long)W - x0) * cn;
+    for (int idx = threadIdx.x; idx < rows * outw; idx += kThreads) {
+        int r = idx / outw, e = idx - r * outw;
+        const VT *vr = V + r * ldw + e;
+        long ge = x0 * cn + e;
+        D o;
+        if constexpr (F32) {
+            auto v = [&](int i) -> float { return vr[i * cn]; };
+            float a = ge < p.body ? taps_fold(p.kxf, p.nkx, p.hsym, v) : taps_seq(p.kxf, p.nkx, p.const_kernel, v);
+            o = finish_f<D>(a, p.fin, p.scale);
+        } else {
+            AT a = 0;
+            for (int i = 0; i < p.nkx; i++) a += (AT)p.kxi[i] * (AT)vr[i * cn];
+            o = finish_int<D, AT>(a, p.fin, p.scale);
+        }
+        row_ptr<D>(p.dst, y0 + r)[ge] = o;
+    }
+}
+
+// Fallback when a tile's halo does not fit LDS (very wide kernels on many
+// channels): every output recomputes the vertical values it needs.
+template <typename S, typename D, typename VT, typename AT, bool F32>
+__global__ __launch_bounds__(kThreads) void sep_direct(SepParams p) {
+    const int cn = p.src.cn, W = p.src.w, H = p.src.h;
+    const int rx = p.nkx / 2, ry = p.nky / 2;
+    const long n = (long)W * cn;
+    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    long y = blockIdx.y;
+    if (gid >= n) return;
+    int rmap[kMaxTaps];
+    for (int j = 0; j < p.nky; j++) rmap[j] = (int)border_map(y + j - ry, H, p.border);
+    long x = gid / cn, ch = gid - x * cn;
+    auto vcol = [&](int i) -> VT {
+        long mx = border_map(x + i - rx, W, p.border);
+        return mx < 0 ? (VT)0 : vert<S, VT, F32>(p, rmap, mx * cn + ch);
+    };
+    D o;
+    if constexpr (F32) {
+        auto v = [&](int i) -> float { return (float)vcol(i); };
+        float a = gid < p.body ? taps_fold(p.kxf, p.nkx, p.hsym, v) : taps_seq(p.kxf, p.nkx, p.const_kernel, v);
+        o = finish_f<D>(a, p.fin, p.scale);
+    } else {
+        AT a = 0;
+        for (int i = 0; i < p.nkx; i++) a += (AT)p.kxi[i] * (AT)vcol(i);
+        o = finish_int<D, AT>(a, p.fin, p.scale);
+    }
+    row_ptr<D>(p.dst, y)[gid] = o;
+}
+
+template <typename S, typename D, typename VT, typename AT, bool F32>
+int run_sep(const SepParams &p, hipStream_t st) {
+    const int cn = p.src.cn;
+    const int rx = p.nkx / 2, ry = p.nky / 2;
+    const size_t lds_cap = 60 * 1024;
+    int twpx = cn == 1 ? 256 : cn == 2 ? 128 : cn <= 4 ? 64 : 16;
+    int th = 16;
+    auto need = [&](int tw, int t) {
+        return (((size_t)(t + 2 * ry) * sizeof(int) + 15) & ~(size_t)15) + (size_t)t * (tw + 2 * rx) * cn * sizeof(VT);
+    };
+    while (need(twpx, th) > lds_cap && th > 1) th /= 2;
+    while (need(twpx, th) > lds_cap && twpx > 1) twpx /= 2;
+    if (need(twpx, th) <= lds_cap) {
+        dim3 grid((unsigned)((p.src.w + twpx - 1) / twpx), (unsigned)((p.src.h + th - 1) / th));
+        hipLaunchKernelGGL((sep_tile<S, D, VT, AT, F32>), grid, dim3(kThreads), need(twpx, th), st, p, twpx, th);
+    } else {
+        long n = (long)p.src.w * cn;
+        dim3 grid((unsigned)((n + kThreads - 1) / kThreads), (unsigned)p.src.h);
+        hipLaunchKernelGGL((sep_direct<S, D, VT, AT, F32>), grid, dim3(kThreads), 0, st, p);
+    }
+    return hipGetLastError() == hipSuccess ? OK : INTERNAL;
+}
+
+template <typename S>
+int run_sep_f32_dst(const SepParams &p, hipStream_t st) {
+    switch (p.ddepth) {
+    case 0: return run_sep<S, uint8_t, float, float, true>(p, st);
+    case 2: return run_sep<S, uint16_t, float, float, true>(p, st);
+    case 3: return run_sep<S, int16_t, float, float, true>(p, st);
+    case 5: return run_sep<S, float, float, float, true>(p, st);
+    default: return UNSUPPORTED;
+    }
+}
+
+}  // namespace
+
+int launch_sep(const SepParams &p, hipStream_t st) {
+    if (p.f32mode) {
+        switch (p.sdepth) {
+        case 0: return run_sep_f32_dst<uint8_t>(p, st);
+        case 2: return run_sep_f32_dst<uint16_t>(p, st);
+        case 3: return run_sep_f32_dst<int16_t>(p, st);
+        case 5: return run_sep_f32_dst<float>(p, st);
+        default: return UNSUPPORTED;
+        }
+    }
+    // Integer modes: u8 Gaussian (FIX8), u16 Gaussian (FIX16), u8 -> i16 Sobel.
+    if (p.sdepth == 0 && p.ddepth == 0) return run_sep<uint8_t, uint8_t, int32_t, int32_t, false>(p, st);
+    if (p.sdepth == 2 && p.ddepth == 2) return run_sep<uint16_t, uint16_t, uint32_t, uint64_t, false>(p, st);
+    if (p.sdepth == 0 && p.ddepth == 3) return run_sep<uint8_t, int16_t, int32_t, int32_t, false>(p, st);
+    return UNSUPPORTED;
+}
+
+}  // namespace visar

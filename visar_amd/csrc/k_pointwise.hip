@@ -1,0 +1,379 @@
+// k_pointwise.hip -- streaming kernels: BGR->gray, threshold, the background
+// update, exact-half / integer-area / nearest / bilinear / fractional-area
+// resize, saturating subtraction and the 8-bit histogram.
+//
+// Each kernel maps one thread to a run of consecutive samples of one row so
+// loads and stores are coalesced; the 16-pixel u8 paths use 128-bit loads
+// when the row segment is 16-byte aligned.
+#include "kernels.hpp"
+
+namespace visar {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+// ---------------------------------------------------------------- gray
+
+// color.zig:126-129, 392-396: y = (b*3735 + g*19235 + r*9798 + 2^14) >> 15
+__device__ inline uint32_t gray_fix(uint32_t b, uint32_t g, uint32_t r) {
+    return (b * 3735u + g * 19235u + r * 9798u + (1u << 14)) >> 15;
+}
+
+// 16 pixels per thread; 3-channel rows are read as three 128-bit words.
+template <int SCN>
+__global__ __launch_bounds__(kThreads) void gray_u8(DImg s, DImg d, int sb, long groups_per_row) {
+    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    long y = gid / groups_per_row;
+    long x = (gid - y * groups_per_row) * 16;
+    if (y >= s.h) return;
+    const uint8_t *sr = row_ptr<const uint8_t>(s, y) + x * SCN;
+    uint8_t *dr = row_ptr<uint8_t>(d, y) + x;
+    const int bi = sb, ri = 2 - sb;
+    if (x + 16 <= s.w && aligned16(sr) && aligned16(dr)) {
+        uint4 w[SCN];
+#pragma unroll
+        for (int k = 0; k < SCN; k++) w[k] = reinterpret_cast<const uint4 *>(sr)[k];
+        const uint8_t *b = reinterpret_cast<const uint8_t *>(w);
+        uint32_t o[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int px = q * 4 + k;
+                v |= gray_fix(b[px * SCN + bi], b[px * SCN + 1], b[px * SCN + ri]) << (8 * k);
+            }
+            o[q] = v;
+        }
+        *reinterpret_cast<uint4 *>(dr) = make_uint4(o[0], o[1], o[2], o[3]);
+        return;
+    }
+    long n = min(16L, (long)s.w - x);
+    for (long i = 0; i < n; i++) dr[i] = (uint8_t)gray_fix(sr[i * SCN + bi], sr[i * SCN + 1], sr[i * SCN + ri]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gray_generic(DImg s, DImg d, int scn, int sb) {
+    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    long y = gid / s.w, x = gid - y * s.w;
+    if (y >= s.h) return;
+    const T *p = row_ptr<const T>(s, y) + x * scn;
+    T *o = row_ptr<T>(d, y) + x;
+    const int bi = sb, ri = 2 - sb;
+    if constexpr (std::is_same<T, float>::value) {
+        *o = p[bi] * 0.114f + p[1] * 0.587f + p[ri] * 0.299f;
+    } else {
+        *o = (T)gray_fix(p[bi], p[1], p[ri]);
+    }
+}
+
+// ----------------------------------------------------------- threshold
+
+// threshold.zig:85-119 (threshOne) and the u8 LUT (threshold.zig:44-60).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void threshold_k(ThreshParams p, long n_per_row) {
+    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    long y = gid / n_per_row, i = gid - y * n_per_row;
+    if (y >= p.s.h) return;
+    T v = row_ptr<const T>(p.s, y)[i];
+    T o;
+    if constexpr (std::is_same<T, uint8_t>::value) {
+        o = p.lut[v];
+    } else {
+        T t = (T)p.t, m = (T)p.m;
+        bool gt = p.below || v > t;
+        switch (p.kind) {
+        case 0: o = gt ? m : (T)0; break;
+        case 1: o = gt ? (T)0 : m; break;
+        case 2: o = gt ? (p.below ? (T)0 : t) : v; break;
+        case 3: o = gt ? v : (T)0; break;
+        default: o = gt ? (T)0 : v; break;
+        }
+    }
+    row_ptr<T>(p.d, y)[i] = o;
+}
+
+// ---------------------------------------------------------- background
+
+// track.zig:33-81. Model u16 in 8.8 fixed point, per pixel in i32.
+__global__ __launch_bounds__(kThreads) void background_k(DImg f, DImg m, uint16_t *model, long mstride, int shift,
+                                                         int thr, int prime) {
+    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    long y = gid / f.w, x = gid - y * f.w;
+    if (y >= f.h) return;
+    int32_t fv = row_ptr<const uint8_t>(f, y)[x];
+    uint16_t *bp = reinterpret_cast<uint16_t *>(reinterpret_cast<uint8_t *>(model) + y * mstride) + x;
+    uint8_t *mp = row_ptr<uint8_t>(m, y) + x;
+    if (prime) {
+        *bp = (uint16_t)(fv << 8);
+        *mp = 0;
+        return;
+    }
+    int32_t bv = *bp;
+    int32_t diff = fv - (bv >> 8);
+    bool fg = (diff < 0 ? -diff : diff) > thr;
+    *mp = fg ? 255 : 0;
+    int sh = shift + (fg ? 2 : 0);
+    *bp = (uint16_t)(bv + (((fv << 8) - bv) >> sh));
+}
+
+// -------------------------------------------------------------- resize
+
+// resize_u8.zig:381-447: (a + b + c + d + 2) >> 2 per channel.
+__global__ __launch_bounds__(kThreads) void half_u8(DImg s, DImg d, long n_per_row) {
+    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    long y = gid / n_per_row, e = gid - y * n_per_row;
+    if (y >= d.h) return;
+    const int cn = d.cn;
+    long x = e / cn, ch = e - x * cn;
+    long i = 2 * x * cn + ch;
+    const uint8_t *a = row_ptr<const uint8_t>(s, 2 * y), *b = row_ptr<const uint8_t>(s, 2 * y + 1);
+    uint32_t v = (uint32_t)a[i] + a[i + cn] + b[i] + b[i + cn];
+    row_ptr<uint8_t>(d, y)[e] = (uint8_t)((v + 2) >> 2);
+}
+
+// Single-channel exact half, 16 output pixels (32 input bytes per row) per thread.
+__global__ __launch_bounds__(kThreads) void half_u8_c1(DImg s, DImg d, long groups_per_row) {
+    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    long y = gid / groups_per_row;
+    long x = (gid - y * groups_per_row) * 16;
+    if (y >= d.h) return;
+    const uint8_t *a = row_ptr<const uint8_t>(s, 2 * y) + 2 * x, *b = row_ptr<const uint8_t>(s, 2 * y + 1) + 2 * x;
+    uint8_t *o = row_ptr<uint8_t>(d, y) + x;
+    if (x + 16 <= d.w && aligned16(a) && aligned16(b) && aligned16(o)) {
+        uint4 a0 = reinterpret_cast<const uint4 *>(a)[0], a1 = reinterpret_cast<const uint4 *>(a)[1];
+        uint4 b0 = reinterpret_cast<const uint4 *>(b)[0], b1 = reinterpret_cast<const uint4 *>(b)[1];
+        uint32_t aw[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+        uint32_t bw[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+        uint32_t out[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int px = q * 4 + k;  // output pixel, input bytes 2px, 2px+1
+                uint32_t wa = aw[px / 2], wb = bw[px / 2];
+                int sh = (px & 1) * 16;
+                uint32_t sum = ((wa >> sh) & 255) + ((wa >> (sh + 8)) & 255) + ((wb >> sh) & 255) + ((wb >> (sh + 8)) & 255);
+                v |= ((sum + 2) >> 2) << (8 * k);
+            }
+            out[q] = v;
+        }
+        *reinterpret_cast<uint4 *>(o) = make_uint4(out[0], out[1], out[2], out[3]);
+        return;
+    }
+    long n = min(16L, (long)d.w - x);
+    for (long i = 0; i < n; i++) {
+        uint32_t v = (uint32_t)a[2 * i] + a[2 * i + 1] + b[2 * i] + b[2 * i + 1];
+        o[i] = (uint8_t)((v + 2) >> 2);
+    }
+}
+
+// resize_u8.zig:1213-1277: block sum times the f32 reciprocal, satF32.
+__global__ __launch_bounds__(kThreads) void area_int_u8(DImg s, DImg d, int ix, int iy, float scale, long n_per_row) {
+    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    long y = gid / n_per_row, e = gid - y * n_per_row;
+    if (y >= d.h) return;
+    const int cn = d.cn;
+    long x = e / cn, ch = e - x * cn;
+    uint32_t sum = 0;
+    for (int dy = 0; dy < iy; dy++) {
+        const uint8_t *r = row_ptr<const uint8_t>(s, y * iy + dy);
+        for (int dx = 0; dx < ix; dx++) sum += r[(x * ix + dx) * cn + ch];
+    }
+    row_ptr<uint8_t>(d, y)[e] = sat_f32<uint8_t>((float)sum * scale);
+}
+
+// resize.zig:62-106 nearest: xofs = min(floor(x * sw/dw), sw - 1) in f64.
+__global__ __launch_bounds__(kThreads) void nearest_k(DImg s, DImg d, int elem, double ifx, double ify) {
+    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    long y = gid / d.w, x = gid - y * d.w;
+    if (y >= d.h) return;
+    long sy = (long)floor((double)y * ify);
+    if (sy > s.h - 1) sy = s.h - 1;
+    long sx = (long)floor((double)x * ifx);
+    if (sx > s.w - 1) sx = s.w - 1;
+    const uint8_t *sp = s.p + sy * s.stride + sx * (long)s.cn * elem;
+    uint8_t *dp = d.p + y * d.stride + x * (long)d.cn * elem;
+    for (long i = 0; i < (long)d.cn * elem; i++) dp[i] = sp[i];
+}
+
+// resize_u8.zig:651-711: H = (s[o0]*w0 + s[o1]*w1) >> 4 as i16 per source
+// row, out = sat_u8((((H0*b0) >> 16) + ((H1*b1) >> 16) + 2) >> 2).
+__global__ __launch_bounds__(kThreads) void linear_u8(DImg s, DImg d, const LinTap *xt, const LinTap *yt, long n_per_row) {
+    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    long y = gid / n_per_row, e = gid - y * n_per_row;
+    if (y >= d.h) return;
+    const int cn = d.cn;
+    long x = e / cn, ch = e - x * cn;
+    LinTap tx = xt[x], ty = yt[y];
+    const uint8_t *r0 = row_ptr<const uint8_t>(s, ty.o0), *r1 = row_ptr<const uint8_t>(s, ty.o1);
+    int32_t h0 = (int16_t)(((int32_t)r0[tx.o0 + ch] * tx.w0 + (int32_t)r0[tx.o1 + ch] * tx.w1) >> 4);
+    int32_t h1 = (int16_t)(((int32_t)r1[tx.o0 + ch] * tx.w0 + (int32_t)r1[tx.o1 + ch] * tx.w1) >> 4);
+    int32_t t = ((h0 * ty.w0) >> 16) + ((h1 * ty.w1) >> 16);
+    int32_t o = (t + 2) >> 2;
+    row_ptr<uint8_t>(d, y)[e] = (uint8_t)(o < 0 ? 0 : o > 255 ? 255 : o);
+}
+
+// resize.zig:527-557 (areaGeneric, bit-exact with resize_u8.zig:1058-1208):
+// per source row of the cell, the f32 sum of s*alpha over the cell's
+// columns in table order, then the sum of those times beta, no FMA.
+__global__ __launch_bounds__(kThreads) void area_frac_u8(DImg s, DImg d, const AreaEntry *xt, const int32_t *xs,
+                                                         const AreaEntry *yt, const int32_t *ys, long n_per_row) {
+    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    long y = gid / n_per_row, e = gid - y * n_per_row;
+    if (y >= d.h) return;
+    const int cn = d.cn;
+    long x = e / cn, ch = e - x * cn;
+    int xa = xs[x], xb = xs[x + 1];
+    float sum = 0.0f;
+    for (int k = ys[y]; k < ys[y + 1]; k++) {
+        AreaEntry ye = yt[k];
+        const uint8_t *r = row_ptr<const uint8_t>(s, ye.si);
+        float h = 0.0f;
+        for (int q = xa; q < xb; q++) h = h + (float)r[(long)xt[q].si * cn + ch] * xt[q].alpha;
+        sum = sum + h * ye.alpha;
+    }
+    row_ptr<uint8_t>(d, y)[e] = sat_cast<uint8_t>((double)sum);
+}
+
+// ----------------------------------------------------------- morph helpers
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void sub_sat_k(DImg a, DImg b, DImg d, long n_per_row) {
+    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
+    long y = gid / n_per_row, i = gid - y * n_per_row;
+    if (y >= d.h) return;
+    T x = row_ptr<const T>(a, y)[i], z = row_ptr<const T>(b, y)[i];
+    T o;
+    if constexpr (std::is_same<T, float>::value) o = x - z;
+    else o = x > z ? (T)(x - z) : (T)0;
+    row_ptr<T>(d, y)[i] = o;
+}
+
+// ------------------------------------------------------------ histogram
+
+__global__ __launch_bounds__(kThreads) void hist_u8(DImg s, uint32_t *hist, long n_per_row) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    long total = n_per_row * s.h;
+    for (long gid = (long)blockIdx.x * kThreads + threadIdx.x; gid < total; gid += (long)gridDim.x * kThreads) {
+        long y = gid / n_per_row, i = gid - y * n_per_row;
+        atomicAdd(&h[row_ptr<const uint8_t>(s, y)[i]], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+inline unsigned blocks_for(long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+inline int launched() { return hipGetLastError() == hipSuccess ? OK : INTERNAL; }
+
+}  // namespace
+
+int launch_gray(const DImg &s, const DImg &d, int depth, int scn, int sb, hipStream_t st) {
+    if (depth == 0) {
+        long gpr = (s.w + 15) / 16;
+        long n = gpr * s.h;
+        if (scn == 3) hipLaunchKernelGGL(gray_u8<3>, dim3(blocks_for(n)), dim3(kThreads), 0, st, s, d, sb, gpr);
+        else hipLaunchKernelGGL(gray_u8<4>, dim3(blocks_for(n)), dim3(kThreads), 0, st, s, d, sb, gpr);
+        return launched();
+    }
+    long n = (long)s.w * s.h;
+    if (depth == 2) hipLaunchKernelGGL(gray_generic<uint16_t>, dim3(blocks_for(n)), dim3(kThreads), 0, st, s, d, scn, sb);
+    else if (depth == 5) hipLaunchKernelGGL(gray_generic<float>, dim3(blocks_for(n)), dim3(kThreads), 0, st, s, d, scn, sb);
+    else return UNSUPPORTED;
+    return launched();
+}
+
+int launch_threshold(const ThreshParams &p, hipStream_t st) {
+    long npr = (long)p.s.w * p.s.cn;
+    long n = npr * p.s.h;
+    switch (p.depth) {
+    case 0: hipLaunchKernelGGL(threshold_k<uint8_t>, dim3(blocks_for(n)), dim3(kThreads), 0, st, p, npr); break;
+    case 2: hipLaunchKernelGGL(threshold_k<uint16_t>, dim3(blocks_for(n)), dim3(kThreads), 0, st, p, npr); break;
+    case 3: hipLaunchKernelGGL(threshold_k<int16_t>, dim3(blocks_for(n)), dim3(kThreads), 0, st, p, npr); break;
+    case 5: hipLaunchKernelGGL(threshold_k<float>, dim3(blocks_for(n)), dim3(kThreads), 0, st, p, npr); break;
+    default: return UNSUPPORTED;
+    }
+    return launched();
+}
+
+int launch_background(const DImg &frame, const DImg &mask, uint16_t *model, long model_stride, int shift, int thr,
+                      int prime, hipStream_t st) {
+    long n = (long)frame.w * frame.h;
+    hipLaunchKernelGGL(background_k, dim3(blocks_for(n)), dim3(kThreads), 0, st, frame, mask, model, model_stride, shift,
+                       thr, prime);
+    return launched();
+}
+
+int launch_copy(const DImg &s, const DImg &d, size_t elem, hipStream_t st) {
+    size_t rl = (size_t)s.w * s.cn * elem;
+    return hipMemcpy2DAsync(d.p, d.stride, s.p, s.stride, rl, s.h, hipMemcpyDeviceToDevice, st) == hipSuccess ? OK
+                                                                                                               : INTERNAL;
+}
+
+int launch_half_u8(const DImg &s, const DImg &d, hipStream_t st) {
+    if (d.cn == 1) {
+        long gpr = (d.w + 15) / 16;
+        hipLaunchKernelGGL(half_u8_c1, dim3(blocks_for(gpr * d.h)), dim3(kThreads), 0, st, s, d, gpr);
+        return launched();
+    }
+    long npr = (long)d.w * d.cn;
+    hipLaunchKernelGGL(half_u8, dim3(blocks_for(npr * d.h)), dim3(kThreads), 0, st, s, d, npr);
+    return launched();
+}
+
+int launch_area_int_u8(const DImg &s, const DImg &d, int ix, int iy, hipStream_t st) {
+    long npr = (long)d.w * d.cn;
+    float scale = 1.0f / (float)(ix * iy);
+    hipLaunchKernelGGL(area_int_u8, dim3(blocks_for(npr * d.h)), dim3(kThreads), 0, st, s, d, ix, iy, scale, npr);
+    return launched();
+}
+
+int launch_nearest(const DImg &s, const DImg &d, size_t elem, hipStream_t st) {
+    long n = (long)d.w * d.h;
+    double ifx = (double)s.w / (double)d.w, ify = (double)s.h / (double)d.h;
+    hipLaunchKernelGGL(nearest_k, dim3(blocks_for(n)), dim3(kThreads), 0, st, s, d, (int)elem, ifx, ify);
+    return launched();
+}
+
+int launch_linear_u8(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, hipStream_t st) {
+    long npr = (long)d.w * d.cn;
+    hipLaunchKernelGGL(linear_u8, dim3(blocks_for(npr * d.h)), dim3(kThreads), 0, st, s, d, xt, yt, npr);
+    return launched();
+}
+
+int launch_area_frac_u8(const DImg &s, const DImg &d, const AreaEntry *xt, const int32_t *xstart, const AreaEntry *yt,
+                        const int32_t *ystart, hipStream_t st) {
+    long npr = (long)d.w * d.cn;
+    hipLaunchKernelGGL(area_frac_u8, dim3(blocks_for(npr * d.h)), dim3(kThreads), 0, st, s, d, xt, xstart, yt, ystart, npr);
+    return launched();
+}
+
+int launch_sub_sat(const DImg &a, const DImg &b, const DImg &d, int depth, hipStream_t st) {
+    long npr = (long)d.w * d.cn;
+    long n = npr * d.h;
+    switch (depth) {
+    case 0: hipLaunchKernelGGL(sub_sat_k<uint8_t>, dim3(blocks_for(n)), dim3(kThreads), 0, st, a, b, d, npr); break;
+    case 2: hipLaunchKernelGGL(sub_sat_k<uint16_t>, dim3(blocks_for(n)), dim3(kThreads), 0, st, a, b, d, npr); break;
+    case 3: hipLaunchKernelGGL(sub_sat_k<int16_t>, dim3(blocks_for(n)), dim3(kThreads), 0, st, a, b, d, npr); break;
+    case 5: hipLaunchKernelGGL(sub_sat_k<float>, dim3(blocks_for(n)), dim3(kThreads), 0, st, a, b, d, npr); break;
+    default: return UNSUPPORTED;
+    }
+    return launched();
+}
+
+int launch_hist_u8(const DImg &s, uint32_t *hist, hipStream_t st) {
+    long npr = (long)s.w * s.cn;
+    long n = npr * s.h;
+    unsigned blocks = blocks_for(n);
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(hist_u8, dim3(blocks), dim3(kThreads), 0, st, s, hist, npr);
+    return launched();
+}
+
+}  // namespace visar

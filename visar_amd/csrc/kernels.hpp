@@ -1,0 +1,123 @@
+// kernels.hpp -- launch interface between the C-ABI layer and the HIP kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace visar {
+
+// ---------------------------------------------------------------- filters
+
+// Finish kinds of reference src/imgproc/filter.zig:61-116.
+enum Finish : int { FIN_FIX8 = 0, FIN_FIX16 = 1, FIN_CAST = 2, FIN_SCALED = 3 };
+enum Sym : int { SYM_NONE = 0, SYM_EVEN = 1, SYM_ODD = 2 };
+
+constexpr int kMaxTaps = 128;
+
+// One separable filter call (filter.zig:120-205). Integer modes use the
+// integer taps; the f32 mode uses the float taps and reproduces the
+// reference's order of operations: lanes of a row before `body` fold
+// symmetric taps, the scalar tail sums them in order.
+struct SepParams {
+    DImg src, dst;
+    int sdepth, ddepth;
+    int nkx, nky;
+    int border;
+    int fin;
+    float scale;
+    int f32mode;
+    int vsym, hsym, const_kernel;
+    long body;
+    int32_t kxi[kMaxTaps], kyi[kMaxTaps];
+    float kxf[kMaxTaps], kyf[kMaxTaps];
+};
+
+int launch_sep(const SepParams &p, hipStream_t st);
+
+// -------------------------------------------------------------- pointwise
+
+// color.zig:324-403 gray conversion: scn 3/4, sb 0/2 (blue index).
+int launch_gray(const DImg &s, const DImg &d, int depth, int scn, int sb, hipStream_t st);
+
+struct ThreshParams {
+    DImg s, d;
+    int depth, kind;
+    int below;  // 16-bit thresholds below the range: every value is above
+    uint8_t lut[256];
+    double t, m;  // threshold / maxval already converted to the sample type
+};
+int launch_threshold(const ThreshParams &p, hipStream_t st);
+
+// video/track.zig:33-81 (primed update) and the priming pass.
+int launch_background(const DImg &frame, const DImg &mask, uint16_t *model, long model_stride, int shift, int thr,
+                      int prime, hipStream_t st);
+
+// ----------------------------------------------------------------- resize
+
+int launch_copy(const DImg &s, const DImg &d, size_t elem, hipStream_t st);
+int launch_half_u8(const DImg &s, const DImg &d, hipStream_t st);
+int launch_area_int_u8(const DImg &s, const DImg &d, int ix, int iy, hipStream_t st);
+int launch_nearest(const DImg &s, const DImg &d, size_t elem, hipStream_t st);
+
+// resize.zig:186-214 taps, prepared on the host: per destination column the
+// two source sample offsets (channel 0, in samples) and 11-bit weights; per
+// destination row the two source rows and weights.
+struct LinTap {
+    int32_t o0, o1;
+    int32_t w0, w1;
+};
+int launch_linear_u8(const DImg &s, const DImg &d, const LinTap *xt_dev, const LinTap *yt_dev, hipStream_t st);
+
+// resize.zig:470-524 area tables: entries grouped per destination index,
+// start offsets in `xstart` / `ystart` (length dsize + 1).
+struct AreaEntry {
+    int32_t si;
+    float alpha;
+};
+int launch_area_frac_u8(const DImg &s, const DImg &d, const AreaEntry *xt, const int32_t *xstart, const AreaEntry *yt,
+                        const int32_t *ystart, hipStream_t st);
+
+// ------------------------------------------------------------------ warps
+
+struct AffineParams {
+    DImg s, d;
+    int depth, interp, border;
+    float m[6];  // inverse matrix rounded to f32
+    float cv;    // border value as f32 (toF of the stored sample)
+    double cvd;  // border value as stored sample
+};
+int launch_warp_affine(const AffineParams &p, hipStream_t st);
+
+struct PerspParams {
+    DImg s, d;
+    int depth, interp, border;
+    double m[9];  // inverse matrix (f64)
+    double cvd;
+};
+int launch_warp_perspective(const PerspParams &p, hipStream_t st);
+
+int launch_remap(const DImg &s, const DImg &d, const DImg &mx, const DImg &my, int bv, hipStream_t st);
+
+// ------------------------------------------------------------- morphology
+
+// One horizontal run of "on" cells of the structuring element: element row
+// ky, columns [x0, x0 + len).
+struct MorphRun {
+    int16_t ky, x0, len, pad;
+};
+constexpr int kMaxRuns = 256;  // rect, cross and ellipse have at most one run per row
+struct MorphParams {
+    DImg s, d;
+    int depth;
+    int op_max;
+    int kw, kh, ax, ay;
+    int nruns;
+    MorphRun runs[kMaxRuns];
+};
+int launch_morph_pass(const MorphParams &p, hipStream_t st);
+// d = saturate(a - b) (morph.zig:574-593)
+int launch_sub_sat(const DImg &a, const DImg &b, const DImg &d, int depth, hipStream_t st);
+
+// --------------------------------------------------------------- histogram
+int launch_hist_u8(const DImg &s, uint32_t *hist_dev, hipStream_t st);
+
+}  // namespace visar

@@ -1,0 +1,59 @@
+// runtime.hpp -- per-call staging of visar_image descriptors onto the device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <utility>
+#include <vector>
+
+#include "../../include/visar.h"
+#include "common.hpp"
+
+namespace visar {
+
+int fail(int status, const char *name);
+int ensure_runtime();
+hipStream_t current_stream();
+bool sync_mode();
+bool is_device_ptr(const void *p);
+int check_image(const visar_image *c, uint32_t depth);
+void *pool_get(size_t bytes);
+void pool_put(void *p);
+
+// One C-ABI call: resolves its images to device views (device buffers in
+// place, host memory staged through pooled device buffers), owns the
+// temporaries, and on finish() copies staged outputs back and waits when
+// the call's semantics require it. Temporaries return to the pool when the
+// Call is destroyed, after the stream has drained.
+class Call {
+  public:
+    Call() = default;
+    Call(const Call &) = delete;
+    Call &operator=(const Call &) = delete;
+    ~Call();
+
+    int begin();
+    int input(const visar_image *c, DImg &out);
+    int output(const visar_image *c, DImg &out);
+    int temp_like(const DImg &like, size_t elem, DImg &out);
+    void *scratch(size_t bytes);
+    int finish(int launch_status);
+    hipStream_t stream() const { return st_; }
+
+  private:
+    struct Pending {
+        const visar_image *desc;
+        void *buf;
+        size_t pitch;
+        bool device_target;
+    };
+    DImg view(const visar_image *c, void *p, size_t stride);
+
+    hipStream_t st_ = nullptr;
+    bool staged_ = false;
+    std::vector<void *> owned_;
+    std::vector<Pending> pending_;
+    std::vector<std::pair<const uint8_t *, size_t>> dev_ranges_;
+};
+
+}  // namespace visar

@@ -1,0 +1,218 @@
+"""Benchmark of the MI355X imgproc hot path (one GPU).
+
+Default workload: the single-image chain on a 3840x2160 3-channel u8 BGR
+frame of uniform bytes: cvtColor BGR->GRAY, resize INTER_LINEAR to
+1920x1080 (exact half), Sobel dx=1 dy=0 ksize=3 to int16, warpAffine
+INTER_LINEAR BORDER_CONSTANT=0 with a seeded matrix (rotation U(-30, 30) deg,
+scale U(0.8, 1.2), shift U(+-10 %)), and warpPerspective with the corners
+jittered by +-5 %. One step = the whole chain on one frame, every output
+written. Inputs are resident in HBM and rotate through 24 distinct frames
+(597 MB, more than twice the 256 MB MALL). The chain is enqueued through the
+C ABI on torch's stream and replayed from a HIP graph.
+
+Prints one JSON line: value = input Gpix/s of the whole chain, a per-kernel
+table (HIP-event durations, algorithmic bytes, GB/s), the roofline of the
+dominant kernel, and the CPU oracle timed on a bounded sample of the same
+workload. `--config 1` times the plain 1080p GaussianBlur 5x5 instead.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=240)
+    ap.add_argument("--warmup", type=int, default=24)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def per_launch_ms(torch, fn, n, stream):
+    """Average duration of one launch of `fn(i)`: n launches are queued
+    behind a spin kernel (so host enqueue time never shows) and bracketed by
+    one pair of events on the launch stream; the figure includes the
+    dispatch gap between back-to-back kernels."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    torch.cuda._sleep(50_000_000)
+    e0.record(stream)
+    for i in range(n):
+        fn(i)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / n
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); images and streams shard "
+              "across GPUs without exchange, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import visar_amd as va
+    from visar_amd import synth
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    va.set_stream(stream)
+    va.set_sync(False)
+
+    R = 24
+    if args.config == 2:
+        W, H = 3840, 2160
+        frames = [torch.from_numpy(synth.uniform_bytes(i, (H, W, 3))).to(dev) for i in range(R)]
+        gray = [torch.empty((H, W), dtype=torch.uint8, device=dev) for _ in range(2)]
+        small = [torch.empty((H // 2, W // 2), dtype=torch.uint8, device=dev) for _ in range(2)]
+        sob = [torch.empty((H // 2, W // 2), dtype=torch.int16, device=dev) for _ in range(2)]
+        wa = [torch.empty((H // 2, W // 2), dtype=torch.uint8, device=dev) for _ in range(2)]
+        wp = [torch.empty((H // 2, W // 2), dtype=torch.uint8, device=dev) for _ in range(2)]
+        ma = synth.affine_matrix(0, W // 2, H // 2)
+        mp = synth.perspective_matrix(0, W // 2, H // 2)
+        sz = (W // 2, H // 2)
+        ops = {
+            "cvt_color_bgr2gray": (lambda i: va.cvt_color(frames[i % R], 6, dst=gray[i % 2]), W * H * 3 + W * H),
+            "resize_linear_half": (lambda i: va.resize(gray[i % 2], sz, 1, dst=small[i % 2]), W * H + W * H // 4),
+            "sobel_dx_i16": (lambda i: va.sobel(small[i % 2], np.int16, 1, 0, 3, dst=sob[i % 2]), W * H // 4 * 3),
+            "warp_affine_linear": (lambda i: va.warp_affine(small[i % 2], ma, sz, 1, 0, 0, dst=wa[i % 2]), W * H // 2),
+            "warp_perspective_linear": (lambda i: va.warp_perspective(small[i % 2], mp, sz, 1, 0, 0, dst=wp[i % 2]),
+                                        W * H // 2),
+        }
+        pix_per_step = W * H
+        workload = "chain 3840x2160x3 u8: gray, resize linear 1/2, sobel dx i16, warpAffine, warpPerspective"
+    else:
+        W, H = 1920, 1080
+        frames = [torch.from_numpy(synth.uniform_bytes(i, (H, W))).to(dev) for i in range(R * 6)]
+        outs = [torch.empty((H, W), dtype=torch.uint8, device=dev) for _ in range(2)]
+        ops = {"gaussian_blur_5x5": (lambda i: va.gaussian_blur(frames[i % len(frames)], (5, 5), 0, dst=outs[i % 2]),
+                                     2 * W * H)}
+        pix_per_step = W * H
+        workload = "GaussianBlur 5x5 reflect-101 1920x1080 u8"
+        R = len(frames)
+
+    def step(i):
+        for fn, _ in ops.values():
+            fn(i)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+
+    # One graph per input slot: the chain's launches replay without host work.
+    graphs = []
+    cap_stream = torch.cuda.Stream()
+    cap_stream.wait_stream(stream)
+    with torch.cuda.stream(cap_stream):
+        va.set_stream(cap_stream)
+        for i in range(R):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=cap_stream):
+                step(i)
+            graphs.append(g)
+    va.set_stream(stream)
+    stream.wait_stream(cap_stream)
+    torch.cuda.synchronize()
+    for i in range(R):  # graph warmup
+        graphs[i].replay()
+    torch.cuda.synchronize()
+
+    K = args.steps
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for i in range(K):
+        graphs[i % R].replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_time(e1)
+    ms_per_step = max(gpu_ms, wall * 1e3) / K
+    value = pix_per_step / (ms_per_step * 1e-3) / 1e9
+
+    # Per-kernel durations, live, with HIP events on the launch stream.
+    kernels = {}
+    for name, (fn, nbytes) in ops.items():
+        ms = per_launch_ms(torch, fn, 96, stream)
+        kernels[name] = {"ms": round(ms, 5), "bytes": int(nbytes), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                         "frac_hbm_peak": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    achieved = kernels[dom]["GB/s"]
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get(dom)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args, synth)
+
+    line = {
+        "metric": "Gpix/s (input pixels through the chain) + HBM GB/s per kernel",
+        "value": round(value, 4),
+        "unit": "Gpix/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (splitmix64 uniform bytes, 24 rotating device frames)",
+        "config": {"workload": workload, "width": W, "height": H, "rotating_inputs": R, "graph_replay": True},
+        "kernels": kernels,
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline(args, synth):
+    """The oracle (scalar C restatement, one thread) on a bounded sample of the
+    same workload: whole steps until about --cpu-seconds of CPU work."""
+    import oracle
+    done, t0 = 0, time.perf_counter()
+    if args.config == 2:
+        W, H = 3840, 2160
+        a = synth.uniform_bytes(0, (H, W, 3))
+        ma = synth.affine_matrix(0, W // 2, H // 2)
+        mp = synth.perspective_matrix(0, W // 2, H // 2)
+        while time.perf_counter() - t0 < args.cpu_seconds or done == 0:
+            g = oracle.cvt_color(a, 6)
+            s = oracle.resize(g, (W // 2, H // 2), 1)
+            oracle.sobel(s, np.int16, 1, 0, 3)
+            oracle.warp_affine(s, ma, (W // 2, H // 2), 1, 0, 0)
+            oracle.warp_perspective(s, mp, (W // 2, H // 2), 1, 0, 0)
+            done += 1
+        sample = f"{done} full chains of 3840x2160x3 on one frame"
+    else:
+        W, H = 1920, 1080
+        a = synth.uniform_bytes(0, (H, W))
+        while time.perf_counter() - t0 < args.cpu_seconds or done == 0:
+            oracle.gaussian_blur(a, (5, 5), 0)
+            done += 1
+        sample = f"{done} GaussianBlur 5x5 calls on 1920x1080"
+    dt = time.perf_counter() - t0
+    return {"value": round(done * W * H / dt / 1e9, 6), "unit": "Gpix/s", "cores": 1, "kind": "port",
+            "sample": sample + f" in {dt:.1f} s (scalar C oracle, -O2, one thread)"}
+
+
+if __name__ == "__main__":
+    main()

@@ -52,10 +52,11 @@ def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-def both(fn_va, fn_or, *arrays):
-    """Runs the product on host and device copies; returns (host, device, oracle)."""
+def both(fn_va, fn_or, *arrays, n_images=1):
+    """Runs the product on host arrays and on device copies of the first
+    `n_images` arguments; returns (host, device, oracle)."""
     host = fn_va(*arrays)
-    dres = fn_va(*[dev(a) if isinstance(a, np.ndarray) else a for a in arrays])
+    dres = fn_va(*[dev(a) if i < n_images else a for i, a in enumerate(arrays)])
     dres = dres.cpu().numpy() if hasattr(dres, "cpu") else dres
     return host, dres, fn_or(*arrays)
 
@@ -76,7 +77,8 @@ def assert_same(got, want, msg=""):
 def check(va, orc, name, *arrays, **kw):
     f_va = getattr(va, name)
     f_or = getattr(orc, name)
-    host, dres, want = both(lambda *a: f_va(*a, **kw), lambda *a: f_or(*a, **kw), *arrays)
+    n_images = 3 if name == "remap" else 1
+    host, dres, want = both(lambda *a: f_va(*a, **kw), lambda *a: f_or(*a, **kw), *arrays, n_images=n_images)
     assert_same(host, want, f"{name} host {kw}")
     assert_same(dres, want, f"{name} device {kw}")
 

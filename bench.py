@@ -69,7 +69,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()  # a dedicated stream (not the legacy default stream)
+    torch.cuda.set_stream(stream)
     va.set_stream(stream)
     va.set_sync(False)
 

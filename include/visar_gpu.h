@@ -17,9 +17,11 @@
 extern "C" {
 #endif
 
-/* Stream used by the calling thread's subsequent calls (a hipStream_t); NULL
- * selects the library's own non-blocking stream. */
+/* Stream used by the calling thread's subsequent calls (a hipStream_t; NULL is
+ * HIP's default stream). visar_gpu_reset_stream() returns the thread to the
+ * library's own non-blocking stream, which is used until a stream is set. */
 int visar_gpu_set_stream(void *hip_stream);
+int visar_gpu_reset_stream(void);
 void *visar_gpu_get_stream(void);
 
 /* 1 (default): a call whose images are all device buffers returns after its
@@ -32,6 +34,12 @@ int visar_gpu_synchronize(void);
 int visar_gpu_malloc(size_t bytes, void **out);
 int visar_gpu_free(void *p);
 int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D, 2 D2H, 3 D2D */);
+
+/* Launch-shape knobs read by the kernels (the autotuner's interface):
+ * "stencil_rows" output rows per thread of the small separable stencils
+ * (1, 2, 4; 0 = automatic). Unknown keys return VISAR_INVALID_ARGUMENT. */
+int visar_gpu_set_tuning(const char *key, int value);
+int visar_gpu_get_tuning(const char *key, int *value);
 
 /* Name and compute units of the current device, e.g. "gfx950 256 CUs". */
 int visar_gpu_device_info(char *buf, size_t n);

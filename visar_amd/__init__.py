@@ -145,10 +145,14 @@ def get_num_threads():
 
 
 def set_stream(stream):
-    """Stream for this thread's calls: a torch.cuda.Stream, a raw hipStream_t int, or None (library stream)."""
-    if stream is not None and hasattr(stream, "cuda_stream"):
+    """Stream for this thread's calls: a torch.cuda.Stream or a raw hipStream_t
+    int (0 is HIP's default stream); None returns to the library's own stream."""
+    if stream is None:
+        _check(lib.visar_gpu_reset_stream())
+        return
+    if hasattr(stream, "cuda_stream"):
         stream = stream.cuda_stream
-    _check(lib.visar_gpu_set_stream(stream))
+    _check(lib.visar_gpu_set_stream(ctypes.c_void_p(stream)))
 
 
 def get_stream():

@@ -276,7 +276,13 @@ int visar_get_num_threads(void) {
 
 int visar_gpu_set_stream(void *s) {
     t_stream = (hipStream_t)s;
-    t_stream_set = s != nullptr;
+    t_stream_set = true;
+    return VISAR_OK;
+}
+
+int visar_gpu_reset_stream(void) {
+    t_stream = nullptr;
+    t_stream_set = false;
     return VISAR_OK;
 }
 
@@ -319,6 +325,18 @@ int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind) {
     if (kind < 1 || kind > 3 || !dst || !src) return fail(INVALID_ARGUMENT, "InvalidArgument");
     if (hipMemcpyAsync(dst, src, bytes, k, current_stream()) != hipSuccess) return fail(INTERNAL, "CopyFailed");
     return hipStreamSynchronize(current_stream()) == hipSuccess ? VISAR_OK : fail(INTERNAL, "CopyFailed");
+}
+
+int visar_gpu_set_tuning(const char *key, int value) {
+    if (!key) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (!set_tuning(key, value)) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    return VISAR_OK;
+}
+
+int visar_gpu_get_tuning(const char *key, int *value) {
+    if (!key || !value) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (!get_tuning(key, value)) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    return VISAR_OK;
 }
 
 int visar_gpu_device_info(char *buf, size_t n) {
@@ -609,6 +627,8 @@ int visar_threshold(const visar_image *src, const visar_image *dst, double thres
         long ti = fl < -2147483648.0 ? -2147483648L : fl > 2147483647.0 ? 2147483647L : (long)fl;
         int m = sat_cast<uint8_t>(maxval);
         int tc = ti < 0 ? 0 : ti > 255 ? 255 : (int)ti;
+        p.ti = (int)(ti < -1 ? -1 : ti > 256 ? 256 : ti);
+        p.mi = m;
         for (int i = 0; i < 256; i++) {
             bool gt = i > ti;
             int v;

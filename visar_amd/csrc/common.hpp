@@ -40,25 +40,36 @@ __host__ __device__ inline T *row_ptr(const DImg &im, long y) {
 }
 
 // border.zig:22-39: maps a coordinate into [0, n), or -1 for a constant border.
-__host__ __device__ inline long border_map(long p, long n, int border) {
-    if (p >= 0 && p < n) return p;
+// 32-bit arithmetic: image sides are at most 2^30 and coordinates within
+// +-1e9 (the warps clamp them there first). The reference reflects
+// repeatedly; the results are periodic (period 2n for reflect, 2n - 2 for
+// reflect-101), which the closed forms below use.
+__host__ __device__ __attribute__((noinline)) inline int border_map_outside(int p, int n, int border) {
     switch (border) {
     case BORDER_CONSTANT: return -1;
     case BORDER_REPLICATE: return p < 0 ? 0 : n - 1;
-    case BORDER_REFLECT:
+    case BORDER_REFLECT: {
+        int q = p % (2 * n);
+        if (q < 0) q += 2 * n;
+        return q >= n ? 2 * n - 1 - q : q;
+    }
     case BORDER_REFLECT101: {
         if (n == 1) return 0;
-        long d = border == BORDER_REFLECT101 ? 1 : 0;
-        long q = p;
-        while (q < 0 || q >= n) q = q < 0 ? -q - 1 + d : n - 1 - (q - n) - d;
-        return q;
+        int q = p % (2 * n - 2);
+        if (q < 0) q += 2 * n - 2;
+        return q >= n ? 2 * n - 2 - q : q;
     }
     case BORDER_WRAP: {
-        long m = p % n;
+        int m = p % n;
         return m < 0 ? m + n : m;
     }
     default: return -1;
     }
+}
+
+__host__ __device__ inline int border_map(int p, int n, int border) {
+    if ((unsigned)p < (unsigned)n) return p;
+    return border_map_outside(p, n, border);
 }
 
 // saturate.zig:8-18: round half to even with the 1.5*2^k magic constant.

@@ -212,6 +212,8 @@ int run_sep_f32_dst(const SepParams &p, hipStream_t st) {
 }  // namespace
 
 int launch_sep(const SepParams &p, hipStream_t st) {
+    const int th = tuning().stencil_rows ? tuning().stencil_rows : stencil_rows_per_thread((long)p.src.w * p.src.cn, p.src.h);
+    if (launch_sep_small_u8(p, st, th)) return OK;
     if (p.f32mode) {
         switch (p.sdepth) {
         case 0: return run_sep_f32_dst<uint8_t>(p, st);

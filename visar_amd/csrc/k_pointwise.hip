@@ -15,6 +15,14 @@ constexpr int kThreads = 256;
 
 __device__ inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// 128-bit streaming load (read once: the nontemporal hint keeps it out of the caches' way).
+__device__ inline uint4 ld_stream(const void *p) {
+    u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // ---------------------------------------------------------------- gray
 
 // color.zig:126-129, 392-396: y = (b*3735 + g*19235 + r*9798 + 2^14) >> 15
@@ -22,37 +30,63 @@ __device__ inline uint32_t gray_fix(uint32_t b, uint32_t g, uint32_t r) {
     return (b * 3735u + g * 19235u + r * 9798u + (1u << 14)) >> 15;
 }
 
-// 16 pixels per thread; 3-channel rows are read as three 128-bit words.
-template <int SCN>
-__global__ __launch_bounds__(kThreads) void gray_u8(DImg s, DImg d, int sb, long groups_per_row) {
-    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
-    long y = gid / groups_per_row;
-    long x = (gid - y * groups_per_row) * 16;
-    if (y >= s.h) return;
-    const uint8_t *sr = row_ptr<const uint8_t>(s, y) + x * SCN;
-    uint8_t *dr = row_ptr<uint8_t>(d, y) + x;
-    const int bi = sb, ri = 2 - sb;
-    if (x + 16 <= s.w && aligned16(sr) && aligned16(dr)) {
+// 16 pixels per thread; a 3-channel group is three 128-bit loads. SB is the
+// blue channel's index (0 BGR, 2 RGB), fixed at compile time so every byte
+// index below is a constant. When both images are contiguous the grid runs
+// over the flat pixel range (no per-row indexing); otherwise blockIdx.y is
+// the row.
+template <int SCN, int SB, bool FLAT>
+__global__ __launch_bounds__(kThreads) void gray_u8(DImg s, DImg d, long npix) {
+    const long g = (long)blockIdx.x * kThreads + threadIdx.x;
+    long x = g * 16;
+    const uint8_t *sr;
+    uint8_t *dr;
+    long limit;
+    if constexpr (FLAT) {
+        if (x >= npix) return;
+        sr = s.p + x * SCN;
+        dr = d.p + x;
+        limit = npix;
+    } else {
+        const int y = blockIdx.y;
+        if (x >= s.w) return;
+        sr = row_ptr<const uint8_t>(s, y) + x * SCN;
+        dr = row_ptr<uint8_t>(d, y) + x;
+        limit = s.w;
+    }
+    constexpr int BI = SB, RI = 2 - SB;
+    if (x + 16 <= limit && aligned16(sr) && aligned16(dr)) {
         uint4 w[SCN];
 #pragma unroll
-        for (int k = 0; k < SCN; k++) w[k] = reinterpret_cast<const uint4 *>(sr)[k];
-        const uint8_t *b = reinterpret_cast<const uint8_t *>(w);
+        for (int k = 0; k < SCN; k++) w[k] = ld_stream(sr + 16 * k);
+        uint32_t words[4 * SCN];
+#pragma unroll
+        for (int k = 0; k < SCN; k++) {
+            words[4 * k + 0] = w[k].x;
+            words[4 * k + 1] = w[k].y;
+            words[4 * k + 2] = w[k].z;
+            words[4 * k + 3] = w[k].w;
+        }
         uint32_t o[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             uint32_t v = 0;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                int px = q * 4 + k;
-                v |= gray_fix(b[px * SCN + bi], b[px * SCN + 1], b[px * SCN + ri]) << (8 * k);
+                const int px = q * 4 + k;
+                const int ib = px * SCN + BI, ig = px * SCN + 1, ir = px * SCN + RI;
+                uint32_t b = (words[ib >> 2] >> ((ib & 3) * 8)) & 255u;
+                uint32_t gg = (words[ig >> 2] >> ((ig & 3) * 8)) & 255u;
+                uint32_t r = (words[ir >> 2] >> ((ir & 3) * 8)) & 255u;
+                v |= gray_fix(b, gg, r) << (8 * k);
             }
             o[q] = v;
         }
         *reinterpret_cast<uint4 *>(dr) = make_uint4(o[0], o[1], o[2], o[3]);
         return;
     }
-    long n = min(16L, (long)s.w - x);
-    for (long i = 0; i < n; i++) dr[i] = (uint8_t)gray_fix(sr[i * SCN + bi], sr[i * SCN + 1], sr[i * SCN + ri]);
+    const long n = min(16L, limit - x);
+    for (long i = 0; i < n; i++) dr[i] = (uint8_t)gray_fix(sr[i * SCN + BI], sr[i * SCN + 1], sr[i * SCN + RI]);
 }
 
 template <typename T>
@@ -96,6 +130,40 @@ __global__ __launch_bounds__(kThreads) void threshold_k(ThreshParams p, long n_p
     row_ptr<T>(p.d, y)[i] = o;
 }
 
+// u8 threshold on contiguous images, 16 samples per thread: the LUT of
+// threshold.zig:44-60 is "v > ti" with ti = floor(thresh) as an integer, so
+// the bytes are compared directly (ti may lie outside 0..254).
+__global__ __launch_bounds__(kThreads) void threshold_u8_flat(const uint8_t *s, uint8_t *d, long n, int ti, int m,
+                                                              int kind) {
+    const long x = ((long)blockIdx.x * kThreads + threadIdx.x) * 16;
+    if (x >= n) return;
+    const int tc = ti < 0 ? 0 : ti > 255 ? 255 : ti;
+    auto one = [&](uint32_t v) -> uint32_t {
+        const bool gt = (int)v > ti;
+        switch (kind) {
+        case 0: return gt ? (uint32_t)m : 0u;
+        case 1: return gt ? 0u : (uint32_t)m;
+        case 2: return gt ? (uint32_t)tc : v;
+        case 3: return gt ? v : 0u;
+        default: return gt ? 0u : v;
+        }
+    };
+    if (x + 16 <= n && aligned16(s + x) && aligned16(d + x)) {
+        uint4 w = *reinterpret_cast<const uint4 *>(s + x);
+        uint32_t in[4] = {w.x, w.y, w.z, w.w}, o[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint32_t r = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) r |= one((in[q] >> (8 * b)) & 255u) << (8 * b);
+            o[q] = r;
+        }
+        *reinterpret_cast<uint4 *>(d + x) = make_uint4(o[0], o[1], o[2], o[3]);
+        return;
+    }
+    for (long i = x; i < n && i < x + 16; i++) d[i] = (uint8_t)one(s[i]);
+}
+
 // ---------------------------------------------------------- background
 
 // track.zig:33-81. Model u16 in 8.8 fixed point, per pixel in i32.
@@ -120,6 +188,56 @@ __global__ __launch_bounds__(kThreads) void background_k(DImg f, DImg m, uint16_
     *bp = (uint16_t)(bv + (((fv << 8) - bv) >> sh));
 }
 
+// The same update on contiguous frame / mask / model, 16 pixels per thread
+// (128-bit frame and mask accesses, 2 x 128-bit model accesses).
+__global__ __launch_bounds__(kThreads) void background_flat(const uint8_t *f, uint8_t *m, uint16_t *model, long n,
+                                                            int shift, int thr, int prime) {
+    const long x = ((long)blockIdx.x * kThreads + threadIdx.x) * 16;
+    if (x >= n) return;
+    if (x + 16 <= n && aligned16(f + x) && aligned16(m + x) && aligned16(model + x)) {
+        uint4 fw = *reinterpret_cast<const uint4 *>(f + x);
+        uint4 b0 = reinterpret_cast<const uint4 *>(model + x)[0], b1 = reinterpret_cast<const uint4 *>(model + x)[1];
+        uint32_t fv[4] = {fw.x, fw.y, fw.z, fw.w};
+        uint32_t bw[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+        uint32_t mo[4] = {0, 0, 0, 0}, bo[8];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int32_t fi = (int32_t)((fv[i >> 2] >> (8 * (i & 3))) & 255u);
+            const int32_t bi = (int32_t)((bw[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+            int32_t nb;
+            uint32_t mk;
+            if (prime) {
+                nb = fi << 8;
+                mk = 0;
+            } else {
+                const int32_t diff = fi - (bi >> 8);
+                const bool fg = (diff < 0 ? -diff : diff) > thr;
+                mk = fg ? 255u : 0u;
+                nb = bi + (((fi << 8) - bi) >> (shift + (fg ? 2 : 0)));
+            }
+            mo[i >> 2] |= mk << (8 * (i & 3));
+            if (i & 1) bo[i >> 1] |= ((uint32_t)nb & 0xffffu) << 16;
+            else bo[i >> 1] = (uint32_t)nb & 0xffffu;
+        }
+        *reinterpret_cast<uint4 *>(m + x) = make_uint4(mo[0], mo[1], mo[2], mo[3]);
+        reinterpret_cast<uint4 *>(model + x)[0] = make_uint4(bo[0], bo[1], bo[2], bo[3]);
+        reinterpret_cast<uint4 *>(model + x)[1] = make_uint4(bo[4], bo[5], bo[6], bo[7]);
+        return;
+    }
+    for (long i = x; i < n && i < x + 16; i++) {
+        const int32_t fi = f[i], bi = model[i];
+        if (prime) {
+            model[i] = (uint16_t)(fi << 8);
+            m[i] = 0;
+            continue;
+        }
+        const int32_t diff = fi - (bi >> 8);
+        const bool fg = (diff < 0 ? -diff : diff) > thr;
+        m[i] = fg ? 255 : 0;
+        model[i] = (uint16_t)(bi + (((fi << 8) - bi) >> (shift + (fg ? 2 : 0))));
+    }
+}
+
 // -------------------------------------------------------------- resize
 
 // resize_u8.zig:381-447: (a + b + c + d + 2) >> 2 per channel.
@@ -135,12 +253,12 @@ __global__ __launch_bounds__(kThreads) void half_u8(DImg s, DImg d, long n_per_r
     row_ptr<uint8_t>(d, y)[e] = (uint8_t)((v + 2) >> 2);
 }
 
-// Single-channel exact half, 16 output pixels (32 input bytes per row) per thread.
-__global__ __launch_bounds__(kThreads) void half_u8_c1(DImg s, DImg d, long groups_per_row) {
-    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
-    long y = gid / groups_per_row;
-    long x = (gid - y * groups_per_row) * 16;
-    if (y >= d.h) return;
+// Single-channel exact half, 16 output pixels (32 input bytes of each of
+// the two source rows) per thread; grid x over 16-pixel groups, y over rows.
+__global__ __launch_bounds__(kThreads) void half_u8_c1(DImg s, DImg d) {
+    const int y = blockIdx.y;
+    const int x = (blockIdx.x * kThreads + threadIdx.x) * 16;
+    if (x >= d.w) return;
     const uint8_t *a = row_ptr<const uint8_t>(s, 2 * y) + 2 * x, *b = row_ptr<const uint8_t>(s, 2 * y + 1) + 2 * x;
     uint8_t *o = row_ptr<uint8_t>(d, y) + x;
     if (x + 16 <= d.w && aligned16(a) && aligned16(b) && aligned16(o)) {
@@ -154,7 +272,7 @@ __global__ __launch_bounds__(kThreads) void half_u8_c1(DImg s, DImg d, long grou
             uint32_t v = 0;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                int px = q * 4 + k;  // output pixel, input bytes 2px, 2px+1
+                int px = q * 4 + k;  // output pixel px reads input bytes 2px, 2px+1
                 uint32_t wa = aw[px / 2], wb = bw[px / 2];
                 int sh = (px & 1) * 16;
                 uint32_t sum = ((wa >> sh) & 255) + ((wa >> (sh + 8)) & 255) + ((wb >> sh) & 255) + ((wb >> (sh + 8)) & 255);
@@ -165,8 +283,8 @@ __global__ __launch_bounds__(kThreads) void half_u8_c1(DImg s, DImg d, long grou
         *reinterpret_cast<uint4 *>(o) = make_uint4(out[0], out[1], out[2], out[3]);
         return;
     }
-    long n = min(16L, (long)d.w - x);
-    for (long i = 0; i < n; i++) {
+    int n = min(16, d.w - x);
+    for (int i = 0; i < n; i++) {
         uint32_t v = (uint32_t)a[2 * i] + a[2 * i + 1] + b[2 * i] + b[2 * i + 1];
         o[i] = (uint8_t)((v + 2) >> 2);
     }
@@ -276,10 +394,17 @@ inline int launched() { return hipGetLastError() == hipSuccess ? OK : INTERNAL; 
 
 int launch_gray(const DImg &s, const DImg &d, int depth, int scn, int sb, hipStream_t st) {
     if (depth == 0) {
-        long gpr = (s.w + 15) / 16;
-        long n = gpr * s.h;
-        if (scn == 3) hipLaunchKernelGGL(gray_u8<3>, dim3(blocks_for(n)), dim3(kThreads), 0, st, s, d, sb, gpr);
-        else hipLaunchKernelGGL(gray_u8<4>, dim3(blocks_for(n)), dim3(kThreads), 0, st, s, d, sb, gpr);
+        const bool flat = s.stride == (long)s.w * scn && d.stride == (long)d.w;
+        const long npix = (long)s.w * s.h;
+        dim3 grid = flat ? dim3(blocks_for((npix + 15) / 16)) : dim3((unsigned)((s.w + 16 * kThreads - 1) / (16 * kThreads)), (unsigned)s.h);
+#define GRAY(SCN, SB)                                                                                       \
+    if (flat) hipLaunchKernelGGL((gray_u8<SCN, SB, true>), grid, dim3(kThreads), 0, st, s, d, npix); \
+    else hipLaunchKernelGGL((gray_u8<SCN, SB, false>), grid, dim3(kThreads), 0, st, s, d, npix);
+        if (scn == 3 && sb == 0) { GRAY(3, 0) }
+        else if (scn == 3) { GRAY(3, 2) }
+        else if (sb == 0) { GRAY(4, 0) }
+        else { GRAY(4, 2) }
+#undef GRAY
         return launched();
     }
     long n = (long)s.w * s.h;
@@ -291,6 +416,12 @@ int launch_gray(const DImg &s, const DImg &d, int depth, int scn, int sb, hipStr
 
 int launch_threshold(const ThreshParams &p, hipStream_t st) {
     long npr = (long)p.s.w * p.s.cn;
+    if (p.depth == 0 && p.s.stride == npr && p.d.stride == npr) {
+        const long n = npr * p.s.h;
+        hipLaunchKernelGGL(threshold_u8_flat, dim3(blocks_for((n + 15) / 16)), dim3(kThreads), 0, st, p.s.p, p.d.p, n,
+                           p.ti, p.mi, p.kind);
+        return launched();
+    }
     long n = npr * p.s.h;
     switch (p.depth) {
     case 0: hipLaunchKernelGGL(threshold_k<uint8_t>, dim3(blocks_for(n)), dim3(kThreads), 0, st, p, npr); break;
@@ -305,6 +436,11 @@ int launch_threshold(const ThreshParams &p, hipStream_t st) {
 int launch_background(const DImg &frame, const DImg &mask, uint16_t *model, long model_stride, int shift, int thr,
                       int prime, hipStream_t st) {
     long n = (long)frame.w * frame.h;
+    if (frame.stride == frame.w && mask.stride == mask.w && model_stride == 2L * frame.w) {
+        hipLaunchKernelGGL(background_flat, dim3(blocks_for((n + 15) / 16)), dim3(kThreads), 0, st, frame.p, mask.p, model,
+                           n, shift, thr, prime);
+        return launched();
+    }
     hipLaunchKernelGGL(background_k, dim3(blocks_for(n)), dim3(kThreads), 0, st, frame, mask, model, model_stride, shift,
                        thr, prime);
     return launched();
@@ -318,8 +454,8 @@ int launch_copy(const DImg &s, const DImg &d, size_t elem, hipStream_t st) {
 
 int launch_half_u8(const DImg &s, const DImg &d, hipStream_t st) {
     if (d.cn == 1) {
-        long gpr = (d.w + 15) / 16;
-        hipLaunchKernelGGL(half_u8_c1, dim3(blocks_for(gpr * d.h)), dim3(kThreads), 0, st, s, d, gpr);
+        dim3 grid((unsigned)((d.w + 16 * kThreads - 1) / (16 * kThreads)), (unsigned)d.h);
+        hipLaunchKernelGGL(half_u8_c1, grid, dim3(kThreads), 0, st, s, d);
         return launched();
     }
     long npr = (long)d.w * d.cn;

@@ -5,6 +5,14 @@
 
 namespace visar {
 
+// Launch-shape knobs (see visar_gpu_set_tuning); 0 means automatic.
+struct Tuning {
+    int stencil_rows = 0;
+};
+Tuning &tuning();
+bool set_tuning(const char *key, int value);
+bool get_tuning(const char *key, int *value);
+
 // ---------------------------------------------------------------- filters
 
 // Finish kinds of reference src/imgproc/filter.zig:61-116.
@@ -33,6 +41,12 @@ struct SepParams {
 
 int launch_sep(const SepParams &p, hipStream_t st);
 
+// Register-window fast path for u8 sources with square 3/5/7 kernels
+// (k_stencil.hip); returns false for shapes it does not cover. `th` is the
+// number of output rows per thread (1, 2 or 4).
+bool launch_sep_small_u8(const SepParams &p, hipStream_t st, int th);
+int stencil_rows_per_thread(long n, int h);
+
 // -------------------------------------------------------------- pointwise
 
 // color.zig:324-403 gray conversion: scn 3/4, sb 0/2 (blue index).
@@ -42,6 +56,7 @@ struct ThreshParams {
     DImg s, d;
     int depth, kind;
     int below;  // 16-bit thresholds below the range: every value is above
+    int ti, mi; // u8: floor(thresh) clamped to the int range, maxval as u8
     uint8_t lut[256];
     double t, m;  // threshold / maxval already converted to the sample type
 };

@@ -6,6 +6,28 @@
 
 namespace visar {
 
+Tuning &tuning() {
+    static Tuning t;
+    return t;
+}
+
+bool set_tuning(const char *key, int value) {
+    if (std::strcmp(key, "stencil_rows") == 0) {
+        if (value != 0 && value != 1 && value != 2 && value != 4) return false;
+        tuning().stencil_rows = value;
+        return true;
+    }
+    return false;
+}
+
+bool get_tuning(const char *key, int *value) {
+    if (std::strcmp(key, "stencil_rows") == 0) {
+        *value = tuning().stencil_rows;
+        return true;
+    }
+    return false;
+}
+
 namespace {
 double bits_to_double(uint64_t b) {
     double d;

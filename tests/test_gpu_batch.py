@@ -1,0 +1,60 @@
+"""Batched persistent launch vs plain per-call GPU results vs the oracle (GPU).
+
+The seeded mixed workload (sides U{64..2048}, 1 or 3 channels, ops GaussianBlur
+3/5/7, resize AREA 1/2, resize LINEAR 1.5x, warpAffine, dilate 5x5 rect) runs
+as one batch; every image must equal its plain C-ABI call bit for bit, and a
+sample of images must equal the oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def va():
+    import visar_amd
+    assert torch.cuda.is_available()
+    return visar_amd
+
+
+def _compare_batch(va, orc, n, lo, hi, seed, oracle_every, use_graph):
+    from visar_amd import batch
+    jobs = batch.workload(seed, n, lo, hi)
+    b = batch.Batch().plan(batch.items(jobs))
+    tiles, wgs = b.info()
+    assert tiles > 0 and wgs > 0
+    for j in jobs:
+        j["dst"].fill_(7)
+    b.run(use_graph=use_graph)
+    b.run(use_graph=use_graph)  # a second run (graph replay) gives the same bytes
+    torch.cuda.synchronize()
+    for i, j in enumerate(jobs):
+        plain = batch.run_plain(va, j, dst=torch.empty_like(j["dst"]))
+        got = j["dst"].cpu().numpy()
+        assert np.array_equal(got, plain.cpu().numpy()), f"item {i} {j['spec']} differs from the plain call"
+        if i % oracle_every == 0:
+            assert np.array_equal(got, batch.run_oracle(orc, j)), f"item {i} {j['spec']} differs from the oracle"
+
+
+def test_batch_small_mixed_graph(va, orc):
+    _compare_batch(va, orc, 96, 16, 300, 11, 1, True)
+
+
+def test_batch_small_mixed_eager(va, orc):
+    _compare_batch(va, orc, 64, 2, 129, 12, 1, False)
+
+
+def test_batch_config_512(va, orc):
+    _compare_batch(va, orc, 512, 64, 2048, 0, 32, True)
+
+
+def test_batch_rejects_bad_items(va):
+    from visar_amd import batch
+    a = torch.zeros((32, 32), dtype=torch.uint8, device="cuda")
+    o = torch.zeros((32, 32), dtype=torch.uint8, device="cuda")
+    with pytest.raises(RuntimeError):
+        batch.Batch().plan([batch.item(batch.GAUSSIAN_BLUR, a, o, ksize=9)])
+    with pytest.raises(RuntimeError):
+        batch.Batch().plan([batch.item(batch.GAUSSIAN_BLUR, a.cpu().numpy(), o, ksize=3)])

@@ -1,0 +1,527 @@
+// batch.hip -- many images, mixed operations, one persistent launch.
+//
+// A plan turns each item into a job: a tile function of tiles.hpp (the same
+// code the plain entry points launch, so results are identical), the job's
+// tile grid and its arguments. Jobs are grouped by kind so that concurrently
+// running workgroups mostly execute the same code. The global tile list is
+// cut into 8 contiguous segments, one per XCD: workgroup w is dispatched to
+// XCD w % 8 and first drains that XCD's segment (neighbouring tiles of an
+// image, and their halo rows, stay in one L2), then steals tiles from the
+// other segments until all are done. Every queue is a counter bumped with
+// atomicAdd; a small kernel clears the counters before each run (no
+// hipMemsetAsync, so the run can be captured in a HIP graph).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/visar_batch.h"
+#include "runtime.hpp"
+#include "tables.hpp"
+#include "tiles.hpp"
+
+namespace visar {
+
+namespace {
+
+constexpr int kXcds = 8;
+
+enum JobKind : int {
+    JK_STENCIL = 0,
+    JK_MORPH = 1,
+    JK_AFFINE_LIN4 = 2,
+    JK_AFFINE_PX = 3,
+    JK_HALF_C1 = 4,
+    JK_HALF = 5,
+    JK_LINEAR = 6,
+    JK_AREA_INT = 7,
+    JK_AREA_FRAC = 8,
+};
+
+struct ResizeArgs {
+    DImg s, d;
+    const LinTap *xt, *yt;
+    const AreaEntry *xe, *ye;
+    const int32_t *xs, *ys;
+    int ix, iy;
+    float scale;
+};
+
+struct BatchJob {
+    int kind;
+    int variant;  // template selector (kernel size, channels, rows per thread, min/max)
+    int tiles_x;
+    int tile0;
+    union {
+        StencilArgs st;
+        MorphSmallArgs mo;
+        AffineParams af;
+        ResizeArgs rs;
+    };
+};
+
+// Variants compiled into the persistent kernel.
+constexpr int stencil_variant(int k, int cn, int th) { return k * 100 + cn * 10 + th; }
+constexpr int morph_variant(int k, int cn, int mx) { return k * 100 + cn * 10 + mx; }
+
+// The largest LDS use among the compiled variants.
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int kStencilLds =
+    cmax(StencilShape<7, 3, 2>::LDS_WORDS, cmax(StencilShape<7, 1, 2>::LDS_WORDS, StencilShape<5, 3, 2>::LDS_WORDS));
+constexpr int kMorphLds = cmax(MorphShape<7, 3, 2>::LDS_WORDS, MorphShape<7, 1, 2>::LDS_WORDS);
+constexpr int kBatchLds = cmax(kStencilLds, kMorphLds);
+
+template <int K, int CN>
+__device__ inline void run_stencil_th(const StencilArgs &a, int th, int bx, int by, uint32_t *lds) {
+    if (th == 2) stencil_tile<K, CN, 2, uint8_t>(a, bx, by, lds);
+    else stencil_tile<K, CN, 1, uint8_t>(a, bx, by, lds);
+}
+
+__device__ inline void run_stencil(const BatchJob &j, int bx, int by, uint32_t *lds) {
+    const int th = j.variant % 10, cn = (j.variant / 10) % 10, k = j.variant / 100;
+    if (cn == 1) {
+        if (k == 3) run_stencil_th<3, 1>(j.st, th, bx, by, lds);
+        else if (k == 5) run_stencil_th<5, 1>(j.st, th, bx, by, lds);
+        else run_stencil_th<7, 1>(j.st, th, bx, by, lds);
+    } else {
+        if (k == 3) run_stencil_th<3, 3>(j.st, th, bx, by, lds);
+        else if (k == 5) run_stencil_th<5, 3>(j.st, th, bx, by, lds);
+        else run_stencil_th<7, 3>(j.st, th, bx, by, lds);
+    }
+}
+
+template <int K, int CN>
+__device__ inline void run_morph_mx(const MorphSmallArgs &a, int mx, int bx, int by, uint32_t *lds) {
+    if (mx) morph_small_tile<K, CN, 2, true>(a, bx, by, lds);
+    else morph_small_tile<K, CN, 2, false>(a, bx, by, lds);
+}
+
+__device__ inline void run_morph(const BatchJob &j, int bx, int by, uint32_t *lds) {
+    const int mx = j.variant % 10, cn = (j.variant / 10) % 10, k = j.variant / 100;
+    if (cn == 1) {
+        if (k == 3) run_morph_mx<3, 1>(j.mo, mx, bx, by, lds);
+        else if (k == 5) run_morph_mx<5, 1>(j.mo, mx, bx, by, lds);
+        else run_morph_mx<7, 1>(j.mo, mx, bx, by, lds);
+    } else {
+        if (k == 3) run_morph_mx<3, 3>(j.mo, mx, bx, by, lds);
+        else if (k == 5) run_morph_mx<5, 3>(j.mo, mx, bx, by, lds);
+        else run_morph_mx<7, 3>(j.mo, mx, bx, by, lds);
+    }
+}
+
+__device__ inline void run_tile(const BatchJob &j, int t, uint32_t *lds) {
+    const int lt = t - j.tile0;
+    const int by = lt / j.tiles_x, bx = lt - by * j.tiles_x;
+    switch (j.kind) {
+    case JK_STENCIL: run_stencil(j, bx, by, lds); break;
+    case JK_MORPH: run_morph(j, bx, by, lds); break;
+    case JK_AFFINE_LIN4: affine_lin4_tile(j.af, bx, by); break;
+    case JK_AFFINE_PX: affine_px_tile<uint8_t>(j.af, bx, by); break;
+    case JK_HALF_C1: half_c1_tile(j.rs.s, j.rs.d, bx, by); break;
+    case JK_HALF: half_tile(j.rs.s, j.rs.d, bx, by); break;
+    case JK_LINEAR: linear_tile(j.rs.s, j.rs.d, j.rs.xt, j.rs.yt, bx, by); break;
+    case JK_AREA_INT: area_int_tile(j.rs.s, j.rs.d, j.rs.ix, j.rs.iy, j.rs.scale, bx, by); break;
+    default: area_frac_tile(j.rs.s, j.rs.d, j.rs.xe, j.rs.xs, j.rs.ye, j.rs.ys, bx, by); break;
+    }
+}
+
+// Counters: [0, kXcds) next tile of each segment. Segment q is the tile
+// range [seg[q], seg[q + 1]).
+__global__ void batch_reset(int *counters) {
+    if (threadIdx.x < kXcds) counters[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(kTileThreads) void batch_kernel(const BatchJob *jobs, int njobs, const int *seg,
+                                                             int *counters) {
+    __shared__ uint32_t lds[kBatchLds];
+    __shared__ int next;
+    const int home = blockIdx.x % kXcds;
+    for (int k = 0; k < kXcds; k++) {
+        const int q = (home + k) % kXcds;
+        const int base = seg[q], size = seg[q + 1] - base;
+        for (;;) {
+            if (threadIdx.x == 0) next = atomicAdd(&counters[q], 1);
+            __syncthreads();
+            const int i = __builtin_amdgcn_readfirstlane(next);
+            __syncthreads();
+            if (i >= size) break;
+            const int t = base + i;
+            int lo = 0, hi = njobs - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (jobs[mid].tile0 <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            run_tile(jobs[lo], t, lds);
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace
+
+}  // namespace visar
+
+using namespace visar;
+
+struct visar_batch {
+    std::vector<BatchJob> jobs;
+    std::vector<int> seg;
+    size_t tiles = 0;
+    int workgroups = 0;
+    void *dev = nullptr;  // jobs | segments | counters | tables
+    size_t dev_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+namespace {
+
+void drop_graph(visar_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+DImg dview(const visar_image &c) {
+    DImg d;
+    d.p = static_cast<uint8_t *>(c.data);
+    d.w = (int)c.width;
+    d.h = (int)c.height;
+    d.cn = (int)c.channels;
+    d.stride = (long)c.stride;
+    return d;
+}
+
+struct Planned {
+    BatchJob job;
+    std::vector<uint8_t> table;  // resize tables, appended to the device arena
+    int table_kind = 0;          // 1 linear, 2 area
+    size_t nx = 0, ny = 0, nxs = 0;
+};
+
+int tiles256(const DImg &d, int rows) {
+    return (int)(((long)d.w * d.cn + 255) / 256) * ((d.h + rows - 1) / rows);
+}
+
+// Validates one item and fills its job; returns a visar status.
+int plan_item(const visar_batch_item &it, Planned &pl) {
+    BatchJob &j = pl.job;
+    std::memset(&j, 0, sizeof j);
+    int rc;
+    if ((rc = check_image(&it.src, 0)) || (rc = check_image(&it.dst, 0))) return rc;
+    if (!is_device_ptr(it.src.data) || !is_device_ptr(it.dst.data)) return fail(UNSUPPORTED, "HostMemoryInBatch");
+    if (it.src.channels != it.dst.channels || (it.src.channels != 1 && it.src.channels != 3))
+        return fail(UNSUPPORTED, "UnsupportedChannels");
+    const DImg s = dview(it.src), d = dview(it.dst);
+    const bool same = it.src.width == it.dst.width && it.src.height == it.dst.height;
+    switch (it.op) {
+    case VISAR_BATCH_GAUSSIAN_BLUR: {
+        if (it.border < 0 || it.border > 4) return fail(UNSUPPORTED, "Unsupported");
+        if (!same) return fail(SIZE_MISMATCH, "SizeMismatch");
+        const int k = it.ksize;
+        if (k != 3 && k != 5 && k != 7) return fail(UNSUPPORTED, "Unsupported");
+        // A one-pixel-wide (or -high) image maps every tap of that axis to
+        // the same sample, which equals the reference's k = 1 there.
+        uint32_t taps[kMaxTaps];
+        gaussian_kernel_fixed(taps, k, it.sigma, 8);
+        j.kind = JK_STENCIL;
+        j.st.src = s;
+        j.st.dst = d;
+        j.st.border = it.border;
+        j.st.fin = FIN_FIX8;
+        j.st.scale = 1.0f;
+        for (int i = 0; i < 7; i++) j.st.kx[i] = j.st.ky[i] = i < k ? (int)taps[i] : 0;
+        int th = tuning().batch_stencil_rows ? tuning().batch_stencil_rows : 1;
+        th = th >= 2 ? 2 : 1;
+        j.variant = k * 100 + (int)it.src.channels * 10 + th;
+        j.tiles_x = (int)(((long)s.w * s.cn + 255) / 256);
+        j.tile0 = tiles256(s, 4 * th);
+        return OK;
+    }
+    case VISAR_BATCH_MORPHOLOGY: {
+        if (it.morph_op != 0 && it.morph_op != 1) return fail(UNSUPPORTED, "Unsupported");
+        if (it.shape < 0 || it.shape > 2) return fail(UNSUPPORTED, "Unsupported");
+        if (it.ksize <= 0 || it.ksize > 255 || it.iterations < 0) return fail(INVALID_ARGUMENT, "InvalidArgument");
+        if (it.iterations != 1 || it.ksize > 7) return fail(UNSUPPORTED, "Unsupported");
+        if (!same) return fail(SIZE_MISMATCH, "SizeMismatch");
+        std::vector<uint8_t> mask;
+        structuring_element(it.shape, it.ksize, it.ksize, mask);
+        MorphParams mp;
+        std::memset(&mp, 0, sizeof mp);
+        mp.s = s;
+        mp.d = d;
+        mp.depth = 0;
+        mp.kw = mp.kh = it.ksize;
+        mp.ax = mp.ay = it.ksize / 2;
+        for (int ky = 0; ky < it.ksize; ky++) {
+            int x = 0;
+            while (x < it.ksize) {
+                if (!mask[(size_t)ky * it.ksize + x]) {
+                    x++;
+                    continue;
+                }
+                int x0 = x;
+                while (x < it.ksize && mask[(size_t)ky * it.ksize + x]) x++;
+                mp.runs[mp.nruns++] = MorphRun{(int16_t)ky, (int16_t)x0, (int16_t)(x - x0), 0};
+            }
+        }
+        const int kf = morph_small_args(mp, j.mo);
+        if (!kf) return fail(UNSUPPORTED, "Unsupported");
+        j.kind = JK_MORPH;
+        j.variant = kf * 100 + (int)it.src.channels * 10 + it.morph_op;
+        j.tiles_x = (int)(((long)s.w * s.cn + 255) / 256);
+        j.tile0 = tiles256(s, 8);
+        return OK;
+    }
+    case VISAR_BATCH_WARP_AFFINE: {
+        if (it.border < 0 || it.border > 4) return fail(UNSUPPORTED, "Unsupported");
+        double inv[6];
+        if (!invert_affine(it.m, inv)) return fail(UNSUPPORTED, "Unsupported");
+        AffineParams &p = j.af;
+        p.s = s;
+        p.d = d;
+        p.depth = 0;
+        p.interp = it.interpolation == 0 ? 0 : 1;
+        p.border = it.border;
+        for (int i = 0; i < 6; i++) p.m[i] = (float)inv[i];
+        p.cvd = sat_cast<uint8_t>(it.value);
+        p.cv = (float)p.cvd;
+        if (it.src.channels == 1 && p.interp == 1) {
+            j.kind = JK_AFFINE_LIN4;
+            j.tiles_x = (d.w + 255) / 256;
+        } else {
+            j.kind = JK_AFFINE_PX;
+            j.tiles_x = (d.w + 63) / 64;
+        }
+        j.tile0 = j.tiles_x * ((d.h + 3) / 4);
+        return OK;
+    }
+    case VISAR_BATCH_RESIZE: {
+        if (it.interpolation != 1 && it.interpolation != 3) return fail(UNSUPPORTED, "Unsupported");
+        if (same) return fail(UNSUPPORTED, "Unsupported");
+        ResizeArgs &r = j.rs;
+        r.s = s;
+        r.d = d;
+        const bool exact_half = it.src.width == 2 * it.dst.width && it.src.height == 2 * it.dst.height;
+        const bool area_up = it.interpolation == 3 && (it.dst.width > it.src.width || it.dst.height > it.src.height);
+        if (exact_half && !area_up) {
+            if (d.cn == 1) {
+                j.kind = JK_HALF_C1;
+                j.tiles_x = (d.w + 1023) / 1024;
+            } else {
+                j.kind = JK_HALF;
+                j.tiles_x = (d.w * d.cn + 255) / 256;
+            }
+            j.tile0 = j.tiles_x * ((d.h + 3) / 4);
+            return OK;
+        }
+        j.tiles_x = (d.w * d.cn + 255) / 256;
+        j.tile0 = j.tiles_x * ((d.h + 3) / 4);
+        if (it.interpolation == 1 || area_up) {
+            j.kind = JK_LINEAR;
+            std::vector<LinTap> t(it.dst.width + it.dst.height);
+            for (uint32_t x = 0; x < it.dst.width; x++) {
+                LinTap q = linear_tap_u8(x, it.src.width, it.dst.width, area_up, true);
+                q.o0 *= (int32_t)it.src.channels;
+                q.o1 *= (int32_t)it.src.channels;
+                t[x] = q;
+            }
+            for (uint32_t y = 0; y < it.dst.height; y++)
+                t[it.dst.width + y] = linear_tap_u8(y, it.src.height, it.dst.height, area_up, false);
+            pl.table.resize(t.size() * sizeof(LinTap));
+            std::memcpy(pl.table.data(), t.data(), pl.table.size());
+            pl.table_kind = 1;
+            pl.nx = it.dst.width;
+            return OK;
+        }
+        const uint32_t ix = it.src.width / it.dst.width, iy = it.src.height / it.dst.height;
+        if (ix * it.dst.width == it.src.width && iy * it.dst.height == it.src.height) {
+            j.kind = JK_AREA_INT;
+            r.ix = (int)ix;
+            r.iy = (int)iy;
+            r.scale = 1.0f / (float)(ix * iy);
+            return OK;
+        }
+        j.kind = JK_AREA_FRAC;
+        std::vector<AreaEntry> xe, ye;
+        std::vector<int32_t> xs, ys;
+        area_tab(it.src.width, it.dst.width, (double)it.src.width / (double)it.dst.width, xe, xs);
+        area_tab(it.src.height, it.dst.height, (double)it.src.height / (double)it.dst.height, ye, ys);
+        size_t bytes = (xe.size() + ye.size()) * sizeof(AreaEntry) + (xs.size() + ys.size()) * sizeof(int32_t);
+        pl.table.resize(bytes);
+        uint8_t *o = pl.table.data();
+        std::memcpy(o, xe.data(), xe.size() * sizeof(AreaEntry));
+        o += xe.size() * sizeof(AreaEntry);
+        std::memcpy(o, ye.data(), ye.size() * sizeof(AreaEntry));
+        o += ye.size() * sizeof(AreaEntry);
+        std::memcpy(o, xs.data(), xs.size() * sizeof(int32_t));
+        o += xs.size() * sizeof(int32_t);
+        std::memcpy(o, ys.data(), ys.size() * sizeof(int32_t));
+        pl.table_kind = 2;
+        pl.nx = xe.size();
+        pl.ny = ye.size();
+        pl.nxs = xs.size();
+        return OK;
+    }
+    default: return fail(UNSUPPORTED, "Unsupported");
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int visar_batch_create(visar_batch **out) {
+    if (!out) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    int rc = ensure_runtime();
+    if (rc) return rc;
+    *out = new (std::nothrow) visar_batch();
+    return *out ? VISAR_OK : fail(OUT_OF_MEMORY, "OutOfMemory");
+}
+
+int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, size_t *bad_item) {
+    if (!b || (!items && n)) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    int rc = ensure_runtime();
+    if (rc) return rc;
+    std::vector<Planned> pl(n);
+    for (size_t i = 0; i < n; i++) {
+        if ((rc = plan_item(items[i], pl[i]))) {
+            if (bad_item) *bad_item = i;
+            return rc;
+        }
+    }
+    // Group jobs by kind and variant (stable: the caller's order within a group).
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) {
+        if (pl[a].job.kind != pl[c].job.kind) return pl[a].job.kind < pl[c].job.kind;
+        return pl[a].job.variant < pl[c].job.variant;
+    });
+    // Device arena: jobs, segment bounds, counters, tables.
+    size_t jobs_bytes = (n ? n : 1) * sizeof(BatchJob);
+    size_t seg_off = (jobs_bytes + 255) & ~(size_t)255;
+    size_t cnt_off = seg_off + 256;
+    size_t tab_off = cnt_off + 256;
+    size_t total = tab_off;
+    std::vector<size_t> tab_at(n, 0);
+    for (size_t i = 0; i < n; i++) {
+        if (!pl[i].table.empty()) {
+            tab_at[i] = total;
+            total += (pl[i].table.size() + 255) & ~(size_t)255;
+        }
+    }
+    drop_graph(b);
+    if (b->dev_bytes < total) {
+        if (b->dev) (void)hipFree(b->dev);
+        b->dev = nullptr;
+        b->dev_bytes = 0;
+        if (hipMalloc(&b->dev, total) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(OUT_OF_MEMORY, "OutOfMemory");
+        }
+        b->dev_bytes = total;
+    }
+    uint8_t *base = static_cast<uint8_t *>(b->dev);
+    std::vector<uint8_t> host(total, 0);
+    b->jobs.clear();
+    int tile = 0;
+    for (size_t k = 0; k < n; k++) {
+        size_t i = order[k];
+        BatchJob j = pl[i].job;
+        const int ntiles = j.tile0;  // plan_item leaves the job's tile count here
+        if (pl[i].table_kind == 1) {
+            const LinTap *t = reinterpret_cast<const LinTap *>(base + tab_at[i]);
+            j.rs.xt = t;
+            j.rs.yt = t + pl[i].nx;
+        } else if (pl[i].table_kind == 2) {
+            const uint8_t *p = base + tab_at[i];
+            j.rs.xe = reinterpret_cast<const AreaEntry *>(p);
+            j.rs.ye = j.rs.xe + pl[i].nx;
+            j.rs.xs = reinterpret_cast<const int32_t *>(j.rs.ye + pl[i].ny);
+            j.rs.ys = j.rs.xs + pl[i].nxs;
+        }
+        if (!pl[i].table.empty()) std::memcpy(host.data() + tab_at[i], pl[i].table.data(), pl[i].table.size());
+        j.tile0 = tile;
+        tile += ntiles;
+        b->jobs.push_back(j);
+    }
+    b->tiles = (size_t)tile;
+    // Segments: contiguous eighths of the tile list.
+    b->seg.assign(kXcds + 1, 0);
+    for (int q = 0; q <= kXcds; q++) b->seg[q] = (int)((long)tile * q / kXcds);
+    if (n) std::memcpy(host.data(), b->jobs.data(), n * sizeof(BatchJob));
+    std::memcpy(host.data() + seg_off, b->seg.data(), b->seg.size() * sizeof(int));
+    hipStream_t st = current_stream();
+    if (hipMemcpyAsync(b->dev, host.data(), total, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(INTERNAL, "CopyFailed");
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+    }
+    const int per_cu = tuning().batch_wg_per_cu ? tuning().batch_wg_per_cu : 4;
+    long wg = (long)cus * per_cu;
+    b->workgroups = (int)std::max(1L, std::min<long>(wg, (long)tile));
+    // A multiple of the XCD count keeps every XCD's home segment served.
+    if (b->workgroups > kXcds) b->workgroups -= b->workgroups % kXcds;
+    return VISAR_OK;
+}
+
+static int launch_batch(visar_batch *b, hipStream_t st) {
+    uint8_t *base = static_cast<uint8_t *>(b->dev);
+    size_t jobs_bytes = (b->jobs.size() ? b->jobs.size() : 1) * sizeof(BatchJob);
+    size_t seg_off = (jobs_bytes + 255) & ~(size_t)255;
+    int *seg = reinterpret_cast<int *>(base + seg_off);
+    int *counters = reinterpret_cast<int *>(base + seg_off + 256);
+    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, st, counters);
+    hipLaunchKernelGGL(batch_kernel, dim3((unsigned)b->workgroups), dim3(kTileThreads), 0, st,
+                       reinterpret_cast<const BatchJob *>(base), (int)b->jobs.size(), seg, counters);
+    return hipGetLastError() == hipSuccess ? OK : INTERNAL;
+}
+
+int visar_batch_run(visar_batch *b, int use_graph) {
+    if (!b) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (b->jobs.empty()) return VISAR_OK;
+    hipStream_t st = current_stream();
+    if (use_graph) {
+        if (!b->exec || b->graph_stream != st) {
+            drop_graph(b);
+            hipStream_t cap = nullptr;
+            if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) return fail(INTERNAL, "StreamFailed");
+            bool ok = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            ok = ok && launch_batch(b, cap) == OK;
+            ok = hipStreamEndCapture(cap, &b->graph) == hipSuccess && ok;
+            ok = ok && hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) == hipSuccess;
+            (void)hipStreamDestroy(cap);
+            if (!ok) {
+                drop_graph(b);
+                return fail(INTERNAL, "GraphCaptureFailed");
+            }
+            b->graph_stream = st;
+        }
+        if (hipGraphLaunch(b->exec, st) != hipSuccess) return fail(INTERNAL, "LaunchFailed");
+    } else if (launch_batch(b, st) != OK) {
+        return fail(INTERNAL, "LaunchFailed");
+    }
+    if (sync_mode() && hipStreamSynchronize(st) != hipSuccess) return fail(INTERNAL, "KernelFailed");
+    return VISAR_OK;
+}
+
+int visar_batch_info(const visar_batch *b, size_t *tiles, size_t *workgroups) {
+    if (!b) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (tiles) *tiles = b->tiles;
+    if (workgroups) *workgroups = (size_t)b->workgroups;
+    return VISAR_OK;
+}
+
+void visar_batch_free(visar_batch *b) {
+    if (!b) return;
+    drop_graph(b);
+    if (b->dev) (void)hipFree(b->dev);
+    delete b;
+}
+
+}  // extern "C"

@@ -7,9 +7,11 @@
 // workgroup stages the tile plus its halo in LDS with out-of-image cells set
 // to the identity (255/0 for u8), which ignores them exactly, then each
 // output reduces over the element's horizontal runs.
-#include "kernels.hpp"
+#include "tiles.hpp"
 
 namespace visar {
+
+int morph_small_args(const MorphParams &p, MorphSmallArgs &a);
 
 namespace {
 
@@ -84,141 +86,44 @@ __global__ __launch_bounds__(kThreads) void morph_direct(MorphParams p) {
     row_ptr<T>(d, y)[e] = acc;
 }
 
-// u8 fast path for elements up to 7 x 7 with 1-4 channels: a 64 x 4
-// workgroup stages (4 * TH + KH - 1) rows x (256 + halo) samples of the
-// source in LDS, out-of-image cells set to the identity (which ignores them
-// exactly, morph.zig:2-3, 257), then each thread reduces 4 consecutive
-// samples over the element's cells. The element's on-cells are a per-row
-// bit mask passed as a kernel argument, so the tap loop is fully unrolled
-// and its conditions are uniform across the wave.
-template <int KW, int KH, int CN, int TH, bool MAX>
-__global__ __launch_bounds__(256) void morph_small_u8(DImg s, DImg d, uint32_t rowmask0, uint32_t rowmask1, int ax,
-                                                       int ay) {
-    constexpr int HL = ((KW / 2) * CN + 3) / 4 * 4;  // bytes of halo on the left (>= ax * CN for centred anchors)
-    constexpr int HW = ((KW - 1) * CN + 3) / 4 + HL / 4;  // halo dwords in total
-    constexpr int PW = 64 + HW;                       // dwords per staged row
-    constexpr int TR = 4 * TH + KH - 1;
-    __shared__ uint32_t tile[TR][PW];
-    const int W = s.w, H = s.h, n = W * CN;
-    const int x0 = blockIdx.x * 256, y0 = blockIdx.y * 4 * TH;
-    const int tid = threadIdx.y * 64 + threadIdx.x;
-    const uint32_t id = MAX ? 0u : 0xffffffffu;
-    const bool aligned = ((s.stride | (uintptr_t)s.p) & 3) == 0;
-    for (int idx = tid; idx < TR * PW; idx += 256) {
-        const int r = idx / PW, k = idx - r * PW;
-        const int sy = y0 - ay + r;
-        const int c = x0 - HL + 4 * k;
-        uint32_t v = id;
-        if (sy >= 0 && sy < H) {
-            const uint8_t *row = s.p + (long)sy * s.stride;
-            if (aligned && c >= 0 && c + 4 <= n) {
-                v = *reinterpret_cast<const uint32_t *>(row + c);
-            } else {
-                v = 0;
-                for (int b = 0; b < 4; b++) {
-                    const int cc = c + b;
-                    const uint32_t byte = (cc >= 0 && cc < n) ? row[cc] : (MAX ? 0u : 255u);
-                    v |= byte << (8 * b);
-                }
-            }
-        }
-        tile[r][k] = v;
-    }
-    __syncthreads();
-    const int x = x0 + threadIdx.x * 4;
-    const int yt = threadIdx.y * TH;
-    if (x >= n || y0 + yt >= H) return;
-    // Columns of the element relative to the thread's first sample: tap i of
-    // output j reads byte HL + j + (i - ax) * CN of the thread's window.
-    uint32_t acc[TH][4];
-#pragma unroll
-    for (int t = 0; t < TH; t++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[t][j] = MAX ? 0u : 255u;
-#pragma unroll
-    for (int r = 0; r < TH + KH - 1; r++) {
-        constexpr int NWT = (HL + 4 + (KW - 1) * CN + 3) / 4 + 1;
-        uint32_t w[NWT];
-#pragma unroll
-        for (int k = 0; k < NWT; k++) w[k] = (threadIdx.x + k < PW) ? tile[yt + r][threadIdx.x + k] : 0u;
-#pragma unroll
-        for (int t = 0; t < TH; t++) {
-            const int ky = r - t;
-            if (ky < 0 || ky >= KH) continue;
-            const uint32_t bits = ky < 4 ? (rowmask0 >> (8 * ky)) & 255u : (rowmask1 >> (8 * (ky - 4))) & 255u;
-#pragma unroll
-            for (int i = 0; i < KW; i++) {
-                if (!((bits >> i) & 1u)) continue;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int o = HL + j + (i - ax) * CN;
-                    const uint32_t v = (w[o >> 2] >> ((o & 3) * 8)) & 255u;
-                    acc[t][j] = MAX ? max(acc[t][j], v) : min(acc[t][j], v);
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < TH; t++) {
-        const int y = y0 + yt + t;
-        if (y >= H) break;
-        uint8_t *out = d.p + (long)y * d.stride + x;
-        if (x + 4 <= n && (((uintptr_t)out | (uintptr_t)d.stride) & 3) == 0) {
-            *reinterpret_cast<uint32_t *>(out) = acc[t][0] | (acc[t][1] << 8) | (acc[t][2] << 16) | (acc[t][3] << 24);
-        } else {
-            for (int j = 0; j < 4 && x + j < n; j++) out[j] = (uint8_t)acc[t][j];
-        }
-    }
+template <int KW, int CN, int TH, bool MAX>
+__global__ __launch_bounds__(256) void morph_small_u8(MorphSmallArgs a) {
+    __shared__ uint32_t lds[MorphShape<KW, CN, TH>::LDS_WORDS];
+    morph_small_tile<KW, CN, TH, MAX>(a, blockIdx.x, blockIdx.y, lds);
 }
 
-template <int KW, int KH, int CN, bool MAX>
-void launch_morph_small(const MorphParams &p, uint32_t m0, uint32_t m1, hipStream_t st) {
-    const long n = (long)p.s.w * CN;
-    const int th = p.s.h >= 1024 ? 2 : 1;
-    dim3 block(64, 4);
-    if (th == 2) {
-        dim3 grid((unsigned)((n + 255) / 256), (unsigned)((p.s.h + 7) / 8));
-        hipLaunchKernelGGL((morph_small_u8<KW, KH, CN, 2, MAX>), grid, block, 0, st, p.s, p.d, m0, m1, p.ax, p.ay);
+template <int K, int CN, bool MAX>
+void launch_morph_small(const MorphSmallArgs &a, hipStream_t st) {
+    const long n = (long)a.src.w * CN;
+    if (a.src.h >= 1024) {
+        dim3 grid((unsigned)((n + 255) / 256), (unsigned)((a.src.h + 7) / 8));
+        hipLaunchKernelGGL((morph_small_u8<K, CN, 2, MAX>), grid, dim3(256), 0, st, a);
     } else {
-        dim3 grid((unsigned)((n + 255) / 256), (unsigned)((p.s.h + 3) / 4));
-        hipLaunchKernelGGL((morph_small_u8<KW, KH, CN, 1, MAX>), grid, block, 0, st, p.s, p.d, m0, m1, p.ax, p.ay);
+        dim3 grid((unsigned)((n + 255) / 256), (unsigned)((a.src.h + 3) / 4));
+        hipLaunchKernelGGL((morph_small_u8<K, CN, 1, MAX>), grid, dim3(256), 0, st, a);
     }
 }
 
 template <int K, bool MAX>
-bool morph_small_cn(const MorphParams &p, uint32_t m0, uint32_t m1, hipStream_t st) {
-    switch (p.s.cn) {
-    case 1: launch_morph_small<K, K, 1, MAX>(p, m0, m1, st); return true;
-    case 2: launch_morph_small<K, K, 2, MAX>(p, m0, m1, st); return true;
-    case 3: launch_morph_small<K, K, 3, MAX>(p, m0, m1, st); return true;
-    case 4: launch_morph_small<K, K, 4, MAX>(p, m0, m1, st); return true;
+bool morph_small_cn(const MorphSmallArgs &a, hipStream_t st) {
+    switch (a.src.cn) {
+    case 1: launch_morph_small<K, 1, MAX>(a, st); return true;
+    case 2: launch_morph_small<K, 2, MAX>(a, st); return true;
+    case 3: launch_morph_small<K, 3, MAX>(a, st); return true;
+    case 4: launch_morph_small<K, 4, MAX>(a, st); return true;
     default: return false;
     }
 }
 
-// Elements up to 7 x 7 run in a 3 x 3, 5 x 5 or 7 x 7 frame (a smaller
-// element is placed in the frame's centre rows/columns, anchors centred).
 template <bool MAX>
 bool try_morph_small(const MorphParams &p, hipStream_t st) {
-    if (p.depth != 0 || p.s.cn > 4 || p.kw > 7 || p.kh > 7) return false;
-    if (p.ax != p.kw / 2 || p.ay != p.kh / 2) return false;
-    const int K = (p.kw > p.kh ? p.kw : p.kh) | 1;
-    const int dx = K / 2 - p.ax, dy = K / 2 - p.ay;
-    uint32_t rows[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < p.nruns; k++) {
-        const MorphRun &r = p.runs[k];
-        for (int i = 0; i < r.len; i++) rows[r.ky + dy] |= 1u << (r.x0 + i + dx);
-    }
-    const uint32_t m0 = rows[0] | rows[1] << 8 | rows[2] << 16 | rows[3] << 24;
-    const uint32_t m1 = rows[4] | rows[5] << 8 | rows[6] << 16 | rows[7] << 24;
-    MorphParams q = p;
-    q.ax = K / 2;
-    q.ay = K / 2;
-    switch (K) {
-    case 1:
-    case 3: return morph_small_cn<3, MAX>(q, m0, m1, st);
-    case 5: return morph_small_cn<5, MAX>(q, m0, m1, st);
-    case 7: return morph_small_cn<7, MAX>(q, m0, m1, st);
+    MorphSmallArgs a;
+    int k = morph_small_args(p, a);
+    if (!k) return false;
+    switch (k) {
+    case 3: return morph_small_cn<3, MAX>(a, st);
+    case 5: return morph_small_cn<5, MAX>(a, st);
+    case 7: return morph_small_cn<7, MAX>(a, st);
     default: return false;
     }
 }
@@ -251,6 +156,27 @@ int run_morph_op(const MorphParams &p, hipStream_t st) {
 }
 
 }  // namespace
+
+// Elements up to 7 x 7 on u8 images with 1-4 channels run in a 3 x 3, 5 x 5
+// or 7 x 7 frame (the element placed so that its centred anchor is the
+// frame's centre). Returns the frame size, or 0 when the fast path does not apply.
+int morph_small_args(const MorphParams &p, MorphSmallArgs &a) {
+    if (p.depth != 0 || p.s.cn > 4 || p.kw > 7 || p.kh > 7) return 0;
+    if (p.ax != p.kw / 2 || p.ay != p.kh / 2) return 0;
+    int K = (p.kw > p.kh ? p.kw : p.kh) | 1;
+    if (K == 1) K = 3;  // a single cell runs in the 3 x 3 frame
+    const int dx = K / 2 - p.ax, dy = K / 2 - p.ay;
+    uint32_t rows[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < p.nruns; k++) {
+        const MorphRun &r = p.runs[k];
+        for (int i = 0; i < r.len; i++) rows[r.ky + dy] |= 1u << (r.x0 + i + dx);
+    }
+    a.src = p.s;
+    a.dst = p.d;
+    a.mask0 = rows[0] | rows[1] << 8 | rows[2] << 16 | rows[3] << 24;
+    a.mask1 = rows[4] | rows[5] << 8 | rows[6] << 16 | rows[7] << 24;
+    return K;
+}
 
 int launch_morph_pass(const MorphParams &p, hipStream_t st) {
     switch (p.depth) {

@@ -5,7 +5,7 @@
 // Each kernel maps one thread to a run of consecutive samples of one row so
 // loads and stores are coalesced; the 16-pixel u8 paths use 128-bit loads
 // when the row segment is 16-byte aligned.
-#include "kernels.hpp"
+#include "tiles.hpp"
 
 namespace visar {
 
@@ -13,7 +13,6 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__device__ inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
@@ -240,69 +239,12 @@ __global__ __launch_bounds__(kThreads) void background_flat(const uint8_t *f, ui
 
 // -------------------------------------------------------------- resize
 
-// resize_u8.zig:381-447: (a + b + c + d + 2) >> 2 per channel.
-__global__ __launch_bounds__(kThreads) void half_u8(DImg s, DImg d, long n_per_row) {
-    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
-    long y = gid / n_per_row, e = gid - y * n_per_row;
-    if (y >= d.h) return;
-    const int cn = d.cn;
-    long x = e / cn, ch = e - x * cn;
-    long i = 2 * x * cn + ch;
-    const uint8_t *a = row_ptr<const uint8_t>(s, 2 * y), *b = row_ptr<const uint8_t>(s, 2 * y + 1);
-    uint32_t v = (uint32_t)a[i] + a[i + cn] + b[i] + b[i + cn];
-    row_ptr<uint8_t>(d, y)[e] = (uint8_t)((v + 2) >> 2);
-}
-
-// Single-channel exact half, 16 output pixels (32 input bytes of each of
-// the two source rows) per thread; grid x over 16-pixel groups, y over rows.
-__global__ __launch_bounds__(kThreads) void half_u8_c1(DImg s, DImg d) {
-    const int y = blockIdx.y;
-    const int x = (blockIdx.x * kThreads + threadIdx.x) * 16;
-    if (x >= d.w) return;
-    const uint8_t *a = row_ptr<const uint8_t>(s, 2 * y) + 2 * x, *b = row_ptr<const uint8_t>(s, 2 * y + 1) + 2 * x;
-    uint8_t *o = row_ptr<uint8_t>(d, y) + x;
-    if (x + 16 <= d.w && aligned16(a) && aligned16(b) && aligned16(o)) {
-        uint4 a0 = reinterpret_cast<const uint4 *>(a)[0], a1 = reinterpret_cast<const uint4 *>(a)[1];
-        uint4 b0 = reinterpret_cast<const uint4 *>(b)[0], b1 = reinterpret_cast<const uint4 *>(b)[1];
-        uint32_t aw[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-        uint32_t bw[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-        uint32_t out[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            uint32_t v = 0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                int px = q * 4 + k;  // output pixel px reads input bytes 2px, 2px+1
-                uint32_t wa = aw[px / 2], wb = bw[px / 2];
-                int sh = (px & 1) * 16;
-                uint32_t sum = ((wa >> sh) & 255) + ((wa >> (sh + 8)) & 255) + ((wb >> sh) & 255) + ((wb >> (sh + 8)) & 255);
-                v |= ((sum + 2) >> 2) << (8 * k);
-            }
-            out[q] = v;
-        }
-        *reinterpret_cast<uint4 *>(o) = make_uint4(out[0], out[1], out[2], out[3]);
-        return;
-    }
-    int n = min(16, d.w - x);
-    for (int i = 0; i < n; i++) {
-        uint32_t v = (uint32_t)a[2 * i] + a[2 * i + 1] + b[2 * i] + b[2 * i + 1];
-        o[i] = (uint8_t)((v + 2) >> 2);
-    }
-}
-
-// resize_u8.zig:1213-1277: block sum times the f32 reciprocal, satF32.
-__global__ __launch_bounds__(kThreads) void area_int_u8(DImg s, DImg d, int ix, int iy, float scale, long n_per_row) {
-    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
-    long y = gid / n_per_row, e = gid - y * n_per_row;
-    if (y >= d.h) return;
-    const int cn = d.cn;
-    long x = e / cn, ch = e - x * cn;
-    uint32_t sum = 0;
-    for (int dy = 0; dy < iy; dy++) {
-        const uint8_t *r = row_ptr<const uint8_t>(s, y * iy + dy);
-        for (int dx = 0; dx < ix; dx++) sum += r[(x * ix + dx) * cn + ch];
-    }
-    row_ptr<uint8_t>(d, y)[e] = sat_f32<uint8_t>((float)sum * scale);
+// Resize tiles (tiles.hpp): 256 output samples x 4 rows per workgroup, the
+// one-channel exact half 1024 x 4.
+__global__ __launch_bounds__(kThreads) void half_u8(DImg s, DImg d) { half_tile(s, d, blockIdx.x, blockIdx.y); }
+__global__ __launch_bounds__(kThreads) void half_u8_c1(DImg s, DImg d) { half_c1_tile(s, d, blockIdx.x, blockIdx.y); }
+__global__ __launch_bounds__(kThreads) void area_int_u8(DImg s, DImg d, int ix, int iy, float scale) {
+    area_int_tile(s, d, ix, iy, scale, blockIdx.x, blockIdx.y);
 }
 
 // resize.zig:62-106 nearest: xofs = min(floor(x * sw/dw), sw - 1) in f64.
@@ -319,43 +261,13 @@ __global__ __launch_bounds__(kThreads) void nearest_k(DImg s, DImg d, int elem, 
     for (long i = 0; i < (long)d.cn * elem; i++) dp[i] = sp[i];
 }
 
-// resize_u8.zig:651-711: H = (s[o0]*w0 + s[o1]*w1) >> 4 as i16 per source
-// row, out = sat_u8((((H0*b0) >> 16) + ((H1*b1) >> 16) + 2) >> 2).
-__global__ __launch_bounds__(kThreads) void linear_u8(DImg s, DImg d, const LinTap *xt, const LinTap *yt, long n_per_row) {
-    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
-    long y = gid / n_per_row, e = gid - y * n_per_row;
-    if (y >= d.h) return;
-    const int cn = d.cn;
-    long x = e / cn, ch = e - x * cn;
-    LinTap tx = xt[x], ty = yt[y];
-    const uint8_t *r0 = row_ptr<const uint8_t>(s, ty.o0), *r1 = row_ptr<const uint8_t>(s, ty.o1);
-    int32_t h0 = (int16_t)(((int32_t)r0[tx.o0 + ch] * tx.w0 + (int32_t)r0[tx.o1 + ch] * tx.w1) >> 4);
-    int32_t h1 = (int16_t)(((int32_t)r1[tx.o0 + ch] * tx.w0 + (int32_t)r1[tx.o1 + ch] * tx.w1) >> 4);
-    int32_t t = ((h0 * ty.w0) >> 16) + ((h1 * ty.w1) >> 16);
-    int32_t o = (t + 2) >> 2;
-    row_ptr<uint8_t>(d, y)[e] = (uint8_t)(o < 0 ? 0 : o > 255 ? 255 : o);
+__global__ __launch_bounds__(kThreads) void linear_u8(DImg s, DImg d, const LinTap *xt, const LinTap *yt) {
+    linear_tile(s, d, xt, yt, blockIdx.x, blockIdx.y);
 }
 
-// resize.zig:527-557 (areaGeneric, bit-exact with resize_u8.zig:1058-1208):
-// per source row of the cell, the f32 sum of s*alpha over the cell's
-// columns in table order, then the sum of those times beta, no FMA.
 __global__ __launch_bounds__(kThreads) void area_frac_u8(DImg s, DImg d, const AreaEntry *xt, const int32_t *xs,
-                                                         const AreaEntry *yt, const int32_t *ys, long n_per_row) {
-    long gid = (long)blockIdx.x * kThreads + threadIdx.x;
-    long y = gid / n_per_row, e = gid - y * n_per_row;
-    if (y >= d.h) return;
-    const int cn = d.cn;
-    long x = e / cn, ch = e - x * cn;
-    int xa = xs[x], xb = xs[x + 1];
-    float sum = 0.0f;
-    for (int k = ys[y]; k < ys[y + 1]; k++) {
-        AreaEntry ye = yt[k];
-        const uint8_t *r = row_ptr<const uint8_t>(s, ye.si);
-        float h = 0.0f;
-        for (int q = xa; q < xb; q++) h = h + (float)r[(long)xt[q].si * cn + ch] * xt[q].alpha;
-        sum = sum + h * ye.alpha;
-    }
-    row_ptr<uint8_t>(d, y)[e] = sat_cast<uint8_t>((double)sum);
+                                                         const AreaEntry *yt, const int32_t *ys) {
+    area_frac_tile(s, d, xt, xs, yt, ys, blockIdx.x, blockIdx.y);
 }
 
 // ----------------------------------------------------------- morph helpers
@@ -452,21 +364,21 @@ int launch_copy(const DImg &s, const DImg &d, size_t elem, hipStream_t st) {
                                                                                                                : INTERNAL;
 }
 
+inline dim3 tiles256(const DImg &d) { return dim3((unsigned)(((long)d.w * d.cn + 255) / 256), (unsigned)((d.h + 3) / 4)); }
+
 int launch_half_u8(const DImg &s, const DImg &d, hipStream_t st) {
     if (d.cn == 1) {
-        dim3 grid((unsigned)((d.w + 16 * kThreads - 1) / (16 * kThreads)), (unsigned)d.h);
-        hipLaunchKernelGGL(half_u8_c1, grid, dim3(kThreads), 0, st, s, d);
+        hipLaunchKernelGGL(half_u8_c1, dim3((unsigned)((d.w + 1023) / 1024), (unsigned)((d.h + 3) / 4)), dim3(kThreads), 0, st,
+                           s, d);
         return launched();
     }
-    long npr = (long)d.w * d.cn;
-    hipLaunchKernelGGL(half_u8, dim3(blocks_for(npr * d.h)), dim3(kThreads), 0, st, s, d, npr);
+    hipLaunchKernelGGL(half_u8, tiles256(d), dim3(kThreads), 0, st, s, d);
     return launched();
 }
 
 int launch_area_int_u8(const DImg &s, const DImg &d, int ix, int iy, hipStream_t st) {
-    long npr = (long)d.w * d.cn;
     float scale = 1.0f / (float)(ix * iy);
-    hipLaunchKernelGGL(area_int_u8, dim3(blocks_for(npr * d.h)), dim3(kThreads), 0, st, s, d, ix, iy, scale, npr);
+    hipLaunchKernelGGL(area_int_u8, tiles256(d), dim3(kThreads), 0, st, s, d, ix, iy, scale);
     return launched();
 }
 
@@ -478,15 +390,13 @@ int launch_nearest(const DImg &s, const DImg &d, size_t elem, hipStream_t st) {
 }
 
 int launch_linear_u8(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, hipStream_t st) {
-    long npr = (long)d.w * d.cn;
-    hipLaunchKernelGGL(linear_u8, dim3(blocks_for(npr * d.h)), dim3(kThreads), 0, st, s, d, xt, yt, npr);
+    hipLaunchKernelGGL(linear_u8, tiles256(d), dim3(kThreads), 0, st, s, d, xt, yt);
     return launched();
 }
 
 int launch_area_frac_u8(const DImg &s, const DImg &d, const AreaEntry *xt, const int32_t *xstart, const AreaEntry *yt,
                         const int32_t *ystart, hipStream_t st) {
-    long npr = (long)d.w * d.cn;
-    hipLaunchKernelGGL(area_frac_u8, dim3(blocks_for(npr * d.h)), dim3(kThreads), 0, st, s, d, xt, xstart, yt, ystart, npr);
+    hipLaunchKernelGGL(area_frac_u8, tiles256(d), dim3(kThreads), 0, st, s, d, xt, xstart, yt, ystart);
     return launched();
 }
 

@@ -8,185 +8,13 @@
 // One thread computes one output pixel (all channels): stores are
 // coalesced along the row, the four taps are gathers that hit L2/MALL
 // because neighbouring threads read neighbouring source pixels.
-#include "kernels.hpp"
+#include "tiles.hpp"
 
 namespace visar {
 
 namespace {
 
 constexpr int kThreads = 256;
-
-template <typename T>
-__device__ inline float ld_f(const DImg &s, long y, long i) {
-    return (float)row_ptr<const T>(s, y)[i];
-}
-
-// geometry.zig:504-506 roundTo
-template <typename T>
-__device__ inline T round_to(float v) {
-    return sat_cast<T>((double)v);
-}
-
-// std.math.clamp(v, lo, hi) = @max(lo, @min(v, hi)), NaN-ignoring.
-__device__ inline double clampd(double v, double lo, double hi) { return max_num(lo, min_num(v, hi)); }
-
-// geometry.zig:492-494 toIdx
-__device__ inline long to_idx(float f) { return (long)min_num(max_num(f, -1.0e9f), 1.0e9f); }
-
-// One output pixel of warpAffine (all channels into o[0..cn)).
-template <typename T>
-__device__ inline void affine_pixel(const AffineParams &p, int x, float bx, float by, T *o) {
-    const DImg &s = p.s;
-    const int cn = s.cn;
-    const long W = s.w, H = s.h;
-    float xf = (float)x;
-    float sx = fmaf(p.m[0], xf, bx);
-    float sy = fmaf(p.m[3], xf, by);
-    if (p.interp == 0) {
-        // geometry.zig:496-502 nearestPixel
-        long ix = to_idx(rint_magic(min_num(max_num(sx, -4.0e6f), 4.0e6f)));
-        long iy = to_idx(rint_magic(min_num(max_num(sy, -4.0e6f), 4.0e6f)));
-        long mx = border_map(ix, W, p.border), my = border_map(iy, H, p.border);
-        if (mx >= 0 && my >= 0) {
-            const T *sp = row_ptr<const T>(s, my) + mx * cn;
-            for (int c = 0; c < cn; c++) o[c] = sp[c];
-        } else {
-            for (int c = 0; c < cn; c++) o[c] = (T)p.cvd;
-        }
-        return;
-    }
-    // geometry.zig:508-533 linearPixel (the 16-pixel fast block computes the same values)
-    float fx = floorf(sx), fy = floorf(sy);
-    float ax = sx - fx, ay = sy - fy;
-    long ix = to_idx(fx), iy = to_idx(fy);
-    if (p.border == BORDER_CONSTANT && (ix < -1 || iy < -1 || ix >= W || iy >= H)) {
-        for (int c = 0; c < cn; c++) o[c] = (T)p.cvd;
-        return;
-    }
-    long x0, x1, y0, y1;
-    if (ix >= 0 && ix + 1 < W && iy >= 0 && iy + 1 < H) {
-        x0 = ix, x1 = ix + 1, y0 = iy, y1 = iy + 1;
-    } else {
-        x0 = border_map(ix, W, p.border), x1 = border_map(ix + 1, W, p.border);
-        y0 = border_map(iy, H, p.border), y1 = border_map(iy + 1, H, p.border);
-    }
-    for (int c = 0; c < cn; c++) {
-        float p00 = (x0 >= 0 && y0 >= 0) ? ld_f<T>(s, y0, x0 * cn + c) : p.cv;
-        float p01 = (x1 >= 0 && y0 >= 0) ? ld_f<T>(s, y0, x1 * cn + c) : p.cv;
-        float p10 = (x0 >= 0 && y1 >= 0) ? ld_f<T>(s, y1, x0 * cn + c) : p.cv;
-        float p11 = (x1 >= 0 && y1 >= 0) ? ld_f<T>(s, y1, x1 * cn + c) : p.cv;
-        float v0 = fmaf(ax, p01 - p00, p00);
-        float v1 = fmaf(ax, p11 - p10, p10);
-        o[c] = round_to<T>(fmaf(ay, v1 - v0, v0));
-    }
-}
-
-// Bilinear tap addresses of one pixel for u8 one-channel images: border-
-// mapped tap columns/rows (-1 = constant border) and load addresses clamped
-// into the image, so the four byte loads can be issued unconditionally and
-// batched across the thread's pixels (they are gathers, latency-bound).
-struct Taps {
-    int x0, x1, y0, y1;  // mapped coordinates, -1 for a constant-border tap
-    bool out;            // constant border and the whole footprint is outside
-};
-
-__device__ inline Taps map_taps(long ix, long iy, int W, int H, int border) {
-    Taps t;
-    t.out = border == BORDER_CONSTANT && (ix < -1 || iy < -1 || ix >= W || iy >= H);
-    if (ix >= 0 && ix + 1 < W && iy >= 0 && iy + 1 < H) {
-        t.x0 = (int)ix, t.x1 = (int)ix + 1, t.y0 = (int)iy, t.y1 = (int)iy + 1;
-    } else {
-        t.x0 = border_map((int)ix, W, border), t.x1 = border_map((int)ix + 1, W, border);
-        t.y0 = border_map((int)iy, H, border), t.y1 = border_map((int)iy + 1, H, border);
-    }
-    return t;
-}
-
-// Round half to even and clamp to u8 for a blend of u8 samples: such a value
-// lies in [0, 255] (up to rounding), where the 1.5*2^23 trick in f32 equals
-// sat.cast(u8, v) (geometry.zig:563-566 uses the same trick).
-__device__ inline uint32_t round_u8(float v) {
-    float r = rint_magic(v);
-    r = min_num(max_num(r, 0.0f), 255.0f);
-    return (uint32_t)r;
-}
-
-struct Quad {
-    float p00, p01, p10, p11;
-};
-
-__device__ inline Quad load_quad_u8(const DImg &s, const Taps &t, float cv) {
-    const uint8_t *r0 = s.p + (long)max(t.y0, 0) * s.stride, *r1 = s.p + (long)max(t.y1, 0) * s.stride;
-    const int xa = max(t.x0, 0), xb = max(t.x1, 0);
-    uint32_t a = r0[xa], b = r0[xb], c = r1[xa], d = r1[xb];
-    Quad q;
-    q.p00 = (t.x0 >= 0 && t.y0 >= 0) ? (float)a : cv;
-    q.p01 = (t.x1 >= 0 && t.y0 >= 0) ? (float)b : cv;
-    q.p10 = (t.x0 >= 0 && t.y1 >= 0) ? (float)c : cv;
-    q.p11 = (t.x1 >= 0 && t.y1 >= 0) ? (float)d : cv;
-    return q;
-}
-
-// Four consecutive bilinear u8 pixels of warpAffine (geometry.zig:508-533).
-__device__ inline uint32_t affine4_u8(const AffineParams &p, int x, float bx, float by) {
-    const int W = p.s.w, H = p.s.h;
-    Taps t[4];
-    float ax[4], ay[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const float xf = (float)(x + j);
-        const float sx = fmaf(p.m[0], xf, bx), sy = fmaf(p.m[3], xf, by);
-        const float fx = floorf(sx), fy = floorf(sy);
-        ax[j] = sx - fx;
-        ay[j] = sy - fy;
-        t[j] = map_taps(to_idx(fx), to_idx(fy), W, H, p.border);
-    }
-    Quad q[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) q[j] = load_quad_u8(p.s, t[j], p.cv);
-    uint32_t out = 0;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const float v0 = fmaf(ax[j], q[j].p01 - q[j].p00, q[j].p00);
-        const float v1 = fmaf(ax[j], q[j].p11 - q[j].p10, q[j].p10);
-        uint32_t v = round_u8(fmaf(ay[j], v1 - v0, v0));
-        if (t[j].out) v = (uint32_t)p.cvd;
-        out |= v << (8 * j);
-    }
-    return out;
-}
-
-// Any depth and channel count: one output pixel per thread, blocks of 64 x 4.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void warp_affine_k(AffineParams p) {
-    const DImg &d = p.d;
-    const int y = blockIdx.y * 4 + threadIdx.y;
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    if (y >= d.h || x >= d.w) return;
-    float yf = (float)y;
-    float bx = p.m[1] * yf + p.m[2];
-    float by = p.m[4] * yf + p.m[5];
-    affine_pixel<T>(p, x, bx, by, row_ptr<T>(d, y) + (long)x * d.cn);
-}
-
-// One-channel u8 bilinear warpAffine, 4 pixels per thread; a partial group
-// at the row end computes all four and stores the ones inside the row.
-__global__ __launch_bounds__(kThreads) void warp_affine_lin4_u8(AffineParams p) {
-    const DImg &d = p.d;
-    const int y = blockIdx.y * 4 + threadIdx.y;
-    const int x = (blockIdx.x * 64 + threadIdx.x) * 4;
-    if (y >= d.h || x >= d.w) return;
-    const float yf = (float)y;
-    const float bx = p.m[1] * yf + p.m[2];
-    const float by = p.m[4] * yf + p.m[5];
-    const uint32_t v = affine4_u8(p, x, bx, by);
-    uint8_t *o = row_ptr<uint8_t>(d, y) + x;
-    if (x + 4 <= d.w && ((uintptr_t)o & 3) == 0) {
-        *reinterpret_cast<uint32_t *>(o) = v;
-    } else {
-        for (int j = 0; j < 4 && x + j < d.w; j++) o[j] = (uint8_t)(v >> (8 * j));
-    }
-}
 
 // One output pixel of warpPerspective (geometry.zig:615-672).
 template <typename T>
@@ -236,7 +64,7 @@ __device__ inline void persp_pixel(const PerspParams &p, int x, int y, T *o) {
         acc += ((x1 >= 0 && y0 >= 0) ? ld_f<T>(s, y0, x1 * cn + c) : cv) * w1;
         acc += ((x0 >= 0 && y1 >= 0) ? ld_f<T>(s, y1, x0 * cn + c) : cv) * w2;
         acc += ((x1 >= 0 && y1 >= 0) ? ld_f<T>(s, y1, x1 * cn + c) : cv) * w3;
-        o[c] = round_to<T>(acc);
+        o[c] = sat_cast<T>((double)acc);
     }
 }
 
@@ -284,16 +112,16 @@ __device__ inline uint32_t persp4_u8(const PerspParams &p, int x, int y) {
 template <typename T>
 __global__ __launch_bounds__(kThreads) void warp_persp_k(PerspParams p) {
     const DImg &d = p.d;
-    const int y = blockIdx.y * 4 + threadIdx.y;
-    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + tile_ty();
+    const int x = blockIdx.x * 64 + tile_tx();
     if (y >= d.h || x >= d.w) return;
     persp_pixel<T>(p, x, y, row_ptr<T>(d, y) + (long)x * d.cn);
 }
 
 __global__ __launch_bounds__(kThreads) void warp_persp_lin4_u8(PerspParams p) {
     const DImg &d = p.d;
-    const int y = blockIdx.y * 4 + threadIdx.y;
-    const int x = (blockIdx.x * 64 + threadIdx.x) * 4;
+    const int y = blockIdx.y * 4 + tile_ty();
+    const int x = (blockIdx.x * 64 + tile_tx()) * 4;
     if (y >= d.h || x >= d.w) return;
     const uint32_t v = persp4_u8(p, x, y);
     uint8_t *o = row_ptr<uint8_t>(d, y) + x;
@@ -337,6 +165,13 @@ __global__ __launch_bounds__(kThreads) void remap_k(DImg s, DImg d, DImg mxi, DI
     }
 }
 
+__global__ __launch_bounds__(kThreads) void warp_affine_lin4_u8(AffineParams p) { affine_lin4_tile(p, blockIdx.x, blockIdx.y); }
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void warp_affine_k(AffineParams p) {
+    affine_px_tile<T>(p, blockIdx.x, blockIdx.y);
+}
+
 inline unsigned blocks_for(long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 inline int launched() { return hipGetLastError() == hipSuccess ? OK : INTERNAL; }
 
@@ -345,7 +180,7 @@ inline dim3 grid2d(int w, int h, int px) { return dim3((unsigned)((w + 64 * px -
 }  // namespace
 
 int launch_warp_affine(const AffineParams &p, hipStream_t st) {
-    const dim3 b(64, 4);
+    const dim3 b(kTileThreads);
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1) {
         hipLaunchKernelGGL(warp_affine_lin4_u8, grid2d(p.d.w, p.d.h, 4), b, 0, st, p);
         return launched();
@@ -362,7 +197,7 @@ int launch_warp_affine(const AffineParams &p, hipStream_t st) {
 }
 
 int launch_warp_perspective(const PerspParams &p, hipStream_t st) {
-    const dim3 b(64, 4);
+    const dim3 b(kTileThreads);
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1) {
         hipLaunchKernelGGL(warp_persp_lin4_u8, grid2d(p.d.w, p.d.h, 4), b, 0, st, p);
         return launched();

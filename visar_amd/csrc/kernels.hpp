@@ -7,7 +7,9 @@ namespace visar {
 
 // Launch-shape knobs (see visar_gpu_set_tuning); 0 means automatic.
 struct Tuning {
-    int stencil_rows = 0;
+    int stencil_rows = 0;        // plain stencils: output rows per thread (1, 2, 4)
+    int batch_stencil_rows = 0;  // batched stencils: output rows per thread (1, 2)
+    int batch_wg_per_cu = 0;     // persistent workgroups per compute unit
 };
 Tuning &tuning();
 bool set_tuning(const char *key, int value);
@@ -46,6 +48,8 @@ int launch_sep(const SepParams &p, hipStream_t st);
 // number of output rows per thread (1, 2 or 4).
 bool launch_sep_small_u8(const SepParams &p, hipStream_t st, int th);
 int stencil_rows_per_thread(long n, int h);
+struct StencilArgs;
+bool stencil_args(const SepParams &p, StencilArgs &a);
 
 // -------------------------------------------------------------- pointwise
 
@@ -129,6 +133,8 @@ struct MorphParams {
     MorphRun runs[kMaxRuns];
 };
 int launch_morph_pass(const MorphParams &p, hipStream_t st);
+struct MorphSmallArgs;
+int morph_small_args(const MorphParams &p, MorphSmallArgs &a);
 // d = saturate(a - b) (morph.zig:574-593)
 int launch_sub_sat(const DImg &a, const DImg &b, const DImg &d, int depth, hipStream_t st);
 

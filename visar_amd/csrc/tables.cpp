@@ -11,20 +11,35 @@ Tuning &tuning() {
     return t;
 }
 
+namespace {
+struct Knob {
+    const char *key;
+    int Tuning::*field;
+    int lo, hi;
+};
+const Knob kKnobs[] = {
+    {"stencil_rows", &Tuning::stencil_rows, 0, 4},
+    {"batch_stencil_rows", &Tuning::batch_stencil_rows, 0, 2},
+    {"batch_wg_per_cu", &Tuning::batch_wg_per_cu, 0, 32},
+};
+}  // namespace
+
 bool set_tuning(const char *key, int value) {
-    if (std::strcmp(key, "stencil_rows") == 0) {
-        if (value != 0 && value != 1 && value != 2 && value != 4) return false;
-        tuning().stencil_rows = value;
-        return true;
-    }
+    for (const Knob &k : kKnobs)
+        if (std::strcmp(key, k.key) == 0) {
+            if (value < k.lo || value > k.hi || (k.field == &Tuning::stencil_rows && value == 3)) return false;
+            tuning().*(k.field) = value;
+            return true;
+        }
     return false;
 }
 
 bool get_tuning(const char *key, int *value) {
-    if (std::strcmp(key, "stencil_rows") == 0) {
-        *value = tuning().stencil_rows;
-        return true;
-    }
+    for (const Knob &k : kKnobs)
+        if (std::strcmp(key, k.key) == 0) {
+            *value = tuning().*(k.field);
+            return true;
+        }
     return false;
 }
 

@@ -3,13 +3,16 @@
 // A plan turns each item into a job: a tile function of tiles.hpp (the same
 // code the plain entry points launch, so results are identical), the job's
 // tile grid and its arguments. Jobs are grouped by kind so that concurrently
-// running workgroups mostly execute the same code. The global tile list is
-// cut into 8 contiguous segments, one per XCD: workgroup w is dispatched to
-// XCD w % 8 and first drains that XCD's segment (neighbouring tiles of an
-// image, and their halo rows, stay in one L2), then steals tiles from the
-// other segments until all are done. Every queue is a counter bumped with
-// atomicAdd; a small kernel clears the counters before each run (no
-// hipMemsetAsync, so the run can be captured in a HIP graph).
+// running workgroups mostly execute the same code. Each job's tiles are cut
+// into work units of about the same number of bytes (a run of consecutive
+// tiles of one job), and the unit list is cut into 8 contiguous segments of
+// equal bytes, one per XCD: workgroup w is dispatched to XCD w % 8 and first
+// drains that XCD's segment (neighbouring tiles of an image, and their halo
+// rows, stay in one L2), then steals units from the other segments until all
+// are done. Every queue is a counter bumped with atomicAdd; a workgroup claims
+// its next unit before it works through the current one, so the atomic's
+// latency hides behind the work. A small kernel clears the counters before
+// each run (no hipMemsetAsync, so the run can be captured in a HIP graph).
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -50,7 +53,7 @@ struct BatchJob {
     int kind;
     int variant;  // template selector (kernel size, channels, rows per thread, min/max)
     int tiles_x;
-    int tile0;
+    int ntiles;
     union {
         StencilArgs st;
         MorphSmallArgs mo;
@@ -108,9 +111,12 @@ __device__ inline void run_morph(const BatchJob &j, int bx, int by, uint32_t *ld
     }
 }
 
-__device__ inline void run_tile(const BatchJob &j, int t, uint32_t *lds) {
-    const int lt = t - j.tile0;
-    const int by = lt / j.tiles_x, bx = lt - by * j.tiles_x;
+// A run of consecutive tiles [first, first + count) of job `job`.
+struct BatchUnit {
+    int job, first, count, pad;
+};
+
+__device__ inline void run_tile(const BatchJob &j, int bx, int by, uint32_t *lds) {
     switch (j.kind) {
     case JK_STENCIL: run_stencil(j, bx, by, lds); break;
     case JK_MORPH: run_morph(j, bx, by, lds); break;
@@ -124,34 +130,42 @@ __device__ inline void run_tile(const BatchJob &j, int t, uint32_t *lds) {
     }
 }
 
-// Counters: [0, kXcds) next tile of each segment. Segment q is the tile
+// Counters: [0, kXcds) next unit of each segment. Segment q is the unit
 // range [seg[q], seg[q + 1]).
 __global__ void batch_reset(int *counters) {
     if (threadIdx.x < kXcds) counters[threadIdx.x] = 0;
 }
 
-__global__ __launch_bounds__(kTileThreads) void batch_kernel(const BatchJob *jobs, int njobs, const int *seg,
-                                                             int *counters) {
+__global__ __launch_bounds__(kTileThreads) void batch_kernel(const BatchJob *jobs, const BatchUnit *units,
+                                                             const int *seg, int *counters) {
     __shared__ uint32_t lds[kBatchLds];
     __shared__ int next;
     const int home = blockIdx.x % kXcds;
     for (int k = 0; k < kXcds; k++) {
         const int q = (home + k) % kXcds;
         const int base = seg[q], size = seg[q + 1] - base;
-        for (;;) {
-            if (threadIdx.x == 0) next = atomicAdd(&counters[q], 1);
-            __syncthreads();
-            const int i = __builtin_amdgcn_readfirstlane(next);
-            __syncthreads();
-            if (i >= size) break;
-            const int t = base + i;
-            int lo = 0, hi = njobs - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (jobs[mid].tile0 <= t) lo = mid;
-                else hi = mid - 1;
+        if (size <= 0) continue;
+        if (threadIdx.x == 0) next = atomicAdd(&counters[q], 1);
+        __syncthreads();
+        int i = __builtin_amdgcn_readfirstlane(next);
+        __syncthreads();
+        while (i < size) {
+            int claim = 0;
+            if (threadIdx.x == 0) claim = atomicAdd(&counters[q], 1);  // the next unit, claimed early
+            const BatchUnit u = units[base + i];
+            const BatchJob &j = jobs[u.job];
+            int by = u.first / j.tiles_x, bx = u.first - by * j.tiles_x;
+            for (int c = 0; c < u.count; c++) {
+                run_tile(j, bx, by, lds);
+                __syncthreads();
+                if (++bx == j.tiles_x) {
+                    bx = 0;
+                    by++;
+                }
             }
-            run_tile(jobs[lo], t, lds);
+            if (threadIdx.x == 0) next = claim;
+            __syncthreads();
+            i = __builtin_amdgcn_readfirstlane(next);
             __syncthreads();
         }
     }
@@ -165,10 +179,12 @@ using namespace visar;
 
 struct visar_batch {
     std::vector<BatchJob> jobs;
+    std::vector<BatchUnit> units;
     std::vector<int> seg;
     size_t tiles = 0;
     int workgroups = 0;
-    void *dev = nullptr;  // jobs | segments | counters | tables
+    void *dev = nullptr;  // jobs | units | segments | counters | tables
+    size_t units_off = 0, seg_off = 0;
     size_t dev_bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -237,7 +253,7 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
         th = th >= 2 ? 2 : 1;
         j.variant = k * 100 + (int)it.src.channels * 10 + th;
         j.tiles_x = (int)(((long)s.w * s.cn + 255) / 256);
-        j.tile0 = tiles256(s, 4 * th);
+        j.ntiles = tiles256(s, 4 * th);
         return OK;
     }
     case VISAR_BATCH_MORPHOLOGY: {
@@ -272,7 +288,7 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
         j.kind = JK_MORPH;
         j.variant = kf * 100 + (int)it.src.channels * 10 + it.morph_op;
         j.tiles_x = (int)(((long)s.w * s.cn + 255) / 256);
-        j.tile0 = tiles256(s, 8);
+        j.ntiles = tiles256(s, 8);
         return OK;
     }
     case VISAR_BATCH_WARP_AFFINE: {
@@ -295,7 +311,7 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
             j.kind = JK_AFFINE_PX;
             j.tiles_x = (d.w + 63) / 64;
         }
-        j.tile0 = j.tiles_x * ((d.h + 3) / 4);
+        j.ntiles = j.tiles_x * ((d.h + 3) / 4);
         return OK;
     }
     case VISAR_BATCH_RESIZE: {
@@ -314,11 +330,11 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
                 j.kind = JK_HALF;
                 j.tiles_x = (d.w * d.cn + 255) / 256;
             }
-            j.tile0 = j.tiles_x * ((d.h + 3) / 4);
+            j.ntiles = j.tiles_x * ((d.h + 3) / 4);
             return OK;
         }
         j.tiles_x = (d.w * d.cn + 255) / 256;
-        j.tile0 = j.tiles_x * ((d.h + 3) / 4);
+        j.ntiles = j.tiles_x * ((d.h + 3) / 4);
         if (it.interpolation == 1 || area_up) {
             j.kind = JK_LINEAR;
             std::vector<LinTap> t(it.dst.width + it.dst.height);
@@ -399,17 +415,64 @@ int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, si
         if (pl[a].job.kind != pl[c].job.kind) return pl[a].job.kind < pl[c].job.kind;
         return pl[a].job.variant < pl[c].job.variant;
     });
-    // Device arena: jobs, segment bounds, counters, tables.
-    size_t jobs_bytes = (n ? n : 1) * sizeof(BatchJob);
-    size_t seg_off = (jobs_bytes + 255) & ~(size_t)255;
-    size_t cnt_off = seg_off + 256;
-    size_t tab_off = cnt_off + 256;
-    size_t total = tab_off;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+    }
+    const int per_cu = tuning().batch_wg_per_cu ? tuning().batch_wg_per_cu : 4;
+    const long wg_max = (long)cus * per_cu;
+    // Work units: runs of consecutive tiles of one job, each about `target`
+    // bytes of input + output, small enough that every workgroup gets many
+    // (the tail balances) and large enough that a claim's atomic is rare.
+    double total_bytes = 0;
+    std::vector<double> job_bytes(n);
+    for (size_t k = 0; k < n; k++) {
+        const visar_batch_item &it = items[order[k]];
+        job_bytes[k] = (double)it.src.width * it.src.height * it.src.channels +
+                       (double)it.dst.width * it.dst.height * it.dst.channels;
+        total_bytes += job_bytes[k];
+    }
+    const double target = std::min(256.0 * 1024, std::max(16.0 * 1024, total_bytes / ((double)wg_max * 16)));
+    b->units.clear();
+    std::vector<double> unit_bytes;
+    long tiles = 0;
+    for (size_t k = 0; k < n; k++) {
+        const int nt = pl[order[k]].job.ntiles;
+        const double tb = job_bytes[k] / std::max(1, nt);
+        const int per = (int)std::max(1.0, std::min((double)nt, std::floor(target / std::max(tb, 1.0))));
+        for (int f = 0; f < nt; f += per) {
+            const int c = std::min(per, nt - f);
+            b->units.push_back(BatchUnit{(int)k, f, c, 0});
+            unit_bytes.push_back(tb * c);
+        }
+        tiles += nt;
+    }
+    b->tiles = (size_t)tiles;
+    const size_t nu = b->units.size();
+    // Segments: contiguous runs of units with equal bytes.
+    b->seg.assign(kXcds + 1, (int)nu);
+    b->seg[0] = 0;
+    {
+        double acc = 0;
+        int q = 1;
+        for (size_t u = 0; u < nu && q < kXcds; u++) {
+            acc += unit_bytes[u];
+            while (q < kXcds && acc >= total_bytes * q / kXcds) b->seg[q++] = (int)u + 1;
+        }
+    }
+    // Device arena: jobs | units | segment bounds | counters | tables.
+    auto up256 = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t jobs_bytes = up256((n ? n : 1) * sizeof(BatchJob));
+    const size_t units_off = jobs_bytes;
+    const size_t seg_off = units_off + up256((nu ? nu : 1) * sizeof(BatchUnit));
+    const size_t cnt_off = seg_off + 256;
+    size_t total = cnt_off + 256;
     std::vector<size_t> tab_at(n, 0);
     for (size_t i = 0; i < n; i++) {
         if (!pl[i].table.empty()) {
             tab_at[i] = total;
-            total += (pl[i].table.size() + 255) & ~(size_t)255;
+            total += up256(pl[i].table.size());
         }
     }
     drop_graph(b);
@@ -419,18 +482,20 @@ int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, si
         b->dev_bytes = 0;
         if (hipMalloc(&b->dev, total) != hipSuccess) {
             (void)hipGetLastError();
+            b->jobs.clear();
+            b->units.clear();
             return fail(OUT_OF_MEMORY, "OutOfMemory");
         }
         b->dev_bytes = total;
     }
+    b->units_off = units_off;
+    b->seg_off = seg_off;
     uint8_t *base = static_cast<uint8_t *>(b->dev);
     std::vector<uint8_t> host(total, 0);
     b->jobs.clear();
-    int tile = 0;
     for (size_t k = 0; k < n; k++) {
         size_t i = order[k];
         BatchJob j = pl[i].job;
-        const int ntiles = j.tile0;  // plan_item leaves the job's tile count here
         if (pl[i].table_kind == 1) {
             const LinTap *t = reinterpret_cast<const LinTap *>(base + tab_at[i]);
             j.rs.xt = t;
@@ -443,28 +508,16 @@ int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, si
             j.rs.ys = j.rs.xs + pl[i].nxs;
         }
         if (!pl[i].table.empty()) std::memcpy(host.data() + tab_at[i], pl[i].table.data(), pl[i].table.size());
-        j.tile0 = tile;
-        tile += ntiles;
         b->jobs.push_back(j);
     }
-    b->tiles = (size_t)tile;
-    // Segments: contiguous eighths of the tile list.
-    b->seg.assign(kXcds + 1, 0);
-    for (int q = 0; q <= kXcds; q++) b->seg[q] = (int)((long)tile * q / kXcds);
     if (n) std::memcpy(host.data(), b->jobs.data(), n * sizeof(BatchJob));
+    if (nu) std::memcpy(host.data() + units_off, b->units.data(), nu * sizeof(BatchUnit));
     std::memcpy(host.data() + seg_off, b->seg.data(), b->seg.size() * sizeof(int));
     hipStream_t st = current_stream();
     if (hipMemcpyAsync(b->dev, host.data(), total, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return fail(INTERNAL, "CopyFailed");
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-    }
-    const int per_cu = tuning().batch_wg_per_cu ? tuning().batch_wg_per_cu : 4;
-    long wg = (long)cus * per_cu;
-    b->workgroups = (int)std::max(1L, std::min<long>(wg, (long)tile));
+    b->workgroups = (int)std::max(1L, std::min<long>(wg_max, (long)nu));
     // A multiple of the XCD count keeps every XCD's home segment served.
     if (b->workgroups > kXcds) b->workgroups -= b->workgroups % kXcds;
     return VISAR_OK;
@@ -472,19 +525,18 @@ int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, si
 
 static int launch_batch(visar_batch *b, hipStream_t st) {
     uint8_t *base = static_cast<uint8_t *>(b->dev);
-    size_t jobs_bytes = (b->jobs.size() ? b->jobs.size() : 1) * sizeof(BatchJob);
-    size_t seg_off = (jobs_bytes + 255) & ~(size_t)255;
-    int *seg = reinterpret_cast<int *>(base + seg_off);
-    int *counters = reinterpret_cast<int *>(base + seg_off + 256);
+    int *seg = reinterpret_cast<int *>(base + b->seg_off);
+    int *counters = reinterpret_cast<int *>(base + b->seg_off + 256);
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, st, counters);
     hipLaunchKernelGGL(batch_kernel, dim3((unsigned)b->workgroups), dim3(kTileThreads), 0, st,
-                       reinterpret_cast<const BatchJob *>(base), (int)b->jobs.size(), seg, counters);
+                       reinterpret_cast<const BatchJob *>(base),
+                       reinterpret_cast<const BatchUnit *>(base + b->units_off), seg, counters);
     return hipGetLastError() == hipSuccess ? OK : INTERNAL;
 }
 
 int visar_batch_run(visar_batch *b, int use_graph) {
     if (!b) return fail(INVALID_ARGUMENT, "InvalidArgument");
-    if (b->jobs.empty()) return VISAR_OK;
+    if (b->units.empty()) return VISAR_OK;
     hipStream_t st = current_stream();
     if (use_graph) {
         if (!b->exec || b->graph_stream != st) {

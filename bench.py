@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 4])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy (scales the HBM byte counters of a profile)")
     return ap.parse_args()
 
 
@@ -152,6 +154,15 @@ def main():
     va.set_stream(stream)
     va.set_sync(False)
 
+    if args.copy_calibration:
+        n = 256 << 20
+        a = torch.empty(n, dtype=torch.uint8, device=dev).fill_(1)
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n, "launches": 4}))
+        return
     if args.config == 4:
         return bench_batch(args, torch, va)
 
@@ -238,7 +249,9 @@ def main():
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get(dom)
+        # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+        # passes (scaled by a plain copy's reading), keyed by kernel.
+        traffic = json.load(open(tf)).get("traffic_bytes_per_launch", {}).get(dom)
 
     cpu = None
     if not args.no_cpu:

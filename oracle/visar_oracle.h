@@ -9,6 +9,16 @@
  * results are bit-exact with VISAR (plyght/VISAR 0.1.0, Zig). The image
  * descriptor has the layout of visar_image (include/visar.h:34-41 in the
  * reference) so the same ctypes structure drives both.
+ *
+ * Pinning. The reference cannot be built here (Zig), so each function is
+ * checked by tests/test_oracle_golden.py against the reference's own known
+ * answers, closed forms, brute force or invariants. Parity partially
+ * unpinned: the exact f32 operation order of orc_warp_affine,
+ * orc_warp_perspective, orc_remap, the fractional AREA path of orc_resize,
+ * the generic LINEAR taps and orc_sep_filter_2d follows the cited Zig but has
+ * no reference vectors (the reference checks these only against cv2 or not
+ * at all); the invariants catch dropped terms, sign and index errors but not
+ * FMA-versus-separate rounding.
  */
 #ifndef VISAR_ORACLE_H
 #define VISAR_ORACLE_H

@@ -128,6 +128,41 @@ def test_gaussian_strided_views(va, orc):
                 assert_same(va.gaussian_blur(dv, (k, k), s).cpu().numpy(), want, f"device view k{k}")
 
 
+def test_stencil_and_morph_misaligned_rows(va, orc):
+    """Sources and destinations at every byte offset mod 4 with odd row
+    strides, wide enough for several 256-byte tiles: every output byte equals
+    the oracle and the bytes around the destination view stay untouched."""
+    for cn in (1, 3):
+        for (h, w) in [(19, 347), (6, 601), (33, 97)]:
+            a = natural(h, w, cn, h + w + cn)
+            for off in range(4):
+                for pad in (0, 1, 5):
+                    pitch = w * cn + pad
+                    sbig = torch.zeros(h * pitch + 8, dtype=torch.uint8, device="cuda")
+                    sv = sbig[off:off + h * pitch].view(h, pitch)[:, :w * cn]
+                    sv.copy_(dev(a.reshape(h, w * cn)))
+                    sv = sv.view(h, w, cn) if cn == 3 else sv
+                    dbig = torch.full((h * pitch + 8,), 77, dtype=torch.uint8, device="cuda")
+                    dv = dbig[(off + 1) % 4:(off + 1) % 4 + h * pitch].view(h, pitch)[:, :w * cn]
+                    dv = dv.view(h, w, cn) if cn == 3 else dv
+                    keep = torch.ones_like(dbig, dtype=torch.bool)
+                    o = (off + 1) % 4
+                    for r in range(h):
+                        keep[o + r * pitch:o + r * pitch + w * cn] = False
+                    for k in (3, 5, 7):
+                        want = orc.gaussian_blur(a, (k, k), 0, border=4)
+                        dbig.fill_(77)
+                        va.gaussian_blur(sv, (k, k), 0, border=4, dst=dv)
+                        assert_same(dv.cpu().numpy(), want, f"gauss k{k} cn{cn} {h}x{w} off{off} pad{pad}")
+                        assert bool((dbig[keep] == 77).all()), f"gauss k{k} wrote outside its view"
+                    for op, shape, k in [(1, 0, 5), (0, 2, 7), (1, 1, 3)]:
+                        want = orc.morphology(a, op, shape, (k, k))
+                        dbig.fill_(77)
+                        va.morphology(sv, op, shape, (k, k), dst=dv)
+                        assert_same(dv.cpu().numpy(), want, f"morph op{op} k{k} cn{cn} off{off} pad{pad}")
+                        assert bool((dbig[keep] == 77).all()), f"morph op{op} wrote outside its view"
+
+
 # --------------------------------------------------------------- Sobel
 
 @pytest.mark.parametrize("dx,dy,ks", [(1, 0, 3), (0, 1, 3), (1, 1, 3), (1, 0, 5), (2, 0, 5), (1, 0, -1), (0, 1, 1),

@@ -2,17 +2,20 @@
 //
 // A plan turns each item into a job: a tile function of tiles.hpp (the same
 // code the plain entry points launch, so results are identical), the job's
-// tile grid and its arguments. Jobs are grouped by kind so that concurrently
-// running workgroups mostly execute the same code. Each job's tiles are cut
-// into work units of about the same number of bytes (a run of consecutive
-// tiles of one job), and the unit list is cut into 8 contiguous segments of
-// equal bytes, one per XCD: workgroup w is dispatched to XCD w % 8 and first
-// drains that XCD's segment (neighbouring tiles of an image, and their halo
-// rows, stay in one L2), then steals units from the other segments until all
-// are done. Every queue is a counter bumped with atomicAdd; a workgroup claims
-// its next unit before it works through the current one, so the atomic's
-// latency hides behind the work. A small kernel clears the counters before
-// each run (no hipMemsetAsync, so the run can be captured in a HIP graph).
+// tile grid and its arguments. Jobs are grouped by tile variant (kind,
+// kernel size, channels, rows per thread); each group gets its own
+// persistent kernel, specialised for that one variant so its registers and
+// code are only what the variant needs, and the groups' kernels run back to
+// back in one launch (one HIP graph). Each job's tiles are cut into work
+// units of about the same number of bytes (a run of consecutive tiles of one
+// job), and a group's unit list is cut into 8 contiguous segments of equal
+// bytes, one per XCD: workgroup w is dispatched to XCD w % 8 and first drains
+// that XCD's segment (neighbouring tiles of an image, and their halo rows,
+// stay in one L2), then steals units from the other segments until all are
+// done. Every queue is a counter bumped with atomicAdd; a workgroup claims its
+// next unit before it works through the current one, so the atomic's latency
+// hides behind the work. A small kernel clears the counters before each run
+// (no hipMemsetAsync, so the run can be captured in a HIP graph).
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -62,114 +65,121 @@ struct BatchJob {
     };
 };
 
-// Variants compiled into the persistent kernel.
-constexpr int stencil_variant(int k, int cn, int th) { return k * 100 + cn * 10 + th; }
-constexpr int morph_variant(int k, int cn, int mx) { return k * 100 + cn * 10 + mx; }
-
-// The largest LDS use among the compiled variants.
-constexpr int cmax(int a, int b) { return a > b ? a : b; }
-constexpr int kStencilLds =
-    cmax(StencilShape<7, 3, 2>::LDS_WORDS, cmax(StencilShape<7, 1, 2>::LDS_WORDS, StencilShape<5, 3, 2>::LDS_WORDS));
-constexpr int kMorphLds = cmax(MorphShape<7, 3, 2>::LDS_WORDS, MorphShape<7, 1, 2>::LDS_WORDS);
-constexpr int kBatchLds = cmax(kStencilLds, kMorphLds);
-
-template <int K, int CN>
-__device__ inline void run_stencil_th(const StencilArgs &a, int th, int bx, int by, uint32_t *lds) {
-    if (th == 2) stencil_tile<K, CN, 2, uint8_t>(a, bx, by, lds);
-    else stencil_tile<K, CN, 1, uint8_t>(a, bx, by, lds);
-}
-
-__device__ inline void run_stencil(const BatchJob &j, int bx, int by, uint32_t *lds) {
-    const int th = j.variant % 10, cn = (j.variant / 10) % 10, k = j.variant / 100;
-    if (cn == 1) {
-        if (k == 3) run_stencil_th<3, 1>(j.st, th, bx, by, lds);
-        else if (k == 5) run_stencil_th<5, 1>(j.st, th, bx, by, lds);
-        else run_stencil_th<7, 1>(j.st, th, bx, by, lds);
-    } else {
-        if (k == 3) run_stencil_th<3, 3>(j.st, th, bx, by, lds);
-        else if (k == 5) run_stencil_th<5, 3>(j.st, th, bx, by, lds);
-        else run_stencil_th<7, 3>(j.st, th, bx, by, lds);
-    }
-}
-
-template <int K, int CN>
-__device__ inline void run_morph_mx(const MorphSmallArgs &a, int mx, int bx, int by, uint32_t *lds) {
-    if (mx) morph_small_tile<K, CN, 2, true>(a, bx, by, lds);
-    else morph_small_tile<K, CN, 2, false>(a, bx, by, lds);
-}
-
-__device__ inline void run_morph(const BatchJob &j, int bx, int by, uint32_t *lds) {
-    const int mx = j.variant % 10, cn = (j.variant / 10) % 10, k = j.variant / 100;
-    if (cn == 1) {
-        if (k == 3) run_morph_mx<3, 1>(j.mo, mx, bx, by, lds);
-        else if (k == 5) run_morph_mx<5, 1>(j.mo, mx, bx, by, lds);
-        else run_morph_mx<7, 1>(j.mo, mx, bx, by, lds);
-    } else {
-        if (k == 3) run_morph_mx<3, 3>(j.mo, mx, bx, by, lds);
-        else if (k == 5) run_morph_mx<5, 3>(j.mo, mx, bx, by, lds);
-        else run_morph_mx<7, 3>(j.mo, mx, bx, by, lds);
-    }
-}
-
 // A run of consecutive tiles [first, first + count) of job `job`.
 struct BatchUnit {
     int job, first, count, pad;
 };
 
-__device__ inline void run_tile(const BatchJob &j, int bx, int by, uint32_t *lds) {
-    switch (j.kind) {
-    case JK_STENCIL: run_stencil(j, bx, by, lds); break;
-    case JK_MORPH: run_morph(j, bx, by, lds); break;
-    case JK_AFFINE_LIN4: affine_lin4_tile(j.af, bx, by); break;
-    case JK_AFFINE_PX: affine_px_tile<uint8_t>(j.af, bx, by); break;
-    case JK_HALF_C1: half_c1_tile(j.rs.s, j.rs.d, bx, by); break;
-    case JK_HALF: half_tile(j.rs.s, j.rs.d, bx, by); break;
-    case JK_LINEAR: linear_tile(j.rs.s, j.rs.d, j.rs.xt, j.rs.yt, bx, by); break;
-    case JK_AREA_INT: area_int_tile(j.rs.s, j.rs.d, j.rs.ix, j.rs.iy, j.rs.scale, bx, by); break;
-    default: area_frac_tile(j.rs.s, j.rs.d, j.rs.xe, j.rs.xs, j.rs.ye, j.rs.ys, bx, by); break;
-    }
-}
-
-// Counters: [0, kXcds) next unit of each segment. Segment q is the unit
-// range [seg[q], seg[q + 1]).
-__global__ void batch_reset(int *counters) {
-    if (threadIdx.x < kXcds) counters[threadIdx.x] = 0;
-}
-
-__global__ __launch_bounds__(kTileThreads) void batch_kernel(const BatchJob *jobs, const BatchUnit *units,
-                                                             const int *seg, int *counters) {
-    __shared__ uint32_t lds[kBatchLds];
-    __shared__ int next;
+// Drains one group's queues: seg[0..8] are the group's 8 segment bounds in
+// the unit list, counters[0..7] its queue counters.
+template <typename Tile>
+__device__ __attribute__((always_inline)) inline void drain(const BatchJob *__restrict__ jobs, const BatchUnit *__restrict__ units,
+                                                            const int *__restrict__ seg, int *counters, int *next, Tile tile) {
     const int home = blockIdx.x % kXcds;
     for (int k = 0; k < kXcds; k++) {
         const int q = (home + k) % kXcds;
         const int base = seg[q], size = seg[q + 1] - base;
         if (size <= 0) continue;
-        if (threadIdx.x == 0) next = atomicAdd(&counters[q], 1);
+        if (threadIdx.x == 0) *next = atomicAdd(&counters[q], 1);
         __syncthreads();
-        int i = __builtin_amdgcn_readfirstlane(next);
+        int i = __builtin_amdgcn_readfirstlane(*next);
         __syncthreads();
         while (i < size) {
             int claim = 0;
             if (threadIdx.x == 0) claim = atomicAdd(&counters[q], 1);  // the next unit, claimed early
             const BatchUnit u = units[base + i];
             const BatchJob &j = jobs[u.job];
-            int by = u.first / j.tiles_x, bx = u.first - by * j.tiles_x;
+            const int tiles_x = j.tiles_x;
+            int by = u.first / tiles_x, bx = u.first - by * tiles_x;
             for (int c = 0; c < u.count; c++) {
-                run_tile(j, bx, by, lds);
+                tile(j, bx, by);
                 __syncthreads();
-                if (++bx == j.tiles_x) {
+                if (++bx == tiles_x) {
                     bx = 0;
                     by++;
                 }
             }
-            if (threadIdx.x == 0) next = claim;
+            if (threadIdx.x == 0) *next = claim;
             __syncthreads();
-            i = __builtin_amdgcn_readfirstlane(next);
+            i = __builtin_amdgcn_readfirstlane(*next);
             __syncthreads();
         }
     }
 }
+
+#define BATCH_ARGS const BatchJob *__restrict__ jobs, const BatchUnit *__restrict__ units, const int *__restrict__ seg, int *counters
+
+template <int K, int CN, int TH>
+__global__ __launch_bounds__(kTileThreads) void batch_stencil(BATCH_ARGS) {
+    __shared__ uint32_t lds[StencilShape<K, CN, TH>::LDS_WORDS];
+    __shared__ int next;
+    drain(jobs, units, seg, counters, &next,
+          [&](const BatchJob &j, int bx, int by) { stencil_tile<K, CN, TH, uint8_t>(j.st, bx, by, lds); });
+}
+
+template <int K, int CN, bool MAX>
+__global__ __launch_bounds__(kTileThreads) void batch_morph(BATCH_ARGS) {
+    __shared__ uint32_t lds[MorphShape<K, CN, 2>::LDS_WORDS];
+    __shared__ int next;
+    drain(jobs, units, seg, counters, &next,
+          [&](const BatchJob &j, int bx, int by) { morph_small_tile<K, CN, 2, MAX>(j.mo, bx, by, lds); });
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kTileThreads) void batch_other(BATCH_ARGS) {
+    __shared__ int next;
+    drain(jobs, units, seg, counters, &next, [&](const BatchJob &j, int bx, int by) {
+        if constexpr (KIND == JK_AFFINE_LIN4) affine_lin4_tile(j.af, bx, by);
+        else if constexpr (KIND == JK_AFFINE_PX) affine_px_tile<uint8_t>(j.af, bx, by);
+        else if constexpr (KIND == JK_HALF_C1) half_c1_tile(j.rs.s, j.rs.d, bx, by);
+        else if constexpr (KIND == JK_HALF) half_tile(j.rs.s, j.rs.d, bx, by);
+        else if constexpr (KIND == JK_LINEAR) linear_tile(j.rs.s, j.rs.d, j.rs.xt, j.rs.yt, bx, by);
+        else if constexpr (KIND == JK_AREA_INT) area_int_tile(j.rs.s, j.rs.d, j.rs.ix, j.rs.iy, j.rs.scale, bx, by);
+        else area_frac_tile(j.rs.s, j.rs.d, j.rs.xe, j.rs.xs, j.rs.ye, j.rs.ys, bx, by);
+    });
+}
+
+__global__ void batch_reset(int *counters, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) counters[i] = 0;
+}
+
+using GroupKernel = void (*)(BATCH_ARGS);
+
+template <int K, int CN>
+GroupKernel stencil_kernel(int th) {
+    return th == 2 ? batch_stencil<K, CN, 2> : batch_stencil<K, CN, 1>;
+}
+
+template <int K, int CN>
+GroupKernel morph_kernel(int mx) {
+    return mx ? batch_morph<K, CN, true> : batch_morph<K, CN, false>;
+}
+
+// The kernel of a (kind, variant) group; variant = k * 100 + cn * 10 + th
+// (stencils) or + max (morphology).
+GroupKernel group_kernel(int kind, int variant) {
+    const int k = variant / 100, cn = (variant / 10) % 10, last = variant % 10;
+    switch (kind) {
+    case JK_STENCIL:
+        if (cn == 1) return k == 3 ? stencil_kernel<3, 1>(last) : k == 5 ? stencil_kernel<5, 1>(last) : stencil_kernel<7, 1>(last);
+        return k == 3 ? stencil_kernel<3, 3>(last) : k == 5 ? stencil_kernel<5, 3>(last) : stencil_kernel<7, 3>(last);
+    case JK_MORPH:
+        if (cn == 1) return k == 3 ? morph_kernel<3, 1>(last) : k == 5 ? morph_kernel<5, 1>(last) : morph_kernel<7, 1>(last);
+        return k == 3 ? morph_kernel<3, 3>(last) : k == 5 ? morph_kernel<5, 3>(last) : morph_kernel<7, 3>(last);
+    case JK_AFFINE_LIN4: return batch_other<JK_AFFINE_LIN4>;
+    case JK_AFFINE_PX: return batch_other<JK_AFFINE_PX>;
+    case JK_HALF_C1: return batch_other<JK_HALF_C1>;
+    case JK_HALF: return batch_other<JK_HALF>;
+    case JK_LINEAR: return batch_other<JK_LINEAR>;
+    case JK_AREA_INT: return batch_other<JK_AREA_INT>;
+    default: return batch_other<JK_AREA_FRAC>;
+    }
+}
+
+struct Group {
+    int kind, variant;
+    int workgroups;
+};
 
 }  // namespace
 
@@ -180,11 +190,12 @@ using namespace visar;
 struct visar_batch {
     std::vector<BatchJob> jobs;
     std::vector<BatchUnit> units;
-    std::vector<int> seg;
+    std::vector<Group> groups;
+    std::vector<int> seg;  // 9 bounds per group
     size_t tiles = 0;
     int workgroups = 0;
     void *dev = nullptr;  // jobs | units | segments | counters | tables
-    size_t units_off = 0, seg_off = 0;
+    size_t units_off = 0, seg_off = 0, cnt_off = 0;
     size_t dev_bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -450,24 +461,47 @@ int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, si
     }
     b->tiles = (size_t)tiles;
     const size_t nu = b->units.size();
-    // Segments: contiguous runs of units with equal bytes.
-    b->seg.assign(kXcds + 1, (int)nu);
-    b->seg[0] = 0;
+    // Groups: runs of jobs with one (kind, variant); each gets 8 segments of
+    // equal bytes over its units, and a launch width that fits its units.
+    b->groups.clear();
+    b->seg.clear();
+    b->workgroups = 0;
     {
-        double acc = 0;
-        int q = 1;
-        for (size_t u = 0; u < nu && q < kXcds; u++) {
-            acc += unit_bytes[u];
-            while (q < kXcds && acc >= total_bytes * q / kXcds) b->seg[q++] = (int)u + 1;
+        size_t u = 0;
+        while (u < nu) {
+            const BatchJob &j0 = pl[order[b->units[u].job]].job;
+            size_t e = u;
+            double gbytes = 0;
+            while (e < nu) {
+                const BatchJob &je = pl[order[b->units[e].job]].job;
+                if (je.kind != j0.kind || je.variant != j0.variant) break;
+                gbytes += unit_bytes[e];
+                e++;
+            }
+            std::vector<int> sg(kXcds + 1, (int)e);
+            sg[0] = (int)u;
+            double acc = 0;
+            int q = 1;
+            for (size_t x = u; x < e && q < kXcds; x++) {
+                acc += unit_bytes[x];
+                while (q < kXcds && acc >= gbytes * q / kXcds) sg[q++] = (int)x + 1;
+            }
+            b->seg.insert(b->seg.end(), sg.begin(), sg.end());
+            int wg = (int)std::max(1L, std::min<long>(wg_max, (long)(e - u)));
+            if (wg > kXcds) wg -= wg % kXcds;  // every XCD's home segment is served
+            b->groups.push_back(Group{j0.kind, j0.variant, wg});
+            b->workgroups += wg;
+            u = e;
         }
     }
+    const size_t ng = b->groups.size();
     // Device arena: jobs | units | segment bounds | counters | tables.
     auto up256 = [](size_t v) { return (v + 255) & ~(size_t)255; };
     const size_t jobs_bytes = up256((n ? n : 1) * sizeof(BatchJob));
     const size_t units_off = jobs_bytes;
     const size_t seg_off = units_off + up256((nu ? nu : 1) * sizeof(BatchUnit));
-    const size_t cnt_off = seg_off + 256;
-    size_t total = cnt_off + 256;
+    const size_t cnt_off = seg_off + up256((ng ? ng : 1) * (kXcds + 1) * sizeof(int));
+    size_t total = cnt_off + up256((ng ? ng : 1) * kXcds * sizeof(int));
     std::vector<size_t> tab_at(n, 0);
     for (size_t i = 0; i < n; i++) {
         if (!pl[i].table.empty()) {
@@ -484,12 +518,14 @@ int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, si
             (void)hipGetLastError();
             b->jobs.clear();
             b->units.clear();
+            b->groups.clear();
             return fail(OUT_OF_MEMORY, "OutOfMemory");
         }
         b->dev_bytes = total;
     }
     b->units_off = units_off;
     b->seg_off = seg_off;
+    b->cnt_off = cnt_off;
     uint8_t *base = static_cast<uint8_t *>(b->dev);
     std::vector<uint8_t> host(total, 0);
     b->jobs.clear();
@@ -517,20 +553,22 @@ int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, si
     if (hipMemcpyAsync(b->dev, host.data(), total, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return fail(INTERNAL, "CopyFailed");
-    b->workgroups = (int)std::max(1L, std::min<long>(wg_max, (long)nu));
-    // A multiple of the XCD count keeps every XCD's home segment served.
-    if (b->workgroups > kXcds) b->workgroups -= b->workgroups % kXcds;
     return VISAR_OK;
 }
 
 static int launch_batch(visar_batch *b, hipStream_t st) {
     uint8_t *base = static_cast<uint8_t *>(b->dev);
-    int *seg = reinterpret_cast<int *>(base + b->seg_off);
-    int *counters = reinterpret_cast<int *>(base + b->seg_off + 256);
-    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, st, counters);
-    hipLaunchKernelGGL(batch_kernel, dim3((unsigned)b->workgroups), dim3(kTileThreads), 0, st,
-                       reinterpret_cast<const BatchJob *>(base),
-                       reinterpret_cast<const BatchUnit *>(base + b->units_off), seg, counters);
+    const int *seg = reinterpret_cast<const int *>(base + b->seg_off);
+    int *counters = reinterpret_cast<int *>(base + b->cnt_off);
+    const int ng = (int)b->groups.size();
+    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(256), 0, st, counters, ng * kXcds);
+    const BatchJob *jobs = reinterpret_cast<const BatchJob *>(base);
+    const BatchUnit *units = reinterpret_cast<const BatchUnit *>(base + b->units_off);
+    for (int g = 0; g < ng; g++) {
+        const Group &gr = b->groups[g];
+        hipLaunchKernelGGL(group_kernel(gr.kind, gr.variant), dim3((unsigned)gr.workgroups), dim3(kTileThreads), 0, st,
+                           jobs, units, seg + g * (kXcds + 1), counters + g * kXcds);
+    }
     return hipGetLastError() == hipSuccess ? OK : INTERNAL;
 }
 

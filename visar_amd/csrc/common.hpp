@@ -44,7 +44,7 @@ __host__ __device__ inline T *row_ptr(const DImg &im, long y) {
 // +-1e9 (the warps clamp them there first). The reference reflects
 // repeatedly; the results are periodic (period 2n for reflect, 2n - 2 for
 // reflect-101), which the closed forms below use.
-__host__ __device__ __attribute__((noinline)) inline int border_map_outside(int p, int n, int border) {
+__host__ __device__ inline int border_map_far(int p, int n, int border) {
     switch (border) {
     case BORDER_CONSTANT: return -1;
     case BORDER_REPLICATE: return p < 0 ? 0 : n - 1;
@@ -67,10 +67,52 @@ __host__ __device__ __attribute__((noinline)) inline int border_map_outside(int 
     }
 }
 
-__host__ __device__ inline int border_map(int p, int n, int border) {
-    if ((unsigned)p < (unsigned)n) return p;
-    return border_map_outside(p, n, border);
+__host__ __device__ __attribute__((noinline)) inline int border_map_outside(int p, int n, int border) {
+    return border_map_far(p, n, border);
 }
+
+// One reflection covers everything a stencil halo or a warp footprint
+// reaches on images at least as wide as the halo: p in [-(n - 1), 2n - 2].
+template <bool FAR_INLINE>
+__host__ __device__ inline int border_map_t(int p, int n, int border) {
+    if ((unsigned)p < (unsigned)n) return p;
+    if (border == BORDER_CONSTANT) return -1;
+    if (border == BORDER_REPLICATE) return p < 0 ? 0 : n - 1;
+    if (p >= -(n - 1) && p <= 2 * n - 2) {
+        if (border == BORDER_REFLECT101) return p < 0 ? -p : 2 * n - 2 - p;
+        if (border == BORDER_REFLECT) return p < 0 ? -p - 1 : 2 * n - 1 - p;
+        if (border == BORDER_WRAP) return p < 0 ? p + n : p - n;
+    }
+    if constexpr (FAR_INLINE) return border_map_far(p, n, border);
+    else return border_map_outside(p, n, border);
+}
+
+// Generic kernels: the far case is an out-of-line call (smaller code).
+__host__ __device__ inline int border_map(int p, int n, int border) { return border_map_t<false>(p, n, border); }
+
+// Stencil and morphology halos (p within a few pixels of the image, or far
+// out only on images narrower than the halo): no call, no division, small
+// code. Reflects repeatedly as border.zig does; the loop runs once per image
+// width the coordinate lies outside.
+__host__ __device__ inline int border_map_halo(int p, int n, int border) {
+    if ((unsigned)p < (unsigned)n) return p;
+    if (border == BORDER_CONSTANT) return -1;
+    if (border == BORDER_REPLICATE) return p < 0 ? 0 : n - 1;
+    if (border == BORDER_WRAP) {
+        while (p < 0) p += n;
+        while (p >= n) p -= n;
+        return p;
+    }
+    if (n == 1) return 0;
+    const int d = border == BORDER_REFLECT101 ? 1 : 0;
+    while (p < 0 || p >= n) p = p < 0 ? -p - 1 + d : 2 * n - 1 - d - p;
+    return p;
+}
+
+// Tile code: no call at all. A call site makes the compiler drain every
+// outstanding load of the wave before it, which would serialise a tile's
+// loads even when no lane takes the call.
+__host__ __device__ inline int border_map_inl(int p, int n, int border) { return border_map_t<true>(p, n, border); }
 
 // saturate.zig:8-18: round half to even with the 1.5*2^k magic constant.
 __host__ __device__ inline float rint_magic(float x) { return (x + 12582912.0f) - 12582912.0f; }

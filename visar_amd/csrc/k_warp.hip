@@ -32,7 +32,7 @@ __device__ inline void persp_pixel(const PerspParams &p, int x, int y, T *o) {
     if (p.interp == 0) {
         long ix = (long)round(clampd(sx, -1e9, 1e9));
         long iy = (long)round(clampd(sy, -1e9, 1e9));
-        long mx = border_map(ix, W, p.border), my = border_map(iy, H, p.border);
+        long mx = border_map_inl(ix, W, p.border), my = border_map_inl(iy, H, p.border);
         if (mx >= 0 && my >= 0) {
             const T *sp = row_ptr<const T>(s, my) + mx * cn;
             for (int c = 0; c < cn; c++) o[c] = sp[c];
@@ -53,8 +53,8 @@ __device__ inline void persp_pixel(const PerspParams &p, int x, int y, T *o) {
     if (ix >= 0 && ix + 1 < W && iy >= 0 && iy + 1 < H) {
         x0 = ix, x1 = ix + 1, y0 = iy, y1 = iy + 1;
     } else {
-        x0 = border_map(ix, W, p.border), x1 = border_map(ix + 1, W, p.border);
-        y0 = border_map(iy, H, p.border), y1 = border_map(iy + 1, H, p.border);
+        x0 = border_map_inl(ix, W, p.border), x1 = border_map_inl(ix + 1, W, p.border);
+        y0 = border_map_inl(iy, H, p.border), y1 = border_map_inl(iy + 1, H, p.border);
     }
     float w0 = (1 - ax) * (1 - ay), w1 = ax * (1 - ay), w2 = (1 - ax) * ay, w3 = ax * ay;
     float cv = (float)p.cvd;

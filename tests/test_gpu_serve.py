@@ -1,0 +1,119 @@
+"""Served front end vs the oracle and vs the plain calls (GPU).
+
+Each frame's mask must equal the oracle's pipeline (GaussianBlur 5x5 ->
+Background -> threshold -> open 3x3) bit for bit, a stream's results must be
+the same when it runs alone and when its frames share launches with other
+streams, and streams are admitted only while the HBM budget holds them."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def va():
+    import visar_amd
+    assert torch.cuda.is_available()
+    return visar_amd
+
+
+def frames_for(seed, n, w, h):
+    from visar_amd.serve import CameraSet
+    cams = CameraSet(1, w, h, seed=seed, pool=4)
+    return [cams.next_frame(0).copy() for _ in range(n)]
+
+
+def run_streams(srv, streams, interleave=True):
+    """streams: list of frame lists, one per stream. Returns per stream the
+    (masks, results) in frame order."""
+    ids = [srv.admit() for _ in streams]
+    h, w = streams[0][0].shape
+    masks = [[np.full((h, w), 7, np.uint8) for _ in fl] for fl in streams]
+    n = max(len(fl) for fl in streams)
+    order = [(k, i) for k in range(n) for i in range(len(streams)) if k < len(streams[i])]
+    if not interleave:
+        order = [(k, i) for i in range(len(streams)) for k in range(len(streams[i]))]
+    for k, i in order:
+        srv.submit(ids[i], streams[i][k], tag=k, mask_out=masks[i][k])
+    srv.drain()
+    res = {}
+    while True:
+        got = srv.poll(1024)
+        if not got:
+            break
+        for r in got:
+            res[(r["stream_id"], r["frame_index"])] = r
+    out = []
+    for i, sid in enumerate(ids):
+        rs = [res[(sid, k)] for k in range(len(streams[i]))]
+        assert all(r["status"] == 0 for r in rs)
+        assert [r["tag"] for r in rs] == list(range(len(streams[i])))
+        out.append((masks[i], rs))
+    for sid in ids:
+        srv.release(sid)
+    return out
+
+
+@pytest.mark.parametrize("w,h", [(301, 197), (64, 33), (5, 3), (1, 1), (257, 17)])
+def test_serve_matches_oracle_pipeline(va, orc, w, h):
+    from visar_amd.serve import Server, pipeline_oracle
+    streams = [frames_for(s, 6, w, h) for s in range(3)]
+    with Server(width=w, height=h, max_batch=2, queue_depth=3, min_area=5) as srv:
+        out = run_streams(srv, streams)
+    for fl, (masks, rs) in zip(streams, out):
+        want = pipeline_oracle(orc, fl)
+        for k, (m, wm) in enumerate(zip(masks, want)):
+            assert np.array_equal(m, wm), f"{w}x{h} frame {k}: {(m != wm).sum()} pixels differ"
+            assert rs[k]["components"] == orc.connected_components_with_stats(wm)[0]
+
+
+def test_serve_matches_plain_calls_full_size(va):
+    from visar_amd.serve import PlainPipeline, Server
+    streams = [frames_for(10 + s, 5, 1280, 720) for s in range(2)]
+    with Server() as srv:
+        out = run_streams(srv, streams)
+    for fl, (masks, rs) in zip(streams, out):
+        plain = PlainPipeline(va)
+        for k, f in enumerate(fl):
+            m, n = plain.run(f)
+            assert np.array_equal(masks[k], m), f"frame {k} differs from the plain calls"
+            assert rs[k]["components"] == n
+
+
+def test_serve_alone_equals_in_mix(va):
+    from visar_amd.serve import Server
+    a = frames_for(42, 8, 1280, 720)
+    others = [frames_for(100 + s, 8, 1280, 720) for s in range(7)]
+    with Server(max_batch=1) as srv:
+        (alone_masks, alone_rs), = run_streams(srv, [a])
+    with Server(max_batch=8, max_inflight=2) as srv:
+        mixed = run_streams(srv, [others[0], others[1], a] + others[2:])
+    mix_masks, mix_rs = mixed[2]
+    assert max(r["batch_size"] for r in mix_rs) > 1, "the stream never shared a launch"
+    for k in range(8):
+        assert np.array_equal(alone_masks[k], mix_masks[k])
+        assert alone_rs[k]["checksum"] == mix_rs[k]["checksum"]
+        assert alone_rs[k]["components"] == mix_rs[k]["components"]
+
+
+def test_serve_admission_by_budget(va):
+    from visar_amd.serve import Server
+    with Server(width=320, height=240, max_streams=8) as probe:
+        per = probe.stats()["bytes_per_stream"]
+    with Server(width=320, height=240, max_streams=8, hbm_budget=int(2.5 * per)) as srv:
+        a, b = srv.admit(), srv.admit()
+        assert srv.try_admit() is None
+        st = srv.stats()
+        assert st["active_streams"] == 2 and st["rejected"] == 1 and st["device_bytes"] == 2 * per
+        f = frames_for(1, 2, 320, 240)
+        for k in range(2):
+            srv.submit(a, f[k], tag=k)
+            srv.submit(b, f[k], tag=k)
+        srv.release(a)
+        c = srv.try_admit()
+        assert c is not None
+        srv.submit(c, f[0])
+        srv.drain()
+        assert len(srv.poll(100)) == 5

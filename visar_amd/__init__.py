@@ -265,16 +265,21 @@ class Background:
 
 
 def connected_components_with_stats(a, connectivity=8):
-    """(count, labels i32, stats (count, 5) i32, centroids (count, 2) f64) like the reference driver."""
+    """(count, labels i32, stats (count, 5) i32, centroids (count, 2) f64) like the reference driver
+    (tests/py/visar_ctypes.py:345-361): room for 4096 labels first, then exactly the reported count."""
     h, w = a.shape[:2]
     labels = np.empty((h, w), np.int32)
-    cap = min(h * w + 1, 1 << 22)
-    stats = np.empty((cap, 5), np.int32)
-    cent = np.empty((cap, 2), np.float64)
-    count = ctypes.c_size_t(0)
     src = a.cpu().numpy() if _is_tensor(a) else a
-    _check(lib.visar_connected_components(_ref(src), _ref(labels), connectivity, stats.ctypes.data_as(ctypes.c_void_p),
-                                          cent.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(cap),
-                                          ctypes.byref(count)))
-    n = count.value
-    return n, labels, stats[:n].copy(), cent[:n].copy()
+    cap = min(h * w + 1, 4096)
+    while True:
+        stats = np.empty((cap, 5), np.int32)
+        cent = np.empty((cap, 2), np.float64)
+        count = ctypes.c_size_t(0)
+        _check(lib.visar_connected_components(_ref(src), _ref(labels), connectivity,
+                                              stats.ctypes.data_as(ctypes.c_void_p),
+                                              cent.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(cap),
+                                              ctypes.byref(count)))
+        n = count.value
+        if n <= cap:
+            return n, labels, stats[:n].copy(), cent[:n].copy()
+        cap = n

@@ -315,7 +315,6 @@ __device__ __attribute__((always_inline)) inline void morph_small_tile(const Mor
         },
         [&](int c) { return (c >= 0 && c < n) ? c : -1; });
     const int tx = tile_tx();
-    const int x = x0 + tx * 4;
     const int yt = tile_ty() * TH;
     if (y0 + yt >= H) return;  // the same for the whole wave
     uint32_t acc[TH][4];

@@ -20,7 +20,11 @@ torch = pytest.importorskip("torch")
 def va():
     import visar_amd
     assert torch.cuda.is_available()
-    return visar_amd
+    # The library's calls go on torch's stream, ordered with the tensors' own
+    # fills and copies (the library's default stream is a separate one).
+    visar_amd.set_stream(torch.cuda.current_stream())
+    yield visar_amd
+    visar_amd.set_stream(None)
 
 
 def rnd(shape, seed, dtype=np.uint8):
@@ -368,6 +372,45 @@ def test_morphology_mask_config(va, orc):
         for shape, ks in [(0, (3, 3)), (2, (5, 5)), (1, (7, 7))]:
             for it in (1, 3):
                 check(va, orc, "morphology", m, op, shape, ks, iterations=it)
+
+
+@pytest.fixture(scope="module")
+def mask4k():
+    from visar_amd import synth
+    return synth.morph_mask(0, 4096)
+
+
+def test_morphology_4096_sample_vs_oracle(va, orc, mask4k):
+    """Full-size cases of the 4096x4096 morphology workload against the oracle."""
+    d = dev(mask4k)
+    for op, shape, ks, it in [(2, 2, 5, 2), (5, 1, 7, 3), (4, 0, 3, 1), (1, 1, 7, 1)]:
+        want = orc.morphology(mask4k, op, shape, (ks, ks), it)
+        assert_same(va.morphology(d, op, shape, (ks, ks), it).cpu().numpy(), want, f"op{op} shape{shape} k{ks} it{it}")
+
+
+def test_morphology_4096_all_cases_relations(va, mask4k):
+    """All 63 cases at full size, tied to the single passes by the identities
+    of morphologyEx (morph.zig:532-571): open = dilate^n(erode^n), close =
+    erode^n(dilate^n), gradient = dilate^n - erode^n, tophat = src - open,
+    blackhat = close - src (saturating), erode^n <= src <= dilate^n, and
+    open/close idempotent; the single passes are checked against the oracle
+    at smaller sizes."""
+    d = dev(mask4k)
+    for shape, ks in [(0, 3), (2, 5), (1, 7)]:
+        k = (ks, ks)
+        for it in (1, 2, 3):
+            r = {op: va.morphology(d, op, shape, k, it) for op in range(7)}
+            er, di = r[0], r[1]
+            assert bool((er <= d).all()) and bool((d <= di).all())
+            assert torch.equal(r[2], va.morphology(er, 1, shape, k, it))
+            assert torch.equal(r[3], va.morphology(di, 0, shape, k, it))
+            assert torch.equal(r[4], di - er)
+            assert torch.equal(r[5], d - r[2])
+            assert torch.equal(r[6], r[3] - d)
+            assert bool((r[2] <= d).all()) and bool((d <= r[3]).all())
+            if it == 1:
+                assert torch.equal(va.morphology(r[2], 2, shape, k, 1), r[2])
+                assert torch.equal(va.morphology(r[3], 3, shape, k, 1), r[3])
 
 
 # ---------------------------------------------------------- background

@@ -16,7 +16,11 @@ torch = pytest.importorskip("torch")
 def va():
     import visar_amd
     assert torch.cuda.is_available()
-    return visar_amd
+    # The library's calls go on torch's stream, ordered with the tensors' own
+    # fills and copies (the library's default stream is a separate one).
+    visar_amd.set_stream(torch.cuda.current_stream())
+    yield visar_amd
+    visar_amd.set_stream(None)
 
 
 def frames_for(seed, n, w, h):

@@ -9,6 +9,7 @@ Importing fails loudly when the HIP library has not been built.
 """
 import ctypes
 import os
+import threading
 
 import numpy as np
 
@@ -102,7 +103,22 @@ def desc(a):
     return Image(w, h, c, _DEPTH[t], stride, a.ctypes.data)
 
 
+_tls = threading.local()
+
+
+def _follow_torch(a):
+    """Until the caller picks a stream with set_stream, calls that take torch
+    tensors run on torch's current stream of the tensor's device, so they are
+    ordered after the tensors' own fills and copies and before what torch
+    does with the results."""
+    if not getattr(_tls, "explicit", False):
+        import torch
+        lib.visar_gpu_set_stream(ctypes.c_void_p(torch.cuda.current_stream(a.device).cuda_stream))
+
+
 def _ref(a):
+    if _is_tensor(a) and a.is_cuda:
+        _follow_torch(a)
     return ctypes.byref(desc(a))
 
 
@@ -146,12 +162,15 @@ def get_num_threads():
 
 def set_stream(stream):
     """Stream for this thread's calls: a torch.cuda.Stream or a raw hipStream_t
-    int (0 is HIP's default stream); None returns to the library's own stream."""
+    int (0 is HIP's default stream); None returns to the library's own stream
+    (and to following torch's current stream for calls on torch tensors)."""
     if stream is None:
+        _tls.explicit = False
         _check(lib.visar_gpu_reset_stream())
         return
     if hasattr(stream, "cuda_stream"):
         stream = stream.cuda_stream
+    _tls.explicit = True
     _check(lib.visar_gpu_set_stream(ctypes.c_void_p(stream)))
 
 

@@ -290,6 +290,29 @@ def test_warp_affine(va, orc):
                             check(va, orc, "warp_affine", a, m, size, interpolation=interp, border=border, value=37.6)
 
 
+def test_warp_affine_u8_box_cases(va, orc):
+    """One-channel u8 bilinear BORDER_CONSTANT (the LDS-box path): odd
+    strides, offset views, tiny images, rotations up to 90 deg, scales from
+    0.3 (box too wide: global taps) to 3, and maps that leave the image."""
+    from visar_amd import synth
+    big = natural(260, 420, 1, 91)
+    views = [big[:197, :301], big[3:200, 1:306], big[5:6, 7:8], big[:2, :3], np.ascontiguousarray(big[:150, :257])]
+    for a in views:
+        h, w = a.shape
+        ms = [synth.rotation_matrix(w / 2, h / 2, ang, sc) for ang, sc in [(8.2, 0.91), (-30, 1.2), (90, 1.0),
+                                                                          (45, 0.3), (12, 3.0)]]
+        ms += [np.array([[1, 0, -w * 0.6], [0, 1, h * 0.3]]), np.array([[0.5, 0.2, 3.5], [-0.1, 0.7, -2.25]])]
+        for m in ms:
+            for size in [(w, h), (w + 67, max(1, h - 5))]:
+                want = orc.warp_affine(np.ascontiguousarray(a), m, size, 1, 0, 17.0)
+                assert_same(va.warp_affine(a, m, size, 1, 0, 17.0), want, f"host {a.shape} {m.tolist()} {size}")
+                dbig = dev(big)
+                ys, xs = (0, 0) if a.base is None else np.unravel_index((a.__array_interface__['data'][0] -
+                                                                         big.__array_interface__['data'][0]), big.shape)
+                dv = dbig[ys:ys + h, xs:xs + w] if a.base is not None else dev(a)
+                assert_same(va.warp_affine(dv, m, size, 1, 0, 17.0).cpu().numpy(), want, f"device {a.shape} {size}")
+
+
 def test_warp_affine_1080p_config(va, orc):
     from visar_amd import synth
     a = synth.uniform_bytes(7, (1080, 1920))

@@ -69,14 +69,13 @@ __device__ inline void persp_pixel(const PerspParams &p, int x, int y, T *o) {
 }
 
 // Four consecutive bilinear u8 pixels of warpPerspective (geometry.zig:615-672).
-__device__ inline uint32_t persp4_u8(const PerspParams &p, int x, int y) {
-    const int W = p.s.w, H = p.s.h;
-    Taps t[4];
+__device__ inline uint32_t persp4_u8(const PerspParams &p, int x, int dx, int y) {
+    int ix[4], iy[4];
     float ax[4], ay[4];
     const double fy = (double)y;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        const double fx = (double)(x + j);
+        const double fx = (double)(x + j * dx);
         double sx = p.m[0] * fx + p.m[1] * fy + p.m[2];
         double sy = p.m[3] * fx + p.m[4] * fy + p.m[5];
         const double w = p.m[6] * fx + p.m[7] * fy + p.m[8];
@@ -86,13 +85,13 @@ __device__ inline uint32_t persp4_u8(const PerspParams &p, int x, int y) {
         const double x0f = floor(clampd(sx, -1e9, 1e9)), y0f = floor(clampd(sy, -1e9, 1e9));
         ax[j] = (float)(sx - x0f);
         ay[j] = (float)(sy - y0f);
-        t[j] = map_taps((long)x0f, (long)y0f, W, H, p.border);
+        ix[j] = (int)x0f;
+        iy[j] = (int)y0f;
     }
-    const float cv = (float)p.cvd;
     Quad q[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) q[j] = load_quad_u8(p.s, t[j], cv);
-    uint32_t out = 0;
+    bool out[4];
+    quads_u8(p.s, p.border, (float)p.cvd, ix, iy, q, out);
+    uint32_t res = 0;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const float a = ax[j], b = ay[j];
@@ -103,10 +102,10 @@ __device__ inline uint32_t persp4_u8(const PerspParams &p, int x, int y) {
         acc += q[j].p10 * w2;
         acc += q[j].p11 * w3;
         uint32_t v = round_u8(acc);
-        if (t[j].out) v = (uint32_t)p.cvd;
-        out |= v << (8 * j);
+        if (out[j]) v = (uint32_t)p.cvd;
+        res |= v << (8 * j);
     }
-    return out;
+    return res;
 }
 
 template <typename T>
@@ -121,15 +120,9 @@ __global__ __launch_bounds__(kThreads) void warp_persp_k(PerspParams p) {
 __global__ __launch_bounds__(kThreads) void warp_persp_lin4_u8(PerspParams p) {
     const DImg &d = p.d;
     const int y = blockIdx.y * 4 + tile_ty();
-    const int x = (blockIdx.x * 64 + tile_tx()) * 4;
-    if (y >= d.h || x >= d.w) return;
-    const uint32_t v = persp4_u8(p, x, y);
-    uint8_t *o = row_ptr<uint8_t>(d, y) + x;
-    if (x + 4 <= d.w && ((uintptr_t)o & 3) == 0) {
-        *reinterpret_cast<uint32_t *>(o) = v;
-    } else {
-        for (int j = 0; j < 4 && x + j < d.w; j++) o[j] = (uint8_t)(v >> (8 * j));
-    }
+    const int x0 = blockIdx.x * 256;
+    if (y >= d.h || x0 + tile_tx() >= d.w) return;
+    store4_strided_u8(d, x0, y, persp4_u8(p, x0 + tile_tx(), 64, y));
 }
 
 // undistort.zig:365-394 linearF32 (the 16-lane all-inside block is identical).
@@ -167,6 +160,11 @@ __global__ __launch_bounds__(kThreads) void remap_k(DImg s, DImg d, DImg mxi, DI
 
 __global__ __launch_bounds__(kThreads) void warp_affine_lin4_u8(AffineParams p) { affine_lin4_tile(p, blockIdx.x, blockIdx.y); }
 
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void warp_affine_box_u8(AffineParams p) {
+    __shared__ uint32_t lds[kWarpLdsWords];
+    affine_box_tile(p, blockIdx.x, blockIdx.y, lds);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void warp_affine_k(AffineParams p) {
     affine_px_tile<T>(p, blockIdx.x, blockIdx.y);
@@ -181,6 +179,11 @@ inline dim3 grid2d(int w, int h, int px) { return dim3((unsigned)((w + 64 * px -
 
 int launch_warp_affine(const AffineParams &p, hipStream_t st) {
     const dim3 b(kTileThreads);
+    if (p.depth == 0 && p.d.cn == 1 && p.interp == 1 && p.border == BORDER_CONSTANT) {
+        const dim3 g((unsigned)((p.d.w + kWarpTW - 1) / kWarpTW), (unsigned)((p.d.h + kWarpTH - 1) / kWarpTH));
+        hipLaunchKernelGGL(warp_affine_box_u8, g, b, 0, st, p);
+        return launched();
+    }
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1) {
         hipLaunchKernelGGL(warp_affine_lin4_u8, grid2d(p.d.w, p.d.h, 4), b, 0, st, p);
         return launched();

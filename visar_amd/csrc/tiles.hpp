@@ -376,8 +376,9 @@ __device__ __attribute__((always_inline)) inline float ld_f(const DImg &s, long 
 // lies in [0, 255] (up to rounding), where the 1.5*2^23 trick in f32 equals
 // sat.cast(u8, v) (geometry.zig:563-566 uses the same trick).
 __device__ __attribute__((always_inline)) inline uint32_t round_u8(float v) {
-    float r = rint_magic(v);
-    r = min_num(max_num(r, 0.0f), 255.0f);
+    // v is a blend of u8 samples, never NaN, so IEEE min/max equal the
+    // reference's NaN-aware ones here.
+    const float r = fminf(fmaxf(rint_magic(v), 0.0f), 255.0f);
     return (uint32_t)r;
 }
 
@@ -388,7 +389,7 @@ struct Taps {
     bool out;
 };
 
-__device__ __attribute__((always_inline)) inline Taps map_taps(long ix, long iy, int W, int H, int border) {
+__device__ __attribute__((always_inline)) inline Taps map_taps(int ix, int iy, int W, int H, int border) {
     Taps t;
     t.out = border == BORDER_CONSTANT && (ix < -1 || iy < -1 || ix >= W || iy >= H);
     if (ix >= 0 && ix + 1 < W && iy >= 0 && iy + 1 < H) {
@@ -418,33 +419,260 @@ __device__ __attribute__((always_inline)) inline Quad load_quad_u8(const DImg &s
     return q;
 }
 
-// Four consecutive bilinear pixels of a one-channel u8 warpAffine.
-__device__ __attribute__((always_inline)) inline uint32_t affine4_u8(const AffineParams &p, int x, float bx, float by) {
-    const int W = p.s.w, H = p.s.h;
+// toIdx (geometry.zig:492-494) for the coordinates a warp produces, which
+// are finite: IEEE min/max equal the reference's NaN-aware ones on them.
+__device__ __attribute__((always_inline)) inline int to_i32(float f) { return (int)fminf(fmaxf(f, -1.0e9f), 1.0e9f); }
+
+// Whether a bilinear footprint at (ix, iy) lies wholly inside the image.
+__device__ __attribute__((always_inline)) inline bool inner_tap(int ix, int iy, int W, int H) {
+    return (unsigned)ix < (unsigned)(W - 1) && (unsigned)iy < (unsigned)(H - 1);
+}
+
+// The taps of four one-channel u8 pixels. When every footprint of the wave
+// lies inside the image (the common case), the taps are plain loads at
+// 32-bit offsets from the image base; otherwise each goes through the
+// border rules.
+__device__ __attribute__((always_inline)) inline void quads_u8(const DImg &s, int border, float cv, const int (&ix)[4],
+                                                               const int (&iy)[4], Quad (&q)[4], bool (&out)[4]) {
+    const int W = s.w, H = s.h;
+    bool inner = (unsigned long)s.h * (unsigned long)s.stride < (1ul << 32);
+#pragma unroll
+    for (int j = 0; j < 4; j++) inner = inner && inner_tap(ix[j], iy[j], W, H);
+    if (__all(inner)) {
+        const gptr<const uint8_t> base = gp((const uint8_t *)s.p);
+        const uint32_t st = (uint32_t)s.stride;
+        uint32_t a[4], b[4], c[4], d[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t o = (uint32_t)iy[j] * st + (uint32_t)ix[j];
+            a[j] = base[o];
+            b[j] = base[o + 1];
+            c[j] = base[o + st];
+            d[j] = base[o + st + 1];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            q[j].p00 = (float)a[j];
+            q[j].p01 = (float)b[j];
+            q[j].p10 = (float)c[j];
+            q[j].p11 = (float)d[j];
+            out[j] = false;
+        }
+        return;
+    }
     Taps t[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) t[j] = map_taps(ix[j], iy[j], W, H, border);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        q[j] = load_quad_u8(s, t[j], cv);
+        out[j] = t[j].out;
+    }
+}
+
+// Four bilinear pixels of a one-channel u8 warpAffine, at x, x + dx,
+// x + 2dx, x + 3dx of one row.
+__device__ __attribute__((always_inline)) inline uint32_t affine4_u8(const AffineParams &p, int x, int dx, float bx,
+                                                                     float by) {
+    int ix[4], iy[4];
     float ax[4], ay[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        const float xf = (float)(x + j);
+        const float xf = (float)(x + j * dx);
         const float sx = fmaf(p.m[0], xf, bx), sy = fmaf(p.m[3], xf, by);
         const float fx = floorf(sx), fy = floorf(sy);
         ax[j] = sx - fx;
         ay[j] = sy - fy;
-        t[j] = map_taps(to_idx(fx), to_idx(fy), W, H, p.border);
+        ix[j] = to_i32(fx);
+        iy[j] = to_i32(fy);
     }
     Quad q[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) q[j] = load_quad_u8(p.s, t[j], p.cv);
-    uint32_t out = 0;
+    bool out[4];
+    quads_u8(p.s, p.border, p.cv, ix, iy, q, out);
+    uint32_t res = 0;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const float v0 = fmaf(ax[j], q[j].p01 - q[j].p00, q[j].p00);
         const float v1 = fmaf(ax[j], q[j].p11 - q[j].p10, q[j].p10);
         uint32_t v = round_u8(fmaf(ay[j], v1 - v0, v0));
-        if (t[j].out) v = (uint32_t)p.cvd;
-        out |= v << (8 * j);
+        if (out[j]) v = (uint32_t)p.cvd;
+        res |= v << (8 * j);
     }
-    return out;
+    return res;
+}
+
+// Four output bytes at (x, y): one dword store when it fits and is aligned.
+__device__ __attribute__((always_inline)) inline void store4_u8(const DImg &d, int x, int y, uint32_t v) {
+    const gptr<uint8_t> o = grow<uint8_t>(d, y) + x;
+    if (x + 4 <= d.w && ((uintptr_t)o & 3) == 0) {
+        *(gptr<uint32_t>)o = v;
+    } else {
+        for (int j = 0; j < 4 && x + j < d.w; j++) o[j] = (uint8_t)(v >> (8 * j));
+    }
+}
+
+// ---------------------------------------------- LDS-staged bilinear warps
+//
+// One-channel u8 bilinear warps with BORDER_CONSTANT, 64 x 16 output
+// pixels per workgroup, 4 adjacent pixels per thread. Gathering taps from
+// global memory costs one address per byte in the texture addresser, which
+// bounds these kernels; instead the workgroup stages the source box its
+// taps can reach (the corners' taps, widened by one pixel, clipped to the
+// image) in LDS with dword loads, and gathers from there. Every thread then
+// checks that each of its in-image taps lies in the box; if one does not
+// (or the box is larger than the LDS tile), the workgroup gathers from
+// global memory instead, so the result never depends on the box estimate.
+
+constexpr int kWarpTW = 64, kWarpTH = 16;            // output pixels per workgroup
+constexpr int kWarpBoxRows = 64, kWarpBoxPitch = 128;  // staged source rows, bytes per staged row
+constexpr int kWarpBoxBytes = 124;                     // widest staged source span (+3 alignment bytes)
+constexpr int kWarpLdsWords = kWarpBoxRows * kWarpBoxPitch / 4;
+
+__device__ __attribute__((always_inline)) inline int wtile_x(int bx) { return bx * kWarpTW + (threadIdx.x & 15) * 4; }
+__device__ __attribute__((always_inline)) inline int wtile_y(int by) { return by * kWarpTH + (threadIdx.x >> 4); }
+
+struct WarpCoord {
+    int ix, iy;
+    float ax, ay;
+};
+
+// warpAffine coordinates of pixel (x, y) (geometry.zig:432-533).
+struct AffineCoord {
+    const float *m;
+    __device__ __attribute__((always_inline)) WarpCoord operator()(int x, int y) const {
+        const float yf = (float)y;
+        const float bxv = m[1] * yf + m[2];
+        const float byv = m[4] * yf + m[5];
+        const float xf = (float)x;
+        const float sx = fmaf(m[0], xf, bxv), sy = fmaf(m[3], xf, byv);
+        const float fx = floorf(sx), fy = floorf(sy);
+        return WarpCoord{to_i32(fx), to_i32(fy), sx - fx, sy - fy};
+    }
+};
+
+// Quads of a thread's 4 pixels for a one-channel u8 image with
+// BORDER_CONSTANT (value cv), from the LDS box when the workgroup can use
+// it, from global memory otherwise. Every thread of the workgroup calls
+// this (it votes and may synchronise); `live` marks threads whose pixels are
+// in the output.
+template <typename Coord>
+__device__ __attribute__((always_inline)) inline void warp_box_quads(const DImg &s, float cv, Coord coord, int bx,
+                                                                     int by, bool live, const WarpCoord (&c)[4],
+                                                                     uint32_t *lds, Quad (&q)[4], bool (&out)[4]) {
+    const int W = s.w, H = s.h;
+    int xlo = 0x7fffffff, xhi = -0x7fffffff, ylo = 0x7fffffff, yhi = -0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const WarpCoord e = coord(bx * kWarpTW + (k & 1) * (kWarpTW - 1), by * kWarpTH + (k >> 1) * (kWarpTH - 1));
+        xlo = min(xlo, e.ix);
+        xhi = max(xhi, e.ix);
+        ylo = min(ylo, e.iy);
+        yhi = max(yhi, e.iy);
+    }
+    xlo = max(xlo - 1, 0);
+    ylo = max(ylo - 1, 0);
+    xhi = min(xhi + 2, W - 1);
+    yhi = min(yhi + 2, H - 1);
+    const int nb = xhi - xlo + 1, nr = yhi - ylo + 1;
+    const bool fits = nb > 0 && nr > 0 && nb <= kWarpBoxBytes && nr <= kWarpBoxRows &&
+                      (unsigned long)s.h * (unsigned long)s.stride < (1ul << 31);
+    bool bad = false;
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int x0 = c[j].ix, y0 = c[j].iy;
+            bad = bad || ((unsigned)x0 < (unsigned)W && (x0 < xlo || x0 > xhi)) ||
+                  ((unsigned)(x0 + 1) < (unsigned)W && (x0 + 1 < xlo || x0 + 1 > xhi)) ||
+                  ((unsigned)y0 < (unsigned)H && (y0 < ylo || y0 > yhi)) ||
+                  ((unsigned)(y0 + 1) < (unsigned)H && (y0 + 1 < ylo || y0 + 1 > yhi));
+        }
+    }
+    const bool use = __syncthreads_or(bad) == 0 && fits;
+    if (!use) {
+        int ix[4], iy[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            ix[j] = c[j].ix;
+            iy[j] = c[j].iy;
+        }
+        quads_u8(s, BORDER_CONSTANT, cv, ix, iy, q, out);
+        return;
+    }
+    // Stage rows ylo..yhi, bytes xlo..xhi: row r's dwords start at the
+    // aligned address at or below its byte xlo (shift a_r), 8 rows x 32
+    // dwords per round.
+    const uintptr_t a0 = (uintptr_t)s.p + (uintptr_t)ylo * (uintptr_t)s.stride + (uintptr_t)xlo;
+    uint32_t v[kWarpBoxRows / 8];
+#pragma unroll
+    for (int t = 0; t < kWarpBoxRows / 8; t++) {
+        const int r = t * 8 + (int)(threadIdx.x >> 5), k = threadIdx.x & 31;
+        v[t] = 0;
+        if (r < nr) {
+            const uint8_t *row = s.p + (long)(ylo + r) * s.stride;
+            const int a = (int)((a0 + (uintptr_t)r * (uintptr_t)s.stride) & 3);
+            const int c0 = xlo - a + 4 * k;  // row offset of this dword
+            if (4 * k < nb + a) {
+                if (c0 >= 0 && c0 + 4 <= W) {
+                    v[t] = *(gptr<const uint32_t>)gp(row + c0);
+                } else {
+                    uint32_t w = 0;
+                    for (int b = 0; b < 4; b++)
+                        if (c0 + b >= 0 && c0 + b < W) w |= (uint32_t)gp(row)[c0 + b] << (8 * b);
+                    v[t] = w;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kWarpBoxRows / 8; t++) {
+        const int r = t * 8 + (int)(threadIdx.x >> 5), k = threadIdx.x & 31;
+        if (r < nr) lds[r * (kWarpBoxPitch / 4) + k] = v[t];
+    }
+    __syncthreads();
+    const uint8_t *box = reinterpret_cast<const uint8_t *>(lds);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int x0 = c[j].ix, y0 = c[j].iy;
+        const int r0 = min(max(y0 - ylo, 0), nr - 1), r1 = min(max(y0 + 1 - ylo, 0), nr - 1);
+        const int k0 = min(max(x0 - xlo, 0), nb - 1), k1 = min(max(x0 + 1 - xlo, 0), nb - 1);
+        const int s0 = (int)((a0 + (uintptr_t)r0 * (uintptr_t)s.stride) & 3);
+        const int s1 = (int)((a0 + (uintptr_t)r1 * (uintptr_t)s.stride) & 3);
+        const uint32_t b00 = box[r0 * kWarpBoxPitch + s0 + k0], b01 = box[r0 * kWarpBoxPitch + s0 + k1];
+        const uint32_t b10 = box[r1 * kWarpBoxPitch + s1 + k0], b11 = box[r1 * kWarpBoxPitch + s1 + k1];
+        const bool vx0 = (unsigned)x0 < (unsigned)W, vx1 = (unsigned)(x0 + 1) < (unsigned)W;
+        const bool vy0 = (unsigned)y0 < (unsigned)H, vy1 = (unsigned)(y0 + 1) < (unsigned)H;
+        q[j].p00 = vx0 && vy0 ? (float)b00 : cv;
+        q[j].p01 = vx1 && vy0 ? (float)b01 : cv;
+        q[j].p10 = vx0 && vy1 ? (float)b10 : cv;
+        q[j].p11 = vx1 && vy1 ? (float)b11 : cv;
+        out[j] = x0 < -1 || y0 < -1 || x0 >= W || y0 >= H;
+    }
+}
+
+// One-channel u8 bilinear warpAffine, BORDER_CONSTANT: a 64 x 16 tile.
+__device__ __attribute__((always_inline)) inline void affine_box_tile(const AffineParams &p, int bx, int by,
+                                                                      uint32_t *lds) {
+    const DImg &d = p.d;
+    const int x = wtile_x(bx), y = wtile_y(by);
+    const bool live = x < d.w && y < d.h;
+    const AffineCoord coord{p.m};
+    WarpCoord c[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) c[j] = coord(x + j, y);
+    Quad q[4];
+    bool out[4];
+    warp_box_quads(p.s, p.cv, coord, bx, by, live, c, lds, q, out);
+    if (!live) return;
+    uint32_t res = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const float v0 = fmaf(c[j].ax, q[j].p01 - q[j].p00, q[j].p00);
+        const float v1 = fmaf(c[j].ax, q[j].p11 - q[j].p10, q[j].p10);
+        uint32_t v = round_u8(fmaf(c[j].ay, v1 - v0, v0));
+        if (out[j]) v = (uint32_t)p.cvd;
+        res |= v << (8 * j);
+    }
+    store4_u8(d, x, y, res);
 }
 
 // One output pixel of warpAffine, any depth and channel count.
@@ -488,23 +716,30 @@ __device__ __attribute__((always_inline)) inline void affine_pixel(const AffineP
     }
 }
 
-// One-channel u8 bilinear warpAffine tile: 256 x 4 pixels, 4 per thread; a
-// partial group at the row end computes all four, stores those in the row.
+// Four output bytes of lane tx at x0 + tx + 64j (j = 0..3) of row y: each
+// store instruction writes 64 consecutive bytes.
+__device__ __attribute__((always_inline)) inline void store4_strided_u8(const DImg &d, int x0, int y, uint32_t v) {
+    const gptr<uint8_t> o = grow<uint8_t>(d, y);
+    const int x = x0 + tile_tx();
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (x + 64 * j < d.w) o[x + 64 * j] = (uint8_t)(v >> (8 * j));
+}
+
+// One-channel u8 bilinear warp tiles are 256 x 4 pixels; lane tx of a wave
+// computes pixels x0 + tx + 64j, so one gather instruction covers 64
+// neighbouring output pixels, whose taps lie on few source rows under a
+// rotation (a lane with 4 adjacent pixels would spread one instruction over
+// 256 pixels and many more cache lines).
 __device__ __attribute__((always_inline)) inline void affine_lin4_tile(const AffineParams &p, int bx, int by) {
     const DImg &d = p.d;
     const int y = by * 4 + tile_ty();
-    const int x = (bx * 64 + tile_tx()) * 4;
-    if (y >= d.h || x >= d.w) return;
+    const int x0 = bx * 256;
+    if (y >= d.h || x0 + tile_tx() >= d.w) return;
     const float yf = (float)y;
     const float bxv = p.m[1] * yf + p.m[2];
     const float byv = p.m[4] * yf + p.m[5];
-    const uint32_t v = affine4_u8(p, x, bxv, byv);
-    const gptr<uint8_t> o = grow<uint8_t>(d, y) + x;
-    if (x + 4 <= d.w && ((uintptr_t)o & 3) == 0) {
-        *(gptr<uint32_t>)o = v;
-    } else {
-        for (int j = 0; j < 4 && x + j < d.w; j++) o[j] = (uint8_t)(v >> (8 * j));
-    }
+    store4_strided_u8(d, x0, y, affine4_u8(p, x0 + tile_tx(), 64, bxv, byv));
 }
 
 // Any depth / channel count: 64 x 4 pixels per tile, one per thread.

@@ -41,6 +41,8 @@ enum JobKind : int {
     JK_LINEAR = 6,
     JK_AREA_INT = 7,
     JK_AREA_FRAC = 8,
+    JK_STENCIL_SWAR = 9,
+    JK_AFFINE_BOX = 10,
 };
 
 struct ResizeArgs {
@@ -117,6 +119,21 @@ __global__ __launch_bounds__(kTileThreads) void batch_stencil(BATCH_ARGS) {
           [&](const BatchJob &j, int bx, int by) { stencil_tile<K, CN, TH, uint8_t>(j.st, bx, by, lds); });
 }
 
+template <int MODE, int CN, int TH>
+__global__ __launch_bounds__(kTileThreads) void batch_stencil_swar(BATCH_ARGS) {
+    __shared__ uint32_t lds[StencilShape<SwarTaps<MODE>::K, CN, TH>::LDS_WORDS];
+    __shared__ int next;
+    drain(jobs, units, seg, counters, &next,
+          [&](const BatchJob &j, int bx, int by) { stencil_swar_tile<MODE, CN, TH, uint8_t>(j.st, bx, by, lds); });
+}
+
+__global__ __launch_bounds__(kTileThreads) void batch_affine_box(BATCH_ARGS) {
+    __shared__ uint32_t lds[kWarpLdsWords];
+    __shared__ int next;
+    drain(jobs, units, seg, counters, &next,
+          [&](const BatchJob &j, int bx, int by) { affine_box_tile(j.af, bx, by, lds); });
+}
+
 template <int K, int CN, bool MAX>
 __global__ __launch_bounds__(kTileThreads) void batch_morph(BATCH_ARGS) {
     __shared__ uint32_t lds[MorphShape<K, CN, 2>::LDS_WORDS];
@@ -150,6 +167,12 @@ GroupKernel stencil_kernel(int th) {
     return th == 2 ? batch_stencil<K, CN, 2> : batch_stencil<K, CN, 1>;
 }
 
+template <int MODE>
+GroupKernel swar_kernel(int cn, int th) {
+    if (cn == 1) return th == 2 ? batch_stencil_swar<MODE, 1, 2> : batch_stencil_swar<MODE, 1, 1>;
+    return th == 2 ? batch_stencil_swar<MODE, 3, 2> : batch_stencil_swar<MODE, 3, 1>;
+}
+
 template <int K, int CN>
 GroupKernel morph_kernel(int mx) {
     return mx ? batch_morph<K, CN, true> : batch_morph<K, CN, false>;
@@ -166,6 +189,8 @@ GroupKernel group_kernel(int kind, int variant) {
     case JK_MORPH:
         if (cn == 1) return k == 3 ? morph_kernel<3, 1>(last) : k == 5 ? morph_kernel<5, 1>(last) : morph_kernel<7, 1>(last);
         return k == 3 ? morph_kernel<3, 3>(last) : k == 5 ? morph_kernel<5, 3>(last) : morph_kernel<7, 3>(last);
+    case JK_STENCIL_SWAR: return k == SWAR_G3 ? swar_kernel<SWAR_G3>(cn, last) : swar_kernel<SWAR_G5>(cn, last);
+    case JK_AFFINE_BOX: return batch_affine_box;
     case JK_AFFINE_LIN4: return batch_other<JK_AFFINE_LIN4>;
     case JK_AFFINE_PX: return batch_other<JK_AFFINE_PX>;
     case JK_HALF_C1: return batch_other<JK_HALF_C1>;
@@ -263,6 +288,11 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
         int th = tuning().batch_stencil_rows ? tuning().batch_stencil_rows : 1;
         th = th >= 2 ? 2 : 1;
         j.variant = k * 100 + (int)it.src.channels * 10 + th;
+        const int mode = swar_mode(j.st, k, 0);
+        if (mode == SWAR_G3 || mode == SWAR_G5) {
+            j.kind = JK_STENCIL_SWAR;
+            j.variant = mode * 100 + (int)it.src.channels * 10 + th;
+        }
         j.tiles_x = (int)(((long)s.w * s.cn + 255) / 256);
         j.ntiles = tiles256(s, 4 * th);
         return OK;
@@ -315,6 +345,12 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
         for (int i = 0; i < 6; i++) p.m[i] = (float)inv[i];
         p.cvd = sat_cast<uint8_t>(it.value);
         p.cv = (float)p.cvd;
+        if (it.src.channels == 1 && p.interp == 1 && p.border == BORDER_CONSTANT) {
+            j.kind = JK_AFFINE_BOX;
+            j.tiles_x = (d.w + kWarpTW - 1) / kWarpTW;
+            j.ntiles = j.tiles_x * ((d.h + kWarpTH - 1) / kWarpTH);
+            return OK;
+        }
         if (it.src.channels == 1 && p.interp == 1) {
             j.kind = JK_AFFINE_LIN4;
             j.tiles_x = (d.w + 255) / 256;

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kTileThreads) void serve_k(const ServeLaunch a) {
     const gptr<const uint8_t> bg_in = gp((const uint8_t *)ss.bg[par]);
     const gptr<uint8_t> bg_out = gp((uint8_t *)ss.bg[par ^ 1]);
 
-    const auto st = stage_rows<kFR, kFW>(
+    const auto st = stage_rows<kFR, kFW, 5>(
         f, W, x0 - 8, 0u, 0u, fr, [&](int r) { return border_map_halo(y0 - 4 + r, H, BORDER_REFLECT101); },
         [&](int c) { return mapped_pos<1>(c, W, BORDER_REFLECT101); });
 

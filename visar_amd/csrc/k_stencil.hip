@@ -2,6 +2,8 @@
 // 3/5/7 kernels (GaussianBlur u8, Sobel u8 -> i16), the shapes of every
 // benchmark configuration. The per-tile work is stencil_tile (tiles.hpp),
 // shared with the persistent batch kernel.
+#include <initializer_list>
+
 #include "tiles.hpp"
 
 namespace visar {
@@ -12,6 +14,31 @@ template <int K, int CN, int TH, typename D>
 __global__ __launch_bounds__(256) void sep_small_u8(StencilArgs p) {
     __shared__ uint32_t lds[StencilShape<K, CN, TH>::LDS_WORDS];
     stencil_tile<K, CN, TH, D>(p, blockIdx.x, blockIdx.y, lds);
+}
+
+template <int MODE, int CN, int TH, typename D>
+__global__ __launch_bounds__(256) void sep_swar(StencilArgs p) {
+    __shared__ uint32_t lds[StencilShape<SwarTaps<MODE>::K, CN, TH>::LDS_WORDS];
+    stencil_swar_tile<MODE, CN, TH, D>(p, blockIdx.x, blockIdx.y, lds);
+}
+
+template <int MODE, int CN, typename D>
+void launch_swar_th(const StencilArgs &p, hipStream_t st, int th) {
+    const long n = (long)p.src.w * CN;
+    auto grid = [&](int t) { return dim3((unsigned)((n + 255) / 256), (unsigned)((p.src.h + 4 * t - 1) / (4 * t))); };
+    switch (th) {
+    case 1: hipLaunchKernelGGL((sep_swar<MODE, CN, 1, D>), grid(1), dim3(256), 0, st, p); break;
+    case 2: hipLaunchKernelGGL((sep_swar<MODE, CN, 2, D>), grid(2), dim3(256), 0, st, p); break;
+    default: hipLaunchKernelGGL((sep_swar<MODE, CN, 4, D>), grid(4), dim3(256), 0, st, p); break;
+    }
+}
+
+template <int MODE, typename D>
+bool launch_swar(const StencilArgs &p, hipStream_t st, int th) {
+    if (p.src.cn == 1) launch_swar_th<MODE, 1, D>(p, st, th);
+    else if (p.src.cn == 3) launch_swar_th<MODE, 3, D>(p, st, th);
+    else return false;
+    return true;
 }
 
 template <int K, int CN, typename D>
@@ -75,11 +102,38 @@ bool stencil_args(const SepParams &p, StencilArgs &a) {
     return true;
 }
 
+// The packed 16-bit mode whose taps the call has exactly, or -1.
+int swar_mode(const StencilArgs &a, int k, int ddepth) {
+    auto is = [&](const int *t, std::initializer_list<int> v) {
+        int i = 0;
+        for (int x : v)
+            if (t[i++] != x) return false;
+        return true;
+    };
+    if (a.src.cn != 1 && a.src.cn != 3) return -1;
+    if (ddepth == 0 && a.fin == FIN_FIX8) {
+        if (k == 3 && is(a.kx, {64, 128, 64}) && is(a.ky, {64, 128, 64})) return SWAR_G3;
+        if (k == 5 && is(a.kx, {16, 64, 96, 64, 16}) && is(a.ky, {16, 64, 96, 64, 16})) return SWAR_G5;
+    }
+    if (ddepth == 3 && a.fin == FIN_CAST && k == 3) {
+        if (is(a.kx, {-1, 0, 1}) && is(a.ky, {1, 2, 1})) return SWAR_SOBEL_DX;
+        if (is(a.kx, {1, 2, 1}) && is(a.ky, {-1, 0, 1})) return SWAR_SOBEL_DY;
+    }
+    return -1;
+}
+
 // Returns false when the call is not a shape this kernel covers.
 bool launch_sep_small_u8(const SepParams &p, hipStream_t st, int th) {
     StencilArgs a;
     if (!stencil_args(p, a)) return false;
-    const bool ok = p.ddepth == 0 ? launch_d<uint8_t>(a, p.nkx, st, th) : launch_d<int16_t>(a, p.nkx, st, th);
+    bool ok;
+    switch (swar_mode(a, p.nkx, p.ddepth)) {
+    case SWAR_G3: ok = launch_swar<SWAR_G3, uint8_t>(a, st, th); break;
+    case SWAR_G5: ok = launch_swar<SWAR_G5, uint8_t>(a, st, th); break;
+    case SWAR_SOBEL_DX: ok = launch_swar<SWAR_SOBEL_DX, int16_t>(a, st, th); break;
+    case SWAR_SOBEL_DY: ok = launch_swar<SWAR_SOBEL_DY, int16_t>(a, st, th); break;
+    default: ok = p.ddepth == 0 ? launch_d<uint8_t>(a, p.nkx, st, th) : launch_d<int16_t>(a, p.nkx, st, th);
+    }
     return ok && hipGetLastError() == hipSuccess;
 }
 

@@ -57,67 +57,76 @@ struct Staged {
 // row_of(r): source row of staged row r, or -1 for a row of `fill` words.
 // edge_pos(c): where byte c of a row comes from (the identity inside
 // [0, n), a border-mapped index outside, or -1 for the byte `fill_byte`).
-template <int TR, int PW2, typename RowOf, typename EdgePos>
-__device__ __attribute__((always_inline)) inline Staged<TR, PW2> stage_rows(const DImg &s, int n, int c0, uint32_t fill, uint32_t fill_byte,
-                                             uint32_t *lds, RowOf row_of, EdgePos edge_pos) {
+// EW: dwords at each end of a staged row that may cross the row's ends and
+// are still read by a thread with output in the row (left: the dwords
+// before byte 0; right: from the first dword that reaches past byte n - 1). Dwords wholly inside
+// the row are one aligned load each; the crossing ones (at most EW per end
+// and row, one per thread) are four byte loads from their mapped positions,
+// issued right after the dword loads so the workgroup waits once.
+template <int TR, int PW2, int EW, typename RowOf, typename EdgePos>
+__device__ __attribute__((always_inline)) inline Staged<TR, PW2> stage_rows(const DImg &s, int n, int c0, uint32_t fill,
+                                                                            uint32_t fill_byte, uint32_t *lds,
+                                                                            RowOf row_of, EdgePos edge_pos) {
     constexpr int TOT = TR * PW2;
     constexpr int NI = (TOT + kTileThreads - 1) / kTileThreads;
+    constexpr int SLOTS = TR * 2 * EW;
+    static_assert(SLOTS <= kTileThreads, "one edge slot per thread");
     uint32_t(*tile)[PW2] = reinterpret_cast<uint32_t(*)[PW2]>(lds);
     uint32_t *shift = lds + TOT;
     uint32_t v[NI];
-    // Whether any staged dword crosses a row end (the same for the whole
-    // workgroup). Inner tiles only issue aligned dword loads; edge tiles load
-    // every byte from its border-mapped position. Either way each thread
-    // issues all its loads before it waits for any, and no load sits in a
-    // branch of its own (the compiler would wait inside such a branch).
-    const bool edge_tile = c0 - 3 < 0 || c0 + 4 * PW2 > n;
-    if (!edge_tile) {
+    bool put[NI];
 #pragma unroll
-        for (int i = 0; i < NI; i++) {
-            const int idx = threadIdx.x + i * kTileThreads;
-            const int r = idx / PW2, k = idx - r * PW2;
-            const int sy = idx < TOT ? row_of(r) : -1;
-            const gptr<const uint8_t> row = grow<const uint8_t>(s, max(sy, 0));
-            const int c = c0 - (int)((uintptr_t)row & 3) + 4 * k;
-            v[i] = fill;
-            if (sy >= 0) v[i] = *(gptr<const uint32_t>)(row + c);
-        }
-    } else {
-        uint32_t eb[NI][4];
-        int q[NI][4];
+    for (int i = 0; i < NI; i++) {
+        const int idx = threadIdx.x + i * kTileThreads;
+        const int r = idx / PW2, k = idx - r * PW2;
+        const int sy = idx < TOT ? row_of(r) : -1;
+        const gptr<const uint8_t> row = grow<const uint8_t>(s, max(sy, 0));
+        const int c = c0 - (int)((uintptr_t)row & 3) + 4 * k;
+        const bool in = sy >= 0 && c >= 0 && c + 4 <= n;
+        put[i] = idx < TOT && (in || sy < 0);
+        v[i] = fill;
+        if (in) v[i] = *(gptr<const uint32_t>)(row + c);
+    }
+    // Edge slot of this thread: row r, left dword j or right dword j.
+    const int slot = threadIdx.x;
+    int er = -1, ek = 0;
+    uint32_t ev = 0;
+    if (slot < SLOTS) {
+        const int r = slot / (2 * EW), j = slot - r * 2 * EW;
+        const int sy = row_of(r);
+        if (sy >= 0) {
+            const gptr<const uint8_t> row = grow<const uint8_t>(s, sy);
+            const int a = (int)((uintptr_t)row & 3);
+            int k = j;
+            if (j >= EW) k = ((n - 4 - c0 + a) >> 2) + 1 + (j - EW);  // first dword past the row end, onwards
+            const int c = c0 - a + 4 * k;
+            const bool mine = k >= 0 && k < PW2 && (j < EW ? c < 0 : (c >= 0 && c + 4 > n));
+            if (mine) {
+                int q[4];
+                uint32_t b[4];
 #pragma unroll
-        for (int i = 0; i < NI; i++) {
-            const int idx = threadIdx.x + i * kTileThreads;
-            const int r = idx / PW2, k = idx - r * PW2;
-            const int sy = idx < TOT ? row_of(r) : -1;
-            const gptr<const uint8_t> row = grow<const uint8_t>(s, max(sy, 0));
-            const int c = c0 - (int)((uintptr_t)row & 3) + 4 * k;
+                for (int t = 0; t < 4; t++) q[t] = edge_pos(c + t);
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                q[i][j] = sy >= 0 ? edge_pos(c + j) : -2;
-                eb[i][j] = row[max(q[i][j], 0)];
+                for (int t = 0; t < 4; t++) b[t] = row[max(q[t], 0)];
+#pragma unroll
+                for (int t = 0; t < 4; t++) ev |= (q[t] >= 0 ? b[t] : fill_byte) << (8 * t);
+                er = r;
+                ek = k;
             }
-        }
-#pragma unroll
-        for (int i = 0; i < NI; i++) {
-            uint32_t w = 0;
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                w |= (q[i][j] >= 0 ? eb[i][j] : (q[i][j] == -1 ? fill_byte : (fill >> (8 * j)) & 255u)) << (8 * j);
-            v[i] = w;
         }
     }
 #pragma unroll
     for (int i = 0; i < NI; i++) {
         const int idx = threadIdx.x + i * kTileThreads;
-        if (idx < TOT) {
+        if (put[i]) {
             const int r = idx / PW2, k = idx - r * PW2;
             tile[r][k] = v[i];
-            if (k == 0) {
-                const int sy = row_of(r);
-                shift[r] = sy >= 0 ? (uint32_t)((uintptr_t)(s.p + (long)sy * s.stride) & 3) : 0u;
-            }
         }
+    }
+    if (er >= 0) tile[er][ek] = ev;
+    if (threadIdx.x < TR) {
+        const int sy = row_of(threadIdx.x);
+        shift[threadIdx.x] = sy >= 0 ? (uint32_t)((uintptr_t)(s.p + (long)sy * s.stride) & 3) : 0u;
     }
     __syncthreads();
     return Staged<TR, PW2>{tile, shift};
@@ -233,7 +242,7 @@ __device__ __attribute__((always_inline)) inline void stencil_tile(const Stencil
     const int x0 = bx * 256;
     const int y0 = by * 4 * TH;
     const int border = p.border;
-    const auto st = stage_rows<TR, PW2>(
+    const auto st = stage_rows<TR, PW2, HW + 2>(
         p.src, n, x0 - 4 * HW, 0u, 0u, lds, [&](int r) { return border_map_halo(y0 - R + r, H, border); },
         [&](int c) { return mapped_pos<CN>(c, W, border); });
     const int tx = tile_tx();
@@ -277,6 +286,149 @@ __device__ __attribute__((always_inline)) inline void stencil_tile(const Stencil
     }
 }
 
+// -------------------------------------------------- packed 16-bit stencils
+//
+// The stencils of the benchmark shapes in pairs of 16-bit lanes: GaussianBlur
+// u8 3x3 and 5x5 with sigma 0 (binomial taps 64 * [1 2 1] and
+// 16 * [1 4 6 4 1], which reduce to [1 2 1] and [1 4 6 4 1]; the full sum is
+// 4096 or 256 times the reduced one, so the reference's (sum + 2^15) >> 16 is
+// (v + 8) >> 4 or (v + 128) >> 8 exactly, and every partial sum stays below
+// 2^16), and Sobel 3x3 dx or dy to i16 with scale 1 (partial sums within
+// +-1020, exact in wrapping 16-bit arithmetic). A thread's 4 samples are
+// handled as two pairs, samples (0, 2) and (1, 3); a pair of source bytes
+// (b[e], b[e + 2]) comes from the staged dwords with one v_perm.
+
+enum SwarMode : int { SWAR_G3 = 0, SWAR_G5 = 1, SWAR_SOBEL_DX = 2, SWAR_SOBEL_DY = 3 };
+
+// The packed mode whose taps a u8 stencil call has exactly (k taps per
+// axis, destination depth ddepth), or -1 (host side, k_stencil.hip).
+int swar_mode(const StencilArgs &a, int k, int ddepth);
+
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+typedef int16_t i16x2 __attribute__((ext_vector_type(2)));
+
+template <int MODE>
+struct SwarTaps;
+template <>
+struct SwarTaps<SWAR_G3> {
+    static constexpr int K = 3;
+    static constexpr int h[3] = {1, 2, 1}, v[3] = {1, 2, 1};
+};
+template <>
+struct SwarTaps<SWAR_G5> {
+    static constexpr int K = 5;
+    static constexpr int h[5] = {1, 4, 6, 4, 1}, v[5] = {1, 4, 6, 4, 1};
+};
+template <>
+struct SwarTaps<SWAR_SOBEL_DX> {
+    static constexpr int K = 3;
+    static constexpr int h[3] = {-1, 0, 1}, v[3] = {1, 2, 1};
+};
+template <>
+struct SwarTaps<SWAR_SOBEL_DY> {
+    static constexpr int K = 3;
+    static constexpr int h[3] = {1, 2, 1}, v[3] = {-1, 0, 1};
+};
+
+// (b[e], b[e + 2]) as two zero-extended 16-bit lanes; byte e is relative to
+// the thread's own dword w[HW], so it lives at index e + 4 HW of w[].
+template <int HW, int NW>
+__device__ __attribute__((always_inline)) inline uint32_t byte_pair(const uint32_t (&w)[NW], int e) {
+    const int q = e + 4 * HW, i = q >> 2, o = q & 3;
+    if (o <= 1) return __builtin_amdgcn_perm(w[i], w[i], 0x0c000c00u | ((uint32_t)(o + 2) << 16) | (uint32_t)o);
+    return __builtin_amdgcn_perm(w[i + 1], w[i], 0x0c000c00u | ((uint32_t)(o + 2) << 16) | (uint32_t)o);
+}
+
+template <int MODE>
+__device__ __attribute__((always_inline)) inline uint32_t swar_mac(uint32_t acc, uint32_t x, int k) {
+    if (k == 0) return acc;
+    if constexpr (MODE == SWAR_SOBEL_DX || MODE == SWAR_SOBEL_DY) {
+        const i16x2 a = __builtin_bit_cast(i16x2, acc), b = __builtin_bit_cast(i16x2, x);
+        i16x2 r;
+        if (k == 1) r = a + b;
+        else if (k == -1) r = a - b;
+        else r = a + b * (int16_t)k;
+        return __builtin_bit_cast(uint32_t, r);
+    } else {
+        const u16x2 a = __builtin_bit_cast(u16x2, acc), b = __builtin_bit_cast(u16x2, x);
+        const u16x2 r = k == 1 ? a + b : a + b * (uint16_t)k;
+        return __builtin_bit_cast(uint32_t, r);
+    }
+}
+
+template <int MODE, int CN, int TH, typename D>
+__device__ __attribute__((always_inline)) inline void stencil_swar_tile(const StencilArgs &p, int bx, int by,
+                                                                        uint32_t *lds) {
+    using T = SwarTaps<MODE>;
+    constexpr int K = T::K;
+    using S = StencilShape<K, CN, TH>;
+    constexpr int R = S::R, HW = S::HW, NW = S::NW, PW2 = S::PW2, TR = S::TR;
+    const int W = p.src.w, H = p.src.h;
+    const int n = W * CN;
+    const int x0 = bx * 256;
+    const int y0 = by * 4 * TH;
+    const int border = p.border;
+    const auto st = stage_rows<TR, PW2, HW + 2>(
+        p.src, n, x0 - 4 * HW, 0u, 0u, lds, [&](int r) { return border_map_halo(y0 - R + r, H, border); },
+        [&](int c) { return mapped_pos<CN>(c, W, border); });
+    const int tx = tile_tx();
+    const int x = x0 + tx * 4;
+    const int yt = tile_ty() * TH;
+    if (y0 + yt >= H) return;  // the same for the whole wave
+    uint32_t he[TH + 2 * R], ho[TH + 2 * R];  // horizontal sums of samples (0, 2) and (1, 3)
+#pragma unroll
+    for (int r = 0; r < TH + 2 * R; r++) {
+        const uint32_t a = st.shift[yt + r];
+        uint32_t d[NW + 1];
+#pragma unroll
+        for (int k = 0; k <= NW; k++) d[k] = st.tile[yt + r][tx + k];
+        uint32_t w[NW];
+#pragma unroll
+        for (int k = 0; k < NW; k++) w[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], a);
+        uint32_t se = 0, so = 0;
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+            se = swar_mac<MODE>(se, byte_pair<HW>(w, (i - R) * CN), T::h[i]);
+            so = swar_mac<MODE>(so, byte_pair<HW>(w, 1 + (i - R) * CN), T::h[i]);
+        }
+        he[r] = se;
+        ho[r] = so;
+    }
+#pragma unroll
+    for (int t = 0; t < TH; t++) {
+        const int y = y0 + yt + t;
+        if (y >= H) break;  // uniform across the wave
+        uint32_t ve = 0, vo = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            ve = swar_mac<MODE>(ve, he[t + k], T::v[k]);
+            vo = swar_mac<MODE>(vo, ho[t + k], T::v[k]);
+        }
+        D *row = reinterpret_cast<D *>(p.dst.p + (long)y * p.dst.stride);
+        if constexpr (MODE == SWAR_G3 || MODE == SWAR_G5) {
+            constexpr uint16_t add = MODE == SWAR_G3 ? 8 : 128, sh = MODE == SWAR_G3 ? 4 : 8;
+            const u16x2 fe = (__builtin_bit_cast(u16x2, ve) + add) >> sh;
+            const u16x2 fo = (__builtin_bit_cast(u16x2, vo) + add) >> sh;
+            // bytes: sample 0 = fe.lo, 1 = fo.lo, 2 = fe.hi, 3 = fo.hi
+            const uint32_t v4 = __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, fo), __builtin_bit_cast(uint32_t, fe),
+                                                      0x06020400u);
+            store_row_u8(reinterpret_cast<uint8_t *>(row), x0, min(x0 + 256, n), v4);
+        } else {
+            // i16 samples: dword 0 = (sample 0, sample 1), dword 1 = (2, 3)
+            const uint32_t lo = __builtin_amdgcn_perm(vo, ve, 0x05040100u);
+            const uint32_t hi = __builtin_amdgcn_perm(vo, ve, 0x07060302u);
+            if (x >= n) continue;
+            const gptr<int16_t> out = gp(reinterpret_cast<int16_t *>(row) + x);
+            if (x + 4 <= n && ((uintptr_t)out & 7) == 0) {
+                *(gptr<u32x2>)out = u32x2{lo, hi};
+            } else {
+                const int16_t vals[4] = {(int16_t)lo, (int16_t)(lo >> 16), (int16_t)hi, (int16_t)(hi >> 16)};
+                for (int j = 0; j < 4 && x + j < n; j++) out[j] = vals[j];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------- morphology
 //
 // u8 elements up to 7 x 7 in a KW x KW frame with a centred anchor. The
@@ -307,7 +459,7 @@ __device__ __attribute__((always_inline)) inline void morph_small_tile(const Mor
     const int H = s.h, n = s.w * CN;
     const int x0 = bx * 256, y0 = by * 4 * TH;
     const uint32_t id = MAX ? 0u : 0xffffffffu;
-    const auto st = stage_rows<TR, PW2>(
+    const auto st = stage_rows<TR, PW2, (NWT - HL / 4 + 1 > HL / 4 + 1 ? NWT - HL / 4 + 1 : HL / 4 + 1)>(
         s, n, x0 - HL, id, id & 255u, lds,
         [&](int r) {
             const int sy = y0 - A + r;

@@ -61,7 +61,7 @@ def per_launch_ms(torch, fn, n, stream):
 
 
 def bench_batch(args, torch, va):
-    """512 images of uneven sizes and mixed ops: one persistent launch
+    """512 images of uneven sizes and mixed ops: one graph launch
     (graph replay, autotuned launch shape) against 512 plain C-ABI calls."""
     from visar_amd import autotune, batch
     stream = torch.cuda.current_stream()
@@ -105,7 +105,7 @@ def bench_batch(args, torch, va):
     tiles, wgs = b.info()
     gbs = nbytes / (ms * 1e-3) / 1e9
     line = {
-        "metric": "images/s of the 512-image mixed batch (one persistent launch)",
+        "metric": "images/s of the 512-image mixed batch (one graph launch)",
         "value": round(512 / (ms * 1e-3), 1),
         "unit": "images/s",
         "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": round(ms, 4),

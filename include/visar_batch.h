@@ -1,14 +1,15 @@
 /*
  * visar_batch.h -- many images of uneven sizes and mixed operations in one
- * persistent GPU launch (no counterpart in the reference, whose hot path
- * runs one CPU call per image).
+ * GPU launch (no counterpart in the reference, whose hot path runs one CPU
+ * call per image).
  *
  * Each item is one of the hot-path calls on device buffers; the result of an
  * item is bit for bit the result of the matching visar.h call. A plan
  * validates the items (with the checks of the matching visar.h call), splits
- * every image into tiles and uploads the tile jobs; a run resets the queue and
- * launches one persistent kernel whose workgroups pull tiles from the queue
- * until it is empty. Runs after the first replay a captured HIP graph.
+ * every image into tiles, groups the items by tile variant, cuts each image's
+ * tiles into work units of about equal bytes and uploads them; a run
+ * launches one kernel per variant in which every workgroup runs one unit.
+ * Runs after the first replay a captured HIP graph of those launches.
  */
 #ifndef VISAR_BATCH_H
 #define VISAR_BATCH_H

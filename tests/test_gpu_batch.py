@@ -1,4 +1,4 @@
-"""Batched persistent launch vs plain per-call GPU results vs the oracle (GPU).
+"""Batched launch vs plain per-call GPU results vs the oracle (GPU).
 
 The seeded mixed workload (sides U{64..2048}, 1 or 3 channels, ops GaussianBlur
 3/5/7, resize AREA 1/2, resize LINEAR 1.5x, warpAffine, dilate 5x5 rect) runs

@@ -268,6 +268,49 @@ def test_resize_area_cases(va, orc):
             check(va, orc, "resize", a, (dw, dh), 3)
 
 
+def test_resize_linear_scales_near_tile_limits(va, orc):
+    """Linear (and area-up) scales on both sides of the LDS tile's source-box
+    limits, 16-row tile bands with ragged ends, and 256-sample bands with a
+    ragged last tile: the staged tiles and the direct-gather tiles alike."""
+    for cn in (1, 3, 4):
+        a = natural(83, 301, cn, 5 + cn)
+        for (dw, dh) in [(302, 84), (451, 124), (2107, 581), (151, 42), (150, 41), (181, 50), (120, 33), (301, 250),
+                         (700, 83), (3, 900), (900, 2)]:
+            for interp in (1, 3):
+                if interp == 3 and dw <= 301 and dh <= 83:
+                    continue  # area shrink, not the linear taps
+                check(va, orc, "resize", a, (dw, dh), interp)
+
+
+def test_resize_linear_misaligned_rows(va, orc):
+    """Sources and destinations at every byte offset mod 4 with odd row
+    strides: every output byte equals the oracle and the bytes around the
+    destination view stay untouched."""
+    for cn in (1, 3):
+        h, w = 37, 301
+        a = natural(h, w, cn, 77 + cn)
+        for (dw, dh) in [(452, 56), (180, 22)]:
+            want = orc.resize(a, (dw, dh), 1)
+            for off in range(4):
+                for pad in (0, 1, 5):
+                    pitch = w * cn + pad
+                    sbig = torch.zeros(h * pitch + 8, dtype=torch.uint8, device="cuda")
+                    sv = sbig[off:off + h * pitch].view(h, pitch)[:, :w * cn]
+                    sv.copy_(dev(a.reshape(h, w * cn)))
+                    sv = sv.view(h, w, cn) if cn == 3 else sv
+                    dp = dw * cn + pad
+                    o = (off + 1) % 4
+                    dbig = torch.full((dh * dp + 8,), 77, dtype=torch.uint8, device="cuda")
+                    dv = dbig[o:o + dh * dp].view(dh, dp)[:, :dw * cn]
+                    dv = dv.view(dh, dw, cn) if cn == 3 else dv
+                    keep = torch.ones_like(dbig, dtype=torch.bool)
+                    for r in range(dh):
+                        keep[o + r * dp:o + r * dp + dw * cn] = False
+                    va.resize(sv, (dw, dh), 1, dst=dv)
+                    assert_same(dv.cpu().numpy(), want, f"linear {w}x{h}->{dw}x{dh} cn{cn} off{off} pad{pad}")
+                    assert bool((dbig[keep] == 77).all()), "resize wrote outside its view"
+
+
 def test_resize_linear_4k_half_config(va, orc):
     from visar_amd import synth
     a = synth.uniform_bytes(5, (2160, 3840))

@@ -123,12 +123,12 @@ def batch_signature(specs):
     return ",".join(f"{k}:{counts[k]}" for k in sorted(counts))
 
 
-BATCH_SPACE = {"batch_stencil_rows": [1, 2], "batch_wg_per_cu": [1, 2, 3, 4, 8]}
+BATCH_SPACE = {"batch_stencil_rows": [1, 2], "batch_unit_kb": [4, 8, 16, 32, 64]}
 STENCIL_SPACE = {"stencil_rows": [1, 2, 4]}
 
 
 def tune_batch(batch_obj, items, specs, force=False):
-    """Tunes the persistent batch launch of `items` (visar_amd.batch.Item list)."""
+    """Tunes the batch launch of `items` (visar_amd.batch.Item list)."""
     def prepare(_setting):
         batch_obj.plan(items)
 

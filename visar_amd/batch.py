@@ -1,5 +1,5 @@
 """Batched execution (include/visar_batch.h): many images of uneven sizes and
-mixed operations in one persistent launch, plus the seeded mixed-op workload
+mixed operations in one graph launch, plus the seeded mixed-op workload
 (512 images, sides U{64..2048}, 1 or 3 channels, one op each)."""
 import ctypes
 

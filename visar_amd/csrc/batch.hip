@@ -8,14 +8,12 @@
 // code are only what the variant needs, and the groups' kernels run back to
 // back in one launch (one HIP graph). Each job's tiles are cut into work
 // units of about the same number of bytes (a run of consecutive tiles of one
-// job), and a group's unit list is cut into 8 contiguous segments of equal
-// bytes, one per XCD: workgroup w is dispatched to XCD w % 8 and first drains
-// that XCD's segment (neighbouring tiles of an image, and their halo rows,
-// stay in one L2), then steals units from the other segments until all are
-// done. Every queue is a counter bumped with atomicAdd; a workgroup claims its
-// next unit before it works through the current one, so the atomic's latency
-// hides behind the work. A small kernel clears the counters before each run
-// (no hipMemsetAsync, so the run can be captured in a HIP graph).
+// job), and every unit is one workgroup of its group's launch, so the
+// hardware dispatcher balances the load and no workgroup waits on a shared
+// counter. Workgroup b is dispatched to XCD b % 8; it takes unit
+// (b % 8) * per + b / 8 of its group, so each XCD works in order through one
+// contiguous eighth of the group's units (neighbouring tiles of an image,
+// and the halo rows they share, stay in that XCD's L2).
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -43,6 +41,7 @@ enum JobKind : int {
     JK_AREA_FRAC = 8,
     JK_STENCIL_SWAR = 9,
     JK_AFFINE_BOX = 10,
+    JK_LINEAR_BOX = 11,
 };
 
 struct ResizeArgs {
@@ -72,80 +71,61 @@ struct BatchUnit {
     int job, first, count, pad;
 };
 
-// Drains one group's queues: seg[0..8] are the group's 8 segment bounds in
-// the unit list, counters[0..7] its queue counters.
+// Runs the unit of this workgroup: `count` units start at `units`.
 template <typename Tile>
-__device__ __attribute__((always_inline)) inline void drain(const BatchJob *__restrict__ jobs, const BatchUnit *__restrict__ units,
-                                                            const int *__restrict__ seg, int *counters, int *next, Tile tile) {
-    const int home = blockIdx.x % kXcds;
-    for (int k = 0; k < kXcds; k++) {
-        const int q = (home + k) % kXcds;
-        const int base = seg[q], size = seg[q + 1] - base;
-        if (size <= 0) continue;
-        if (threadIdx.x == 0) *next = atomicAdd(&counters[q], 1);
-        __syncthreads();
-        int i = __builtin_amdgcn_readfirstlane(*next);
-        __syncthreads();
-        while (i < size) {
-            int claim = 0;
-            if (threadIdx.x == 0) claim = atomicAdd(&counters[q], 1);  // the next unit, claimed early
-            const BatchUnit u = units[base + i];
-            const BatchJob &j = jobs[u.job];
-            const int tiles_x = j.tiles_x;
-            int by = u.first / tiles_x, bx = u.first - by * tiles_x;
-            for (int c = 0; c < u.count; c++) {
-                tile(j, bx, by);
-                __syncthreads();
-                if (++bx == tiles_x) {
-                    bx = 0;
-                    by++;
-                }
-            }
-            if (threadIdx.x == 0) *next = claim;
-            __syncthreads();
-            i = __builtin_amdgcn_readfirstlane(*next);
-            __syncthreads();
+__device__ __attribute__((always_inline)) inline void run_unit(const BatchJob *__restrict__ jobs, const BatchUnit *__restrict__ units,
+                                                               int count, Tile tile) {
+    const int per = (count + kXcds - 1) / kXcds;
+    const int i = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+    if (i >= count) return;
+    const BatchUnit u = units[i];
+    const BatchJob &j = jobs[u.job];
+    const int tiles_x = j.tiles_x;
+    int by = u.first / tiles_x, bx = u.first - by * tiles_x;
+    for (int c = 0; c < u.count; c++) {
+        if (c) __syncthreads();  // the previous tile's LDS reads are done
+        tile(j, bx, by);
+        if (++bx == tiles_x) {
+            bx = 0;
+            by++;
         }
     }
 }
 
-#define BATCH_ARGS const BatchJob *__restrict__ jobs, const BatchUnit *__restrict__ units, const int *__restrict__ seg, int *counters
+#define BATCH_ARGS const BatchJob *__restrict__ jobs, const BatchUnit *__restrict__ units, int count
 
 template <int K, int CN, int TH>
 __global__ __launch_bounds__(kTileThreads) void batch_stencil(BATCH_ARGS) {
     __shared__ uint32_t lds[StencilShape<K, CN, TH>::LDS_WORDS];
-    __shared__ int next;
-    drain(jobs, units, seg, counters, &next,
-          [&](const BatchJob &j, int bx, int by) { stencil_tile<K, CN, TH, uint8_t>(j.st, bx, by, lds); });
+    run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) { stencil_tile<K, CN, TH, uint8_t>(j.st, bx, by, lds); });
 }
 
 template <int MODE, int CN, int TH>
 __global__ __launch_bounds__(kTileThreads) void batch_stencil_swar(BATCH_ARGS) {
     __shared__ uint32_t lds[StencilShape<SwarTaps<MODE>::K, CN, TH>::LDS_WORDS];
-    __shared__ int next;
-    drain(jobs, units, seg, counters, &next,
-          [&](const BatchJob &j, int bx, int by) { stencil_swar_tile<MODE, CN, TH, uint8_t>(j.st, bx, by, lds); });
+    run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) { stencil_swar_tile<MODE, CN, TH, uint8_t>(j.st, bx, by, lds); });
 }
 
 __global__ __launch_bounds__(kTileThreads) void batch_affine_box(BATCH_ARGS) {
     __shared__ uint32_t lds[kWarpLdsWords];
-    __shared__ int next;
-    drain(jobs, units, seg, counters, &next,
-          [&](const BatchJob &j, int bx, int by) { affine_box_tile(j.af, bx, by, lds); });
+    run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) { affine_box_tile(j.af, bx, by, lds); });
 }
 
 template <int K, int CN, bool MAX>
 __global__ __launch_bounds__(kTileThreads) void batch_morph(BATCH_ARGS) {
     __shared__ uint32_t lds[MorphShape<K, CN, 2>::LDS_WORDS];
-    __shared__ int next;
-    drain(jobs, units, seg, counters, &next,
-          [&](const BatchJob &j, int bx, int by) { morph_small_tile<K, CN, 2, MAX>(j.mo, bx, by, lds); });
+    run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) { morph_small_tile<K, CN, 2, MAX>(j.mo, bx, by, lds); });
+}
+
+__global__ __launch_bounds__(kTileThreads) void batch_linear(BATCH_ARGS) {
+    __shared__ uint32_t lds[kLinLdsWords];
+    run_unit(jobs, units, count,
+             [&](const BatchJob &j, int bx, int by) { linear_box_tile(j.rs.s, j.rs.d, j.rs.xt, j.rs.yt, bx, by, lds); });
 }
 
 template <int KIND>
 __global__ __launch_bounds__(kTileThreads) void batch_other(BATCH_ARGS) {
-    __shared__ int next;
-    drain(jobs, units, seg, counters, &next, [&](const BatchJob &j, int bx, int by) {
+    run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) {
         if constexpr (KIND == JK_AFFINE_LIN4) affine_lin4_tile(j.af, bx, by);
         else if constexpr (KIND == JK_AFFINE_PX) affine_px_tile<uint8_t>(j.af, bx, by);
         else if constexpr (KIND == JK_HALF_C1) half_c1_tile(j.rs.s, j.rs.d, bx, by);
@@ -154,10 +134,6 @@ __global__ __launch_bounds__(kTileThreads) void batch_other(BATCH_ARGS) {
         else if constexpr (KIND == JK_AREA_INT) area_int_tile(j.rs.s, j.rs.d, j.rs.ix, j.rs.iy, j.rs.scale, bx, by);
         else area_frac_tile(j.rs.s, j.rs.d, j.rs.xe, j.rs.xs, j.rs.ye, j.rs.ys, bx, by);
     });
-}
-
-__global__ void batch_reset(int *counters, int n) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) counters[i] = 0;
 }
 
 using GroupKernel = void (*)(BATCH_ARGS);
@@ -196,6 +172,7 @@ GroupKernel group_kernel(int kind, int variant) {
     case JK_HALF_C1: return batch_other<JK_HALF_C1>;
     case JK_HALF: return batch_other<JK_HALF>;
     case JK_LINEAR: return batch_other<JK_LINEAR>;
+    case JK_LINEAR_BOX: return batch_linear;
     case JK_AREA_INT: return batch_other<JK_AREA_INT>;
     default: return batch_other<JK_AREA_FRAC>;
     }
@@ -203,8 +180,10 @@ GroupKernel group_kernel(int kind, int variant) {
 
 struct Group {
     int kind, variant;
-    int workgroups;
+    int first, count;  // the group's units
 };
+
+int group_workgroups(const Group &g) { return (g.count + kXcds - 1) / kXcds * kXcds; }
 
 }  // namespace
 
@@ -216,11 +195,10 @@ struct visar_batch {
     std::vector<BatchJob> jobs;
     std::vector<BatchUnit> units;
     std::vector<Group> groups;
-    std::vector<int> seg;  // 9 bounds per group
     size_t tiles = 0;
     int workgroups = 0;
-    void *dev = nullptr;  // jobs | units | segments | counters | tables
-    size_t units_off = 0, seg_off = 0, cnt_off = 0;
+    void *dev = nullptr;  // jobs | units | tables
+    size_t units_off = 0;
     size_t dev_bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -393,6 +371,10 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
             }
             for (uint32_t y = 0; y < it.dst.height; y++)
                 t[it.dst.width + y] = linear_tap_u8(y, it.src.height, it.dst.height, area_up, false);
+            if (linear_box_fits(t.data(), (int)it.dst.width, (int)it.src.channels, t.data() + it.dst.width, (int)it.dst.height)) {
+                j.kind = JK_LINEAR_BOX;
+                j.ntiles = j.tiles_x * ((d.h + kLinTH - 1) / kLinTH);
+            }
             pl.table.resize(t.size() * sizeof(LinTap));
             std::memcpy(pl.table.data(), t.data(), pl.table.size());
             pl.table_kind = 1;
@@ -462,82 +444,36 @@ int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, si
         if (pl[a].job.kind != pl[c].job.kind) return pl[a].job.kind < pl[c].job.kind;
         return pl[a].job.variant < pl[c].job.variant;
     });
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-    }
-    const int per_cu = tuning().batch_wg_per_cu ? tuning().batch_wg_per_cu : 4;
-    const long wg_max = (long)cus * per_cu;
     // Work units: runs of consecutive tiles of one job, each about `target`
-    // bytes of input + output, small enough that every workgroup gets many
-    // (the tail balances) and large enough that a claim's atomic is rare.
-    double total_bytes = 0;
-    std::vector<double> job_bytes(n);
-    for (size_t k = 0; k < n; k++) {
-        const visar_batch_item &it = items[order[k]];
-        job_bytes[k] = (double)it.src.width * it.src.height * it.src.channels +
-                       (double)it.dst.width * it.dst.height * it.dst.channels;
-        total_bytes += job_bytes[k];
-    }
-    const double target = std::min(256.0 * 1024, std::max(16.0 * 1024, total_bytes / ((double)wg_max * 16)));
+    // bytes of input + output: large enough to spread a workgroup's start-up
+    // over several tiles, small enough that the last units are short.
+    const double target = 1024.0 * (tuning().batch_unit_kb ? tuning().batch_unit_kb : 16);
     b->units.clear();
-    std::vector<double> unit_bytes;
+    b->groups.clear();
     long tiles = 0;
     for (size_t k = 0; k < n; k++) {
-        const int nt = pl[order[k]].job.ntiles;
-        const double tb = job_bytes[k] / std::max(1, nt);
+        const visar_batch_item &it = items[order[k]];
+        const BatchJob &j = pl[order[k]].job;
+        if (b->groups.empty() || b->groups.back().kind != j.kind || b->groups.back().variant != j.variant)
+            b->groups.push_back(Group{j.kind, j.variant, (int)b->units.size(), 0});
+        const double bytes = (double)it.src.width * it.src.height * it.src.channels +
+                             (double)it.dst.width * it.dst.height * it.dst.channels;
+        const int nt = j.ntiles;
+        const double tb = bytes / std::max(1, nt);
         const int per = (int)std::max(1.0, std::min((double)nt, std::floor(target / std::max(tb, 1.0))));
-        for (int f = 0; f < nt; f += per) {
-            const int c = std::min(per, nt - f);
-            b->units.push_back(BatchUnit{(int)k, f, c, 0});
-            unit_bytes.push_back(tb * c);
-        }
+        for (int f = 0; f < nt; f += per) b->units.push_back(BatchUnit{(int)k, f, std::min(per, nt - f), 0});
+        b->groups.back().count = (int)b->units.size() - b->groups.back().first;
         tiles += nt;
     }
     b->tiles = (size_t)tiles;
     const size_t nu = b->units.size();
-    // Groups: runs of jobs with one (kind, variant); each gets 8 segments of
-    // equal bytes over its units, and a launch width that fits its units.
-    b->groups.clear();
-    b->seg.clear();
     b->workgroups = 0;
-    {
-        size_t u = 0;
-        while (u < nu) {
-            const BatchJob &j0 = pl[order[b->units[u].job]].job;
-            size_t e = u;
-            double gbytes = 0;
-            while (e < nu) {
-                const BatchJob &je = pl[order[b->units[e].job]].job;
-                if (je.kind != j0.kind || je.variant != j0.variant) break;
-                gbytes += unit_bytes[e];
-                e++;
-            }
-            std::vector<int> sg(kXcds + 1, (int)e);
-            sg[0] = (int)u;
-            double acc = 0;
-            int q = 1;
-            for (size_t x = u; x < e && q < kXcds; x++) {
-                acc += unit_bytes[x];
-                while (q < kXcds && acc >= gbytes * q / kXcds) sg[q++] = (int)x + 1;
-            }
-            b->seg.insert(b->seg.end(), sg.begin(), sg.end());
-            int wg = (int)std::max(1L, std::min<long>(wg_max, (long)(e - u)));
-            if (wg > kXcds) wg -= wg % kXcds;  // every XCD's home segment is served
-            b->groups.push_back(Group{j0.kind, j0.variant, wg});
-            b->workgroups += wg;
-            u = e;
-        }
-    }
-    const size_t ng = b->groups.size();
-    // Device arena: jobs | units | segment bounds | counters | tables.
+    for (const Group &g : b->groups) b->workgroups += group_workgroups(g);
+    // Device arena: jobs | units | tables.
     auto up256 = [](size_t v) { return (v + 255) & ~(size_t)255; };
     const size_t jobs_bytes = up256((n ? n : 1) * sizeof(BatchJob));
     const size_t units_off = jobs_bytes;
-    const size_t seg_off = units_off + up256((nu ? nu : 1) * sizeof(BatchUnit));
-    const size_t cnt_off = seg_off + up256((ng ? ng : 1) * (kXcds + 1) * sizeof(int));
-    size_t total = cnt_off + up256((ng ? ng : 1) * kXcds * sizeof(int));
+    size_t total = units_off + up256((nu ? nu : 1) * sizeof(BatchUnit));
     std::vector<size_t> tab_at(n, 0);
     for (size_t i = 0; i < n; i++) {
         if (!pl[i].table.empty()) {
@@ -560,8 +496,6 @@ int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, si
         b->dev_bytes = total;
     }
     b->units_off = units_off;
-    b->seg_off = seg_off;
-    b->cnt_off = cnt_off;
     uint8_t *base = static_cast<uint8_t *>(b->dev);
     std::vector<uint8_t> host(total, 0);
     b->jobs.clear();
@@ -584,7 +518,6 @@ int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, si
     }
     if (n) std::memcpy(host.data(), b->jobs.data(), n * sizeof(BatchJob));
     if (nu) std::memcpy(host.data() + units_off, b->units.data(), nu * sizeof(BatchUnit));
-    std::memcpy(host.data() + seg_off, b->seg.data(), b->seg.size() * sizeof(int));
     hipStream_t st = current_stream();
     if (hipMemcpyAsync(b->dev, host.data(), total, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -594,17 +527,11 @@ int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, si
 
 static int launch_batch(visar_batch *b, hipStream_t st) {
     uint8_t *base = static_cast<uint8_t *>(b->dev);
-    const int *seg = reinterpret_cast<const int *>(base + b->seg_off);
-    int *counters = reinterpret_cast<int *>(base + b->cnt_off);
-    const int ng = (int)b->groups.size();
-    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(256), 0, st, counters, ng * kXcds);
     const BatchJob *jobs = reinterpret_cast<const BatchJob *>(base);
     const BatchUnit *units = reinterpret_cast<const BatchUnit *>(base + b->units_off);
-    for (int g = 0; g < ng; g++) {
-        const Group &gr = b->groups[g];
-        hipLaunchKernelGGL(group_kernel(gr.kind, gr.variant), dim3((unsigned)gr.workgroups), dim3(kTileThreads), 0, st,
-                           jobs, units, seg + g * (kXcds + 1), counters + g * kXcds);
-    }
+    for (const Group &g : b->groups)
+        hipLaunchKernelGGL(group_kernel(g.kind, g.variant), dim3((unsigned)group_workgroups(g)), dim3(kTileThreads), 0, st,
+                           jobs, units + g.first, g.count);
     return hipGetLastError() == hipSuccess ? OK : INTERNAL;
 }
 

@@ -697,7 +697,8 @@ int visar_resize(const visar_image *src, const visar_image *dst, int interp) {
         if (!dev) return fail(OUT_OF_MEMORY, "OutOfMemory");
         if (hipMemcpyAsync(dev, tabs.data(), tabs.size() * sizeof(LinTap), hipMemcpyHostToDevice, st) != hipSuccess)
             return fail(INTERNAL, "CopyFailed");
-        return call.finish(launch_linear_u8(s, d, dev, dev + dst->width, st));
+        const bool box = linear_box_fits(tabs.data(), (int)dst->width, (int)src->channels, tabs.data() + dst->width, (int)dst->height);
+        return call.finish(launch_linear_u8(s, d, dev, dev + dst->width, st, box));
     }
     // INTER_AREA shrinking
     if (exact_half) return call.finish(launch_half_u8(s, d, st));

@@ -265,6 +265,11 @@ __global__ __launch_bounds__(kThreads) void linear_u8(DImg s, DImg d, const LinT
     linear_tile(s, d, xt, yt, blockIdx.x, blockIdx.y);
 }
 
+__global__ __launch_bounds__(kThreads) void linear_box_u8(DImg s, DImg d, const LinTap *xt, const LinTap *yt) {
+    __shared__ uint32_t lds[kLinLdsWords];
+    linear_box_tile(s, d, xt, yt, blockIdx.x, blockIdx.y, lds);
+}
+
 __global__ __launch_bounds__(kThreads) void area_frac_u8(DImg s, DImg d, const AreaEntry *xt, const int32_t *xs,
                                                          const AreaEntry *yt, const int32_t *ys) {
     area_frac_tile(s, d, xt, xs, yt, ys, blockIdx.x, blockIdx.y);
@@ -389,8 +394,13 @@ int launch_nearest(const DImg &s, const DImg &d, size_t elem, hipStream_t st) {
     return launched();
 }
 
-int launch_linear_u8(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, hipStream_t st) {
-    hipLaunchKernelGGL(linear_u8, tiles256(d), dim3(kThreads), 0, st, s, d, xt, yt);
+int launch_linear_u8(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, hipStream_t st, bool box) {
+    if (box) {
+        const dim3 grid((unsigned)(((long)d.w * d.cn + 255) / 256), (unsigned)((d.h + kLinTH - 1) / kLinTH));
+        hipLaunchKernelGGL(linear_box_u8, grid, dim3(kThreads), 0, st, s, d, xt, yt);
+    } else {
+        hipLaunchKernelGGL(linear_u8, tiles256(d), dim3(kThreads), 0, st, s, d, xt, yt);
+    }
     return launched();
 }
 

@@ -9,7 +9,7 @@ namespace visar {
 struct Tuning {
     int stencil_rows = 0;        // plain stencils: output rows per thread (1, 2, 4)
     int batch_stencil_rows = 0;  // batched stencils: output rows per thread (1, 2)
-    int batch_wg_per_cu = 0;     // persistent workgroups per compute unit
+    int batch_unit_kb = 0;       // batched calls: KiB of input + output per workgroup
 };
 Tuning &tuning();
 bool set_tuning(const char *key, int value);
@@ -84,7 +84,18 @@ struct LinTap {
     int32_t o0, o1;
     int32_t w0, w1;
 };
-int launch_linear_u8(const DImg &s, const DImg &d, const LinTap *xt_dev, const LinTap *yt_dev, hipStream_t st);
+int launch_linear_u8(const DImg &s, const DImg &d, const LinTap *xt_dev, const LinTap *yt_dev, hipStream_t st, bool box);
+
+// Tiles of the LDS-staged linear resize: 256 output samples x kLinTH rows,
+// whose source box is at most kLinRows rows of kLinWords dwords.
+constexpr int kLinTH = 16;
+constexpr int kLinRows = 18;    // every upscale fits
+constexpr int kLinWords = 132;  // scales down to 1/2 fit
+constexpr int kLinLdsWords = kLinRows * kLinWords + kLinRows * 128;
+// True when the tables (x taps in samples, premultiplied by cn) suit the
+// LDS tiles: taps that never decrease, o0 <= o1, and every tile's box within
+// the limits above.
+bool linear_box_fits(const LinTap *xt, int dw, int cn, const LinTap *yt, int dh);
 
 // resize.zig:470-524 area tables: entries grouped per destination index,
 // start offsets in `xstart` / `ystart` (length dsize + 1).

@@ -1,6 +1,7 @@
 // tables.cpp -- host-side coefficient tables (built with -ffp-contract=off).
 #include "tables.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -20,7 +21,7 @@ struct Knob {
 const Knob kKnobs[] = {
     {"stencil_rows", &Tuning::stencil_rows, 0, 4},
     {"batch_stencil_rows", &Tuning::batch_stencil_rows, 0, 2},
-    {"batch_wg_per_cu", &Tuning::batch_wg_per_cu, 0, 32},
+    {"batch_unit_kb", &Tuning::batch_unit_kb, 0, 1024},
 };
 }  // namespace
 
@@ -219,6 +220,28 @@ LinTap linear_tap_u8(long d, uint32_t ssize, uint32_t dsize, bool area, bool pin
     t.o0 = (int32_t)(sx < 0 ? 0 : sx > last ? last : sx);
     t.o1 = (int32_t)(sx + 1 < 0 ? 0 : sx + 1 > last ? last : sx + 1);
     return t;
+}
+
+bool linear_box_fits(const LinTap *xt, int dw, int cn, const LinTap *yt, int dh) {
+    auto mono = [](const LinTap *t, int n) {
+        for (int i = 0; i < n; i++) {
+            if (t[i].o0 > t[i].o1) return false;
+            if (i && (t[i].o0 < t[i - 1].o0 || t[i].o1 < t[i - 1].o1)) return false;
+        }
+        return true;
+    };
+    if (dw <= 0 || dh <= 0 || cn <= 0 || !mono(xt, dw) || !mono(yt, dh)) return false;
+    const long n = (long)dw * cn;
+    for (long e = 0; e < n; e += 256) {
+        const long last = std::min(n, e + 256) - 1;
+        const long span = (long)xt[last / cn].o1 + cn - 1 - xt[e / cn].o0 + 1;
+        if ((span + 6) / 4 > kLinWords) return false;
+    }
+    for (int y = 0; y < dh; y += kLinTH) {
+        const int last = std::min(dh, y + kLinTH) - 1;
+        if (yt[last].o1 - yt[y].o0 + 1 > kLinRows) return false;
+    }
+    return true;
 }
 
 void area_tab(uint32_t ssize, uint32_t dsize, double scale, std::vector<AreaEntry> &e, std::vector<int32_t> &start) {

@@ -970,6 +970,16 @@ __device__ __attribute__((always_inline)) inline void half_c1_tile(const DImg &s
 
 // resize_u8.zig:651-711: H = (s[o0]*w0 + s[o1]*w1) >> 4 as i16 per source
 // row, out = sat_u8((((H0*b0) >> 16) + ((H1*b1) >> 16) + 2) >> 2).
+__device__ __attribute__((always_inline)) inline int32_t lin_h(uint32_t s0, uint32_t s1, int32_t w0, int32_t w1) {
+    return (int16_t)(((int32_t)s0 * w0 + (int32_t)s1 * w1) >> 4);
+}
+__device__ __attribute__((always_inline)) inline uint32_t lin_v(int32_t h0, int32_t h1, int32_t b0, int32_t b1) {
+    const int32_t t = ((h0 * b0) >> 16) + ((h1 * b1) >> 16);
+    const int32_t v = (t + 2) >> 2;
+    return (uint32_t)(v < 0 ? 0 : v > 255 ? 255 : v);
+}
+
+// Direct gathers: 256 output samples x 4 rows.
 __device__ __attribute__((always_inline)) inline void linear_tile(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, int bx, int by) {
     const int y = by * 4 + tile_ty();
     const int cn = d.cn;
@@ -982,11 +992,106 @@ __device__ __attribute__((always_inline)) inline void linear_tile(const DImg &s,
     for (int e = e0; e < e0 + 4 && e < n; e++) {
         const int x = e / cn, ch = e - x * cn;
         const LinTap tx = xt[x];
-        const int32_t h0 = (int16_t)(((int32_t)r0[tx.o0 + ch] * tx.w0 + (int32_t)r0[tx.o1 + ch] * tx.w1) >> 4);
-        const int32_t h1 = (int16_t)(((int32_t)r1[tx.o0 + ch] * tx.w0 + (int32_t)r1[tx.o1 + ch] * tx.w1) >> 4);
-        const int32_t t = ((h0 * ty.w0) >> 16) + ((h1 * ty.w1) >> 16);
-        const int32_t v = (t + 2) >> 2;
-        o[e] = (uint8_t)(v < 0 ? 0 : v > 255 ? 255 : v);
+        const int32_t h0 = lin_h(r0[tx.o0 + ch], r0[tx.o1 + ch], tx.w0, tx.w1);
+        const int32_t h1 = lin_h(r1[tx.o0 + ch], r1[tx.o1 + ch], tx.w0, tx.w1);
+        o[e] = (uint8_t)lin_v(h0, h1, ty.w0, ty.w1);
+    }
+}
+
+__device__ __attribute__((always_inline)) inline LinTap load_tap(gptr<const LinTap> p, int i) {
+    const u32x4 v = *(gptr<const u32x4>)(p + i);
+    LinTap t;
+    t.o0 = (int32_t)v.x;
+    t.o1 = (int32_t)v.y;
+    t.w0 = (int32_t)v.z;
+    t.w1 = (int32_t)v.w;
+    return t;
+}
+
+// Through LDS: 256 output samples x kLinTH rows, for tap tables that
+// linear_box_fits() accepted (taps that never decrease, every tile's source
+// box within kLinRows x kLinWords). The box of a tile, rows yt[y0].o0 ..
+// yt[y_last].o1 and samples xt[x_first].o0 .. xt[x_last].o1 + cn - 1, then
+// holds every tap of the tile. It is staged as aligned dwords; pass 1 forms
+// H for every staged row and the tile's 256 samples (as i16 in LDS), pass 2
+// blends the two H rows of each output row and stores whole dwords.
+__device__ __attribute__((always_inline)) inline void linear_box_tile(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt,
+                                                                      int bx, int by, uint32_t *lds) {
+    const int cn = d.cn, n = d.w * cn, ns = s.w * cn;
+    const int e_first = bx * 256, y0 = by * kLinTH;
+    const int e_last = min(n, e_first + 256) - 1, y_last = min(d.h, y0 + kLinTH) - 1;
+    const gptr<const LinTap> xg = gp(xt), yg = gp(yt);
+    const int c_lo = load_tap(xg, e_first / cn).o0;
+    const int ylo = load_tap(yg, y0).o0, yhi = load_tap(yg, y_last).o1;
+    const int nr = yhi - ylo + 1, nd = (load_tap(xg, e_last / cn).o1 + cn - 1 - c_lo + 7) >> 2;
+    uint8_t *box = reinterpret_cast<uint8_t *>(lds);
+    uint32_t *hb = lds + kLinRows * kLinWords;
+    const int g = tile_tx(), wv = __builtin_amdgcn_readfirstlane(tile_ty());
+    // Stage: slot i is dword k of box row r; the float quotient is exact
+    // for these sizes and corrected all the same.
+    const float rcp = 1.0f / (float)nd;
+    for (int i = threadIdx.x; i < nr * nd; i += kTileThreads) {
+        int r = (int)(((float)i + 0.5f) * rcp);
+        int k = i - r * nd;
+        if (k >= nd) r++, k -= nd;
+        if (k < 0) r--, k += nd;
+        const gptr<const uint8_t> row = grow<const uint8_t>(s, ylo + r);
+        const int c = c_lo - (int)((uintptr_t)(row + c_lo) & 3) + 4 * k;
+        uint32_t v = 0;
+        if (c >= 0 && c + 4 <= ns) {
+            v = *(gptr<const uint32_t>)(row + c);
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                if (c + t >= 0 && c + t < ns) v |= (uint32_t)row[c + t] << (8 * t);
+        }
+        lds[r * kLinWords + k] = v;
+    }
+    int32_t b0[4], b1[4], w0[4], w1[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int e = min(e_first + 4 * g + j, e_last);
+        const int x = e / cn, ch = e - x * cn;
+        const LinTap t = load_tap(xg, x);
+        b0[j] = t.o0 + ch - c_lo;
+        b1[j] = t.o1 + ch - c_lo;
+        w0[j] = t.w0;
+        w1[j] = t.w1;
+    }
+    __syncthreads();
+    // Pass 1: H rows.
+    for (int r = wv; r < nr; r += 4) {
+        const int sh = (int)((uintptr_t)(grow<const uint8_t>(s, ylo + r) + c_lo) & 3);
+        const uint8_t *rb = box + r * (kLinWords * 4) + sh;
+        uint32_t p[2] = {0u, 0u};
+#pragma unroll
+        for (int j = 0; j < 4; j++) p[j >> 1] |= ((uint32_t)lin_h(rb[b0[j]], rb[b1[j]], w0[j], w1[j]) & 0xffffu) << (16 * (j & 1));
+        *reinterpret_cast<u32x2 *>(hb + r * 128 + 2 * g) = u32x2{p[0], p[1]};
+    }
+    __syncthreads();
+    // Pass 2: rows y0 + 4 wv .. + 3.
+    const int e = e_first + 4 * g;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int y = y0 + wv * 4 + q;
+        if (y > y_last) break;
+        const LinTap ty = load_tap(yg, y);
+        const u32x2 h0 = *reinterpret_cast<const u32x2 *>(hb + (ty.o0 - ylo) * 128 + 2 * g);
+        const u32x2 h1 = *reinterpret_cast<const u32x2 *>(hb + (ty.o1 - ylo) * 128 + 2 * g);
+        uint32_t out = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t a = j < 2 ? h0.x : h0.y, b = j < 2 ? h1.x : h1.y;
+            const int32_t v0 = (int16_t)(a >> (16 * (j & 1))), v1 = (int16_t)(b >> (16 * (j & 1)));
+            out |= lin_v(v0, v1, ty.w0, ty.w1) << (8 * j);
+        }
+        if (e > e_last) continue;
+        const gptr<uint8_t> o = grow<uint8_t>(d, y) + e;
+        if (e + 3 <= e_last && ((uintptr_t)o & 3) == 0) {
+            *(gptr<uint32_t>)o = out;
+        } else {
+            for (int j = 0; j < 4 && e + j <= e_last; j++) o[j] = (uint8_t)(out >> (8 * j));
+        }
     }
 }
 

@@ -282,15 +282,15 @@ def test_resize_linear_scales_near_tile_limits(va, orc):
                 check(va, orc, "resize", a, (dw, dh), interp)
 
 
-def test_resize_linear_misaligned_rows(va, orc):
-    """Sources and destinations at every byte offset mod 4 with odd row
-    strides: every output byte equals the oracle and the bytes around the
-    destination view stay untouched."""
+def test_resize_misaligned_rows(va, orc):
+    """Linear and area resizes with sources and destinations at every byte
+    offset mod 4 and odd row strides: every output byte equals the oracle and
+    the bytes around the destination view stay untouched."""
     for cn in (1, 3):
         h, w = 37, 301
         a = natural(h, w, cn, 77 + cn)
-        for (dw, dh) in [(452, 56), (180, 22)]:
-            want = orc.resize(a, (dw, dh), 1)
+        for (dw, dh, interp) in [(452, 56, 1), (180, 22, 1), (150, 18, 3), (211, 29, 3)]:
+            want = orc.resize(a, (dw, dh), interp)
             for off in range(4):
                 for pad in (0, 1, 5):
                     pitch = w * cn + pad
@@ -306,8 +306,8 @@ def test_resize_linear_misaligned_rows(va, orc):
                     keep = torch.ones_like(dbig, dtype=torch.bool)
                     for r in range(dh):
                         keep[o + r * dp:o + r * dp + dw * cn] = False
-                    va.resize(sv, (dw, dh), 1, dst=dv)
-                    assert_same(dv.cpu().numpy(), want, f"linear {w}x{h}->{dw}x{dh} cn{cn} off{off} pad{pad}")
+                    va.resize(sv, (dw, dh), interp, dst=dv)
+                    assert_same(dv.cpu().numpy(), want, f"interp {interp} {w}x{h}->{dw}x{dh} cn{cn} off{off} pad{pad}")
                     assert bool((dbig[keep] == 77).all()), "resize wrote outside its view"
 
 

@@ -42,6 +42,7 @@ enum JobKind : int {
     JK_STENCIL_SWAR = 9,
     JK_AFFINE_BOX = 10,
     JK_LINEAR_BOX = 11,
+    JK_AREA_BOX = 12,
 };
 
 struct ResizeArgs {
@@ -123,6 +124,13 @@ __global__ __launch_bounds__(kTileThreads) void batch_linear(BATCH_ARGS) {
              [&](const BatchJob &j, int bx, int by) { linear_box_tile(j.rs.s, j.rs.d, j.rs.xt, j.rs.yt, bx, by, lds); });
 }
 
+__global__ __launch_bounds__(kTileThreads) void batch_area(BATCH_ARGS) {
+    __shared__ uint32_t lds[kAreaLdsWords];
+    run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) {
+        area_box_tile(j.rs.s, j.rs.d, j.rs.xe, j.rs.xs, j.rs.ye, j.rs.ys, bx, by, lds);
+    });
+}
+
 template <int KIND>
 __global__ __launch_bounds__(kTileThreads) void batch_other(BATCH_ARGS) {
     run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) {
@@ -173,6 +181,7 @@ GroupKernel group_kernel(int kind, int variant) {
     case JK_HALF: return batch_other<JK_HALF>;
     case JK_LINEAR: return batch_other<JK_LINEAR>;
     case JK_LINEAR_BOX: return batch_linear;
+    case JK_AREA_BOX: return batch_area;
     case JK_AREA_INT: return batch_other<JK_AREA_INT>;
     default: return batch_other<JK_AREA_FRAC>;
     }
@@ -394,6 +403,10 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
         std::vector<int32_t> xs, ys;
         area_tab(it.src.width, it.dst.width, (double)it.src.width / (double)it.dst.width, xe, xs);
         area_tab(it.src.height, it.dst.height, (double)it.src.height / (double)it.dst.height, ye, ys);
+        if (area_box_fits(xe.data(), xs.data(), (int)it.dst.width, (int)it.src.channels, ye.data(), ys.data(), (int)it.dst.height)) {
+            j.kind = JK_AREA_BOX;
+            j.ntiles = j.tiles_x * ((d.h + kAreaTH - 1) / kAreaTH);
+        }
         size_t bytes = (xe.size() + ye.size()) * sizeof(AreaEntry) + (xs.size() + ys.size()) * sizeof(int32_t);
         pl.table.resize(bytes);
         uint8_t *o = pl.table.data();

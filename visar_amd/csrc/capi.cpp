@@ -727,7 +727,8 @@ int visar_resize(const visar_image *src, const visar_image *dst, int interp) {
     const AreaEntry *dye = dxe + xe.size();
     const int32_t *dxs = (const int32_t *)(dye + ye.size());
     const int32_t *dys = dxs + xs.size();
-    return call.finish(launch_area_frac_u8(s, d, dxe, dxs, dye, dys, st));
+    const bool box = area_box_fits(xe.data(), xs.data(), (int)dst->width, (int)src->channels, ye.data(), ys.data(), (int)dst->height);
+    return call.finish(launch_area_frac_u8(s, d, dxe, dxs, dye, dys, st, box));
 }
 
 // ------------------------------------------------------------------ warps

@@ -275,6 +275,12 @@ __global__ __launch_bounds__(kThreads) void area_frac_u8(DImg s, DImg d, const A
     area_frac_tile(s, d, xt, xs, yt, ys, blockIdx.x, blockIdx.y);
 }
 
+__global__ __launch_bounds__(kThreads) void area_box_u8(DImg s, DImg d, const AreaEntry *xt, const int32_t *xs,
+                                                        const AreaEntry *yt, const int32_t *ys) {
+    __shared__ uint32_t lds[kAreaLdsWords];
+    area_box_tile(s, d, xt, xs, yt, ys, blockIdx.x, blockIdx.y, lds);
+}
+
 // ----------------------------------------------------------- morph helpers
 
 template <typename T>
@@ -405,8 +411,13 @@ int launch_linear_u8(const DImg &s, const DImg &d, const LinTap *xt, const LinTa
 }
 
 int launch_area_frac_u8(const DImg &s, const DImg &d, const AreaEntry *xt, const int32_t *xstart, const AreaEntry *yt,
-                        const int32_t *ystart, hipStream_t st) {
-    hipLaunchKernelGGL(area_frac_u8, tiles256(d), dim3(kThreads), 0, st, s, d, xt, xstart, yt, ystart);
+                        const int32_t *ystart, hipStream_t st, bool box) {
+    if (box) {
+        const dim3 grid((unsigned)(((long)d.w * d.cn + 255) / 256), (unsigned)((d.h + kAreaTH - 1) / kAreaTH));
+        hipLaunchKernelGGL(area_box_u8, grid, dim3(kThreads), 0, st, s, d, xt, xstart, yt, ystart);
+    } else {
+        hipLaunchKernelGGL(area_frac_u8, tiles256(d), dim3(kThreads), 0, st, s, d, xt, xstart, yt, ystart);
+    }
     return launched();
 }
 

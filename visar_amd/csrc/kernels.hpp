@@ -104,7 +104,19 @@ struct AreaEntry {
     float alpha;
 };
 int launch_area_frac_u8(const DImg &s, const DImg &d, const AreaEntry *xt, const int32_t *xstart, const AreaEntry *yt,
-                        const int32_t *ystart, hipStream_t st);
+                        const int32_t *ystart, hipStream_t st, bool box);
+
+// Tiles of the LDS-staged area resize: 256 output samples x kAreaTH rows,
+// cells of at most kAreaMaxCell entries, source boxes of at most kAreaRows
+// rows of kAreaWords dwords (shrinks by up to 2.2 fit; 8 workgroups' LDS
+// fit one CU).
+constexpr int kAreaTH = 4;
+constexpr int kAreaMaxCell = 4;
+constexpr int kAreaRows = 12;
+constexpr int kAreaWords = 148;
+constexpr int kAreaLdsWords = kAreaRows * kAreaWords + kAreaRows * 256;
+// True when the area tables suit those tiles.
+bool area_box_fits(const AreaEntry *xt, const int32_t *xs, int dw, int cn, const AreaEntry *yt, const int32_t *ys, int dh);
 
 // ------------------------------------------------------------------ warps
 

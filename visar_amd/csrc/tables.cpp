@@ -244,6 +244,37 @@ bool linear_box_fits(const LinTap *xt, int dw, int cn, const LinTap *yt, int dh)
     return true;
 }
 
+bool area_box_fits(const AreaEntry *xt, const int32_t *xs, int dw, int cn, const AreaEntry *yt, const int32_t *ys, int dh) {
+    if (dw <= 0 || dh <= 0 || cn <= 0) return false;
+    // One band of cells [c0, c1]: its entries' range must be that of its
+    // first and last entries.
+    auto band = [](const AreaEntry *t, const int32_t *st, int c0, int c1, long *lo, long *hi) {
+        const int a = st[c0], b = st[c1 + 1];
+        if (b <= a) return false;
+        long mn = t[a].si, mx = t[a].si;
+        for (int q = a; q < b; q++) mn = std::min<long>(mn, t[q].si), mx = std::max<long>(mx, t[q].si);
+        *lo = mn;
+        *hi = mx;
+        return mn == t[a].si && mx == t[b - 1].si;
+    };
+    for (int x = 0; x < dw; x++)
+        if (xs[x + 1] - xs[x] < 1 || xs[x + 1] - xs[x] > kAreaMaxCell) return false;
+    for (int y = 0; y < dh; y++)
+        if (ys[y + 1] - ys[y] < 1) return false;
+    const long n = (long)dw * cn;
+    for (long e = 0; e < n; e += 256) {
+        long lo, hi;
+        if (!band(xt, xs, (int)(e / cn), (int)((std::min(n, e + 256) - 1) / cn), &lo, &hi)) return false;
+        if (((hi - lo) * cn + cn + 6) / 4 > kAreaWords) return false;
+    }
+    for (int y = 0; y < dh; y += kAreaTH) {
+        long lo, hi;
+        if (!band(yt, ys, y, std::min(dh, y + kAreaTH) - 1, &lo, &hi)) return false;
+        if (hi - lo + 1 > kAreaRows) return false;
+    }
+    return true;
+}
+
 void area_tab(uint32_t ssize, uint32_t dsize, double scale, std::vector<AreaEntry> &e, std::vector<int32_t> &start) {
     e.clear();
     start.assign(dsize + 1, 0);

@@ -979,6 +979,38 @@ __device__ __attribute__((always_inline)) inline uint32_t lin_v(int32_t h0, int3
     return (uint32_t)(v < 0 ? 0 : v > 255 ? 255 : v);
 }
 
+// Stages source rows ylo .. ylo + nr - 1, bytes c_lo .. c_lo + 4 nd - 4 of
+// each, as aligned dwords: LDS word r * PITCH + k holds the row's bytes from
+// c_lo - a + 4k on, where a = box_shift(s, ylo + r, c_lo). Dwords wholly
+// inside the row are single loads; bytes outside the row stay zero. Slots
+// are spread over the workgroup (the float quotient is exact for these
+// sizes and corrected all the same). The caller synchronises.
+__device__ __attribute__((always_inline)) inline int box_shift(const DImg &s, int y, int c_lo) {
+    return (int)((uintptr_t)(grow<const uint8_t>(s, y) + c_lo) & 3);
+}
+template <int PITCH>
+__device__ __attribute__((always_inline)) inline void stage_box(const DImg &s, int ylo, int nr, int c_lo, int nd, uint32_t *lds) {
+    const int ns = s.w * s.cn;
+    const float rcp = 1.0f / (float)nd;
+    for (int i = threadIdx.x; i < nr * nd; i += kTileThreads) {
+        int r = (int)(((float)i + 0.5f) * rcp);
+        int k = i - r * nd;
+        if (k >= nd) r++, k -= nd;
+        if (k < 0) r--, k += nd;
+        const gptr<const uint8_t> row = grow<const uint8_t>(s, ylo + r);
+        const int c = c_lo - box_shift(s, ylo + r, c_lo) + 4 * k;
+        uint32_t v = 0;
+        if (c >= 0 && c + 4 <= ns) {
+            v = *(gptr<const uint32_t>)(row + c);
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                if (c + t >= 0 && c + t < ns) v |= (uint32_t)row[c + t] << (8 * t);
+        }
+        lds[r * PITCH + k] = v;
+    }
+}
+
 // Direct gathers: 256 output samples x 4 rows.
 __device__ __attribute__((always_inline)) inline void linear_tile(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, int bx, int by) {
     const int y = by * 4 + tile_ty();
@@ -1017,7 +1049,7 @@ __device__ __attribute__((always_inline)) inline LinTap load_tap(gptr<const LinT
 // blends the two H rows of each output row and stores whole dwords.
 __device__ __attribute__((always_inline)) inline void linear_box_tile(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt,
                                                                       int bx, int by, uint32_t *lds) {
-    const int cn = d.cn, n = d.w * cn, ns = s.w * cn;
+    const int cn = d.cn, n = d.w * cn;
     const int e_first = bx * 256, y0 = by * kLinTH;
     const int e_last = min(n, e_first + 256) - 1, y_last = min(d.h, y0 + kLinTH) - 1;
     const gptr<const LinTap> xg = gp(xt), yg = gp(yt);
@@ -1027,26 +1059,7 @@ __device__ __attribute__((always_inline)) inline void linear_box_tile(const DImg
     uint8_t *box = reinterpret_cast<uint8_t *>(lds);
     uint32_t *hb = lds + kLinRows * kLinWords;
     const int g = tile_tx(), wv = __builtin_amdgcn_readfirstlane(tile_ty());
-    // Stage: slot i is dword k of box row r; the float quotient is exact
-    // for these sizes and corrected all the same.
-    const float rcp = 1.0f / (float)nd;
-    for (int i = threadIdx.x; i < nr * nd; i += kTileThreads) {
-        int r = (int)(((float)i + 0.5f) * rcp);
-        int k = i - r * nd;
-        if (k >= nd) r++, k -= nd;
-        if (k < 0) r--, k += nd;
-        const gptr<const uint8_t> row = grow<const uint8_t>(s, ylo + r);
-        const int c = c_lo - (int)((uintptr_t)(row + c_lo) & 3) + 4 * k;
-        uint32_t v = 0;
-        if (c >= 0 && c + 4 <= ns) {
-            v = *(gptr<const uint32_t>)(row + c);
-        } else {
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-                if (c + t >= 0 && c + t < ns) v |= (uint32_t)row[c + t] << (8 * t);
-        }
-        lds[r * kLinWords + k] = v;
-    }
+    stage_box<kLinWords>(s, ylo, nr, c_lo, nd, lds);
     int32_t b0[4], b1[4], w0[4], w1[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -1061,8 +1074,7 @@ __device__ __attribute__((always_inline)) inline void linear_box_tile(const DImg
     __syncthreads();
     // Pass 1: H rows.
     for (int r = wv; r < nr; r += 4) {
-        const int sh = (int)((uintptr_t)(grow<const uint8_t>(s, ylo + r) + c_lo) & 3);
-        const uint8_t *rb = box + r * (kLinWords * 4) + sh;
+        const uint8_t *rb = box + r * (kLinWords * 4) + box_shift(s, ylo + r, c_lo);
         uint32_t p[2] = {0u, 0u};
 #pragma unroll
         for (int j = 0; j < 4; j++) p[j >> 1] |= ((uint32_t)lin_h(rb[b0[j]], rb[b1[j]], w0[j], w1[j]) & 0xffffu) << (16 * (j & 1));
@@ -1119,6 +1131,86 @@ __device__ __attribute__((always_inline)) inline void area_frac_tile(const DImg 
             sum = sum + h * ye.alpha;
         }
         o[e] = sat_cast<uint8_t>((double)sum);
+    }
+}
+
+// area_frac_tile through LDS, for tables that area_box_fits() accepted
+// (every cell with 1 .. kAreaMaxCell entries, each tile's entries within
+// the span of its first and last cell, and that span within kAreaRows x
+// kAreaWords): 256 output samples x kAreaTH rows. The box is staged as
+// aligned dwords; pass 1 forms, for every staged row and the tile's 256
+// samples, the f32 sum of s * alpha over the sample's cell in table order
+// (one sample per thread, the cell's entries in registers); pass 2 sums
+// those rows times beta in table order, the same operations in the same
+// order as area_frac_tile.
+__device__ __attribute__((always_inline)) inline AreaEntry load_entry(gptr<const AreaEntry> p, int i) {
+    const u32x2 v = *(gptr<const u32x2>)(p + i);
+    AreaEntry e;
+    e.si = (int32_t)v.x;
+    e.alpha = __uint_as_float(v.y);
+    return e;
+}
+
+__device__ __attribute__((always_inline)) inline void area_box_tile(const DImg &s, const DImg &d, const AreaEntry *xt, const int32_t *xs,
+                                                                    const AreaEntry *yt, const int32_t *ys, int bx, int by, uint32_t *lds) {
+    const int cn = d.cn, n = d.w * cn;
+    const int e_first = bx * 256, y0 = by * kAreaTH;
+    const int e_last = min(n, e_first + 256) - 1, y_last = min(d.h, y0 + kAreaTH) - 1;
+    const gptr<const AreaEntry> xg = gp(xt), yg = gp(yt);
+    const gptr<const int32_t> xsg = gp(xs), ysg = gp(ys);
+    const int c_lo = xg[xsg[e_first / cn]].si * cn;
+    const int c_hi = xg[xsg[e_last / cn + 1] - 1].si * cn + cn - 1;
+    const int ylo = yg[ysg[y0]].si, yhi = yg[ysg[y_last + 1] - 1].si;
+    const int nr = yhi - ylo + 1, nd = (c_hi - c_lo + 7) >> 2;
+    stage_box<kAreaWords>(s, ylo, nr, c_lo, nd, lds);
+    // This thread's sample of pass 1 and its cell's entries.
+    const int e = min(e_first + (int)threadIdx.x, e_last);
+    const int x = e / cn, ch = e - x * cn;
+    const int q0 = xsg[x], nq = xsg[x + 1] - q0;
+    int xo[kAreaMaxCell];
+    float al[kAreaMaxCell];
+#pragma unroll
+    for (int q = 0; q < kAreaMaxCell; q++) {
+        const AreaEntry en = load_entry(xg, q0 + min(q, nq - 1));
+        xo[q] = en.si * cn + ch - c_lo;
+        al[q] = en.alpha;
+    }
+    __syncthreads();
+    const uint8_t *box = reinterpret_cast<const uint8_t *>(lds);
+    float *hb = reinterpret_cast<float *>(lds + kAreaRows * kAreaWords);
+    for (int r = 0; r < nr; r++) {
+        const uint8_t *rb = box + r * (kAreaWords * 4) + box_shift(s, ylo + r, c_lo);
+        float h = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kAreaMaxCell; q++)
+            if (q < nq) h = h + (float)rb[xo[q]] * al[q];
+        hb[r * 256 + threadIdx.x] = h;
+    }
+    __syncthreads();
+    const int wv = __builtin_amdgcn_readfirstlane(tile_ty()), g = tile_tx();
+    const int e4 = e_first + 4 * g;
+#pragma unroll 1
+    for (int y = y0 + wv; y <= y_last; y += 4) {
+        const int k0 = ysg[y], k1 = ysg[y + 1];
+        float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k = k0; k < k1; k++) {
+            const AreaEntry en = load_entry(yg, k);
+            const float4 hv = *reinterpret_cast<const float4 *>(hb + (en.si - ylo) * 256 + 4 * g);
+            sum[0] = sum[0] + hv.x * en.alpha;
+            sum[1] = sum[1] + hv.y * en.alpha;
+            sum[2] = sum[2] + hv.z * en.alpha;
+            sum[3] = sum[3] + hv.w * en.alpha;
+        }
+        if (e4 > e_last) continue;
+        uint32_t out = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) out |= (uint32_t)sat_cast<uint8_t>((double)sum[j]) << (8 * j);
+        const gptr<uint8_t> o = grow<uint8_t>(d, y) + e4;
+        if (e4 + 3 <= e_last && ((uintptr_t)o & 3) == 0) {
+            *(gptr<uint32_t>)o = out;
+        } else {
+            for (int j = 0; j < 4 && e4 + j <= e_last; j++) o[j] = (uint8_t)(out >> (8 * j));
+        }
     }
 }
 

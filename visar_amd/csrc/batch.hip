@@ -173,7 +173,8 @@ GroupKernel group_kernel(int kind, int variant) {
     case JK_MORPH:
         if (cn == 1) return k == 3 ? morph_kernel<3, 1>(last) : k == 5 ? morph_kernel<5, 1>(last) : morph_kernel<7, 1>(last);
         return k == 3 ? morph_kernel<3, 3>(last) : k == 5 ? morph_kernel<5, 3>(last) : morph_kernel<7, 3>(last);
-    case JK_STENCIL_SWAR: return k == SWAR_G3 ? swar_kernel<SWAR_G3>(cn, last) : swar_kernel<SWAR_G5>(cn, last);
+    case JK_STENCIL_SWAR:
+        return k == SWAR_G3 ? swar_kernel<SWAR_G3>(cn, last) : k == SWAR_G5 ? swar_kernel<SWAR_G5>(cn, last) : swar_kernel<SWAR_G7>(cn, last);
     case JK_AFFINE_BOX: return batch_affine_box;
     case JK_AFFINE_LIN4: return batch_other<JK_AFFINE_LIN4>;
     case JK_AFFINE_PX: return batch_other<JK_AFFINE_PX>;
@@ -276,7 +277,7 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
         th = th >= 2 ? 2 : 1;
         j.variant = k * 100 + (int)it.src.channels * 10 + th;
         const int mode = swar_mode(j.st, k, 0);
-        if (mode == SWAR_G3 || mode == SWAR_G5) {
+        if (mode == SWAR_G3 || mode == SWAR_G5 || mode == SWAR_G7) {
             j.kind = JK_STENCIL_SWAR;
             j.variant = mode * 100 + (int)it.src.channels * 10 + th;
         }

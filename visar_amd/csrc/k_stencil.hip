@@ -114,6 +114,7 @@ int swar_mode(const StencilArgs &a, int k, int ddepth) {
     if (ddepth == 0 && a.fin == FIN_FIX8) {
         if (k == 3 && is(a.kx, {64, 128, 64}) && is(a.ky, {64, 128, 64})) return SWAR_G3;
         if (k == 5 && is(a.kx, {16, 64, 96, 64, 16}) && is(a.ky, {16, 64, 96, 64, 16})) return SWAR_G5;
+        if (k == 7 && is(a.kx, {8, 28, 56, 72, 56, 28, 8}) && is(a.ky, {8, 28, 56, 72, 56, 28, 8})) return SWAR_G7;
     }
     if (ddepth == 3 && a.fin == FIN_CAST && k == 3) {
         if (is(a.kx, {-1, 0, 1}) && is(a.ky, {1, 2, 1})) return SWAR_SOBEL_DX;
@@ -130,6 +131,7 @@ bool launch_sep_small_u8(const SepParams &p, hipStream_t st, int th) {
     switch (swar_mode(a, p.nkx, p.ddepth)) {
     case SWAR_G3: ok = launch_swar<SWAR_G3, uint8_t>(a, st, th); break;
     case SWAR_G5: ok = launch_swar<SWAR_G5, uint8_t>(a, st, th); break;
+    case SWAR_G7: ok = launch_swar<SWAR_G7, uint8_t>(a, st, th); break;
     case SWAR_SOBEL_DX: ok = launch_swar<SWAR_SOBEL_DX, int16_t>(a, st, th); break;
     case SWAR_SOBEL_DY: ok = launch_swar<SWAR_SOBEL_DY, int16_t>(a, st, th); break;
     default: ok = p.ddepth == 0 ? launch_d<uint8_t>(a, p.nkx, st, th) : launch_d<int16_t>(a, p.nkx, st, th);

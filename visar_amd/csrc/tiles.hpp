@@ -289,7 +289,7 @@ __device__ __attribute__((always_inline)) inline void stencil_tile(const Stencil
 // -------------------------------------------------- packed 16-bit stencils
 //
 // The stencils of the benchmark shapes in pairs of 16-bit lanes: GaussianBlur
-// u8 3x3 and 5x5 with sigma 0 (binomial taps 64 * [1 2 1] and
+// u8 7x7 with sigma 0 (below), 3x3 and 5x5 with sigma 0 (binomial taps 64 * [1 2 1] and
 // 16 * [1 4 6 4 1], which reduce to [1 2 1] and [1 4 6 4 1]; the full sum is
 // 4096 or 256 times the reduced one, so the reference's (sum + 2^15) >> 16 is
 // (v + 8) >> 4 or (v + 128) >> 8 exactly, and every partial sum stays below
@@ -298,7 +298,7 @@ __device__ __attribute__((always_inline)) inline void stencil_tile(const Stencil
 // handled as two pairs, samples (0, 2) and (1, 3); a pair of source bytes
 // (b[e], b[e + 2]) comes from the staged dwords with one v_perm.
 
-enum SwarMode : int { SWAR_G3 = 0, SWAR_G5 = 1, SWAR_SOBEL_DX = 2, SWAR_SOBEL_DY = 3 };
+enum SwarMode : int { SWAR_G3 = 0, SWAR_G5 = 1, SWAR_SOBEL_DX = 2, SWAR_SOBEL_DY = 3, SWAR_G7 = 4 };
 
 // The packed mode whose taps a u8 stencil call has exactly (k taps per
 // axis, destination depth ddepth), or -1 (host side, k_stencil.hip).
@@ -318,6 +318,15 @@ template <>
 struct SwarTaps<SWAR_G5> {
     static constexpr int K = 5;
     static constexpr int h[5] = {1, 4, 6, 4, 1}, v[5] = {1, 4, 6, 4, 1};
+};
+// 7 x 7 with sigma 0: taps 4 * {2, 7, 14, 18, 14, 7, 2}, so the result is
+// (R + 2048) >> 12 for the reduced sum R < 2^20. The horizontal sums stay
+// below 2^14 in 16-bit lanes; the vertical pass adds the mirrored rows in
+// 16 bits and takes the weighted sum in 32 bits with v_dot2_u32_u16.
+template <>
+struct SwarTaps<SWAR_G7> {
+    static constexpr int K = 7;
+    static constexpr int h[7] = {2, 7, 14, 18, 14, 7, 2}, v[7] = {2, 7, 14, 18, 14, 7, 2};
 };
 template <>
 struct SwarTaps<SWAR_SOBEL_DX> {
@@ -398,13 +407,39 @@ __device__ __attribute__((always_inline)) inline void stencil_swar_tile(const St
     for (int t = 0; t < TH; t++) {
         const int y = y0 + yt + t;
         if (y >= H) break;  // uniform across the wave
+        D *row = reinterpret_cast<D *>(p.dst.p + (long)y * p.dst.stride);
+        if constexpr (MODE == SWAR_G7) {
+            // per lane: 2 (r0 + r6) + 7 (r1 + r5) + 14 (r2 + r4) + 18 r3
+            uint32_t f[4];  // samples 0, 2 (even pair) and 1, 3 (odd pair)
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const uint32_t *hv = q == 0 ? he : ho;
+                const u16x2 a = __builtin_bit_cast(u16x2, hv[t]) + __builtin_bit_cast(u16x2, hv[t + 6]);
+                const u16x2 b = __builtin_bit_cast(u16x2, hv[t + 1]) + __builtin_bit_cast(u16x2, hv[t + 5]);
+                const u16x2 c = __builtin_bit_cast(u16x2, hv[t + 2]) + __builtin_bit_cast(u16x2, hv[t + 4]);
+                const uint32_t ua = __builtin_bit_cast(uint32_t, a), ub = __builtin_bit_cast(uint32_t, b),
+                               uc = __builtin_bit_cast(uint32_t, c), um = hv[t + 3];
+#pragma unroll
+                for (int l = 0; l < 2; l++) {
+                    const uint32_t sel = l == 0 ? 0x05040100u : 0x07060302u;
+                    const u16x2 ab = __builtin_bit_cast(u16x2, __builtin_amdgcn_perm(ub, ua, sel));
+                    const u16x2 cm = __builtin_bit_cast(u16x2, __builtin_amdgcn_perm(um, uc, sel));
+                    uint32_t r = __builtin_amdgcn_udot2(ab, (u16x2){2, 7}, 2048u, false);
+                    r = __builtin_amdgcn_udot2(cm, (u16x2){14, 18}, r, false);
+                    f[q + 2 * l] = r >> 12;
+                }
+            }
+            // f[0] = sample 0, f[1] = sample 1, f[2] = sample 2, f[3] = sample 3
+            const uint32_t v4 = f[0] | (f[1] << 8) | (f[2] << 16) | (f[3] << 24);
+            store_row_u8(reinterpret_cast<uint8_t *>(row), x0, min(x0 + 256, n), v4);
+            continue;
+        }
         uint32_t ve = 0, vo = 0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             ve = swar_mac<MODE>(ve, he[t + k], T::v[k]);
             vo = swar_mac<MODE>(vo, ho[t + k], T::v[k]);
         }
-        D *row = reinterpret_cast<D *>(p.dst.p + (long)y * p.dst.stride);
         if constexpr (MODE == SWAR_G3 || MODE == SWAR_G5) {
             constexpr uint16_t add = MODE == SWAR_G3 ? 8 : 128, sh = MODE == SWAR_G3 ? 4 : 8;
             const u16x2 fe = (__builtin_bit_cast(u16x2, ve) + add) >> sh;

@@ -43,6 +43,9 @@ typedef struct visar_serve_config {
     uint32_t queue_depth;     /* frames a stream may have submitted and unfinished, 1..64 */
     uint32_t host_workers;    /* connected-components threads, 1..64 */
     uint32_t max_inflight;    /* batches on the GPU at once, 1..8 */
+    uint32_t batch_window_us; /* with fewer than max_batch streams ready, wait up to this long
+                                 after the oldest ready frame arrived for more (0: launch at once;
+                                 at most 1 s) */
 } visar_serve_config;
 
 typedef struct visar_serve_result {
@@ -70,7 +73,7 @@ typedef struct visar_serve visar_serve;
 
 /* Fills the defaults (1280 x 720, shift 4, threshold 28, mask threshold
  * 127, min area 250, no budget, 64 streams, batch 64, depth 4, 4 workers,
- * 2 batches in flight). */
+ * 2 batches in flight, no batching window). */
 void visar_serve_config_default(visar_serve_config *cfg);
 int visar_serve_create(const visar_serve_config *cfg, visar_serve **out);
 /* VISAR_OUT_OF_MEMORY ("HbmBudget") when the stream does not fit the budget

@@ -311,6 +311,38 @@ def test_resize_misaligned_rows(va, orc):
                     assert bool((dbig[keep] == 77).all()), "resize wrote outside its view"
 
 
+def test_queued_calls_keep_their_tables_and_temporaries(va, orc):
+    """Queued calls (no per-call wait) whose tap tables and temporaries come
+    from the shared pool: each buffer is reused only after the work that
+    reads it, so every output equals the oracle after one final wait."""
+    cases = []
+    for i in range(24):
+        cn = 1 + 2 * (i % 2)
+        h, w = 40 + 7 * i, 60 + 13 * i
+        a = natural(h, w, cn, 900 + i)
+        if i % 3 == 0:
+            cases.append((a, ("resize", (w * 3 // 2, h * 3 // 2), 1)))
+        elif i % 3 == 1:
+            cases.append((a, ("resize", (w // 2 + 1, h // 2 + 1), 3)))
+        else:
+            cases.append((a, ("morphology", 2, 0, (3, 3), 2)))
+    srcs = [dev(a) for a, _ in cases]
+    outs = []
+    va.set_sync(False)
+    try:
+        for sv, (_, op) in zip(srcs, cases):
+            if op[0] == "resize":
+                outs.append(va.resize(sv, op[1], op[2]))
+            else:
+                outs.append(va.morphology(sv, op[1], op[2], op[3], iterations=op[4]))
+        va.synchronize()
+    finally:
+        va.set_sync(True)
+    for i, ((a, op), got) in enumerate(zip(cases, outs)):
+        want = orc.resize(a, op[1], op[2]) if op[0] == "resize" else orc.morphology(a, op[1], op[2], op[3], iterations=op[4])
+        assert_same(got.cpu().numpy(), want, f"queued call {i} {op}")
+
+
 def test_resize_linear_4k_half_config(va, orc):
     from visar_amd import synth
     a = synth.uniform_bytes(5, (2160, 3840))

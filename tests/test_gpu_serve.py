@@ -92,7 +92,9 @@ def test_serve_alone_equals_in_mix(va):
     others = [frames_for(100 + s, 8, 1280, 720) for s in range(7)]
     with Server(max_batch=1) as srv:
         (alone_masks, alone_rs), = run_streams(srv, [a])
-    with Server(max_batch=8, max_inflight=2) as srv:
+    # A batching window long enough that frames submitted back to back share
+    # launches however fast the GPU drains them.
+    with Server(max_batch=8, max_inflight=2, batch_window_us=200000) as srv:
         mixed = run_streams(srv, [others[0], others[1], a] + others[2:])
     mix_masks, mix_rs = mixed[2]
     assert max(r["batch_size"] for r in mix_rs) > 1, "the stream never shared a launch"

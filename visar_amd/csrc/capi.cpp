@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -65,13 +66,28 @@ int ensure_runtime() {
 hipStream_t current_stream() { return t_stream_set ? t_stream : g_own_stream; }
 bool sync_mode() { return t_sync != 0; }
 
-// Device buffers are reused by size class: a staged call takes the smallest
-// cached buffer that fits, and returns it once its work has finished.
+// Buffers are reused by size class: a call takes the smallest cached buffer
+// that fits. A call that waited for its stream returns its buffers at once;
+// one that only queued work returns them behind an event recorded on its
+// stream, and they become free when that event has passed, so queued calls
+// never wait for the GPU to reuse memory. The same pool type serves device
+// buffers and pinned host staging buffers.
+struct DeviceAlloc {
+    static hipError_t alloc(void **p, size_t n) { return hipMalloc(p, n); }
+    static void release(void *p) { (void)hipFree(p); }
+};
+struct PinnedAlloc {
+    static hipError_t alloc(void **p, size_t n) { return hipHostMalloc(p, n, hipHostMallocDefault); }
+    static void release(void *p) { (void)hipHostFree(p); }
+};
+
+template <typename A>
 class BufferPool {
   public:
     void *get(size_t bytes) {
         {
             std::lock_guard<std::mutex> g(mu_);
+            reap_locked();
             auto it = free_.lower_bound(bytes);
             if (it != free_.end() && it->first <= bytes * 2 + (1 << 20)) {
                 void *p = it->second;
@@ -81,10 +97,10 @@ class BufferPool {
             }
         }
         void *p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) {
+        if (A::alloc(&p, bytes) != hipSuccess) {
             (void)hipGetLastError();
             trim();
-            if (hipMalloc(&p, bytes) != hipSuccess) {
+            if (A::alloc(&p, bytes) != hipSuccess) {
                 (void)hipGetLastError();
                 return nullptr;
             }
@@ -96,25 +112,72 @@ class BufferPool {
     void put(void *p) {
         if (!p) return;
         std::lock_guard<std::mutex> g(mu_);
+        put_locked(p);
+    }
+    // Returns `bufs` once the work queued on `st` so far has finished.
+    void put_after(const std::vector<void *> &bufs, hipStream_t st) {
+        if (bufs.empty()) return;
+        std::lock_guard<std::mutex> g(mu_);
+        hipEvent_t ev = nullptr;
+        if (!events_.empty()) {
+            ev = events_.back();
+            events_.pop_back();
+        } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            ev = nullptr;
+        }
+        if (!ev || hipEventRecord(ev, st) != hipSuccess) {
+            // No event: wait for the stream instead.
+            (void)hipGetLastError();
+            if (ev) events_.push_back(ev);
+            (void)hipStreamSynchronize(st);
+            for (void *p : bufs) put_locked(p);
+            return;
+        }
+        deferred_.push_back(Deferred{ev, bufs});
+    }
+    void trim() {
+        std::lock_guard<std::mutex> g(mu_);
+        reap_locked();
+        for (auto &kv : free_) A::release(kv.second);
+        free_.clear();
+    }
+
+  private:
+    struct Deferred {
+        hipEvent_t ev;
+        std::vector<void *> bufs;
+    };
+    void put_locked(void *p) {
         auto it = sizes_.find(p);
         if (it == sizes_.end()) return;
         free_.emplace(it->second, p);
         sizes_.erase(it);
     }
-    void trim() {
-        std::lock_guard<std::mutex> g(mu_);
-        for (auto &kv : free_) (void)hipFree(kv.second);
-        free_.clear();
+    void reap_locked() {
+        while (!deferred_.empty()) {
+            const hipError_t q = hipEventQuery(deferred_.front().ev);
+            if (q == hipErrorNotReady) {
+                (void)hipGetLastError();
+                break;
+            }
+            for (void *p : deferred_.front().bufs) put_locked(p);
+            events_.push_back(deferred_.front().ev);
+            deferred_.pop_front();
+        }
     }
-
-  private:
     std::mutex mu_;
     std::multimap<size_t, void *> free_;
     std::map<void *, size_t> sizes_;
+    std::deque<Deferred> deferred_;
+    std::vector<hipEvent_t> events_;
 };
 
-static BufferPool &pool() {
-    static BufferPool *p = new BufferPool();  // never destroyed: outlives static HIP teardown
+static BufferPool<DeviceAlloc> &pool() {
+    static auto *p = new BufferPool<DeviceAlloc>();  // never destroyed: outlives static HIP teardown
+    return *p;
+}
+static BufferPool<PinnedAlloc> &pinned_pool() {
+    static auto *p = new BufferPool<PinnedAlloc>();
     return *p;
 }
 
@@ -149,7 +212,13 @@ int check_image(const visar_image *c, uint32_t depth) {
 }
 
 Call::~Call() {
-    for (void *p : owned_) pool_put(p);
+    if (waited_ || !st_) {
+        for (void *p : owned_) pool_put(p);
+        for (void *p : pinned_) pinned_pool().put(p);
+    } else {
+        pool().put_after(owned_, st_);
+        pinned_pool().put_after(pinned_, st_);
+    }
 }
 
 int Call::begin() {
@@ -173,6 +242,22 @@ void *Call::scratch(size_t bytes) {
     void *p = pool_get(bytes ? bytes : 1);
     if (p) owned_.push_back(p);
     return p;
+}
+
+void *Call::upload(const void *src, size_t bytes) {
+    void *dev = scratch(bytes);
+    void *pin = pinned_pool().get(bytes ? bytes : 1);
+    if (!dev || !pin) {
+        if (pin) pinned_pool().put(pin);
+        return nullptr;
+    }
+    pinned_.push_back(pin);
+    std::memcpy(pin, src, bytes);
+    if (hipMemcpyAsync(dev, pin, bytes, hipMemcpyHostToDevice, st_) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return dev;
 }
 
 int Call::input(const visar_image *c, DImg &out) {
@@ -235,6 +320,7 @@ int Call::temp_like(const DImg &like, size_t elem, DImg &out) {
 int Call::finish(int launch_status) {
     if (launch_status != OK) {
         (void)hipStreamSynchronize(st_);
+        waited_ = true;
         return launch_status == UNSUPPORTED ? fail(UNSUPPORTED, "Unsupported") : fail(INTERNAL, "LaunchFailed");
     }
     for (auto &pd : pending_) {
@@ -243,8 +329,9 @@ int Call::finish(int launch_status) {
         if (hipMemcpy2DAsync(c->data, c->stride, pd.buf, pd.pitch, row_bytes(c), c->height, kind, st_) != hipSuccess)
             return fail(INTERNAL, "CopyFailed");
     }
-    if (staged_ || sync_mode() || !owned_.empty()) {
+    if (staged_ || sync_mode()) {
         if (hipStreamSynchronize(st_) != hipSuccess) return fail(INTERNAL, "KernelFailed");
+        waited_ = true;
     }
     return OK;
 }
@@ -693,10 +780,8 @@ int visar_resize(const visar_image *src, const visar_image *dst, int interp) {
             tabs[x] = t;
         }
         for (uint32_t y = 0; y < dst->height; y++) tabs[dst->width + y] = linear_tap_u8(y, src->height, dst->height, area_up, false);
-        LinTap *dev = (LinTap *)call.scratch(tabs.size() * sizeof(LinTap));
+        LinTap *dev = (LinTap *)call.upload(tabs.data(), tabs.size() * sizeof(LinTap));
         if (!dev) return fail(OUT_OF_MEMORY, "OutOfMemory");
-        if (hipMemcpyAsync(dev, tabs.data(), tabs.size() * sizeof(LinTap), hipMemcpyHostToDevice, st) != hipSuccess)
-            return fail(INTERNAL, "CopyFailed");
         const bool box = linear_box_fits(tabs.data(), (int)dst->width, (int)src->channels, tabs.data() + dst->width, (int)dst->height);
         return call.finish(launch_linear_u8(s, d, dev, dev + dst->width, st, box));
     }
@@ -720,9 +805,8 @@ int visar_resize(const visar_image *src, const visar_image *dst, int interp) {
     put(ye.data(), ye.size() * sizeof(AreaEntry));
     put(xs.data(), xs.size() * sizeof(int32_t));
     put(ys.data(), ys.size() * sizeof(int32_t));
-    uint8_t *dev = (uint8_t *)call.scratch(bytes);
+    uint8_t *dev = (uint8_t *)call.upload(blob.data(), bytes);
     if (!dev) return fail(OUT_OF_MEMORY, "OutOfMemory");
-    if (hipMemcpyAsync(dev, blob.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess) return fail(INTERNAL, "CopyFailed");
     const AreaEntry *dxe = (const AreaEntry *)dev;
     const AreaEntry *dye = dxe + xe.size();
     const int32_t *dxs = (const int32_t *)(dye + ye.size());

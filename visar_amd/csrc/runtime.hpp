@@ -24,7 +24,8 @@ void pool_put(void *p);
 // place, host memory staged through pooled device buffers), owns the
 // temporaries, and on finish() copies staged outputs back and waits when
 // the call's semantics require it. Temporaries return to the pool when the
-// Call is destroyed, after the stream has drained.
+// Call is destroyed: at once when finish() waited for the stream, otherwise
+// once the work queued so far on the call's stream has finished.
 class Call {
   public:
     Call() = default;
@@ -37,6 +38,9 @@ class Call {
     int output(const visar_image *c, DImg &out);
     int temp_like(const DImg &like, size_t elem, DImg &out);
     void *scratch(size_t bytes);
+    // A device copy of `bytes` host bytes, uploaded on the call's stream
+    // through pinned memory (the host data may change once this returns).
+    void *upload(const void *src, size_t bytes);
     int finish(int launch_status);
     hipStream_t stream() const { return st_; }
 
@@ -51,7 +55,9 @@ class Call {
 
     hipStream_t st_ = nullptr;
     bool staged_ = false;
+    bool waited_ = false;
     std::vector<void *> owned_;
+    std::vector<void *> pinned_;
     std::vector<Pending> pending_;
     std::vector<std::pair<const uint8_t *, size_t>> dev_ranges_;
 };

@@ -117,6 +117,7 @@ int check_config(const visar_serve_config &c) {
     if (c.queue_depth < 1 || c.queue_depth > 64) return VISAR_INVALID_ARGUMENT;
     if (c.host_workers < 1 || c.host_workers > 64) return VISAR_INVALID_ARGUMENT;
     if (c.max_inflight < 1 || c.max_inflight > 8) return VISAR_INVALID_ARGUMENT;
+    if (c.batch_window_us > 1000000) return VISAR_INVALID_ARGUMENT;
     return VISAR_OK;
 }
 
@@ -206,6 +207,28 @@ void dispatch_loop(visar_serve *s) {
                 return false;
             });
             if (s->stop) return;
+            // Batching window: with fewer than max_batch streams ready, wait
+            // until max_batch are or the oldest ready frame has waited
+            // batch_window_us.
+            if (s->cfg.batch_window_us > 0) {
+                auto ready = [&](double *oldest) {
+                    size_t n = 0;
+                    for (const Stream &sm : s->streams)
+                        if (sm.active && !sm.queued.empty()) {
+                            n++;
+                            if (oldest) *oldest = std::min(*oldest, sm.queued.front().t_submit);
+                        }
+                    return n;
+                };
+                double oldest = 1e300;
+                if (ready(&oldest) < s->cfg.max_batch) {
+                    const double left = oldest + 1e-6 * s->cfg.batch_window_us - s->now();
+                    if (left > 0)
+                        s->cv_dispatch.wait_for(lk, std::chrono::duration<double>(left),
+                                                [&] { return s->stop || ready(nullptr) >= s->cfg.max_batch; });
+                }
+                if (s->stop) return;
+            }
             const size_t ns = s->streams.size();
             for (size_t k = 0; k < ns && b.reqs.size() < s->cfg.max_batch; k++) {
                 const size_t i = (s->rr + k) % ns;
@@ -309,6 +332,7 @@ void visar_serve_config_default(visar_serve_config *c) {
     c->queue_depth = 4;
     c->host_workers = 4;
     c->max_inflight = 2;
+    c->batch_window_us = 0;
 }
 
 int visar_serve_create(const visar_serve_config *cfg, visar_serve **out) {

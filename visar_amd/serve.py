@@ -21,7 +21,7 @@ class Config(ctypes.Structure):
         ("shift", ctypes.c_int), ("threshold", ctypes.c_int), ("mask_threshold", ctypes.c_int),
         ("min_area", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
         ("max_streams", ctypes.c_uint32), ("max_batch", ctypes.c_uint32), ("queue_depth", ctypes.c_uint32),
-        ("host_workers", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32),
+        ("host_workers", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32), ("batch_window_us", ctypes.c_uint32),
     ]
 
 

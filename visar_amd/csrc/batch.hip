@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kTileThreads) void batch_stencil_swar(BATCH_ARGS) {
     run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) { stencil_swar_tile<MODE, CN, TH, uint8_t>(j.st, bx, by, lds); });
 }
 
-__global__ __launch_bounds__(kTileThreads) void batch_affine_box(BATCH_ARGS) {
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(5))) void batch_affine_box(BATCH_ARGS) {
     __shared__ uint32_t lds[kWarpLdsWords];
     run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) { affine_box_tile(j.af, bx, by, lds); });
 }

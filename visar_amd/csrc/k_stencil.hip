@@ -11,25 +11,32 @@ namespace visar {
 namespace {
 
 template <int K, int CN, int TH, typename D>
-__global__ __launch_bounds__(256) void sep_small_u8(StencilArgs p) {
+__global__ __launch_bounds__(256) void sep_small_u8(StencilArgs p, TileOrder o) {
     __shared__ uint32_t lds[StencilShape<K, CN, TH>::LDS_WORDS];
-    stencil_tile<K, CN, TH, D>(p, blockIdx.x, blockIdx.y, lds);
+    int bx, by;
+    if (tile_of(o, bx, by)) stencil_tile<K, CN, TH, D>(p, bx, by, lds);
 }
 
 template <int MODE, int CN, int TH, typename D>
-__global__ __launch_bounds__(256) void sep_swar(StencilArgs p) {
+__global__ __launch_bounds__(256) void sep_swar(StencilArgs p, TileOrder o) {
     __shared__ uint32_t lds[StencilShape<SwarTaps<MODE>::K, CN, TH>::LDS_WORDS];
-    stencil_swar_tile<MODE, CN, TH, D>(p, blockIdx.x, blockIdx.y, lds);
+    int bx, by;
+    if (tile_of(o, bx, by)) stencil_swar_tile<MODE, CN, TH, D>(p, bx, by, lds);
+}
+
+// Tiles of 256 bytes x 4 th rows.
+inline TileOrder stencil_order(const StencilArgs &p, int cn, int th) {
+    return tile_order((int)(((long)p.src.w * cn + 255) / 256), (p.src.h + 4 * th - 1) / (4 * th));
 }
 
 template <int MODE, int CN, typename D>
 void launch_swar_th(const StencilArgs &p, hipStream_t st, int th) {
-    const long n = (long)p.src.w * CN;
-    auto grid = [&](int t) { return dim3((unsigned)((n + 255) / 256), (unsigned)((p.src.h + 4 * t - 1) / (4 * t))); };
+    const TileOrder o = stencil_order(p, CN, th == 1 || th == 2 ? th : 4);
+    const dim3 g(tile_grid(o.ntiles));
     switch (th) {
-    case 1: hipLaunchKernelGGL((sep_swar<MODE, CN, 1, D>), grid(1), dim3(256), 0, st, p); break;
-    case 2: hipLaunchKernelGGL((sep_swar<MODE, CN, 2, D>), grid(2), dim3(256), 0, st, p); break;
-    default: hipLaunchKernelGGL((sep_swar<MODE, CN, 4, D>), grid(4), dim3(256), 0, st, p); break;
+    case 1: hipLaunchKernelGGL((sep_swar<MODE, CN, 1, D>), g, dim3(256), 0, st, p, o); break;
+    case 2: hipLaunchKernelGGL((sep_swar<MODE, CN, 2, D>), g, dim3(256), 0, st, p, o); break;
+    default: hipLaunchKernelGGL((sep_swar<MODE, CN, 4, D>), g, dim3(256), 0, st, p, o); break;
     }
 }
 
@@ -43,12 +50,12 @@ bool launch_swar(const StencilArgs &p, hipStream_t st, int th) {
 
 template <int K, int CN, typename D>
 bool launch_k(const StencilArgs &p, hipStream_t st, int th) {
-    const long n = (long)p.src.w * CN;
-    auto grid = [&](int t) { return dim3((unsigned)((n + 255) / 256), (unsigned)((p.src.h + 4 * t - 1) / (4 * t))); };
+    const TileOrder o = stencil_order(p, CN, th == 1 || th == 2 ? th : 4);
+    const dim3 g(tile_grid(o.ntiles));
     switch (th) {
-    case 1: hipLaunchKernelGGL((sep_small_u8<K, CN, 1, D>), grid(1), dim3(256), 0, st, p); break;
-    case 2: hipLaunchKernelGGL((sep_small_u8<K, CN, 2, D>), grid(2), dim3(256), 0, st, p); break;
-    default: hipLaunchKernelGGL((sep_small_u8<K, CN, 4, D>), grid(4), dim3(256), 0, st, p); break;
+    case 1: hipLaunchKernelGGL((sep_small_u8<K, CN, 1, D>), g, dim3(256), 0, st, p, o); break;
+    case 2: hipLaunchKernelGGL((sep_small_u8<K, CN, 2, D>), g, dim3(256), 0, st, p, o); break;
+    default: hipLaunchKernelGGL((sep_small_u8<K, CN, 4, D>), g, dim3(256), 0, st, p, o); break;
     }
     return true;
 }

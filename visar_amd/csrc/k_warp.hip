@@ -101,9 +101,7 @@ __device__ inline uint32_t persp4_u8(const PerspParams &p, int x, int dx, int y)
         acc += q[j].p01 * w1;
         acc += q[j].p10 * w2;
         acc += q[j].p11 * w3;
-        uint32_t v = round_u8(acc);
-        if (out[j]) v = (uint32_t)p.cvd;
-        res |= v << (8 * j);
+        res = pack_u8(out[j] ? (float)p.cvd : acc, (uint32_t)j, res);
     }
     return res;
 }
@@ -117,10 +115,12 @@ __global__ __launch_bounds__(kThreads) void warp_persp_k(PerspParams p) {
     persp_pixel<T>(p, x, y, row_ptr<T>(d, y) + (long)x * d.cn);
 }
 
-__global__ __launch_bounds__(kThreads) void warp_persp_lin4_u8(PerspParams p) {
+__global__ __launch_bounds__(kThreads) void warp_persp_lin4_u8(PerspParams p, TileOrder o) {
     const DImg &d = p.d;
-    const int y = blockIdx.y * 4 + tile_ty();
-    const int x0 = blockIdx.x * 256;
+    int bx, by;
+    if (!tile_of(o, bx, by)) return;
+    const int y = by * 4 + tile_ty();
+    const int x0 = bx * 256;
     if (y >= d.h || x0 + tile_tx() >= d.w) return;
     store4_strided_u8(d, x0, y, persp4_u8(p, x0 + tile_tx(), 64, y));
 }
@@ -158,11 +158,15 @@ __global__ __launch_bounds__(kThreads) void remap_k(DImg s, DImg d, DImg mxi, DI
     }
 }
 
-__global__ __launch_bounds__(kThreads) void warp_affine_lin4_u8(AffineParams p) { affine_lin4_tile(p, blockIdx.x, blockIdx.y); }
+__global__ __launch_bounds__(kThreads) void warp_affine_lin4_u8(AffineParams p, TileOrder o) {
+    int bx, by;
+    if (tile_of(o, bx, by)) affine_lin4_tile(p, bx, by);
+}
 
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void warp_affine_box_u8(AffineParams p) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void warp_affine_box_u8(AffineParams p, TileOrder o) {
     __shared__ uint32_t lds[kWarpLdsWords];
-    affine_box_tile(p, blockIdx.x, blockIdx.y, lds);
+    int bx, by;
+    if (tile_of(o, bx, by)) affine_box_tile(p, bx, by, lds);
 }
 
 template <typename T>
@@ -180,12 +184,13 @@ inline dim3 grid2d(int w, int h, int px) { return dim3((unsigned)((w + 64 * px -
 int launch_warp_affine(const AffineParams &p, hipStream_t st) {
     const dim3 b(kTileThreads);
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1 && p.border == BORDER_CONSTANT) {
-        const dim3 g((unsigned)((p.d.w + kWarpTW - 1) / kWarpTW), (unsigned)((p.d.h + kWarpTH - 1) / kWarpTH));
-        hipLaunchKernelGGL(warp_affine_box_u8, g, b, 0, st, p);
+        const TileOrder o = tile_order((p.d.w + kWarpTW - 1) / kWarpTW, (p.d.h + kWarpTH - 1) / kWarpTH);
+        hipLaunchKernelGGL(warp_affine_box_u8, dim3(tile_grid(o.ntiles)), b, 0, st, p, o);
         return launched();
     }
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1) {
-        hipLaunchKernelGGL(warp_affine_lin4_u8, grid2d(p.d.w, p.d.h, 4), b, 0, st, p);
+        const TileOrder o = tile_order((p.d.w + 255) / 256, (p.d.h + 3) / 4);
+        hipLaunchKernelGGL(warp_affine_lin4_u8, dim3(tile_grid(o.ntiles)), b, 0, st, p, o);
         return launched();
     }
     const dim3 g = grid2d(p.d.w, p.d.h, 1);
@@ -202,7 +207,8 @@ int launch_warp_affine(const AffineParams &p, hipStream_t st) {
 int launch_warp_perspective(const PerspParams &p, hipStream_t st) {
     const dim3 b(kTileThreads);
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1) {
-        hipLaunchKernelGGL(warp_persp_lin4_u8, grid2d(p.d.w, p.d.h, 4), b, 0, st, p);
+        const TileOrder o = tile_order((p.d.w + 255) / 256, (p.d.h + 3) / 4, true);
+        hipLaunchKernelGGL(warp_persp_lin4_u8, dim3(tile_grid(o.ntiles)), b, 0, st, p, o);
         return launched();
     }
     const dim3 g = grid2d(p.d.w, p.d.h, 1);

@@ -10,6 +10,7 @@ struct Tuning {
     int stencil_rows = 0;        // plain stencils: output rows per thread (1, 2, 4)
     int batch_stencil_rows = 0;  // batched stencils: output rows per thread (1, 2)
     int batch_unit_kb = 0;       // batched calls: KiB of input + output per workgroup
+    int tile_order = 0;          // plain tile kernels: 0 each kernel's default, 1 row-major, 2 one band of tiles per XCD
 };
 Tuning &tuning();
 bool set_tuning(const char *key, int value);

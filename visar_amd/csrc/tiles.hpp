@@ -32,6 +32,31 @@ __device__ __attribute__((always_inline)) inline gptr<T> grow(const DImg &im, lo
     return (gptr<T>)(im.p + y * im.stride);
 }
 
+// Tile (bx, by) of a plain call launched as a 1-D grid of tile_grid(ntiles)
+// workgroups. Workgroup b runs on XCD b % 8. In XCD order it takes tile
+// (b % 8) * ceil(ntiles / 8) + b / 8 of the row-major tile list, so each XCD
+// works through one contiguous band of the image and the halo rows or
+// source boxes that neighbouring tiles share stay in that XCD's L2; in
+// row-major order it takes tile b. False for the padding workgroups.
+struct TileOrder {
+    int tiles_x, ntiles, xcd;
+};
+inline unsigned tile_grid(int ntiles) { return (unsigned)((ntiles + 7) / 8 * 8); }
+// xcd_default: the order measured faster for the kernel (XCD bands for the
+// gather-bound perspective warp, row-major for the others).
+inline TileOrder tile_order(int tiles_x, int tiles_y, bool xcd_default = false) {
+    const int k = tuning().tile_order;
+    return TileOrder{tiles_x, tiles_x * tiles_y, k == 2 || (k == 0 && xcd_default)};
+}
+__device__ __attribute__((always_inline)) inline bool tile_of(const TileOrder &o, int &bx, int &by) {
+    const int b = (int)blockIdx.x;
+    const int t = o.xcd ? (b & 7) * ((o.ntiles + 7) >> 3) + (b >> 3) : b;
+    if (t >= o.ntiles) return false;
+    by = t / o.tiles_x;
+    bx = t - by * o.tiles_x;
+    return true;
+}
+
 __device__ __attribute__((always_inline)) inline int tile_tx() { return threadIdx.x & 63; }
 __device__ __attribute__((always_inline)) inline int tile_ty() { return threadIdx.x >> 6; }
 
@@ -569,6 +594,13 @@ __device__ __attribute__((always_inline)) inline uint32_t round_u8(float v) {
     return (uint32_t)r;
 }
 
+// round_u8(v) written into byte `byte` of acc, other bytes kept
+// (v_cvt_pk_u8_f32: round to nearest even and saturate to [0, 255], equal
+// to round_u8 for every finite v).
+__device__ __attribute__((always_inline)) inline uint32_t pack_u8(float v, uint32_t byte, uint32_t acc) {
+    return __builtin_amdgcn_cvt_pk_u8_f32(v, byte, acc);
+}
+
 // Bilinear tap coordinates of one pixel: border-mapped (-1 = constant
 // border tap); `out` = constant border and the footprint entirely outside.
 struct Taps {
@@ -677,13 +709,12 @@ __device__ __attribute__((always_inline)) inline uint32_t affine4_u8(const Affin
     bool out[4];
     quads_u8(p.s, p.border, p.cv, ix, iy, q, out);
     uint32_t res = 0;
+    const float cvd = (float)p.cvd;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const float v0 = fmaf(ax[j], q[j].p01 - q[j].p00, q[j].p00);
         const float v1 = fmaf(ax[j], q[j].p11 - q[j].p10, q[j].p10);
-        uint32_t v = round_u8(fmaf(ay[j], v1 - v0, v0));
-        if (out[j]) v = (uint32_t)p.cvd;
-        res |= v << (8 * j);
+        res = pack_u8(out[j] ? cvd : fmaf(ay[j], v1 - v0, v0), (uint32_t)j, res);
     }
     return res;
 }
@@ -723,30 +754,46 @@ struct WarpCoord {
     float ax, ay;
 };
 
-// warpAffine coordinates of pixel (x, y) (geometry.zig:432-533).
+// warpAffine coordinates of pixel (x, y) (geometry.zig:432-533). Without
+// the clamp of to_i32 for pixels whose coordinates are known to lie near
+// the image (the clamp then changes nothing).
 struct AffineCoord {
     const float *m;
-    __device__ __attribute__((always_inline)) WarpCoord operator()(int x, int y) const {
+    template <bool CLAMP = true>
+    __device__ __attribute__((always_inline)) WarpCoord at(int x, int y) const {
         const float yf = (float)y;
         const float bxv = m[1] * yf + m[2];
         const float byv = m[4] * yf + m[5];
         const float xf = (float)x;
         const float sx = fmaf(m[0], xf, bxv), sy = fmaf(m[3], xf, byv);
         const float fx = floorf(sx), fy = floorf(sy);
-        return WarpCoord{to_i32(fx), to_i32(fy), sx - fx, sy - fy};
+        if constexpr (CLAMP) return WarpCoord{to_i32(fx), to_i32(fy), sx - fx, sy - fy};
+        else return WarpCoord{(int)fx, (int)fy, sx - fx, sy - fy};
     }
+    __device__ __attribute__((always_inline)) WarpCoord operator()(int x, int y) const { return at<true>(x, y); }
 };
 
-// Quads of a thread's 4 pixels for a one-channel u8 image with
-// BORDER_CONSTANT (value cv), from the LDS box when the workgroup can use
-// it, from global memory otherwise. Every thread of the workgroup calls
-// this (it votes and may synchronise); `live` marks threads whose pixels are
-// in the output.
-template <typename Coord>
-__device__ __attribute__((always_inline)) inline void warp_box_quads(const DImg &s, float cv, Coord coord, int bx,
-                                                                     int by, bool live, const WarpCoord (&c)[4],
-                                                                     uint32_t *lds, Quad (&q)[4], bool (&out)[4]) {
+// One-channel u8 bilinear warpAffine, BORDER_CONSTANT (value cv): a 64 x 16
+// tile, 4 adjacent pixels per thread.
+//
+// The box is the corners' taps widened by one pixel (the per-pixel
+// coordinates differ from the corners' linear extremes by rounding only),
+// clipped to the image. Every live thread checks that each of its in-image
+// taps lies in the box, and the workgroup takes the global gathers if one
+// does not or the box is larger than the LDS tile. Staged row r holds source
+// bytes xlo .. xlo + 123 of row ylo + r from byte 0 (each lane aligns its
+// dword with its neighbour's). A tile whose widened box lies inside the
+// image ("interior") has only in-image taps: a pixel then reads its four
+// taps at one LDS address plus immediate offsets. Within a thread's 4
+// adjacent pixels, each coordinate is a single rounding of a linear function
+// of x, hence monotone, so the end pixels bound the others.
+__device__ __attribute__((always_inline)) inline void affine_box_tile(const AffineParams &p, int bx, int by,
+                                                                      uint32_t *lds) {
+    const DImg &d = p.d, &s = p.s;
     const int W = s.w, H = s.h;
+    const int x = wtile_x(bx), y = wtile_y(by);
+    const bool live = x < d.w && y < d.h;
+    const AffineCoord coord{p.m};
     int xlo = 0x7fffffff, xhi = -0x7fffffff, ylo = 0x7fffffff, yhi = -0x7fffffff;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -756,6 +803,7 @@ __device__ __attribute__((always_inline)) inline void warp_box_quads(const DImg 
         ylo = min(ylo, e.iy);
         yhi = max(yhi, e.iy);
     }
+    const bool interior = xlo - 1 >= 0 && ylo - 1 >= 0 && xhi + 2 <= W - 1 && yhi + 2 <= H - 1;
     xlo = max(xlo - 1, 0);
     ylo = max(ylo - 1, 0);
     xhi = min(xhi + 2, W - 1);
@@ -763,17 +811,34 @@ __device__ __attribute__((always_inline)) inline void warp_box_quads(const DImg 
     const int nb = xhi - xlo + 1, nr = yhi - ylo + 1;
     const bool fits = nb > 0 && nr > 0 && nb <= kWarpBoxBytes && nr <= kWarpBoxRows &&
                       (unsigned long)s.h * (unsigned long)s.stride < (1ul << 31);
+    // Interior tile: its pixels' coordinates lie within two pixels of the
+    // corners' range, inside the image, so the clamp is not needed.
+    WarpCoord c[4];
+    if (interior) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) c[j] = coord.at<false>(x + j, y);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) c[j] = coord(x + j, y);
+    }
     bool bad = false;
     if (live) {
+        if (interior) {
+            bad = min(c[0].ix, c[3].ix) < xlo || max(c[0].ix, c[3].ix) + 1 > xhi || min(c[0].iy, c[3].iy) < ylo ||
+                  max(c[0].iy, c[3].iy) + 1 > yhi;
+        } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int x0 = c[j].ix, y0 = c[j].iy;
-            bad = bad || ((unsigned)x0 < (unsigned)W && (x0 < xlo || x0 > xhi)) ||
-                  ((unsigned)(x0 + 1) < (unsigned)W && (x0 + 1 < xlo || x0 + 1 > xhi)) ||
-                  ((unsigned)y0 < (unsigned)H && (y0 < ylo || y0 > yhi)) ||
-                  ((unsigned)(y0 + 1) < (unsigned)H && (y0 + 1 < ylo || y0 + 1 > yhi));
+            for (int j = 0; j < 4; j++) {
+                const int x0 = c[j].ix, y0 = c[j].iy;
+                bad = bad || ((unsigned)x0 < (unsigned)W && (x0 < xlo || x0 > xhi)) ||
+                      ((unsigned)(x0 + 1) < (unsigned)W && (x0 + 1 < xlo || x0 + 1 > xhi)) ||
+                      ((unsigned)y0 < (unsigned)H && (y0 < ylo || y0 > yhi)) ||
+                      ((unsigned)(y0 + 1) < (unsigned)H && (y0 + 1 < ylo || y0 + 1 > yhi));
+            }
         }
     }
+    Quad q[4];
+    bool out[4];
     const bool use = __syncthreads_or(bad) == 0 && fits;
     if (!use) {
         int ix[4], iy[4];
@@ -782,82 +847,81 @@ __device__ __attribute__((always_inline)) inline void warp_box_quads(const DImg 
             ix[j] = c[j].ix;
             iy[j] = c[j].iy;
         }
-        quads_u8(s, BORDER_CONSTANT, cv, ix, iy, q, out);
-        return;
-    }
-    // Stage rows ylo..yhi, bytes xlo..xhi: row r's dwords start at the
-    // aligned address at or below its byte xlo (shift a_r), 8 rows x 32
-    // dwords per round.
-    const uintptr_t a0 = (uintptr_t)s.p + (uintptr_t)ylo * (uintptr_t)s.stride + (uintptr_t)xlo;
-    uint32_t v[kWarpBoxRows / 8];
+        quads_u8(s, BORDER_CONSTANT, p.cv, ix, iy, q, out);
+    } else {
+        // Stage rows ylo..yhi, 8 rows x 32 dwords per round: lane k of a
+        // 32-lane group loads the k-th aligned dword at or after the row's
+        // byte xlo - a, then aligns it with lane k + 1's.
+        const int k = threadIdx.x & 31;
+        uint32_t v[kWarpBoxRows / 8];
 #pragma unroll
-    for (int t = 0; t < kWarpBoxRows / 8; t++) {
-        const int r = t * 8 + (int)(threadIdx.x >> 5), k = threadIdx.x & 31;
-        v[t] = 0;
-        if (r < nr) {
-            const uint8_t *row = s.p + (long)(ylo + r) * s.stride;
-            const int a = (int)((a0 + (uintptr_t)r * (uintptr_t)s.stride) & 3);
-            const int c0 = xlo - a + 4 * k;  // row offset of this dword
-            if (4 * k < nb + a) {
-                if (c0 >= 0 && c0 + 4 <= W) {
-                    v[t] = *(gptr<const uint32_t>)gp(row + c0);
-                } else {
-                    uint32_t w = 0;
-                    for (int b = 0; b < 4; b++)
-                        if (c0 + b >= 0 && c0 + b < W) w |= (uint32_t)gp(row)[c0 + b] << (8 * b);
-                    v[t] = w;
+        for (int t = 0; t < kWarpBoxRows / 8; t++) {
+            const int r = t * 8 + (int)(threadIdx.x >> 5);
+            v[t] = 0;
+            if (r < nr) {
+                const gptr<const uint8_t> row = grow<const uint8_t>(s, ylo + r);
+                const int a = (int)((uintptr_t)(row + xlo) & 3);
+                const int c0 = xlo - a + 4 * k;  // row offset of this dword
+                if (4 * k < nb + a) {
+                    if (c0 >= 0 && c0 + 4 <= W) {
+                        v[t] = *(gptr<const uint32_t>)(row + c0);
+                    } else {
+                        uint32_t w = 0;
+                        for (int b = 0; b < 4; b++)
+                            if (c0 + b >= 0 && c0 + b < W) w |= (uint32_t)row[c0 + b] << (8 * b);
+                        v[t] = w;
+                    }
                 }
             }
         }
-    }
 #pragma unroll
-    for (int t = 0; t < kWarpBoxRows / 8; t++) {
-        const int r = t * 8 + (int)(threadIdx.x >> 5), k = threadIdx.x & 31;
-        if (r < nr) lds[r * (kWarpBoxPitch / 4) + k] = v[t];
-    }
-    __syncthreads();
-    const uint8_t *box = reinterpret_cast<const uint8_t *>(lds);
+        for (int t = 0; t < kWarpBoxRows / 8; t++) {
+            const int r = t * 8 + (int)(threadIdx.x >> 5);
+            const uint32_t nx = __shfl_down(v[t], 1);
+            const int a = (int)((uintptr_t)(grow<const uint8_t>(s, ylo + r) + xlo) & 3);
+            if (r < nr && k < 31) lds[r * (kWarpBoxPitch / 4) + k] = __builtin_amdgcn_alignbyte(nx, v[t], (uint32_t)a);
+        }
+        __syncthreads();
+        const uint8_t *box = reinterpret_cast<const uint8_t *>(lds);
+        if (interior) {
+            // Only live threads' taps were checked against the box.
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int x0 = c[j].ix, y0 = c[j].iy;
-        const int r0 = min(max(y0 - ylo, 0), nr - 1), r1 = min(max(y0 + 1 - ylo, 0), nr - 1);
-        const int k0 = min(max(x0 - xlo, 0), nb - 1), k1 = min(max(x0 + 1 - xlo, 0), nb - 1);
-        const int s0 = (int)((a0 + (uintptr_t)r0 * (uintptr_t)s.stride) & 3);
-        const int s1 = (int)((a0 + (uintptr_t)r1 * (uintptr_t)s.stride) & 3);
-        const uint32_t b00 = box[r0 * kWarpBoxPitch + s0 + k0], b01 = box[r0 * kWarpBoxPitch + s0 + k1];
-        const uint32_t b10 = box[r1 * kWarpBoxPitch + s1 + k0], b11 = box[r1 * kWarpBoxPitch + s1 + k1];
-        const bool vx0 = (unsigned)x0 < (unsigned)W, vx1 = (unsigned)(x0 + 1) < (unsigned)W;
-        const bool vy0 = (unsigned)y0 < (unsigned)H, vy1 = (unsigned)(y0 + 1) < (unsigned)H;
-        q[j].p00 = vx0 && vy0 ? (float)b00 : cv;
-        q[j].p01 = vx1 && vy0 ? (float)b01 : cv;
-        q[j].p10 = vx0 && vy1 ? (float)b10 : cv;
-        q[j].p11 = vx1 && vy1 ? (float)b11 : cv;
-        out[j] = x0 < -1 || y0 < -1 || x0 >= W || y0 >= H;
-    }
-}
-
-// One-channel u8 bilinear warpAffine, BORDER_CONSTANT: a 64 x 16 tile.
-__device__ __attribute__((always_inline)) inline void affine_box_tile(const AffineParams &p, int bx, int by,
-                                                                      uint32_t *lds) {
-    const DImg &d = p.d;
-    const int x = wtile_x(bx), y = wtile_y(by);
-    const bool live = x < d.w && y < d.h;
-    const AffineCoord coord{p.m};
-    WarpCoord c[4];
+            for (int j = 0; j < 4; j++) {
+                const uint8_t *b = box + (live ? (c[j].iy - ylo) * kWarpBoxPitch + (c[j].ix - xlo) : 0);
+                q[j].p00 = (float)b[0];
+                q[j].p01 = (float)b[1];
+                q[j].p10 = (float)b[kWarpBoxPitch];
+                q[j].p11 = (float)b[kWarpBoxPitch + 1];
+                out[j] = false;
+            }
+        } else {
+            const float cv = p.cv;
 #pragma unroll
-    for (int j = 0; j < 4; j++) c[j] = coord(x + j, y);
-    Quad q[4];
-    bool out[4];
-    warp_box_quads(p.s, p.cv, coord, bx, by, live, c, lds, q, out);
+            for (int j = 0; j < 4; j++) {
+                const int x0 = c[j].ix, y0 = c[j].iy;
+                const int r0 = min(max(y0 - ylo, 0), nr - 1), r1 = min(max(y0 + 1 - ylo, 0), nr - 1);
+                const int k0 = min(max(x0 - xlo, 0), nb - 1), k1 = min(max(x0 + 1 - xlo, 0), nb - 1);
+                const uint32_t b00 = box[r0 * kWarpBoxPitch + k0], b01 = box[r0 * kWarpBoxPitch + k1];
+                const uint32_t b10 = box[r1 * kWarpBoxPitch + k0], b11 = box[r1 * kWarpBoxPitch + k1];
+                const bool vx0 = (unsigned)x0 < (unsigned)W, vx1 = (unsigned)(x0 + 1) < (unsigned)W;
+                const bool vy0 = (unsigned)y0 < (unsigned)H, vy1 = (unsigned)(y0 + 1) < (unsigned)H;
+                q[j].p00 = vx0 && vy0 ? (float)b00 : cv;
+                q[j].p01 = vx1 && vy0 ? (float)b01 : cv;
+                q[j].p10 = vx0 && vy1 ? (float)b10 : cv;
+                q[j].p11 = vx1 && vy1 ? (float)b11 : cv;
+                out[j] = x0 < -1 || y0 < -1 || x0 >= W || y0 >= H;
+            }
+        }
+    }
     if (!live) return;
     uint32_t res = 0;
+    const float cvd = (float)p.cvd;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const float v0 = fmaf(c[j].ax, q[j].p01 - q[j].p00, q[j].p00);
         const float v1 = fmaf(c[j].ax, q[j].p11 - q[j].p10, q[j].p10);
-        uint32_t v = round_u8(fmaf(c[j].ay, v1 - v0, v0));
-        if (out[j]) v = (uint32_t)p.cvd;
-        res |= v << (8 * j);
+        const float v = fmaf(c[j].ay, v1 - v0, v0);
+        res = pack_u8(out[j] ? cvd : v, (uint32_t)j, res);
     }
     store4_u8(d, x, y, res);
 }

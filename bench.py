@@ -8,7 +8,7 @@ scale U(0.8, 1.2), shift U(+-10 %)), and warpPerspective with the corners
 jittered by +-5 %. One step = the whole chain on one frame, every output
 written. Inputs are resident in HBM and rotate through 24 distinct frames
 (597 MB, more than twice the 256 MB MALL). The chain is enqueued through the
-C ABI on torch's stream and replayed from a HIP graph.
+C ABI on torch's stream and replayed from HIP graphs of 24 steps each.
 
 Prints one JSON line: value = input Gpix/s of the whole chain, a per-kernel
 table (HIP-event durations, algorithmic bytes, GB/s), the roofline of the
@@ -42,6 +42,15 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (scales the HBM byte counters of a profile)")
     return ap.parse_args()
+
+
+def profiled_traffic(key):
+    """HBM bytes per launch of `key` from the committed rocprofv3 counter
+    passes (profiles/traffic.json), or None."""
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(tf):
+        return None
+    return json.load(open(tf)).get("traffic_bytes_per_launch", {}).get(key)
 
 
 def per_launch_ms(torch, fn, n, stream):
@@ -121,7 +130,7 @@ def bench_batch(args, torch, va):
                            "queued_images_per_s": round(512 / plain_async_s, 1),
                            "speedup_vs_queued": round(plain_async_s * 1e3 / ms, 2)},
         "roofline": {"kernel": "batch_kernel", "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": profiled_traffic("batch_graph"),
                      "algorithmic_bytes": nbytes},
         "autotune_ms": {k: round(v, 4) for k, v in timings.items()},
     }
@@ -332,6 +341,9 @@ def bench_serve(args, torch, va):
     }
     # Fused kernel alone: per-frame duration over a full batch (HIP events).
     line["roofline"].update(serve_kernel_roofline(torch, va, W, H, N, fused_bytes))
+    per_frame = profiled_traffic("serve_k_per_frame")
+    if per_frame is not None:
+        line["roofline"]["traffic"] = int(per_frame * line["roofline"]["frames_per_launch"])
     if not args.no_cpu:
         import oracle
         t0 = time.perf_counter()
@@ -443,22 +455,32 @@ def main():
         step(i)
     torch.cuda.synchronize()
 
-    # One graph per input slot: the chain's launches replay without host work.
-    graphs = []
+    # Graphs of G consecutive steps (input slots taken mod R, R a multiple of
+    # G), so the GPU and not the host's launch rate sets the pace: K steps
+    # replay K // G chunk graphs and then K % G single-step graphs.
+    G = 24
+    n_chunks = R // G
+    chunks, singles = [], []
     cap_stream = torch.cuda.Stream()
     cap_stream.wait_stream(stream)
     with torch.cuda.stream(cap_stream):
         va.set_stream(cap_stream)
-        for i in range(R):
+        for c in range(n_chunks):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=cap_stream):
+                for j in range(G):
+                    step(c * G + j)
+            chunks.append(g)
+        for i in range(G):
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g, stream=cap_stream):
                 step(i)
-            graphs.append(g)
+            singles.append(g)
     va.set_stream(stream)
     stream.wait_stream(cap_stream)
     torch.cuda.synchronize()
-    for i in range(R):  # graph warmup
-        graphs[i].replay()
+    for g in chunks + singles:  # graph warmup
+        g.replay()
     torch.cuda.synchronize()
 
     K = args.steps
@@ -466,8 +488,10 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0.record(stream)
-    for i in range(K):
-        graphs[i % R].replay()
+    for c in range(K // G):
+        chunks[c % n_chunks].replay()
+    for i in range(K % G):
+        singles[i].replay()
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -483,12 +507,7 @@ def main():
                          "frac_hbm_peak": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     achieved = kernels[dom]["GB/s"]
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tf):
-        # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
-        # passes (scaled by a plain copy's reading), keyed by kernel.
-        traffic = json.load(open(tf)).get("traffic_bytes_per_launch", {}).get(dom)
+    traffic = profiled_traffic(dom)
 
     cpu = None
     if not args.no_cpu:
@@ -507,7 +526,8 @@ def main():
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic (splitmix64 uniform bytes, 24 rotating device frames)",
-        "config": {"workload": workload, "width": W, "height": H, "rotating_inputs": R, "graph_replay": True},
+        "config": {"workload": workload, "width": W, "height": H, "rotating_inputs": R, "graph_replay": True,
+                   "steps_per_graph": G},
         "kernels": kernels,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

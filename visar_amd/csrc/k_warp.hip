@@ -184,7 +184,7 @@ inline dim3 grid2d(int w, int h, int px) { return dim3((unsigned)((w + 64 * px -
 int launch_warp_affine(const AffineParams &p, hipStream_t st) {
     const dim3 b(kTileThreads);
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1 && p.border == BORDER_CONSTANT) {
-        const TileOrder o = tile_order((p.d.w + kWarpTW - 1) / kWarpTW, (p.d.h + kWarpTH - 1) / kWarpTH);
+        const TileOrder o = tile_order((p.d.w + kWarpTW - 1) / kWarpTW, (p.d.h + kWarpTH - 1) / kWarpTH, true);
         hipLaunchKernelGGL(warp_affine_box_u8, dim3(tile_grid(o.ntiles)), b, 0, st, p, o);
         return launched();
     }

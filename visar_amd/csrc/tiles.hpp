@@ -42,8 +42,10 @@ struct TileOrder {
     int tiles_x, ntiles, xcd;
 };
 inline unsigned tile_grid(int ntiles) { return (unsigned)((ntiles + 7) / 8 * 8); }
-// xcd_default: the order measured faster for the kernel (XCD bands for the
-// gather-bound perspective warp, row-major for the others).
+// xcd_default: the kernel's default order. XCD bands for the warps: the
+// perspective warp runs faster in them, and the affine box warp, whose
+// neighbouring tiles' boxes overlap, reads a quarter of the bytes from
+// memory at the same speed; row-major for the others.
 inline TileOrder tile_order(int tiles_x, int tiles_y, bool xcd_default = false) {
     const int k = tuning().tile_order;
     return TileOrder{tiles_x, tiles_x * tiles_y, k == 2 || (k == 0 && xcd_default)};
@@ -657,6 +659,20 @@ __device__ __attribute__((always_inline)) inline void quads_u8(const DImg &s, in
     bool inner = (unsigned long)s.h * (unsigned long)s.stride < (1ul << 32);
 #pragma unroll
     for (int j = 0; j < 4; j++) inner = inner && inner_tap(ix[j], iy[j], W, H);
+    if (border == BORDER_CONSTANT) {
+        // Every pixel of the wave outside by the constant-border rule: no tap is read.
+        bool none = true;
+#pragma unroll
+        for (int j = 0; j < 4; j++) none = none && (ix[j] < -1 || iy[j] < -1 || ix[j] >= W || iy[j] >= H);
+        if (__all(none)) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                q[j].p00 = q[j].p01 = q[j].p10 = q[j].p11 = cv;
+                out[j] = true;
+            }
+            return;
+        }
+    }
     if (__all(inner)) {
         const gptr<const uint8_t> base = gp((const uint8_t *)s.p);
         const uint32_t st = (uint32_t)s.stride;
@@ -820,6 +836,19 @@ __device__ __attribute__((always_inline)) inline void affine_box_tile(const Affi
     } else {
 #pragma unroll
         for (int j = 0; j < 4; j++) c[j] = coord(x + j, y);
+    }
+    // A tile whose every live pixel lies outside by the constant-border rule
+    // (ix < -1, iy < -1, ix >= W or iy >= H: the value without reading a tap)
+    // is written at once.
+    bool none = true;
+#pragma unroll
+    for (int j = 0; j < 4; j++) none = none && (c[j].ix < -1 || c[j].iy < -1 || c[j].ix >= W || c[j].iy >= H);
+    if (__syncthreads_and(!live || none)) {
+        if (live) {
+            const uint32_t v = (uint32_t)p.cvd;
+            store4_u8(d, x, y, v * 0x01010101u);
+        }
+        return;
     }
     bool bad = false;
     if (live) {

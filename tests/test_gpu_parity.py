@@ -546,3 +546,53 @@ def test_connected_components(va, orc):
             assert_same(lab, rlab)
             assert_same(st, rst)
             np.testing.assert_array_equal(np.nan_to_num(ce, nan=-7), np.nan_to_num(rce, nan=-7))
+
+
+def cc_masks():
+    rng = np.random.default_rng(7)
+    yield "noise2048", ((rng.random((2048, 2048)) > 0.55) * 255).astype(np.uint8)
+    yy, xx = np.mgrid[0:301, 0:257]
+    yield "checker", (((yy + xx) % 2) * 255).astype(np.uint8)
+    yield "checker2", ((((yy // 2) + (xx // 2)) % 2) * 200).astype(np.uint8)
+    yield "diagonals", (((xx - yy) % 5 == 0) * 255).astype(np.uint8)
+    yield "antidiagonals", (((xx + yy) % 7 == 0) * 9).astype(np.uint8)
+    yield "full", np.full((64, 65), 255, np.uint8)
+    yield "empty", np.zeros((33, 70), np.uint8)
+    yield "one", np.full((1, 1), 1, np.uint8)
+    by, bx = np.mgrid[0:720, 0:1280]
+    blobs = np.zeros((720, 1280), np.uint8)
+    for _ in range(40):
+        cy, cx, r = rng.integers(0, 720), rng.integers(0, 1280), rng.integers(3, 60)
+        blobs[(by - cy) ** 2 + (bx - cx) ** 2 < r * r] = 255
+    yield "blobs", blobs
+    wy, wx = np.mgrid[0:200, 0:311]
+    yield "waves", ((np.sin(wy / 3.0) * np.cos(wx / 5.0) > 0.2) * 255).astype(np.uint8)
+
+
+def test_connected_components_device_cases(va, orc):
+    """GPU labelling on device tensors (misaligned strided views included)
+    against the oracle: labels, statistics rows and centroids, both
+    connectivities, down to the maximum-component checkerboards."""
+    for name, m in cc_masks():
+        h, w = m.shape
+        for conn in (4, 8):
+            rn, rlab, rst, rce = orc.connected_components_with_stats(m, conn)
+            big = torch.zeros((h, w + 3), dtype=torch.uint8, device="cuda")
+            big[:, 1:w + 1] = dev(m)
+            mv = big[:, 1:w + 1]
+            lab = torch.full((h, w), -5, dtype=torch.int32, device="cuda")
+            n, _, st, ce = va.connected_components_with_stats(mv, conn)
+            assert n == rn, f"{name} conn {conn}: {n} labels, oracle {rn}"
+            assert_same(st, rst, f"{name} conn {conn} stats")
+            np.testing.assert_array_equal(np.nan_to_num(ce, nan=-7), np.nan_to_num(rce, nan=-7), f"{name} conn {conn}")
+            cap = max(1, min(rn - 1, 5))
+            stats = np.zeros((cap, 5), np.int32)
+            cent = np.zeros((cap, 2), np.float64)
+            count = va.ctypes.c_size_t(0)
+            va._check(va.lib.visar_connected_components(va._ref(mv), va._ref(lab), conn,
+                                                        stats.ctypes.data_as(va.ctypes.c_void_p),
+                                                        cent.ctypes.data_as(va.ctypes.c_void_p), va.ctypes.c_size_t(cap),
+                                                        va.ctypes.byref(count)))
+            assert count.value == rn
+            assert_same(lab.cpu().numpy(), rlab, f"{name} conn {conn} device labels")
+            assert_same(stats, rst[:cap], f"{name} conn {conn} truncated stats")

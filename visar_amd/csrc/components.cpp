@@ -1,4 +1,6 @@
-// components.cpp -- connectedComponentsWithStats on the host.
+// components.cpp -- connectedComponentsWithStats: the C ABI entry point
+// (labelled on the GPU by k_components.hip, finished here) and a host
+// labeller used by the served front end's CPU workers.
 //
 // Reference: src/imgproc/components.zig:408-535, C ABI capi.zig:533-550.
 // Labels follow OpenCV: 4-connected components are numbered in raster order
@@ -17,6 +19,7 @@
 
 #include "../../include/visar.h"
 #include "components.hpp"
+#include "components_gpu.hpp"
 #include "runtime.hpp"
 
 namespace visar {
@@ -185,34 +188,60 @@ extern "C" int visar_connected_components(const visar_image *src, const visar_im
     if (!count) return fail(INVALID_ARGUMENT, "InvalidArgument");
     if (src->channels != 1 || labels->channels != 1 || src->width != labels->width || src->height != labels->height)
         return fail(SIZE_MISMATCH, "SizeMismatch");
-    if ((rc = ensure_runtime())) return rc;
     const uint32_t w = src->width, h = src->height;
-    // Device buffers are brought to the host: labelling is a sequential
-    // union-find over runs and runs on the CPU.
-    std::vector<uint8_t> hsrc;
-    const uint8_t *sp = (const uint8_t *)src->data;
-    size_t sstride = src->stride;
-    if (is_device_ptr(src->data)) {
-        hsrc.resize((size_t)w * h);
-        if (hipMemcpy2D(hsrc.data(), w, src->data, src->stride, w, h, hipMemcpyDeviceToHost) != hipSuccess)
-            return fail(INTERNAL, "CopyFailed");
-        sp = hsrc.data();
-        sstride = w;
+    if ((uint64_t)w * h >= (1ull << 31)) return fail(UNSUPPORTED, "Unsupported");
+    // Labelled on the GPU (k_components.hip); the host turns the integer
+    // sums into the statistics rows and centroids.
+    Call call;
+    DImg s, lv;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, s)) || (rc = call.output(labels, lv))) return rc;
+    CcArgs c{};
+    c.src = s.p;
+    c.sstride = (size_t)s.stride;
+    c.w = w;
+    c.h = h;
+    c.eight = connectivity != 4;
+    c.labels = reinterpret_cast<int32_t *>(lv.p);
+    c.lstride = (size_t)lv.stride;
+    void *scratch = call.scratch(cc_scratch_bytes(w, h));
+    if (!scratch) return fail(OUT_OF_MEMORY, "OutOfMemory");
+    uint32_t n = 0;
+    if (cc_label(c, scratch, call.stream(), &n) != OK) return call.finish(INTERNAL);
+    std::vector<uint64_t> acc(4 * (size_t)n), rows(4 * (size_t)h);
+    std::vector<uint32_t> box(4 * (size_t)n);
+    hipStream_t st = call.stream();
+    if (hipMemcpyAsync(acc.data(), c.acc, acc.size() * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(box.data(), c.box, box.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(rows.data(), c.rows, rows.size() * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return call.finish(INTERNAL);
+    std::vector<CompStats> out(n);
+    CompStats &bg = out[0];
+    bg.x0 = bg.y0 = std::numeric_limits<uint64_t>::max();
+    bg.x1 = bg.y1 = bg.area = bg.sx2 = bg.sy = 0;
+    for (uint32_t y = 0; y < h; y++) {
+        const uint64_t zeros = rows[4 * (size_t)y];
+        if (!zeros) continue;
+        bg.x0 = std::min(bg.x0, rows[4 * (size_t)y + 1]);
+        bg.x1 = std::max(bg.x1, rows[4 * (size_t)y + 2]);
+        bg.y0 = std::min<uint64_t>(bg.y0, y);
+        bg.y1 = std::max<uint64_t>(bg.y1, y);
+        bg.area += zeros;
+        bg.sx2 += (uint64_t)w * (w - 1) - rows[4 * (size_t)y + 3];
+        bg.sy += zeros * y;
     }
-    bool ldev = is_device_ptr(labels->data);
-    std::vector<int32_t> hlab;
-    int32_t *lp = (int32_t *)labels->data;
-    size_t lstride = labels->stride;
-    if (ldev) {
-        hlab.resize((size_t)w * h);
-        lp = hlab.data();
-        lstride = (size_t)w * 4;
+    for (uint32_t l = 1; l < n; l++) {
+        CompStats &o = out[l];
+        o.x0 = box[4 * (size_t)l];
+        o.y0 = box[4 * (size_t)l + 1];
+        o.x1 = box[4 * (size_t)l + 2];
+        o.y1 = box[4 * (size_t)l + 3];
+        o.area = acc[4 * (size_t)l];
+        o.sx2 = acc[4 * (size_t)l + 1];
+        o.sy = acc[4 * (size_t)l + 2];
     }
-    std::vector<CompStats> st;
-    *count = label_components(sp, sstride, lp, lstride, w, h, connectivity != 4, st);
-    write_stats(st, stats, centroids, max);
-    if (ldev && hipMemcpy2D(labels->data, labels->stride, hlab.data(), (size_t)w * 4, (size_t)w * 4, h,
-                            hipMemcpyHostToDevice) != hipSuccess)
-        return fail(INTERNAL, "CopyFailed");
-    return VISAR_OK;
+    *count = n;
+    write_stats(out, stats, centroids, max);
+    return call.finish(OK);
 }

@@ -1,4 +1,5 @@
-// components.hpp -- host connected-component labelling with statistics.
+// components.hpp -- host connected-component labelling with statistics (the
+// served front end's CPU workers) and the C ABI's statistics rows.
 #pragma once
 
 #include <cstddef>

@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--streams", type=int, default=64)
     ap.add_argument("--seconds", type=float, default=6.0, help="served workload: Poisson run length")
+    ap.add_argument("--host-workers", type=int, default=8, help="served workload: connected-components threads")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -51,6 +52,48 @@ def profiled_traffic(key):
     if not os.path.exists(tf):
         return None
     return json.load(open(tf)).get("traffic_bytes_per_launch", {}).get(key)
+
+
+CPU_THREADS = 16  # the GPU host's CPU share for one GPU
+
+
+def cpu_measure(seconds, make_worker, per_unit, unit, what):
+    """The oracle on a bounded sample: `make_worker(k)` returns a callable
+    doing one unit of the workload per call with worker k's own inputs. Runs
+    one worker for half of `seconds`, then CPU_THREADS workers on threads
+    (the oracle's C calls release the GIL) for the other half. `value` is the
+    all-cores rate; `value_1_thread` the single-thread one."""
+    import threading
+
+    def timed(threads, secs):
+        workers = [make_worker(k) for k in range(threads)]
+        counts = [0] * threads
+        t0 = time.perf_counter()
+        stop = t0 + secs
+
+        def run(k):
+            n = 0
+            while True:
+                workers[k](n)
+                n += 1
+                if time.perf_counter() >= stop:
+                    break
+            counts[k] = n
+
+        ts = [threading.Thread(target=run, args=(k,)) for k in range(threads)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        return sum(counts), time.perf_counter() - t0
+
+    n1, dt1 = timed(1, seconds / 2)
+    threads = max(1, min(CPU_THREADS, os.cpu_count() or 1))
+    nn, dtn = timed(threads, seconds / 2)
+    return {"value": round(nn * per_unit / dtn, 6), "unit": unit, "cores": threads, "kind": "port",
+            "value_1_thread": round(n1 * per_unit / dt1, 6),
+            "sample": f"{what}: {nn} units on {threads} threads in {dtn:.1f} s, {n1} on one thread in {dt1:.1f} s "
+                      f"(scalar C oracle, -O2, one call per unit per thread)"}
 
 
 def per_launch_ms(torch, fn, n, stream):
@@ -136,14 +179,9 @@ def bench_batch(args, torch, va):
     }
     if not args.no_cpu:
         import oracle
-        t0 = time.perf_counter()
-        n = 0
-        while time.perf_counter() - t0 < args.cpu_seconds or n == 0:
-            batch.run_oracle(oracle, jobs[n % len(jobs)])
-            n += 1
-        dt = time.perf_counter() - t0
-        line["cpu_baseline"] = {"value": round(n / dt, 2), "unit": "images/s", "cores": 1, "kind": "port",
-                                "sample": f"{n} images of the batch through the scalar C oracle in {dt:.1f} s"}
+        line["cpu_baseline"] = cpu_measure(
+            args.cpu_seconds, lambda k: (lambda n: batch.run_oracle(oracle, jobs[(k * 37 + n) % len(jobs)])), 1,
+            "images/s", "images of the batch")
     print(json.dumps(line))
 
 
@@ -201,17 +239,16 @@ def bench_morph(args, torch, va):
     }
     if not args.no_cpu:
         import oracle
-        t0 = time.perf_counter()
-        n = 0
-        while time.perf_counter() - t0 < args.cpu_seconds or n == 0:
-            op = n % 7
-            shape, ks, _ = MORPH_ELEMENTS[(n // 7) % 3]
-            oracle.morphology(base, op, shape, (ks, ks), 1)
-            n += 1
-        dt = time.perf_counter() - t0
-        line["cpu_baseline"] = {"value": round(n * S * S / dt / 1e9, 5), "unit": "Gpix/s", "cores": 1, "kind": "port",
-                                "sample": f"{n} morphology calls (ops x elements, iteration 1) on the 4096x4096 mask "
-                                          f"through the scalar C oracle in {dt:.1f} s"}
+
+        def morph_worker(k):
+            def one(n):
+                n += k
+                shape, ks, _ = MORPH_ELEMENTS[(n // 7) % 3]
+                oracle.morphology(base, n % 7, shape, (ks, ks), 1)
+            return one
+
+        line["cpu_baseline"] = cpu_measure(args.cpu_seconds, morph_worker, S * S / 1e9, "Gpix/s",
+                                           "morphology calls (ops x elements, iteration 1) on the 4096x4096 mask")
     print(json.dumps(line))
 
 
@@ -227,7 +264,7 @@ def bench_serve(args, torch, va):
     cams = serve.CameraSet(N, W, H, seed=0, pool=32)
     cycle = 30  # frames rendered per stream, replayed in order
     frames = [[cams.next_frame(s).copy() for _ in range(cycle)] for s in range(N)]
-    srv = serve.Server(width=W, height=H, max_streams=N, max_batch=N, queue_depth=4, host_workers=8,
+    srv = serve.Server(width=W, height=H, max_streams=N, max_batch=N, queue_depth=4, host_workers=args.host_workers,
                        max_inflight=2)
     ids = [srv.admit() for _ in range(N)]
     st0 = srv.stats()
@@ -323,7 +360,8 @@ def bench_serve(args, torch, va):
         "data": f"synthetic 1280x720 streams (gradient 60->140 + N(0,4^2) from a pool of 32 noise fields, 5 moving "
                 f"ellipses of 210); {cycle} frames per stream replayed in order",
         "config": {"workload": "served: blur 5x5 -> background (shift 4, thr 28) -> threshold -> open 3x3 -> host CC",
-                   "streams": N, "width": W, "height": H, "max_batch": N, "queue_depth": 4, "host_workers": 8},
+                   "streams": N, "width": W, "height": H, "max_batch": N, "queue_depth": 4,
+                   "host_workers": args.host_workers},
         "capacity": {"frames": N * n_cap, "seconds": round(cap_seconds, 3), "fps": round(cap_fps, 1),
                      "latency_ms_p50": round(float(np.percentile(cap_lat, 50)) * 1e3, 3),
                      "latency_ms_p99": round(float(np.percentile(cap_lat, 99)) * 1e3, 3)},
@@ -346,21 +384,23 @@ def bench_serve(args, torch, va):
         line["roofline"]["traffic"] = int(per_frame * line["roofline"]["frames_per_launch"])
     if not args.no_cpu:
         import oracle
-        t0 = time.perf_counter()
-        n = 0
-        bg = oracle.Background(W, H, 4)
-        while time.perf_counter() - t0 < args.cpu_seconds or n == 0:
-            f = frames[0][n % cycle]
-            b = oracle.gaussian_blur(f, (5, 5), 0)
-            m = np.empty_like(f)
-            bg.apply(b, m, 28)
-            tt = oracle.threshold(m, 127, 255, 0)[1]
-            o = oracle.morphology(tt, 2, 0, (3, 3))
-            oracle.connected_components_with_stats(o)
-            n += 1
-        dt = time.perf_counter() - t0
-        line["cpu_baseline"] = {"value": round(n / dt, 2), "unit": "frames/s", "cores": 1, "kind": "port",
-                                "sample": f"{n} frames of one stream through the scalar C oracle pipeline in {dt:.1f} s"}
+
+        def stream_worker(k):
+            bg = oracle.Background(W, H, 4)
+            fl = frames[k % len(frames)]
+
+            def one(n):
+                f = fl[n % cycle]
+                b = oracle.gaussian_blur(f, (5, 5), 0)
+                m = np.empty_like(f)
+                bg.apply(b, m, 28)
+                tt = oracle.threshold(m, 127, 255, 0)[1]
+                o = oracle.morphology(tt, 2, 0, (3, 3))
+                oracle.connected_components_with_stats(o)
+            return one
+
+        line["cpu_baseline"] = cpu_measure(args.cpu_seconds, stream_worker, 1, "frames/s",
+                                           "frames through the oracle pipeline, one stream per thread")
     print(json.dumps(line))
 
 
@@ -537,33 +577,33 @@ def main():
 
 
 def cpu_baseline(args, synth):
-    """The oracle (scalar C restatement, one thread) on a bounded sample of the
-    same workload: whole steps until about --cpu-seconds of CPU work."""
+    """The oracle on a bounded sample of the same workload (whole steps, each
+    worker on its own input)."""
     import oracle
-    done, t0 = 0, time.perf_counter()
     if args.config == 2:
         W, H = 3840, 2160
-        a = synth.uniform_bytes(0, (H, W, 3))
         ma = synth.affine_matrix(0, W // 2, H // 2)
         mp = synth.perspective_matrix(0, W // 2, H // 2)
-        while time.perf_counter() - t0 < args.cpu_seconds or done == 0:
-            g = oracle.cvt_color(a, 6)
-            s = oracle.resize(g, (W // 2, H // 2), 1)
-            oracle.sobel(s, np.int16, 1, 0, 3)
-            oracle.warp_affine(s, ma, (W // 2, H // 2), 1, 0, 0)
-            oracle.warp_perspective(s, mp, (W // 2, H // 2), 1, 0, 0)
-            done += 1
-        sample = f"{done} full chains of 3840x2160x3 on one frame"
-    else:
-        W, H = 1920, 1080
-        a = synth.uniform_bytes(0, (H, W))
-        while time.perf_counter() - t0 < args.cpu_seconds or done == 0:
-            oracle.gaussian_blur(a, (5, 5), 0)
-            done += 1
-        sample = f"{done} GaussianBlur 5x5 calls on 1920x1080"
-    dt = time.perf_counter() - t0
-    return {"value": round(done * W * H / dt / 1e9, 6), "unit": "Gpix/s", "cores": 1, "kind": "port",
-            "sample": sample + f" in {dt:.1f} s (scalar C oracle, -O2, one thread)"}
+
+        def chain_worker(k):
+            a = synth.uniform_bytes(k, (H, W, 3))
+
+            def one(n):
+                g = oracle.cvt_color(a, 6)
+                s = oracle.resize(g, (W // 2, H // 2), 1)
+                oracle.sobel(s, np.int16, 1, 0, 3)
+                oracle.warp_affine(s, ma, (W // 2, H // 2), 1, 0, 0)
+                oracle.warp_perspective(s, mp, (W // 2, H // 2), 1, 0, 0)
+            return one
+
+        return cpu_measure(args.cpu_seconds, chain_worker, W * H / 1e9, "Gpix/s", "full chains of 3840x2160x3")
+    W, H = 1920, 1080
+
+    def blur_worker(k):
+        a = synth.uniform_bytes(k, (H, W))
+        return lambda n: oracle.gaussian_blur(a, (5, 5), 0)
+
+    return cpu_measure(args.cpu_seconds, blur_worker, W * H / 1e9, "Gpix/s", "GaussianBlur 5x5 calls on 1920x1080")
 
 
 if __name__ == "__main__":

@@ -456,6 +456,26 @@ def test_morphology_cases(va, orc, shape):
     check(va, orc, "morphology", g, 1, 1, (7, 7), iterations=0)
 
 
+def test_morphology_binary_masks(va, orc):
+    """{0, 255} masks (the morphology benchmark's input class), and the same
+    masks with one pixel of 128: every op, element and iteration count, 1 and
+    3 channels, down to one-pixel images."""
+    rng = np.random.default_rng(31)
+    for (h, w) in [(97, 601), (1, 1), (5, 300), (300, 5), (64, 256)]:
+        for cn in (1, 3):
+            shape_ = (h, w) if cn == 1 else (h, w, cn)
+            m = ((rng.random(shape_) > 0.5) * 255).astype(np.uint8)
+            mixed = m.copy()
+            mixed.flat[(h * w * cn) // 2] = 128
+            for a in (m, mixed):
+                for op in range(7):
+                    for shape, ks in [(0, (3, 3)), (1, (5, 5)), (2, (7, 7)), (0, (5, 3))]:
+                        check(va, orc, "morphology", a, op, shape, ks)
+                for it in (2, 3):
+                    check(va, orc, "morphology", a, 2, 0, (3, 3), iterations=it)
+                    check(va, orc, "morphology", a, 3, 1, (5, 5), iterations=it)
+
+
 def test_morphology_depths(va, orc):
     for dt in (np.uint16, np.int16, np.float32):
         a = natural(37, 45, 3, 50, dt)

@@ -13,7 +13,9 @@ C ABI on torch's stream and replayed from HIP graphs of 24 steps each.
 Prints one JSON line: value = input Gpix/s of the whole chain, a per-kernel
 table (HIP-event durations, algorithmic bytes, GB/s), the roofline of the
 dominant kernel, and the CPU oracle timed on a bounded sample of the same
-workload. `--config 1` times the plain 1080p GaussianBlur 5x5 instead.
+workload. `--config 1` times the plain 1080p GaussianBlur 5x5 instead;
+`--config 6` the tracking image of a 3840x2160 BGRA frame (grayDownsample,
+BT.601 gray shrunk by 6 with box averaging, to 640x360).
 """
 import argparse
 import json
@@ -34,7 +36,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=240)
     ap.add_argument("--warmup", type=int, default=24)
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5, 6])
     ap.add_argument("--streams", type=int, default=64)
     ap.add_argument("--seconds", type=float, default=6.0, help="served workload: Poisson run length")
     ap.add_argument("--host-workers", type=int, default=8, help="served workload: connected-components threads")
@@ -477,6 +479,14 @@ def main():
         }
         pix_per_step = W * H
         workload = "chain 3840x2160x3 u8: gray, resize linear 1/2, sobel dx i16, warpAffine, warpPerspective"
+    elif args.config == 6:
+        W, H, K6 = 3840, 2160, 6
+        frames = [torch.from_numpy(synth.uniform_bytes(i, (H, W, 4))).to(dev) for i in range(R)]
+        outs = [torch.empty((H // K6, W // K6), dtype=torch.uint8, device=dev) for _ in range(2)]
+        ops = {"gray_downsample_k6": (lambda i: va.gray_downsample(frames[i % R], K6, dst=outs[i % 2]),
+                                      W * H * 4 + (W // K6) * (H // K6))}
+        pix_per_step = W * H
+        workload = "grayDownsample 3840x2160 BGRA u8 by 6 to 640x360 (tracking image)"
     else:
         W, H = 1920, 1080
         frames = [torch.from_numpy(synth.uniform_bytes(i, (H, W))).to(dev) for i in range(R * 6)]
@@ -554,7 +564,8 @@ def main():
         cpu = cpu_baseline(args, synth)
 
     line = {
-        "metric": "Gpix/s (input pixels through the chain) + HBM GB/s per kernel",
+        "metric": "Gpix/s (input pixels through the chain) + HBM GB/s per kernel" if args.config == 2 else
+                  "Gpix/s (input pixels) + HBM GB/s per kernel",
         "value": round(value, 4),
         "unit": "Gpix/s",
         "n_gpus": 1,
@@ -597,6 +608,15 @@ def cpu_baseline(args, synth):
             return one
 
         return cpu_measure(args.cpu_seconds, chain_worker, W * H / 1e9, "Gpix/s", "full chains of 3840x2160x3")
+    if args.config == 6:
+        W, H = 3840, 2160
+
+        def gd_worker(k):
+            a = synth.uniform_bytes(k, (H, W, 4))
+            return lambda n: oracle.gray_downsample(a, 6)
+
+        return cpu_measure(args.cpu_seconds, gd_worker, W * H / 1e9, "Gpix/s",
+                           "grayDownsample calls on 3840x2160 BGRA, k = 6")
     W, H = 1920, 1080
 
     def blur_worker(k):

@@ -66,6 +66,12 @@ int visar_get_num_threads(void);
  * 11 RGBA2GRAY (8U, 16U, 32F); other codes return VISAR_UNSUPPORTED. */
 int visar_cvt_color(const visar_image *src, const visar_image *dst, int code);
 
+/* reference include/visar.h:53 (capi.zig:176-182 -> videoio/frame.zig:170 grayDownsample):
+ * BT.601 gray of a BGR or BGRA 8U image shrunk by k (1..16) with box averaging
+ * in one pass; dst is (width / k) x (height / k), 1 channel. k <= 0:
+ * VISAR_INVALID_ARGUMENT; k > 16, bad channels or sizes: VISAR_SIZE_MISMATCH. */
+int visar_gray_downsample(const visar_image *src, const visar_image *dst, int k);
+
 /* reference include/visar.h:54. 8U with 1-4 channels: INTER_LINEAR (1) and
  * INTER_AREA (3) in OpenCV's fixed point; INTER_NEAREST (0) for every depth. */
 int visar_resize(const visar_image *src, const visar_image *dst, int interpolation);
@@ -117,6 +123,13 @@ int visar_warp_perspective(const visar_image *src, const visar_image *dst, const
  * ky: []const f32, border); src and dst 8U/16U/16S/32F, f32 accumulation. */
 int visar_sep_filter_2d(const visar_image *src, const visar_image *dst, const float *kx, size_t nkx, const float *ky,
                         size_t nky, int border);
+
+/* reference src/videoio/frame.zig:182 grayDownsampleRect(exec, src, dst, k, x, y, w, h):
+ * only dst's (x, y, w, h) window, bit-identical to visar_gray_downsample (same
+ * argument checks); the rest of dst is left as it is. A window outside dst: VISAR_INVALID_ARGUMENT
+ * (the reference's OutOfBounds); w or h zero does nothing. */
+int visar_gray_downsample_rect(const visar_image *src, const visar_image *dst, int k, uint32_t x, uint32_t y,
+                               uint32_t w, uint32_t h);
 
 #ifdef __cplusplus
 }

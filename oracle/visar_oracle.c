@@ -627,6 +627,65 @@ static double triangle(const uint32_t *hist) {
 }
 
 /* threshold.zig:25-119; THRESH_OTSU (8) / THRESH_TRIANGLE (16) on 8U one-channel images */
+/* videoio/frame.zig:167-186 grayDownsample, Kernel.pairs :292-317 (k = 2:
+ * integer, (sum + 2^15) >> 16) and Kernel.blocks :319-336 (other k: the
+ * weighted block sum as f32 times the f32 reciprocal of k*k*2^14, plus 0.5,
+ * truncated, at most 255). BGR or BGRA (alpha unused); weights 1868, 9617,
+ * 4899 (:186-189); leftover columns and rows are ignored. */
+int orc_gray_downsample(const orc_image *src, const orc_image *dst, uint32_t k) {
+    if (!src || !dst) return ORC_INVALID_ARGUMENT;
+    int rc;
+    if ((rc = check(src, 0)) || (rc = check(dst, 0))) return rc;
+    if (k == 0 || k > 16 || dst->channels != 1 || (src->channels != 3 && src->channels != 4)) return ORC_SIZE_MISMATCH;
+    if (dst->width != src->width / k || dst->height != src->height / k) return ORC_SIZE_MISMATCH;
+    const uint32_t cn = src->channels;
+    const float inv = 1.0f / (float)(k * k * (1u << 14));
+    for (uint32_t oy = 0; oy < dst->height; oy++)
+        for (uint32_t ox = 0; ox < dst->width; ox++) {
+            uint32_t sb = 0, sg = 0, sr = 0;
+            for (uint32_t dy = 0; dy < k; dy++) {
+                const uint8_t *row = (const uint8_t *)src->data + (size_t)(oy * k + dy) * src->stride;
+                for (uint32_t dx = 0; dx < k; dx++) {
+                    const uint8_t *p = row + (size_t)(ox * k + dx) * cn;
+                    sb += p[0];
+                    sg += p[1];
+                    sr += p[2];
+                }
+            }
+            const uint32_t l = sb * 1868u + sg * 9617u + sr * 4899u;
+            uint8_t o;
+            if (k == 2) {
+                o = (uint8_t)((l + (1u << 15)) >> 16);
+            } else {
+                const float v = (float)l * inv + 0.5f;
+                o = (uint8_t)(v < 255.0f ? v : 255.0f);
+            }
+            ((uint8_t *)dst->data)[(size_t)oy * dst->stride + ox] = o;
+        }
+    return ORC_OK;
+}
+
+/* frame.zig:191-199 grayDownsampleRect: the (x, y, w, h) window of dst from
+ * the source blocks under it. */
+int orc_gray_downsample_rect(const orc_image *src, const orc_image *dst, uint32_t k, uint32_t x, uint32_t y, uint32_t w,
+                             uint32_t h) {
+    if (!src || !dst) return ORC_INVALID_ARGUMENT;
+    int rc;
+    if ((rc = check(src, 0)) || (rc = check(dst, 0))) return rc;
+    if (k == 0 || k > 16 || dst->channels != 1 || (src->channels != 3 && src->channels != 4)) return ORC_SIZE_MISMATCH;
+    if (dst->width != src->width / k || dst->height != src->height / k) return ORC_SIZE_MISMATCH;
+    if (w == 0 || h == 0) return ORC_OK;
+    if ((uint64_t)x + w > dst->width || (uint64_t)y + h > dst->height) return ORC_INVALID_ARGUMENT;
+    orc_image d = *dst, s = *src;
+    d.data = (uint8_t *)dst->data + (size_t)y * dst->stride + x;
+    d.width = w;
+    d.height = h;
+    s.data = (uint8_t *)src->data + (size_t)y * k * src->stride + (size_t)x * k * src->channels;
+    s.width = w * k;
+    s.height = h * k;
+    return orc_gray_downsample(&s, &d, k);
+}
+
 int orc_threshold(const orc_image *src, const orc_image *dst, double thresh, double maxval, int type, double *used) {
     int kind = type & 7;
     if (kind > 4) return ORC_UNSUPPORTED;

@@ -65,6 +65,9 @@ int orc_sep_filter_2d(const orc_image *src, const orc_image *dst, const float *k
 
 /* color / threshold */
 int orc_cvt_color(const orc_image *src, const orc_image *dst, int code);
+int orc_gray_downsample(const orc_image *src, const orc_image *dst, uint32_t k);
+int orc_gray_downsample_rect(const orc_image *src, const orc_image *dst, uint32_t k, uint32_t x, uint32_t y, uint32_t w,
+                             uint32_t h);
 int orc_threshold(const orc_image *src, const orc_image *dst, double thresh, double maxval, int type, double *used);
 
 /* resize */

@@ -31,7 +31,7 @@ def test_headers_declare_hot_path():
     for n in ["visar_gaussian_blur", "visar_sobel", "visar_resize", "visar_warp_affine", "visar_remap",
               "visar_morphology", "visar_cvt_color", "visar_threshold", "visar_background_create",
               "visar_background_apply", "visar_background_free", "visar_warp_perspective", "visar_sep_filter_2d",
-              "visar_last_error", "visar_version"]:
+              "visar_gray_downsample", "visar_gray_downsample_rect", "visar_last_error", "visar_version"]:
         assert n in names
 
 
@@ -96,6 +96,37 @@ def test_version_and_errors_without_device():
     assert va.lib.visar_cvt_color(None, None, 6) == -1
     assert va.lib.visar_last_error().decode() == "InvalidArgument"
 
+
+def test_gray_downsample_errors_without_device():
+    """capi.zig:176-182 and videoio/frame.zig:170-189: k <= 0 is InvalidArgument,
+    a non-8U image Unsupported; k > 16, bad channels or sizes are SizeMismatch; a
+    window outside dst is OutOfBounds (INVALID_ARGUMENT here); an empty window
+    does nothing."""
+    import visar_amd as va
+    a = np.zeros((40, 60, 3), np.uint8)
+    for k in (0, -3):  # capi.zig:177: k <= 0 before any image check
+        with pytest.raises(va.VisarError) as e:
+            va.gray_downsample(a, k, dst=np.empty((1, 1, 2), np.uint16))
+        assert e.value.rc == -1 and e.value.name == "InvalidArgument"
+    with pytest.raises(va.VisarError) as e:
+        va.gray_downsample(a.astype(np.uint16), 2, dst=np.empty((20, 30), np.uint16))
+    assert e.value.rc == -3
+    for k, shape, src in [(17, (2, 3), a), (2, (20, 31), a), (2, (20, 30, 3), a),
+                          (2, (20, 30), a[:, :, :1]), (3, (13, 20), a[:, :, :2])]:
+        with pytest.raises(va.VisarError) as e:
+            va.gray_downsample(np.ascontiguousarray(src), k, dst=np.empty(shape, np.uint8))
+        assert e.value.rc == -2 and e.value.name == "SizeMismatch", (k, shape)
+    d = np.full((20, 30), 7, np.uint8)
+    for x, y, w, h in [(25, 0, 6, 1), (0, 19, 1, 2), (2**32 - 1, 0, 2, 1)]:
+        with pytest.raises(va.VisarError) as e:
+            va.gray_downsample_rect(a, 2, d, x, y, w, h)
+        assert e.value.rc == -1 and e.value.name == "OutOfBounds", (x, y, w, h)
+    va.gray_downsample_rect(a, 2, d, 3, 4, 0, 5)
+    va.gray_downsample_rect(a, 2, d, 99, 99, 5, 0)
+    assert (d == 7).all()
+    with pytest.raises(va.VisarError) as e:
+        va.gray_downsample_rect(a, 2, np.empty((20, 29), np.uint8), 0, 0, 0, 0)
+    assert e.value.rc == -2
 
 def test_last_error_is_thread_local():
     import threading

@@ -616,3 +616,67 @@ def test_connected_components_device_cases(va, orc):
             assert count.value == rn
             assert_same(lab.cpu().numpy(), rlab, f"{name} conn {conn} device labels")
             assert_same(stats, rst[:cap], f"{name} conn {conn} truncated stats")
+
+
+# ------------------------------------------------------ gray downsample
+
+def _gd_inputs(cn, k, seed):
+    """The reference's shapes (videoio/frame.zig:384-470: W = 70k + 1,
+    H = 9k; W = 61k + 3, H = 23k + 1) plus saturated and constant blocks."""
+    rng = np.random.default_rng(seed)
+    yield rng.integers(0, 256, (9 * k, 70 * k + 1, cn), dtype=np.uint8)
+    yield rng.integers(0, 256, (23 * k + 1, 61 * k + 3, cn), dtype=np.uint8)
+    yield np.full((3 * k + 2, 5 * k + 1, cn), 255, np.uint8)
+    yield natural(4 * k + 3, 17 * k, cn, seed + 1)
+
+
+@pytest.mark.parametrize("cn", [3, 4])
+def test_gray_downsample(va, orc, cn):
+    for k in range(1, 17):
+        for a in _gd_inputs(cn, k, 100 * cn + k):
+            check(va, orc, "gray_downsample", a, k)
+        # Strided views (every other row of a wider buffer): host views are
+        # staged; device views at unaligned offsets take the byte-load kernel,
+        # 16-byte aligned ones with padded rows the vector-load kernel.
+        big = rnd((6 * k, 9 * k + 5, cn), k)
+        dbig = dev(big)
+        for view, dview in [(big[::2, 3:], dbig[::2, 3:]), (big[1::2, 4:], dbig[1::2, 4:])]:
+            want = orc.gray_downsample(np.ascontiguousarray(view), k)
+            assert_same(va.gray_downsample(view, k), want, f"host view k={k}")
+            assert_same(va.gray_downsample(dview, k).cpu().numpy(), want, f"device view k={k}")
+        padded = rnd((5 * k + 1, 64 * k + 16, cn), k + 50)
+        want = orc.gray_downsample(np.ascontiguousarray(padded[:, :61 * k + 3]), k)
+        assert_same(va.gray_downsample(dev(padded)[:, :61 * k + 3], k).cpu().numpy(), want, f"padded rows k={k}")
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_gray_downsample_frames(va, orc, k):
+    from visar_amd import synth
+    for cn in (3, 4):
+        a = synth.uniform_bytes(k + cn, (1080, 1920, cn))
+        check(va, orc, "gray_downsample", a, k)
+
+
+def test_gray_downsample_rect(va, orc):
+    for cn in (3, 4):
+        for k in (1, 2, 3, 5, 6, 8, 16):
+            rng = np.random.default_rng(k * 7 + cn)
+            a = rng.integers(0, 256, (23 * k + 1, 61 * k + 3, cn), dtype=np.uint8)
+            whole = orc.gray_downsample(a, k)
+            H, W = whole.shape
+            da = dev(a)
+            for _ in range(12):
+                x = int(rng.integers(0, W)); y = int(rng.integers(0, H))
+                w = 1 + int(rng.integers(0, W - x)); h = 1 + int(rng.integers(0, H - y))
+                want = np.full((H, W), 9, np.uint8)
+                want[y:y + h, x:x + w] = whole[y:y + h, x:x + w]
+                assert_same(orc.gray_downsample_rect(a, k, np.full((H, W), 9, np.uint8), x, y, w, h), want, "oracle")
+                got = va.gray_downsample_rect(a, k, np.full((H, W), 9, np.uint8), x, y, w, h)
+                assert_same(got, want, f"rect host cn={cn} k={k} {(x, y, w, h)}")
+                dd = torch.full((H, W), 9, dtype=torch.uint8, device="cuda")
+                va.gray_downsample_rect(da, k, dd, x, y, w, h)
+                assert_same(dd.cpu().numpy(), want, f"rect device cn={cn} k={k} {(x, y, w, h)}")
+            out = np.full((H, W), 9, np.uint8)
+            va.gray_downsample_rect(a, k, out, 0, 0, 0, 5)
+            assert (out == 9).all()
+

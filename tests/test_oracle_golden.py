@@ -201,6 +201,77 @@ def test_gray_lattice(orc):
     assert np.abs(orc.cvt_color(img, 6)[0] - exact).max() <= 0.5 + 1e-9
 
 
+# frame.zig grayDownsample (the tracker's producer of the path's input)
+
+GD_W = np.array([1868, 9617, 4899], np.int64)  # frame.zig:186-189, B, G, R
+
+
+def _block_sums(a, k):
+    h, w = a.shape[0] // k, a.shape[1] // k
+    b = a[:h * k, :w * k, :3].astype(np.int64).reshape(h, k, w, k, 3)
+    return b.sum(axis=(1, 3))
+
+
+@pytest.mark.parametrize("cn", [4, 3])
+def test_gray_downsample_matches_gray_then_area(orc, cn):
+    """frame.zig:384-416: within 0.51 of the exact mean of the full-size lumas,
+    and within 1 of cvtColor's gray at k = 1."""
+    for k in (1, 2, 3, 4, 6, 8):
+        a = rnd((9 * k, 70 * k + 1, cn), k)
+        got = orc.gray_downsample(a, k).astype(np.float64)
+        want = (_block_sums(a, k) @ GD_W) / 16384.0 / (k * k)
+        assert np.abs(got - want).max() <= 0.51
+        if k == 1:
+            gray = orc.cvt_color(a, 6 if cn == 3 else 10)
+            assert np.abs(gray.astype(np.int64) - got.astype(np.int64)).max() <= 1
+
+
+def test_gray_downsample_rounding(orc):
+    """k = 2 is (sum + 2^15) >> 16 exactly; other k round the exact mean half
+    up except within 1e-3 of a half (the f32 reciprocal's error is below 1e-4),
+    and never exceed 255; constant images stay constant."""
+    for cn in (3, 4):
+        for k in (2, 3, 5, 7, 16):
+            a = rnd((5 * k + 3, 23 * k + 2, cn), 100 + k + cn)
+            got = orc.gray_downsample(a, k).astype(np.int64)
+            l = _block_sums(a, k) @ GD_W
+            if k == 2:
+                assert np.array_equal(got, (l + (1 << 15)) >> 16)
+            else:
+                exact = l / float(k * k * 16384) + 0.5
+                frac = exact - np.floor(exact)
+                ok = np.minimum(frac, 1.0 - frac) > 1e-3
+                assert np.array_equal(got[ok], np.minimum(255, np.floor(exact[ok])).astype(np.int64))
+            for v in (0, 1, 128, 254, 255):
+                assert np.unique(orc.gray_downsample(np.full((2 * k, 3 * k, cn), v, np.uint8), k)).tolist() == [v]
+
+
+def test_gray_downsample_rect_equals_window(orc):
+    """frame.zig:418-446: the window of the whole-image result, the rest untouched."""
+    rng = np.random.default_rng(5)
+    for cn in (4, 3):
+        for k in (1, 2, 3, 5, 6, 8):
+            a = rnd((23 * k + 1, 61 * k + 3, cn), k * 7 + cn)
+            want = orc.gray_downsample(a, k)
+            for _ in range(10):
+                x, y = int(rng.integers(0, want.shape[1])), int(rng.integers(0, want.shape[0]))
+                w, h = 1 + int(rng.integers(0, want.shape[1] - x)), 1 + int(rng.integers(0, want.shape[0] - y))
+                got = np.zeros_like(want)
+                orc.gray_downsample_rect(a, k, got, x, y, w, h)
+                inside = np.zeros(want.shape, bool)
+                inside[y:y + h, x:x + w] = True
+                assert np.array_equal(got[inside], want[inside]) and not got[~inside].any()
+
+
+def test_gray_downsample_rejects(orc):
+    a = rnd((8, 8, 3), 1)
+    for k, shape in [(0, (8, 8)), (17, (0, 0)), (2, (4, 5)), (2, (3, 4))]:
+        with pytest.raises(Exception):
+            orc.gray_downsample(a, k, np.zeros(shape if shape[0] else (1, 1), np.uint8))
+    with pytest.raises(Exception):
+        orc.gray_downsample(rnd((8, 8), 1), 2)
+
+
 def test_threshold_known(orc):
     t = KA["threshold"]
     a = np.array([t["in"]], np.uint8)

@@ -265,6 +265,22 @@ def remap(a, mapx, mapy, value=0.0, dst=None):
     return dst
 
 
+def gray_downsample(a, k, dst=None):
+    """BT.601 gray of a BGR/BGRA u8 image shrunk by k (1..16) with box averaging
+    (videoio/frame.zig:170 grayDownsample; tests/py/visar_ctypes.py:102); dst is (h // k, w // k)."""
+    dst = _empty(a, (a.shape[0] // k, a.shape[1] // k)) if dst is None else dst
+    _check(lib.visar_gray_downsample(_ref(a), _ref(dst), int(k)))
+    return dst
+
+
+def gray_downsample_rect(a, k, dst, x, y, w, h):
+    """Only dst's (x, y, w, h) window of gray_downsample (videoio/frame.zig:182
+    grayDownsampleRect); the rest of dst keeps its contents."""
+    _check(lib.visar_gray_downsample_rect(_ref(a), _ref(dst), int(k), ctypes.c_uint32(x),
+                                          ctypes.c_uint32(y), ctypes.c_uint32(w), ctypes.c_uint32(h)))
+    return dst
+
+
 class Background:
     """Running-average background model with its state in device memory."""
 

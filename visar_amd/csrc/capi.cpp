@@ -616,6 +616,53 @@ int visar_cvt_color(const visar_image *src, const visar_image *dst, int code) {
     return call.finish(launch_gray(s, d, (int)T, scn, sb, call.stream()));
 }
 
+// capi.zig:176-182 visar_gray_downsample -> videoio/frame.zig:170-173
+// grayDownsample: BGR / BGRA u8 -> gray shrunk by k (1..16) with box
+// averaging; dst is (src.w / k) x (src.h / k), 1 channel.
+static int gray_down_check(const visar_image *src, const visar_image *dst, int k) {
+    if (k <= 0) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    int rc;
+    if ((rc = check_image(src, 0)) || (rc = check_image(dst, 0))) return rc;
+    if (k > 16 || dst->channels != 1 || (src->channels != 3 && src->channels != 4))
+        return fail(SIZE_MISMATCH, "SizeMismatch");
+    if (dst->width != src->width / (uint32_t)k || dst->height != src->height / (uint32_t)k)
+        return fail(SIZE_MISMATCH, "SizeMismatch");
+    return OK;
+}
+
+static int gray_down_run(const visar_image *src, const visar_image *dst, int k) {
+    Call call;
+    DImg s, d;
+    int rc;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, s)) || (rc = call.output(dst, d))) return rc;
+    return call.finish(launch_gray_down(s, d, k, call.stream()));
+}
+
+int visar_gray_downsample(const visar_image *src, const visar_image *dst, int k) {
+    int rc = gray_down_check(src, dst, k);
+    return rc ? rc : gray_down_run(src, dst, k);
+}
+
+// videoio/frame.zig:179-189 grayDownsampleRect: the (x, y, w, h) window of
+// dst from the k x k source blocks under it; the rest of dst is untouched.
+int visar_gray_downsample_rect(const visar_image *src, const visar_image *dst, int k, uint32_t x, uint32_t y,
+                               uint32_t w, uint32_t h) {
+    int rc = gray_down_check(src, dst, k);
+    if (rc) return rc;
+    if (w == 0 || h == 0) return OK;
+    if ((uint64_t)x + w > dst->width || (uint64_t)y + h > dst->height) return fail(INVALID_ARGUMENT, "OutOfBounds");
+    visar_image sv = *src, dv = *dst;
+    const uint32_t ku = (uint32_t)k;
+    sv.data = (uint8_t *)src->data + (size_t)y * ku * src->stride + (size_t)x * ku * src->channels;
+    sv.width = w * ku;
+    sv.height = h * ku;
+    dv.data = (uint8_t *)dst->data + (size_t)y * dst->stride + x;
+    dv.width = w;
+    dv.height = h;
+    return gray_down_run(&sv, &dv, k);
+}
+
 // threshold.zig:184-207 getThreshVal_Otsu_8u
 static double otsu(const uint32_t *hist, double total) {
     const double scale = 1.0 / total;

@@ -5,6 +5,7 @@
 // Each kernel maps one thread to a run of consecutive samples of one row so
 // loads and stores are coalesced; the 16-pixel u8 paths use 128-bit loads
 // when the row segment is 16-byte aligned.
+#include <utility>
 #include "tiles.hpp"
 
 namespace visar {
@@ -86,6 +87,123 @@ __global__ __launch_bounds__(kThreads) void gray_u8(DImg s, DImg d, long npix) {
     }
     const long n = min(16L, limit - x);
     for (long i = 0; i < n; i++) dr[i] = (uint8_t)gray_fix(sr[i * SCN + BI], sr[i * SCN + 1], sr[i * SCN + RI]);
+}
+
+// videoio/frame.zig:167-177, 292-336 grayDownsample: the k x k block sums of
+// B, G and R weighted 1868 / 9617 / 4899; k = 2 rounds (l + 2^15) >> 16,
+// other k take (f32)l * (f32)(1 / (k*k*2^14)) + 0.5 truncated, at most 255.
+// The weighted sum is linear, so l is also the sum over the block of each
+// pixel's b*1868 + g*9617 + r*4899 (exact in u32: l < 256 * 255 * 2^14 < 2^30).
+__device__ __attribute__((always_inline)) inline uint8_t gray_down_out(uint32_t l, int k, float inv) {
+    if (k == 2) return (uint8_t)((l + (1u << 15)) >> 16);
+    const float v = (float)l * inv + 0.5f;
+    return (uint8_t)(uint32_t)(v < 255.0f ? v : 255.0f);
+}
+
+// Any layout: one output pixel per thread, byte loads; 64 x 4 outputs per
+// tile, tiles walked grid-stride.
+template <int CN>
+__global__ __launch_bounds__(kThreads) void gray_down_u8(DImg s, DImg d, int k, float inv, int tiles_x, long tiles) {
+    for (long t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int ox = (int)(t % tiles_x) * 64 + (threadIdx.x & 63);
+        const long oy = t / tiles_x * 4 + (threadIdx.x >> 6);
+        if (ox >= d.w || oy >= d.h) continue;
+        uint32_t l = 0;
+        for (int dy = 0; dy < k; dy++) {
+            const gptr<const uint8_t> row = grow<const uint8_t>(s, oy * k + dy) + (long)ox * k * CN;
+            for (int dx = 0; dx < k; dx++)
+                l += row[dx * CN] * 1868u + row[dx * CN + 1] * 9617u + row[dx * CN + 2] * 4899u;
+        }
+        grow<uint8_t>(d, oy)[ox] = gray_down_out(l, k, inv);
+    }
+}
+
+// Rows 4-byte (BGR) or 16-byte (BGRA) aligned: a workgroup takes one output
+// row and up to kGdPix of its source pixels. Each thread reads one 4-pixel
+// chunk of each of the K rows with nontemporal vector loads (the frame is
+// read once) and sums every pixel's
+// weighted luma down the rows with v_dot4_u32_u8 (the weights split into
+// high and low bytes: 1868 = 7*256 + 76, 9617 = 37*256 + 145,
+// 4899 = 19*256 + 35); the column sums go to LDS, and each output adds K
+// of them.
+constexpr int kGdPix = 1020;  // <= 4 * 256 - 3: at most 256 chunks per unit
+constexpr uint32_t kGdHi = 7u | 37u << 8 | 19u << 16, kGdLo = 76u | 145u << 8 | 35u << 16;
+
+template <int CN, int K>
+__global__ __launch_bounds__(kThreads) void gray_down_seg(DImg s, DImg d, int per_unit, uint32_t units_x, uint32_t units,
+                                                          float inv) {
+    __shared__ uint32_t col[4 * kThreads];
+    const int tid = threadIdx.x;
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const long oy = u / units_x;
+        const int ox0 = (int)(u % units_x) * per_unit;
+        const int nout = min(per_unit, d.w - ox0);
+        const int p0 = ox0 * K, p1 = p0 + nout * K;  // source pixels [p0, p1)
+        const int c0 = p0 >> 2, nch = ((p1 + 3) >> 2) - c0;
+        const int c = c0 + tid;
+        if (u != blockIdx.x) __syncthreads();  // the previous unit's reads of col
+        if (tid < nch) {
+            uint32_t hi[4] = {0, 0, 0, 0}, lo[4] = {0, 0, 0, 0};
+            if (4 * c + 4 <= s.w) {
+                if constexpr (CN == 4) {
+                    u32x4 v[K];
+#pragma unroll
+                    for (int dy = 0; dy < K; dy++)
+                        v[dy] = __builtin_nontemporal_load((gptr<const u32x4>)(grow<const uint8_t>(s, oy * K + dy) + 16l * c));
+#pragma unroll
+                    for (int dy = 0; dy < K; dy++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            hi[j] = __builtin_amdgcn_udot4(v[dy][j], kGdHi, hi[j], false);
+                            lo[j] = __builtin_amdgcn_udot4(v[dy][j], kGdLo, lo[j], false);
+                        }
+                } else {
+                    // d0 = B0 G0 R0 B1, d1 = G1 R1 B2 G2, d2 = R2 B3 G3 R3 (bytes, low first)
+                    uint32_t v[K][3];
+#pragma unroll
+                    for (int dy = 0; dy < K; dy++) {
+                        const gptr<const uint32_t> q = (gptr<const uint32_t>)(grow<const uint8_t>(s, oy * K + dy) + 12l * c);
+                        v[dy][0] = __builtin_nontemporal_load(q);
+                        v[dy][1] = __builtin_nontemporal_load(q + 1);
+                        v[dy][2] = __builtin_nontemporal_load(q + 2);
+                    }
+                    const uint32_t H = kGdHi, L = kGdLo;
+#pragma unroll
+                    for (int dy = 0; dy < K; dy++) {
+                        const uint32_t d0 = v[dy][0], d1 = v[dy][1], d2 = v[dy][2];
+                        hi[0] = __builtin_amdgcn_udot4(d0, H, hi[0], false);
+                        lo[0] = __builtin_amdgcn_udot4(d0, L, lo[0], false);
+                        hi[1] = __builtin_amdgcn_udot4(d1, H >> 8, __builtin_amdgcn_udot4(d0, H << 24, hi[1], false), false);
+                        lo[1] = __builtin_amdgcn_udot4(d1, L >> 8, __builtin_amdgcn_udot4(d0, L << 24, lo[1], false), false);
+                        hi[2] = __builtin_amdgcn_udot4(d2, H >> 16, __builtin_amdgcn_udot4(d1, H << 16, hi[2], false), false);
+                        lo[2] = __builtin_amdgcn_udot4(d2, L >> 16, __builtin_amdgcn_udot4(d1, L << 16, lo[2], false), false);
+                        hi[3] = __builtin_amdgcn_udot4(d2, H << 8, hi[3], false);
+                        lo[3] = __builtin_amdgcn_udot4(d2, L << 8, lo[3], false);
+                    }
+                }
+            } else {
+                // The chunk crosses the row end: its pixels inside the row only.
+                for (int dy = 0; dy < K; dy++) {
+                    const gptr<const uint8_t> row = grow<const uint8_t>(s, oy * K + dy);
+                    for (int j = 0; j < 4 && 4 * c + j < s.w; j++) {
+                        const gptr<const uint8_t> px = row + (long)(4 * c + j) * CN;
+                        lo[j] += px[0] * 1868u + px[1] * 9617u + px[2] * 4899u;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) col[4 * tid + j] = (hi[j] << 8) + lo[j];
+        }
+        __syncthreads();
+        const int base = p0 - 4 * c0;
+        const gptr<uint8_t> out = grow<uint8_t>(d, oy) + ox0;
+        for (int j = tid; j < nout; j += kThreads) {
+            uint32_t l = 0;
+#pragma unroll
+            for (int t = 0; t < K; t++) l += col[base + j * K + t];
+            out[j] = gray_down_out(l, K, inv);
+        }
+    }
 }
 
 template <typename T>
@@ -376,6 +494,40 @@ int launch_copy(const DImg &s, const DImg &d, size_t elem, hipStream_t st) {
 }
 
 inline dim3 tiles256(const DImg &d) { return dim3((unsigned)(((long)d.w * d.cn + 255) / 256), (unsigned)((d.h + 3) / 4)); }
+
+template <int CN, int K>
+static void launch_gd_seg(const DImg &s, const DImg &d, float inv, hipStream_t st) {
+    const int per = kGdPix / K;
+    const uint32_t units_x = (uint32_t)((d.w + per - 1) / per), units = units_x * (uint32_t)d.h;
+    const unsigned grid = std::min(units, 1u << 20);
+    hipLaunchKernelGGL((gray_down_seg<CN, K>), dim3(grid), dim3(kThreads), 0, st, s, d, per, units_x, units, inv);
+}
+
+template <int CN, int... Ks>
+static void launch_gd_k(const DImg &s, const DImg &d, int k, float inv, hipStream_t st,
+                        std::integer_sequence<int, Ks...>) {
+    ((k == Ks + 1 ? launch_gd_seg<CN, Ks + 1>(s, d, inv, st) : void()), ...);
+}
+
+int launch_gray_down(const DImg &s, const DImg &d, int k, hipStream_t st) {
+    if (k < 1 || k > 16 || (s.cn != 3 && s.cn != 4)) return UNSUPPORTED;
+    const float inv = 1.0f / (float)(k * k * (1 << 14));
+    const unsigned align = s.cn == 4 ? 16 : 4;
+    // Units of one output row and up to kGdPix / k of its pixels, counted in
+    // 32 bits (an image that fits in memory has fewer than 2^32).
+    const uint64_t units = (uint64_t)((d.w + kGdPix / k - 1) / (kGdPix / k)) * (uint64_t)d.h;
+    if ((uintptr_t)s.p % align == 0 && s.stride % align == 0 && units < (1ull << 32)) {
+        if (s.cn == 4) launch_gd_k<4>(s, d, k, inv, st, std::make_integer_sequence<int, 16>());
+        else launch_gd_k<3>(s, d, k, inv, st, std::make_integer_sequence<int, 16>());
+        return launched();
+    }
+    const int tiles_x = (d.w + 63) / 64;
+    const long tiles = (long)tiles_x * ((d.h + 3) / 4);
+    const unsigned grid = (unsigned)std::min<long>(tiles, 1l << 20);
+    if (s.cn == 3) hipLaunchKernelGGL((gray_down_u8<3>), dim3(grid), dim3(kThreads), 0, st, s, d, k, inv, tiles_x, tiles);
+    else hipLaunchKernelGGL((gray_down_u8<4>), dim3(grid), dim3(kThreads), 0, st, s, d, k, inv, tiles_x, tiles);
+    return launched();
+}
 
 int launch_half_u8(const DImg &s, const DImg &d, hipStream_t st) {
     if (d.cn == 1) {

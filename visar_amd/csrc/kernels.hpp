@@ -75,6 +75,8 @@ int launch_background(const DImg &frame, const DImg &mask, uint16_t *model, long
 
 int launch_copy(const DImg &s, const DImg &d, size_t elem, hipStream_t st);
 int launch_half_u8(const DImg &s, const DImg &d, hipStream_t st);
+// BGR / BGRA u8 -> gray shrunk by k with box averaging (videoio/frame.zig:167-177).
+int launch_gray_down(const DImg &s, const DImg &d, int k, hipStream_t st);
 int launch_area_int_u8(const DImg &s, const DImg &d, int ix, int iy, hipStream_t st);
 int launch_nearest(const DImg &s, const DImg &d, size_t elem, hipStream_t st);
 

@@ -15,7 +15,9 @@ table (HIP-event durations, algorithmic bytes, GB/s), the roofline of the
 dominant kernel, and the CPU oracle timed on a bounded sample of the same
 workload. `--config 1` times the plain 1080p GaussianBlur 5x5 instead;
 `--config 6` the tracking image of a 3840x2160 BGRA frame (grayDownsample,
-BT.601 gray shrunk by 6 with box averaging, to 640x360).
+BT.601 gray shrunk by 6 with box averaging, to 640x360); `--config 7` lens
+undistortion of a 1280x720 BGR frame (the reference benchmark's camera and
+5 distortion coefficients: f64 map, 1/32 px fixed point, 15-bit bilinear).
 """
 import argparse
 import json
@@ -36,7 +38,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=240)
     ap.add_argument("--warmup", type=int, default=24)
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5, 6])
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5, 6, 7])
     ap.add_argument("--streams", type=int, default=64)
     ap.add_argument("--seconds", type=float, default=6.0, help="served workload: Poisson run length")
     ap.add_argument("--host-workers", type=int, default=8, help="served workload: connected-components threads")
@@ -57,6 +59,11 @@ def profiled_traffic(key):
 
 
 CPU_THREADS = 16  # the GPU host's CPU share for one GPU
+
+# The camera and distortion of the reference's undistort benchmark
+# (bench/bench_vs_opencv.py:240-241).
+UD_K = np.array([[900.0, 0, 640], [0, 900, 360], [0, 0, 1]])
+UD_DIST = np.array([-0.28, 0.09, 0.001, -0.0005, -0.01])
 
 
 def cpu_measure(seconds, make_worker, per_unit, unit, what):
@@ -487,6 +494,13 @@ def main():
                                       W * H * 4 + (W // K6) * (H // K6))}
         pix_per_step = W * H
         workload = "grayDownsample 3840x2160 BGRA u8 by 6 to 640x360 (tracking image)"
+    elif args.config == 7:
+        W, H = 1280, 720
+        frames = [torch.from_numpy(synth.uniform_bytes(i, (H, W, 3))).to(dev) for i in range(R)]
+        outs = [torch.empty((H, W, 3), dtype=torch.uint8, device=dev) for _ in range(2)]
+        ops = {"undistort_bgr": (lambda i: va.undistort(frames[i % R], UD_K, UD_DIST, dst=outs[i % 2]), 2 * W * H * 3)}
+        pix_per_step = W * H
+        workload = "undistort 1280x720 BGR u8, K = [[900, 0, 640], [0, 900, 360], [0, 0, 1]], 5 coefficients"
     else:
         W, H = 1920, 1080
         frames = [torch.from_numpy(synth.uniform_bytes(i, (H, W))).to(dev) for i in range(R * 6)]
@@ -617,6 +631,15 @@ def cpu_baseline(args, synth):
 
         return cpu_measure(args.cpu_seconds, gd_worker, W * H / 1e9, "Gpix/s",
                            "grayDownsample calls on 3840x2160 BGRA, k = 6")
+    if args.config == 7:
+        W, H = 1280, 720
+
+        def ud_worker(k):
+            a = synth.uniform_bytes(k, (H, W, 3))
+            return lambda n: oracle.undistort(a, UD_K, UD_DIST)
+
+        return cpu_measure(args.cpu_seconds, ud_worker, W * H / 1e9, "Gpix/s",
+                           "undistort calls on 1280x720 BGR (the reference benchmark's camera)")
     W, H = 1920, 1080
 
     def blur_worker(k):

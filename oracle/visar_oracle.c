@@ -1226,6 +1226,167 @@ int orc_morph_pass(const orc_image *s, const orc_image *d, int op_max, const uin
     return ORC_OK;
 }
 
+/* ---------------------------------------------------------- undistort */
+
+/* calib3d/linalg.zig:417-426 inv3: cofactors times 1 / det; 0 when det == 0. */
+static int inv3(const double a[9], double r[9]) {
+    const double det = a[0] * (a[4] * a[8] - a[7] * a[5]) - a[1] * (a[3] * a[8] - a[6] * a[5]) +
+                       a[2] * (a[3] * a[7] - a[6] * a[4]);
+    if (det == 0) return 0;
+    const double d = 1 / det;
+    r[0] = (a[4] * a[8] - a[5] * a[7]) * d;
+    r[1] = (a[2] * a[7] - a[1] * a[8]) * d;
+    r[2] = (a[1] * a[5] - a[2] * a[4]) * d;
+    r[3] = (a[5] * a[6] - a[3] * a[8]) * d;
+    r[4] = (a[0] * a[8] - a[2] * a[6]) * d;
+    r[5] = (a[2] * a[3] - a[0] * a[5]) * d;
+    r[6] = (a[3] * a[7] - a[4] * a[6]) * d;
+    r[7] = (a[1] * a[6] - a[0] * a[7]) * d;
+    r[8] = (a[0] * a[4] - a[1] * a[3]) * d;
+    return 1;
+}
+
+/* linalg.zig:432-438 mul3 */
+static void mul3(const double a[9], const double b[9], double r[9]) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r[i * 3 + j] = a[i * 3] * b[j] + a[i * 3 + 1] * b[3 + j] + a[i * 3 + 2] * b[6 + j];
+}
+
+/* calib3d/camera.zig:17-22 expandDist: 0, 4, 5 or 8 coefficients (k1 k2 p1
+ * p2 [k3 [k4 k5 k6]]), zero-padded to 8. */
+static int expand_dist(const double *d, size_t nd, double k[8]) {
+    if (nd != 0 && nd != 4 && nd != 5 && nd != 8) return ORC_INVALID_ARGUMENT;
+    for (int i = 0; i < 8; i++) k[i] = (size_t)i < nd ? d[i] : 0.0;
+    return ORC_OK;
+}
+
+/* undistort.zig:44-78 MapParams.row for one pixel: destination (j, i) ->
+ * distorted source (u, v), every operation rounded separately. */
+static void map_px(const double ir[9], const double k[8], double fx, double fy, double u0, double v0, double i, double j,
+                   double *u, double *v) {
+    const double _x = (i * ir[1] + ir[2]) + j * ir[0];
+    const double _y = (i * ir[4] + ir[5]) + j * ir[3];
+    double x = _x, y = _y;
+    if (!(ir[6] == 0 && ir[7] == 0 && ir[8] == 1)) {
+        const double _w = (i * ir[7] + ir[8]) + j * ir[6];
+        const double w = 1 / _w;
+        x = _x * w;
+        y = _y * w;
+    }
+    const double x2 = x * x, y2 = y * y, r2 = x2 + y2, _2xy = 2 * x * y;
+    double kr = 1 + ((k[4] * r2 + k[1]) * r2 + k[0]) * r2;
+    if (k[5] != 0 || k[6] != 0 || k[7] != 0) kr /= 1 + ((k[7] * r2 + k[6]) * r2 + k[5]) * r2;
+    const double xd = x * kr + k[2] * _2xy + k[3] * (r2 + 2 * x2);
+    const double yd = y * kr + k[2] * (r2 + 2 * y2) + k[3] * _2xy;
+    *u = fx * xd + u0;
+    *v = fy * yd + v0;
+}
+
+/* undistort.zig:106-115 fix: 1/32-pixel fixed point, clamped, rounded half
+ * to even with the 1.5 * 2^52 trick (max before min: a NaN gives -lim). */
+static int32_t fix32(double v) {
+    const double s = v * 32.0, lim = 2147483647.0;
+    const double c = fmin(fmax(s, -lim), lim);
+    const double r = c + 6755399441055744.0 - 6755399441055744.0;
+    return (int32_t)r;
+}
+
+/* undistort.zig:120-157 sample: 15-bit bilinear weights from the 1/32
+ * fractions, constant border 0; all four taps inside, all outside, or per
+ * tap. (linear16, :159-201, is the same integer factored by rows.) */
+static void undist_px(const orc_image *src, int cn, int32_t iu, int32_t iv, uint8_t *out) {
+    const int32_t sx = iu >> 5, sy = iv >> 5, a = iu & 31, b = iv & 31;
+    const int32_t w[4] = {(32 - a) * (32 - b), a * (32 - b), (32 - a) * b, a * b};
+    const int32_t W = (int32_t)src->width, H = (int32_t)src->height;
+    if (sx >= 0 && sx < W - 1 && sy >= 0 && sy < H - 1) {
+        const uint8_t *p0 = (const uint8_t *)rowp(src, sy) + (size_t)sx * cn;
+        const uint8_t *p1 = p0 + src->stride;
+        for (int ch = 0; ch < cn; ch++) {
+            const int32_t t = p0[ch] * w[0] + p0[ch + cn] * w[1] + p1[ch] * w[2] + p1[ch + cn] * w[3];
+            out[ch] = (uint8_t)((t + 512) >> 10);
+        }
+        return;
+    }
+    if (sx >= W || sx + 1 < 0 || sy >= H || sy + 1 < 0) {
+        for (int ch = 0; ch < cn; ch++) out[ch] = 0;
+        return;
+    }
+    for (int ch = 0; ch < cn; ch++) {
+        int32_t t = 0;
+        for (int q = 0; q < 4; q++) {
+            const int32_t px = sx + (q & 1), py = sy + (q >> 1);
+            const int32_t v = (px >= 0 && px < W && py >= 0 && py < H) ? ((const uint8_t *)rowp(src, py))[(size_t)px * cn + ch] : 0;
+            t += v * w[q];
+        }
+        out[ch] = (uint8_t)((t + 512) >> 10);
+    }
+}
+
+/* undistort.zig:205-272 undistort (u8, 1-4 channels, dst the shape of src,
+ * new_k defaulting to K) and capi_calib3d.zig:186-191: stripes of
+ * max(1, 4096 / width) rows, each with its own inverse of new_k whose
+ * principal point is moved up by the stripe's first row; row i of a stripe
+ * is mapped with i counted from that row. A stripe whose inverse is singular
+ * is left unwritten (the reference returns from its chunk of stripes). */
+int orc_undistort(const orc_image *src, const orc_image *dst, const double K[9], const double *dist, size_t nd,
+                  const double *new_k) {
+    int rc;
+    if ((rc = check(src, 0)) || (rc = check(dst, 0))) return rc;
+    if (src->width != dst->width || src->height != dst->height || src->channels != dst->channels) return ORC_SIZE_MISMATCH;
+    if (src->channels < 1 || src->channels > 4) return ORC_UNSUPPORTED;
+    double k[8];
+    if ((rc = expand_dist(dist, nd, k))) return rc;
+    const double *ar0 = new_k ? new_k : K;
+    const int cn = (int)src->channels;
+    const uint32_t W = src->width, H = src->height;
+    uint32_t stripe = 4096 / W;
+    if (stripe < 1) stripe = 1;
+    if (stripe > H) stripe = H;
+    for (uint32_t y0 = 0; y0 < H; y0 += stripe) {
+        double ar[9], ir[9];
+        memcpy(ar, ar0, sizeof ar);
+        ar[5] = ar0[5] - (double)y0;
+        if (!inv3(ar, ir)) continue;
+        const uint32_t y1 = y0 + stripe < H ? y0 + stripe : H;
+        for (uint32_t y = y0; y < y1; y++) {
+            uint8_t *d = (uint8_t *)rowp(dst, y);
+            for (uint32_t x = 0; x < W; x++) {
+                double u, v;
+                map_px(ir, k, K[0], K[4], K[2], K[5], (double)(y - y0), (double)x, &u, &v);
+                undist_px(src, cn, fix32(u), fix32(v), d + (size_t)x * cn);
+            }
+        }
+    }
+    return ORC_OK;
+}
+
+/* undistort.zig:36-41 MapParams.init (inverse of new_k * R, singular:
+ * InvalidArgument) and :82-103 initUndistortRectifyMap into f32 maps, with
+ * capi_calib3d.zig:195-201 (maps checked as 32F first). */
+int orc_init_undistort_rectify_map(const double K[9], const double *dist, size_t nd, const double *R, const double *new_k,
+                                   const orc_image *mapx, const orc_image *mapy) {
+    int rc;
+    if ((rc = check(mapx, 5)) || (rc = check(mapy, 5))) return rc;
+    static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double m[9], ir[9], k[8];
+    mul3(new_k ? new_k : K, R ? R : I, m);
+    if (!inv3(m, ir)) return ORC_INVALID_ARGUMENT;
+    if ((rc = expand_dist(dist, nd, k))) return rc;
+    if (mapx->width != mapy->width || mapx->height != mapy->height || mapx->channels != mapy->channels ||
+        mapx->channels != 1)
+        return ORC_SIZE_MISMATCH;
+    for (uint32_t y = 0; y < mapx->height; y++) {
+        float *ox = (float *)rowp(mapx, y), *oy = (float *)rowp(mapy, y);
+        for (uint32_t x = 0; x < mapx->width; x++) {
+            double u, v;
+            map_px(ir, k, K[0], K[4], K[2], K[5], (double)y, (double)x, &u, &v);
+            ox[x] = (float)u;
+            oy[x] = (float)v;
+        }
+    }
+    return ORC_OK;
+}
+
 typedef struct { orc_image im; void *buf; } tmp_image;
 
 static int tmp_like(const orc_image *s, tmp_image *t) {

@@ -65,6 +65,10 @@ int orc_sep_filter_2d(const orc_image *src, const orc_image *dst, const float *k
 
 /* color / threshold */
 int orc_cvt_color(const orc_image *src, const orc_image *dst, int code);
+int orc_undistort(const orc_image *src, const orc_image *dst, const double K[9], const double *dist, size_t nd,
+                  const double *new_k);
+int orc_init_undistort_rectify_map(const double K[9], const double *dist, size_t nd, const double *R, const double *new_k,
+                                   const orc_image *mapx, const orc_image *mapy);
 int orc_gray_downsample(const orc_image *src, const orc_image *dst, uint32_t k);
 int orc_gray_downsample_rect(const orc_image *src, const orc_image *dst, uint32_t k, uint32_t x, uint32_t y, uint32_t w,
                              uint32_t h);

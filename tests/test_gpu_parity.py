@@ -680,3 +680,95 @@ def test_gray_downsample_rect(va, orc):
             va.gray_downsample_rect(a, k, out, 0, 0, 0, 5)
             assert (out == 9).all()
 
+
+
+# ------------------------------------------------------------ undistort
+
+UD_DISTS = {  # tests/py/test_calib3d_vs_opencv.py:31-36
+    "none": np.zeros(5),
+    "k1k2": np.array([-0.28, 0.09, 0, 0]),
+    "full5": np.array([-0.28, 0.09, 0.001, -0.0005, -0.01]),
+    "rational8": np.array([0.1, -0.2, 0.002, 0.001, 0.05, 0.01, 0.002, 0.003]),
+}
+
+
+def ud_camera(w, h):
+    """The reference test's camera for a w x h image (test_calib3d_vs_opencv.py:232-235)."""
+    K = np.array([[900.0, 0, 640.5], [0, 880.0, 359.2], [0, 0, 1]])
+    K[0, 2], K[1, 2] = w / 2 - 0.5, h / 2 + 0.3
+    K[0, 0] = K[1, 1] = 0.7 * w
+    return K
+
+
+@pytest.mark.parametrize("name", list(UD_DISTS))
+def test_undistort(va, orc, name):
+    d = UD_DISTS[name]
+    for (h, w) in ((720, 1280), (37, 53)):
+        K = ud_camera(w, h)
+        for cn in (1, 2, 3, 4):
+            check(va, orc, "undistort", natural(h, w, cn, w + cn), K, d)
+        nk = K.copy()
+        nk[0, 0] *= 0.8
+        check(va, orc, "undistort", natural(h, w, 3, 5), K, d, new_k=nk)
+    for (h, w) in ((1, 1), (3, 5), (2, 700), (4, 5000), (300, 13), (45, 300)):
+        check(va, orc, "undistort", natural(h, w, 3, h + w), ud_camera(w, h), d)
+
+
+def test_undistort_4k_frame(va, orc):
+    from visar_amd import synth
+    a = synth.uniform_bytes(3, (2160, 3840, 3))
+    check(va, orc, "undistort", a, ud_camera(3840, 2160), UD_DISTS["full5"])
+
+
+def test_undistort_projective_and_singular_stripes(va, orc):
+    h, w = 45, 300  # stripes of 4096 // 300 = 13 rows
+    K = ud_camera(w, h)
+    a = natural(h, w, 3, 9)
+    # A new_k with a projective row: each stripe's inverse has a full third row.
+    nk = K.copy()
+    nk[2] = [1e-4, 2e-4, 1.0]
+    check(va, orc, "undistort", a, K, UD_DISTS["full5"], new_k=nk)
+    # det(new_k moved up by y0) = 1 - (27 - y0): only the stripe at y0 = 26
+    # is singular and stays unwritten; the others map to huge or non-finite
+    # coordinates (clamped, border 0).
+    sk = np.array([[1.0, 0, 0], [0, 1, 27], [0, 1, 1]])
+    want = orc.undistort(a, K, UD_DISTS["k1k2"], sk, np.full_like(a, 7))
+    assert (want[26:39] == 7).all() and not (want[:26] == 7).all()
+    host = va.undistort(a, K, UD_DISTS["k1k2"], sk, np.full_like(a, 7))
+    assert_same(host, want, "singular stripe host")
+    dd = dev(np.full_like(a, 7))
+    va.undistort(dev(a), K, UD_DISTS["k1k2"], sk, dd)
+    assert_same(dd.cpu().numpy(), want, "singular stripe device")
+    # fx = 0: every stripe is singular, nothing is written.
+    K0 = K.copy()
+    K0[0, 0] = 0
+    out = va.undistort(a, K, None, K0, np.full_like(a, 7))
+    assert (out == 7).all()
+
+
+def test_undistort_in_place(va, orc):
+    a = natural(64, 96, 3, 77)
+    K = ud_camera(96, 64)
+    want = orc.undistort(a, K, UD_DISTS["full5"])
+    da = dev(a)
+    va.undistort(da, K, UD_DISTS["full5"], dst=da)
+    assert_same(da.cpu().numpy(), want, "in place")
+
+
+@pytest.mark.parametrize("name", list(UD_DISTS))
+def test_init_undistort_rectify_map(va, orc, name):
+    d = UD_DISTS[name]
+    c, s = np.cos(0.02), np.sin(0.02)
+    R = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]]) @ np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+    for (h, w) in ((720, 1280), (37, 53), (1, 1), (3, 5000)):
+        K = ud_camera(w, h)
+        nk = K.copy()
+        nk[0, 0] *= 0.8
+        for Rm, nkm in [(None, None), (R, nk), (None, nk), (R, None)]:
+            want = orc.init_undistort_rectify_map(K, d, Rm, nkm, (w, h))
+            got = va.init_undistort_rectify_map(K, d, Rm, nkm, (w, h))
+            dm = (torch.empty((h, w), dtype=torch.float32, device="cuda"), torch.empty((h, w), dtype=torch.float32, device="cuda"))
+            va.init_undistort_rectify_map(K, d, Rm, nkm, (w, h), out=dm)
+            for g, dg, wv, ax in [(got[0], dm[0], want[0], "x"), (got[1], dm[1], want[1], "y")]:
+                assert_same(g, wv, f"map{ax} host {name} {w}x{h}")
+                assert_same(dg.cpu().numpy(), wv, f"map{ax} device {name} {w}x{h}")

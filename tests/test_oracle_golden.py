@@ -597,3 +597,174 @@ def test_otsu_maximises_between_class_variance(orc):
         if s > best * (1 + 1e-12):
             best, arg = s, k
     assert orc.threshold_auto(img, 8) == arg
+
+
+# ------------------------------------------------------------ undistort
+
+UD_DISTS = {  # tests/py/test_calib3d_vs_opencv.py:31-36
+    "none": np.zeros(5),
+    "k1k2": np.array([-0.28, 0.09, 0, 0]),
+    "full5": np.array([-0.28, 0.09, 0.001, -0.0005, -0.01]),
+    "rational8": np.array([0.1, -0.2, 0.002, 0.001, 0.05, 0.01, 0.002, 0.003]),
+}
+
+
+def ud_camera(w, h):
+    """The reference test's camera for a w x h image (test_calib3d_vs_opencv.py:232-235)."""
+    K = np.array([[900.0, 0, 640.5], [0, 880.0, 359.2], [0, 0, 1]])
+    K[0, 2], K[1, 2] = w / 2 - 0.5, h / 2 + 0.3
+    K[0, 0] = K[1, 1] = 0.7 * w
+    return K
+
+
+def rodrigues(r):
+    """Rotation vector -> matrix (the textbook formula)."""
+    r = np.asarray(r, np.float64)
+    th = np.linalg.norm(r)
+    k = r / th
+    kx = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.cos(th) * np.eye(3) + (1 - np.cos(th)) * np.outer(k, k) + np.sin(th) * kx
+
+
+def textbook_map(K, dist, R, new_k, w, h):
+    """OpenCV's documented model: (X, Y, W) = R^T newK^-1 (u, v, 1); x' = X/W;
+    x'' = x' (1 + k1 r2 + k2 r4 + k3 r6) / (1 + k4 r2 + k5 r4 + k6 r6) + 2 p1 x'y' + p2 (r2 + 2x'^2),
+    y'' likewise with p1 and p2 exchanged; map = f x'' + c."""
+    k = np.zeros(8)
+    k[:len(dist)] = dist
+    k1, k2, p1, p2, k3, k4, k5, k6 = k
+    nk = K if new_k is None else new_k
+    Rm = np.eye(3) if R is None else R
+    v, u = np.mgrid[0:h, 0:w].astype(np.float64)
+    P = np.stack([u.ravel(), v.ravel(), np.ones(u.size)])
+    X, Y, Wc = Rm.T @ np.linalg.inv(nk) @ P
+    x, y = X / Wc, Y / Wc
+    r2 = x * x + y * y
+    rad = (1 + k1 * r2 + k2 * r2 ** 2 + k3 * r2 ** 3) / (1 + k4 * r2 + k5 * r2 ** 2 + k6 * r2 ** 3)
+    xd = x * rad + 2 * p1 * x * y + p2 * (r2 + 2 * x * x)
+    yd = y * rad + p1 * (r2 + 2 * y * y) + 2 * p2 * x * y
+    return (K[0, 0] * xd + K[0, 2]).reshape(h, w), (K[1, 1] * yd + K[1, 2]).reshape(h, w)
+
+
+def fixed_point_remap(src, mx, my):
+    """OpenCV's fixed-point INTER_LINEAR remap with a zero border: coordinates
+    rounded to 1/32 px, weights (32 - a)(32 - b), a(32 - b), (32 - a)b, ab,
+    sum + 512 >> 10."""
+    h, w = src.shape[:2]
+    s3 = src.reshape(h, w, -1).astype(np.int64)
+    iu = np.rint(mx * 32).astype(np.int64)
+    iv = np.rint(my * 32).astype(np.int64)
+    sx, sy, a, b = iu >> 5, iv >> 5, iu & 31, iv & 31
+    acc = np.zeros(mx.shape + (s3.shape[2],), np.int64)
+    for dx, dy, wt in [(0, 0, (32 - a) * (32 - b)), (1, 0, a * (32 - b)), (0, 1, (32 - a) * b), (1, 1, a * b)]:
+        px, py = sx + dx, sy + dy
+        ok = (px >= 0) & (px < w) & (py >= 0) & (py < h)
+        val = s3[np.clip(py, 0, h - 1), np.clip(px, 0, w - 1)] * ok[..., None]
+        acc += val * wt[..., None]
+    return ((acc + 512) >> 10).astype(np.uint8).reshape(src.shape)
+
+
+def test_undistort_zero_distortion_is_identity(orc):
+    # undistort.zig:396-407, then every pixel (the identity map is exact).
+    src = (np.arange(23 * 37 * 3) * 7 % 256).astype(np.uint8).reshape(23, 37, 3)
+    K = np.array([[40, 0, 18], [0, 40, 11], [0, 0, 1.0]])
+    dst = orc.undistort(src, K)
+    assert np.array_equal(dst[:22, :36], src[:22, :36])
+    assert np.array_equal(dst, src)
+    for cn in (1, 2, 4):
+        a = np.random.default_rng(cn).integers(0, 256, (9, 700, cn), dtype=np.uint8)[..., 0 if cn == 1 else slice(None)]
+        assert np.array_equal(orc.undistort(a, ud_camera(700, 9), np.zeros(4)), a)
+
+
+@pytest.mark.parametrize("tx,ty", [(3, 0), (-2, 5), (0.5, 0), (0, 0.5), (0.25, 0), (1.75, -0.25)])
+def test_undistort_shifts_closed_form(orc, tx, ty):
+    """With no distortion and new_k = K moved by (-tx, -ty) the map is
+    (x + tx, y + ty): integer shifts copy with a zero border; a half pixel
+    gives (p0 + p1 + 1) >> 1 and a quarter (3 p0 + p1 + 2) >> 2."""
+    h, w = 29, 41
+    a = np.random.default_rng(int(10 * tx + ty + 50)).integers(0, 256, (h, w), dtype=np.uint8)
+    K = ud_camera(w, h)
+    nk = K.copy()
+    nk[0, 2] -= tx
+    nk[1, 2] -= ty
+    got = orc.undistort(a, K, None, nk).astype(np.int64)
+    pad = np.zeros((h + 12, w + 12), np.int64)
+    pad[6:6 + h, 6:6 + w] = a
+
+    def at(dx, dy):  # src(x + dx, y + dy), 0 outside
+        return pad[6 + dy:6 + dy + h, 6 + dx:6 + dx + w]
+
+    def frac_blend(p0, p1, f):  # 1/32 weights along one axis
+        return ((32 - f) * 32 * p0 + f * 32 * p1 + 512) >> 10
+
+    ix, fx = int(np.floor(tx)), int(round((tx - np.floor(tx)) * 32))
+    iy, fy = int(np.floor(ty)), int(round((ty - np.floor(ty)) * 32))
+    if fy == 0:
+        want = frac_blend(at(ix, iy), at(ix + 1, iy), fx)
+    elif fx == 0:
+        want = frac_blend(at(ix, iy), at(ix, iy + 1), fy)
+    else:  # both axes: the product of the 1-D weights
+        want = ((32 - fx) * (32 - fy) * at(ix, iy) + fx * (32 - fy) * at(ix + 1, iy) + (32 - fx) * fy * at(ix, iy + 1)
+                + fx * fy * at(ix + 1, iy + 1) + 512) >> 10
+    if tx == 0.5 and ty == 0:
+        assert np.array_equal(want, (at(0, 0) + at(1, 0) + 1) >> 1)
+    if tx == 0.25:
+        assert np.array_equal(want, (3 * at(0, 0) + at(1, 0) + 2) >> 2)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("name", list(UD_DISTS))
+def test_undistort_map_matches_documented_model(orc, name):
+    for (h, w) in ((720, 1280), (37, 53)):
+        K = ud_camera(w, h)
+        nk = K.copy()
+        nk[0, 0] *= 0.8
+        R = rodrigues([0.01, -0.02, 0.005])
+        for Rm, nkm in [(None, None), (R, nk), (None, nk)]:
+            mx, my = orc.init_undistort_rectify_map(K, UD_DISTS[name], Rm, nkm, (w, h))
+            tx, ty = textbook_map(K, UD_DISTS[name], Rm, nkm, w, h)
+            assert np.abs(mx - tx).max() <= 1e-4 * w / 1280 + 2e-4 and np.abs(my - ty).max() <= 2e-4, name
+            # f32 maps: within half an f32 ulp plus the f64 difference.
+            assert np.abs(mx.astype(np.float64) - tx).max() < 1e-3
+
+
+@pytest.mark.parametrize("name", list(UD_DISTS))
+def test_undistort_equals_fixed_point_remap_of_model(orc, name):
+    """undistort (striped inverses) against the fixed-point remap of the
+    documented model's map computed in one piece, except where a coordinate
+    lies within 1e-6 / 32 px of a rounding boundary."""
+    for (h, w), cn in (((720, 1280), 3), ((37, 53), 1), ((37, 53), 4), ((150, 91), 2)):
+        K = ud_camera(w, h)
+        a = np.random.default_rng(w + cn).integers(0, 256, (h, w, cn), dtype=np.uint8)
+        a = a[..., 0] if cn == 1 else a
+        for nk in (None, np.array([[0.8 * K[0, 0], 0, K[0, 2]], [0, K[1, 1], K[1, 2]], [0, 0, 1]])):
+            got = orc.undistort(a, K, UD_DISTS[name], nk)
+            mx, my = textbook_map(K, UD_DISTS[name], None, nk, w, h)
+            want = fixed_point_remap(a, mx, my)
+            near = lambda m: np.abs(m * 32 - np.floor(m * 32) - 0.5) < 1e-6
+            ok = ~(near(mx) | near(my))
+            assert ok.mean() > 0.999
+            assert np.array_equal(got[ok], want[ok]), (name, w, h, cn)
+
+
+def test_undistort_rejects(orc):
+    a = np.zeros((10, 12, 3), np.uint8)
+    K = ud_camera(12, 10)
+    with pytest.raises(orc.OracleError) as e:
+        orc.undistort(a, K, np.zeros(3))
+    assert e.value.rc == -1
+    with pytest.raises(orc.OracleError) as e:
+        orc.undistort(a, K, None, None, np.zeros((10, 13, 3), np.uint8))
+    assert e.value.rc == -2
+    with pytest.raises(orc.OracleError) as e:
+        orc.undistort(np.zeros((10, 12, 5), np.uint8), K)
+    assert e.value.rc == -3
+    with pytest.raises(orc.OracleError) as e:
+        orc.init_undistort_rectify_map(K, None, None, np.zeros((3, 3)), (12, 10))
+    assert e.value.rc == -1
+    # A singular stripe inverse leaves its rows as they were (fx = 0: all).
+    K0 = K.copy()
+    K0[0, 0] = 0
+    out = np.full((10, 12, 3), 7, np.uint8)
+    orc.undistort(np.ones((10, 12, 3), np.uint8), K, None, K0, out)
+    assert (out == 7).all()

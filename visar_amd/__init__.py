@@ -281,6 +281,39 @@ def gray_downsample_rect(a, k, dst, x, y, w, h):
     return dst
 
 
+def _f64_arg(m, n):
+    """(array, c_void_p) of the first n float64 values of m, or (None, None)."""
+    if m is None:
+        return None, None
+    arr = np.ascontiguousarray(np.asarray(m, np.float64).ravel()[:n])
+    return arr, arr.ctypes.data_as(ctypes.c_void_p)
+
+
+def undistort(a, K, dist=None, new_k=None, dst=None):
+    """Lens undistortion of a u8 image, 1-4 channels (calib3d/undistort.zig:205
+    undistort; tests/py/visar_calib3d.py:146); new_k defaults to K."""
+    dst = _empty(a, tuple(a.shape)) if dst is None else dst
+    kk, kp = _f64_arg(K, 9)
+    dd, dp = _f64_arg(dist, 8)
+    nk, nkp = _f64_arg(new_k, 9)
+    _check(lib.visar_undistort(_ref(a), _ref(dst), kp, dp, ctypes.c_size_t(0 if dd is None else dd.size), nkp))
+    return dst
+
+
+def init_undistort_rectify_map(K, dist, R, new_k, size, out=None):
+    """f32 maps (mapx, mapy) of size (w, h) (undistort.zig:82 initUndistortRectifyMap;
+    tests/py/visar_calib3d.py:155); `out` = (mapx, mapy) arrays or tensors to fill."""
+    w, h = size
+    mx, my = (np.empty((h, w), np.float32), np.empty((h, w), np.float32)) if out is None else out
+    kk, kp = _f64_arg(K, 9)
+    dd, dp = _f64_arg(dist, 8)
+    rr, rp = _f64_arg(R, 9)
+    nk, nkp = _f64_arg(None if new_k is None else np.asarray(new_k, np.float64)[:3, :3], 9)
+    _check(lib.visar_init_undistort_rectify_map(kp, dp, ctypes.c_size_t(0 if dd is None else dd.size), rp, nkp,
+                                                _ref(mx), _ref(my)))
+    return mx, my
+
+
 class Background:
     """Running-average background model with its state in device memory."""
 

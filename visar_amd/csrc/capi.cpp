@@ -23,6 +23,7 @@
 #include "../../include/visar_gpu.h"
 #include "kernels.hpp"
 #include "runtime.hpp"
+#include "undistort.hpp"
 #include "tables.hpp"
 
 namespace visar {
@@ -303,6 +304,15 @@ int Call::output(const visar_image *c, DImg &out) {
     staged_ = true;
     pending_.push_back({c, buf, pitch, false});
     out = view(c, buf, pitch);
+    return OK;
+}
+
+int Call::output_keep(const visar_image *c, DImg &out) {
+    int rc = output(c, out);
+    if (rc || out.p == c->data) return rc;
+    const hipMemcpyKind kind = is_device_ptr(c->data) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (hipMemcpy2DAsync(out.p, (size_t)out.stride, c->data, c->stride, row_bytes(c), c->height, kind, st_) != hipSuccess)
+        return fail(INTERNAL, "CopyFailed");
     return OK;
 }
 
@@ -661,6 +671,80 @@ int visar_gray_downsample_rect(const visar_image *src, const visar_image *dst, i
     dv.width = w;
     dv.height = h;
     return gray_down_run(&sv, &dv, k);
+}
+
+// calib3d/camera.zig:17-22 expandDist with capi_calib3d.zig:93-95
+// distSlice (a null `dist` is no coefficients): 0, 4, 5 or 8 of them.
+static int undistort_lens(const double k[9], const double *dist, size_t nd, UndistortLens &L) {
+    if (!dist) nd = 0;
+    if (nd != 0 && nd != 4 && nd != 5 && nd != 8) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    for (size_t i = 0; i < 8; i++) L.k[i] = i < nd ? dist[i] : 0.0;
+    L.rational = L.k[5] != 0 || L.k[6] != 0 || L.k[7] != 0;
+    L.fx = k[0];
+    L.fy = k[4];
+    L.u0 = k[2];
+    L.v0 = k[5];
+    return OK;
+}
+
+// capi_calib3d.zig:186-191 visar_undistort -> calib3d/undistort.zig:205-272:
+// 8U, 1-4 channels, dst the shape of src, new_k defaulting to K; stripes of
+// max(1, 4096 / width) rows, each with its own inverse of new_k.
+int visar_undistort(const visar_image *src, const visar_image *dst, const double k[9], const double *dist, size_t nd,
+                    const double *new_k) {
+    int rc;
+    if ((rc = check_image(src, 0)) || (rc = check_image(dst, 0))) return rc;
+    if (!k) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (src->width != dst->width || src->height != dst->height || src->channels != dst->channels)
+        return fail(SIZE_MISMATCH, "SizeMismatch");
+    if (src->channels < 1 || src->channels > 4) return fail(UNSUPPORTED, "Unsupported");
+    UndistortArgs p;
+    if ((rc = undistort_lens(k, dist, nd, p.lens))) return rc;
+    for (int i = 0; i < 9; i++) p.ar[i] = new_k ? new_k[i] : k[i];
+    p.stripe = (int)std::min(std::max<uint32_t>(1, 4096 / src->width), src->height);
+    // Rows of a stripe whose inverse is singular stay unwritten; then the
+    // output starts as the destination's contents (the same inv3 as the
+    // kernel's, so host and device agree on which stripes those are). With
+    // a bottom row (0, 0, x) the moved element only meets exact zeros in the
+    // determinant, so the first stripe decides for all.
+    bool partial = false;
+    const uint32_t last = p.ar[6] == 0 && p.ar[7] == 0 ? 1 : src->height;
+    for (uint32_t y0 = 0; y0 < last && !partial; y0 += (uint32_t)p.stripe) {
+        double ar[9], ir[9];
+        for (int i = 0; i < 9; i++) ar[i] = p.ar[i];
+        ar[5] = p.ar[5] - (double)y0;
+        partial = !inv3(ar, ir);
+    }
+    Call call;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, p.s))) return rc;
+    if ((rc = partial ? call.output_keep(dst, p.d) : call.output(dst, p.d))) return rc;
+    return call.finish(launch_undistort(p, call.stream()));
+}
+
+// capi_calib3d.zig:195-201 visar_init_undistort_rectify_map ->
+// undistort.zig:36-41 MapParams.init (inverse of new_k * R; singular:
+// InvalidArgument) and :82-103 initUndistortRectifyMap (32F, one channel).
+int visar_init_undistort_rectify_map(const double k[9], const double *dist, size_t nd, const double *r,
+                                     const double *new_k, const visar_image *mapx, const visar_image *mapy) {
+    int rc;
+    if ((rc = check_image(mapx, 5)) || (rc = check_image(mapy, 5))) return rc;
+    if (!k) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    const double *a = new_k ? new_k : k, *b = r ? r : I;
+    double m[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) m[i * 3 + j] = a[i * 3] * b[j] + a[i * 3 + 1] * b[3 + j] + a[i * 3 + 2] * b[6 + j];
+    UndistortMapArgs p;
+    if (!inv3(m, p.ir)) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if ((rc = undistort_lens(k, dist, nd, p.lens))) return rc;
+    if (mapx->width != mapy->width || mapx->height != mapy->height || mapx->channels != mapy->channels ||
+        mapx->channels != 1)
+        return fail(SIZE_MISMATCH, "SizeMismatch");
+    Call call;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.output(mapx, p.mx)) || (rc = call.output(mapy, p.my))) return rc;
+    return call.finish(launch_undistort_map(p, call.stream()));
 }
 
 // threshold.zig:184-207 getThreshVal_Otsu_8u

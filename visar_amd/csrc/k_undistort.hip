@@ -1,0 +1,171 @@
+// k_undistort.hip -- lens undistortion for gfx950 (calib3d/undistort.zig).
+//
+//   undistort_u8          undistort.zig:205-272: the f64 map of every
+//                         destination pixel, rounded to 1/32 px, then the
+//                         15-bit fixed-point bilinear blend, constant border 0
+//   undistort_map_f32     undistort.zig:82-103 initUndistortRectifyMap into
+//                         f32 maps
+//
+// Every f64 operation is rounded separately (the library builds with
+// -ffp-contract=off) and 1 / x is IEEE division, so the map is the
+// reference's bit for bit.
+#include "tiles.hpp"
+#include "undistort.hpp"
+
+namespace visar {
+
+namespace {
+
+constexpr int kUdThreads = 256;
+constexpr int kUdPx = 4;  // pixels per lane, 64 apart along the row
+constexpr int kUdTileW = 64 * kUdPx, kUdTileH = kUdThreads / 64;
+
+// One destination row of MapParams.row (undistort.zig:44-78): the terms
+// that depend on the row only, then map(j) per pixel.
+struct RowMap {
+    double ir0, ir3, ir6, rx, ry, rw;
+    bool proj;
+
+    __device__ __attribute__((always_inline)) RowMap(const double ir[9], double i) {
+        ir0 = ir[0];
+        ir3 = ir[3];
+        ir6 = ir[6];
+        rx = i * ir[1] + ir[2];
+        ry = i * ir[4] + ir[5];
+        rw = i * ir[7] + ir[8];
+        proj = !(ir[6] == 0 && ir[7] == 0 && ir[8] == 1);
+    }
+
+    __device__ __attribute__((always_inline)) void map(const UndistortLens &L, double j, double &u, double &v) const {
+        const double _x = rx + j * ir0, _y = ry + j * ir3;
+        double x = _x, y = _y;
+        if (proj) {
+            const double w = 1 / (rw + j * ir6);
+            x = _x * w;
+            y = _y * w;
+        }
+        const double *k = L.k;
+        const double x2 = x * x, y2 = y * y, r2 = x2 + y2, _2xy = 2 * x * y;
+        double kr = 1 + ((k[4] * r2 + k[1]) * r2 + k[0]) * r2;
+        if (L.rational) kr /= 1 + ((k[7] * r2 + k[6]) * r2 + k[5]) * r2;
+        const double xd = x * kr + k[2] * _2xy + k[3] * (r2 + 2 * x2);
+        const double yd = y * kr + k[2] * (r2 + 2 * y2) + k[3] * _2xy;
+        u = L.fx * xd + L.u0;
+        v = L.fy * yd + L.v0;
+    }
+};
+
+// undistort.zig:106-115 fix: 1/32 px, clamped to +-(2^31 - 1) with max
+// before min (a NaN becomes -lim), rounded half to even.
+__device__ __attribute__((always_inline)) inline int32_t fix32(double v) {
+    const double lim = 2147483647.0;
+    return (int32_t)rint(fmin(fmax(v * 32.0, -lim), lim));
+}
+
+// undistort.zig:120-157 sample: the inside, outside and per-tap border
+// branches are one formula, sum over taps of (tap inside ? pixel : 0) * w,
+// so every lane takes the same path: tap addresses are clamped into the
+// image (always in bounds) and the taps outside contribute 0.
+template <int CN>
+__global__ __launch_bounds__(kUdThreads) void undistort_u8(UndistortArgs p, TileOrder o) {
+    int bx, by;
+    if (!tile_of(o, bx, by)) return;
+    const int y = by * kUdTileH + tile_ty();
+    if (y >= p.d.h) return;
+    // The stripe's own inverse (undistort.zig:241-247): new_k with its
+    // principal point moved up by the stripe's first row.
+    const int y0 = y / p.stripe * p.stripe;
+    double ar[9], ir[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) ar[q] = p.ar[q];
+    ar[5] = p.ar[5] - (double)y0;
+    if (!inv3(ar, ir)) return;
+    const RowMap rm(ir, (double)(y - y0));
+    const int W = p.s.w, H = p.s.h;
+    int32_t iu[kUdPx], iv[kUdPx];
+    const int xb = bx * kUdTileW + tile_tx();
+#pragma unroll
+    for (int q = 0; q < kUdPx; q++) {
+        double u, v;
+        rm.map(p.lens, (double)(xb + 64 * q), u, v);
+        iu[q] = fix32(u);
+        iv[q] = fix32(v);
+    }
+    uint8_t t[kUdPx][4][CN];
+    int32_t wq[kUdPx][4];
+#pragma unroll
+    for (int q = 0; q < kUdPx; q++) {
+        const int32_t sx = iu[q] >> 5, sy = iv[q] >> 5, a = iu[q] & 31, b = iv[q] & 31;
+        const bool inx0 = sx >= 0 && sx < W, inx1 = sx + 1 >= 0 && sx + 1 < W;
+        const bool iny0 = sy >= 0 && sy < H, iny1 = sy + 1 >= 0 && sy + 1 < H;
+        wq[q][0] = inx0 && iny0 ? (32 - a) * (32 - b) : 0;
+        wq[q][1] = inx1 && iny0 ? a * (32 - b) : 0;
+        wq[q][2] = inx0 && iny1 ? (32 - a) * b : 0;
+        wq[q][3] = inx1 && iny1 ? a * b : 0;
+        const long cx0 = min(max(sx, 0), W - 1), cx1 = min(max(sx + 1, 0), W - 1);
+        const long cy0 = min(max(sy, 0), H - 1), cy1 = min(max(sy + 1, 0), H - 1);
+        const gptr<const uint8_t> r0 = grow<const uint8_t>(p.s, cy0), r1 = grow<const uint8_t>(p.s, cy1);
+#pragma unroll
+        for (int c = 0; c < CN; c++) {
+            t[q][0][c] = r0[cx0 * CN + c];
+            t[q][1][c] = r0[cx1 * CN + c];
+            t[q][2][c] = r1[cx0 * CN + c];
+            t[q][3][c] = r1[cx1 * CN + c];
+        }
+    }
+    const gptr<uint8_t> out = grow<uint8_t>(p.d, y);
+#pragma unroll
+    for (int q = 0; q < kUdPx; q++) {
+        const int x = xb + 64 * q;
+        if (x >= p.d.w) break;
+#pragma unroll
+        for (int c = 0; c < CN; c++) {
+            const int32_t s = t[q][0][c] * wq[q][0] + t[q][1][c] * wq[q][1] + t[q][2][c] * wq[q][2] + t[q][3][c] * wq[q][3];
+            out[(long)x * CN + c] = (uint8_t)((s + 512) >> 10);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kUdThreads) void undistort_map_f32(UndistortMapArgs p, TileOrder o) {
+    int bx, by;
+    if (!tile_of(o, bx, by)) return;
+    const int y = by * kUdTileH + tile_ty();
+    if (y >= p.mx.h) return;
+    const RowMap rm(p.ir, (double)y);
+    const gptr<float> ox = (gptr<float>)grow<uint8_t>(p.mx, y), oy = (gptr<float>)grow<uint8_t>(p.my, y);
+    const int xb = bx * kUdTileW + tile_tx();
+#pragma unroll
+    for (int q = 0; q < kUdPx; q++) {
+        const int x = xb + 64 * q;
+        if (x >= p.mx.w) break;
+        double u, v;
+        rm.map(p.lens, (double)x, u, v);
+        ox[x] = (float)u;
+        oy[x] = (float)v;
+    }
+}
+
+inline int launched() { return hipGetLastError() == hipSuccess ? OK : INTERNAL; }
+
+}  // namespace
+
+int launch_undistort(const UndistortArgs &p, hipStream_t st) {
+    const TileOrder o = tile_order((p.d.w + kUdTileW - 1) / kUdTileW, (p.d.h + kUdTileH - 1) / kUdTileH, true);
+    const dim3 g(tile_grid(o.ntiles)), b(kUdThreads);
+    switch (p.s.cn) {
+    case 1: hipLaunchKernelGGL(undistort_u8<1>, g, b, 0, st, p, o); break;
+    case 2: hipLaunchKernelGGL(undistort_u8<2>, g, b, 0, st, p, o); break;
+    case 3: hipLaunchKernelGGL(undistort_u8<3>, g, b, 0, st, p, o); break;
+    case 4: hipLaunchKernelGGL(undistort_u8<4>, g, b, 0, st, p, o); break;
+    default: return UNSUPPORTED;
+    }
+    return launched();
+}
+
+int launch_undistort_map(const UndistortMapArgs &p, hipStream_t st) {
+    const TileOrder o = tile_order((p.mx.w + kUdTileW - 1) / kUdTileW, (p.mx.h + kUdTileH - 1) / kUdTileH);
+    hipLaunchKernelGGL(undistort_map_f32, dim3(tile_grid(o.ntiles)), dim3(kUdThreads), 0, st, p, o);
+    return launched();
+}
+
+}  // namespace visar

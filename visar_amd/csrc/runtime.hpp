@@ -36,6 +36,9 @@ class Call {
     int begin();
     int input(const visar_image *c, DImg &out);
     int output(const visar_image *c, DImg &out);
+    // An output that the kernels may write only in part: a staged or
+    // temporary buffer starts as a copy of the destination's contents.
+    int output_keep(const visar_image *c, DImg &out);
     int temp_like(const DImg &like, size_t elem, DImg &out);
     void *scratch(size_t bytes);
     // A device copy of `bytes` host bytes, uploaded on the call's stream

@@ -112,6 +112,18 @@ int visar_connected_components(const visar_image *src, const visar_image *labels
 int visar_remap(const visar_image *src, const visar_image *dst, const visar_image *mapx, const visar_image *mapy,
                 double value);
 
+/* reference include/visar.h:333-335 (capi_calib3d.zig:186-191 -> calib3d/undistort.zig:205):
+ * 8U, 1-4 channels, dst the size of src; k and new_k (NULL: k) are row-major 3x3;
+ * dist has nd = 0, 4, 5 or 8 coefficients (k1 k2 p1 p2 [k3 [k4 k5 k6]]), NULL = none.
+ * Rows of a stripe whose shifted new_k is singular are left as they were. */
+int visar_undistort(const visar_image *src, const visar_image *dst, const double k[9], const double *dist, size_t nd,
+                    const double *new_k);
+
+/* reference include/visar.h:336-338 (capi_calib3d.zig:195-201 -> undistort.zig:82):
+ * CV_32FC1 maps (VISAR_32F, one channel, the destination size); r and new_k may be NULL. */
+int visar_init_undistort_rectify_map(const double k[9], const double *dist, size_t nd, const double *r,
+                                     const double *new_k, const visar_image *mapx, const visar_image *mapy);
+
 /* ---- Extensions: Zig-API-only functions of the reference exposed here ---- */
 
 /* reference src/imgproc/geometry.zig:586 warpPerspective(exec, src, dst, m: [9]f64,

@@ -31,7 +31,8 @@ def test_headers_declare_hot_path():
     for n in ["visar_gaussian_blur", "visar_sobel", "visar_resize", "visar_warp_affine", "visar_remap",
               "visar_morphology", "visar_cvt_color", "visar_threshold", "visar_background_create",
               "visar_background_apply", "visar_background_free", "visar_warp_perspective", "visar_sep_filter_2d",
-              "visar_gray_downsample", "visar_gray_downsample_rect", "visar_last_error", "visar_version"]:
+              "visar_gray_downsample", "visar_gray_downsample_rect", "visar_undistort",
+              "visar_init_undistort_rectify_map", "visar_last_error", "visar_version"]:
         assert n in names
 
 
@@ -126,6 +127,31 @@ def test_gray_downsample_errors_without_device():
     assert (d == 7).all()
     with pytest.raises(va.VisarError) as e:
         va.gray_downsample_rect(a, 2, np.empty((20, 29), np.uint8), 0, 0, 0, 0)
+    assert e.value.rc == -2
+
+def test_undistort_errors_without_device():
+    """capi_calib3d.zig:186-201, undistort.zig:36-41, 205-209, camera.zig:17-18."""
+    import visar_amd as va
+    a = np.zeros((10, 12, 3), np.uint8)
+    K = np.array([[10.0, 0, 6], [0, 10, 5], [0, 0, 1]])
+    cases = [((a, K, np.zeros(3)), -1, "InvalidArgument"),
+             ((a, K, None, None, np.zeros((10, 13, 3), np.uint8)), -2, "SizeMismatch"),
+             ((np.zeros((10, 12, 5), np.uint8), K), -3, "Unsupported"),
+             ((a.astype(np.uint16), K), -3, "Unsupported")]
+    for args, rc, name in cases:
+        with pytest.raises(va.VisarError) as e:
+            va.undistort(*args)
+        assert e.value.rc == rc and e.value.name == name, args[2:]
+    with pytest.raises(va.VisarError) as e:  # singular new_k * R
+        va.init_undistort_rectify_map(K, None, None, np.zeros((3, 3)), (12, 10))
+    assert e.value.rc == -1
+    with pytest.raises(va.VisarError) as e:  # maps must be 32F
+        va.init_undistort_rectify_map(K, None, None, None, (12, 10),
+                                      out=(np.zeros((10, 12), np.uint8), np.zeros((10, 12), np.uint8)))
+    assert e.value.rc == -3
+    with pytest.raises(va.VisarError) as e:
+        va.init_undistort_rectify_map(K, None, None, None, (12, 10),
+                                      out=(np.zeros((10, 12), np.float32), np.zeros((10, 11), np.float32)))
     assert e.value.rc == -2
 
 def test_last_error_is_thread_local():

@@ -65,9 +65,12 @@ __device__ __attribute__((always_inline)) inline int32_t fix32(double v) {
 // undistort.zig:120-157 sample: the inside, outside and per-tap border
 // branches are one formula, sum over taps of (tap inside ? pixel : 0) * w,
 // so every lane takes the same path: tap addresses are clamped into the
-// image (always in bounds) and the taps outside contribute 0.
-template <int CN>
+// image (always in bounds) and the taps outside contribute 0. ALIGNED (rows
+// 4-byte aligned): each wave stages its row of the tile in LDS and stores it
+// as 4 * CN contiguous bytes per lane (dword stores) instead of byte stores.
+template <int CN, bool ALIGNED>
 __global__ __launch_bounds__(kUdThreads) void undistort_u8(UndistortArgs p, TileOrder o) {
+    __shared__ uint32_t stage[kUdTileH][kUdTileW * CN / 4];
     int bx, by;
     if (!tile_of(o, bx, by)) return;
     const int y = by * kUdTileH + tile_ty();
@@ -113,15 +116,39 @@ __global__ __launch_bounds__(kUdThreads) void undistort_u8(UndistortArgs p, Tile
             t[q][3][c] = r1[cx1 * CN + c];
         }
     }
+    uint8_t res[kUdPx][CN];
+#pragma unroll
+    for (int q = 0; q < kUdPx; q++)
+#pragma unroll
+        for (int c = 0; c < CN; c++)
+            res[q][c] = (uint8_t)((t[q][0][c] * wq[q][0] + t[q][1][c] * wq[q][1] + t[q][2][c] * wq[q][2] +
+                                   t[q][3][c] * wq[q][3] + 512) >> 10);
     const gptr<uint8_t> out = grow<uint8_t>(p.d, y);
+    if constexpr (ALIGNED) {
+        uint8_t *row = reinterpret_cast<uint8_t *>(stage[tile_ty()]);
 #pragma unroll
-    for (int q = 0; q < kUdPx; q++) {
-        const int x = xb + 64 * q;
-        if (x >= p.d.w) break;
+        for (int q = 0; q < kUdPx; q++)
 #pragma unroll
-        for (int c = 0; c < CN; c++) {
-            const int32_t s = t[q][0][c] * wq[q][0] + t[q][1][c] * wq[q][1] + t[q][2][c] * wq[q][2] + t[q][3][c] * wq[q][3];
-            out[(long)x * CN + c] = (uint8_t)((s + 512) >> 10);
+            for (int c = 0; c < CN; c++) row[(tile_tx() + 64 * q) * CN + c] = res[q][c];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // Lane l stores bytes [4 CN l, 4 CN (l + 1)) of the wave's row span.
+        const int b0 = tile_tx() * 4 * CN, valid = (p.d.w - bx * kUdTileW) * CN;
+        const gptr<uint8_t> dst = out + (long)bx * kUdTileW * CN + b0;
+        if (b0 + 4 * CN <= valid) {
+#pragma unroll
+            for (int k = 0; k < CN; k++) ((gptr<uint32_t>)dst)[k] = stage[tile_ty()][tile_tx() * CN + k];
+        } else {
+            for (int k = 0; k < 4 * CN && b0 + k < valid; k++) dst[k] = row[b0 + k];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kUdPx; q++) {
+            const int x = xb + 64 * q;
+            if (x >= p.d.w) break;
+#pragma unroll
+            for (int c = 0; c < CN; c++) out[(long)x * CN + c] = res[q][c];
         }
     }
 }
@@ -152,11 +179,16 @@ inline int launched() { return hipGetLastError() == hipSuccess ? OK : INTERNAL; 
 int launch_undistort(const UndistortArgs &p, hipStream_t st) {
     const TileOrder o = tile_order((p.d.w + kUdTileW - 1) / kUdTileW, (p.d.h + kUdTileH - 1) / kUdTileH, true);
     const dim3 g(tile_grid(o.ntiles)), b(kUdThreads);
-    switch (p.s.cn) {
-    case 1: hipLaunchKernelGGL(undistort_u8<1>, g, b, 0, st, p, o); break;
-    case 2: hipLaunchKernelGGL(undistort_u8<2>, g, b, 0, st, p, o); break;
-    case 3: hipLaunchKernelGGL(undistort_u8<3>, g, b, 0, st, p, o); break;
-    case 4: hipLaunchKernelGGL(undistort_u8<4>, g, b, 0, st, p, o); break;
+    const bool al = (uintptr_t)p.d.p % 4 == 0 && p.d.stride % 4 == 0;
+    switch (p.s.cn * 2 + al) {
+    case 2: hipLaunchKernelGGL((undistort_u8<1, false>), g, b, 0, st, p, o); break;
+    case 3: hipLaunchKernelGGL((undistort_u8<1, true>), g, b, 0, st, p, o); break;
+    case 4: hipLaunchKernelGGL((undistort_u8<2, false>), g, b, 0, st, p, o); break;
+    case 5: hipLaunchKernelGGL((undistort_u8<2, true>), g, b, 0, st, p, o); break;
+    case 6: hipLaunchKernelGGL((undistort_u8<3, false>), g, b, 0, st, p, o); break;
+    case 7: hipLaunchKernelGGL((undistort_u8<3, true>), g, b, 0, st, p, o); break;
+    case 8: hipLaunchKernelGGL((undistort_u8<4, false>), g, b, 0, st, p, o); break;
+    case 9: hipLaunchKernelGGL((undistort_u8<4, true>), g, b, 0, st, p, o); break;
     default: return UNSUPPORTED;
     }
     return launched();

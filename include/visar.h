@@ -112,6 +112,24 @@ int visar_connected_components(const visar_image *src, const visar_image *labels
 int visar_remap(const visar_image *src, const visar_image *dst, const visar_image *mapx, const visar_image *mapy,
                 double value);
 
+/* reference include/visar.h:56 (capi.zig:208-217 -> imgproc/filter.zig:617 boxFilter):
+ * 8U, 16U and 32F, odd kw, kh; normalize != 0 divides by kw * kh. 16S returns
+ * VISAR_UNSUPPORTED (the reference's i16 sums are undefined for negative
+ * samples), and so do kernels wider or taller than 127 (the reference: 255). */
+int visar_box_filter(const visar_image *src, const visar_image *dst, int kw, int kh, int normalize, int border);
+
+/* reference include/visar.h:58-60 (capi.zig:234-250 -> imgproc/pyramid.zig:65, 403):
+ * 8U, 16U, 16S, 32F; dst ((w + 1) / 2, (h + 1) / 2) for pyr_down (any border
+ * but constant) and (2w, 2h) for pyr_up (border 4, BORDER_DEFAULT, only). */
+int visar_pyr_down(const visar_image *src, const visar_image *dst, int border);
+int visar_pyr_up(const visar_image *src, const visar_image *dst, int border);
+
+/* reference include/visar.h:69-71 (capi.zig:326-337 -> imgproc/threshold.zig:306):
+ * 8U, one channel; method 0 MEAN_C, 1 GAUSSIAN_C; type 0 BINARY, 1 BINARY_INV;
+ * block odd, 3 to 127 on the GPU (larger: VISAR_UNSUPPORTED). */
+int visar_adaptive_threshold(const visar_image *src, const visar_image *dst, double maxval, int method, int type,
+                             int block, double delta);
+
 /* reference include/visar.h:333-335 (capi_calib3d.zig:186-191 -> calib3d/undistort.zig:205):
  * 8U, 1-4 channels, dst the size of src; k and new_k (NULL: k) are row-major 3x3;
  * dist has nd = 0, 4, 5 or 8 coefficients (k1 k2 p1 p2 [k3 [k4 k5 k6]]), NULL = none.

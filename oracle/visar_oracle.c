@@ -412,11 +412,11 @@ static int sep_f32(const orc_image *s, const orc_image *d, const float *kx, int 
     float *V = (float *)malloc(sizeof(float) * (size_t)(n + 2 * rx * cn));
     if (!V) return ORC_OUT_OF_MEMORY;
     float *mid = V + rx * cn;
-    float tv[128];
+    float tv[256];
     /* const_v: value of a constant-border pixel after the vertical pass */
     float const_v = 0.0f;
     for (int j = 0; j < nky; j++) const_v = const_v + ky[j] * 0.0f;
-    long rows[128];
+    long rows[256];
     for (long y = 0; y < H; y++) {
         for (int j = 0; j < nky; j++) rows[j] = orc_border_map(y + j - ry, H, border);
         for (long c = 0; c < n; c++) {
@@ -1223,6 +1223,191 @@ int orc_morph_pass(const orc_image *s, const orc_image *d, int op_max, const uin
                 if (T == 5) st_f(d, y, x * cn + ch, (float)acc);
                 else st_i(d, y, x * cn + ch, (int64_t)acc);
             }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------- box filter, pyramids */
+
+/* capi.zig:208-217 visar_box_filter -> filter.zig:617-633 boxFilter: unit
+ * taps, exact integer sums for u8 and u16 (boxU8's running and prefix sums,
+ * :638-806, give the same integers), f32 through the separable engine's
+ * order; then f32(sum) * f32(scale) saturated (FinishScaled), scale
+ * 1 / (kw * kh) or 1. i16 is rejected: the reference widens i16 samples
+ * into u32 accumulators with @intCast, undefined for negative samples. */
+int orc_box_filter(const orc_image *src, const orc_image *dst, int kw, int kh, int normalize, int border) {
+    if (border < 0 || border > 4) return ORC_UNSUPPORTED;
+    if (kw <= 0 || kh <= 0) return ORC_INVALID_ARGUMENT;
+    if (!src) return ORC_INVALID_ARGUMENT;
+    uint32_t T = src->depth;
+    if (T != 0 && T != 2 && T != 3 && T != 5) return ORC_UNSUPPORTED;
+    int rc;
+    if ((rc = check(src, T)) || (rc = check(dst, T))) return rc;
+    if (kw > 255 || kh > 255 || kw % 2 == 0 || kh % 2 == 0) return ORC_INVALID_ARGUMENT;
+    if (T == 3) return ORC_UNSUPPORTED;
+    if (!same_shape(src, dst)) return ORC_SIZE_MISMATCH;
+    const double scale = normalize ? 1.0 / (double)(kw * kh) : 1.0;
+    if (T == 5) {
+        float ones[255];
+        for (int i = 0; i < 255; i++) ones[i] = 1.0f;
+        return sep_f32(src, dst, ones, kw, ones, kh, border, FIN_SCALED, scale, 0);
+    }
+    int64_t ones[255];
+    for (int i = 0; i < 255; i++) ones[i] = 1;
+    return sep_int(src, dst, ones, kw, ones, kh, border, FIN_SCALED, scale);
+}
+
+/* The integer finish (v + 2^(s-1)) >> s (arithmetic for i16) or, for f32,
+ * v * 2^-s (pyramid.zig:50-61). */
+static void pyr_store(const orc_image *d, long y, long e, double v, int s) {
+    if (d->depth == 5) {
+        st_f(d, y, e, (float)v * (1.0f / (float)(1 << s)));
+        return;
+    }
+    const int64_t iv = (int64_t)v;
+    st_i(d, y, e, (iv + (1 << (s - 1))) >> s);
+}
+
+/* The reference's 5-tap sum (a + f) + (b + e) * 4 + c * 6, in f32 for f32
+ * images and exactly otherwise (pyramid.zig:573-590 vert5, :600-607 h5). */
+static double tap5(int f32, double a, double b, double c, double e, double f) {
+    if (f32) return (double)(((float)a + (float)f) + ((float)b + (float)e) * 4.0f + (float)c * 6.0f);
+    return (a + f) + (b + e) * 4 + c * 6;
+}
+
+/* capi.zig:234-241 visar_pyr_down -> pyramid.zig:65-122 pyrDown: dst is
+ * ((w + 1) / 2, (h + 1) / 2); per output the vertical 5-tap sums of the five
+ * border-mapped source columns 2x - 2 .. 2x + 2 over rows 2y - 2 .. 2y + 2,
+ * then the horizontal 5-tap sum of those, then (v + 128) >> 8 (v / 256 for
+ * f32). Any border but constant. */
+int orc_pyr_down(const orc_image *src, const orc_image *dst, int border) {
+    if (border < 0 || border > 4) return ORC_UNSUPPORTED;
+    if (!src) return ORC_INVALID_ARGUMENT;
+    uint32_t T = src->depth;
+    if (T != 0 && T != 2 && T != 3 && T != 5) return ORC_UNSUPPORTED;
+    int rc;
+    if ((rc = check(src, T)) || (rc = check(dst, T))) return rc;
+    if (border == 0) return ORC_UNSUPPORTED;
+    const long W = src->width, H = src->height, cn = src->channels;
+    if (dst->channels != src->channels || dst->width != (src->width + 1) / 2 || dst->height != (src->height + 1) / 2)
+        return ORC_SIZE_MISMATCH;
+    const int f32 = T == 5;
+    for (long y = 0; y < (long)dst->height; y++) {
+        long ry[5];
+        for (int k = 0; k < 5; k++) ry[k] = orc_border_map(2 * y + k - 2, H, border);
+        for (long x = 0; x < (long)dst->width; x++) {
+            long cx[5];
+            for (int i = 0; i < 5; i++) cx[i] = orc_border_map(2 * x + i - 2, W, border);
+            for (long ch = 0; ch < cn; ch++) {
+                double v[5];
+                for (int i = 0; i < 5; i++) {
+                    const long e = cx[i] * cn + ch;
+                    v[i] = tap5(f32, ldd(src, ry[0], e), ldd(src, ry[1], e), ldd(src, ry[2], e), ldd(src, ry[3], e),
+                                ldd(src, ry[4], e));
+                }
+                pyr_store(dst, y, x * cn + ch, tap5(f32, v[0], v[1], v[2], v[3], v[4]), 8);
+            }
+        }
+    }
+    return ORC_OK;
+}
+
+/* capi.zig:243-250 visar_pyr_up -> pyramid.zig:403-504 pyrUp (reflect101
+ * only): dst is (2w, 2h). Source row y gives output rows 2y (a + 6b + c) and
+ * 2y + 1 (4 (b + c)) from rows a = y - 1, b = y, c = y + 1 of the image
+ * reflected about its upsampled edges (row -1 is row 1, row h is row h - 1;
+ * columns likewise); along the row, output pixel 2p takes
+ * (q[p-1] + q[p+1]) + 6 q[p] and 2p + 1 takes 4 (q[p] + q[p+1]); then
+ * (v + 32) >> 6 (v / 64 for f32). */
+int orc_pyr_up(const orc_image *src, const orc_image *dst, int border) {
+    if (border < 0 || border > 4) return ORC_UNSUPPORTED;
+    if (!src) return ORC_INVALID_ARGUMENT;
+    uint32_t T = src->depth;
+    if (T != 0 && T != 2 && T != 3 && T != 5) return ORC_UNSUPPORTED;
+    int rc;
+    if ((rc = check(src, T)) || (rc = check(dst, T))) return rc;
+    if (border != 4) return ORC_UNSUPPORTED;
+    const long w = src->width, h = src->height, cn = src->channels;
+    if (dst->channels != src->channels || (uint64_t)dst->width != 2 * (uint64_t)w || (uint64_t)dst->height != 2 * (uint64_t)h)
+        return ORC_SIZE_MISMATCH;
+    const int f32 = T == 5;
+    for (long y = 0; y < h; y++) {
+        const long ya = y > 0 ? y - 1 : (h > 1 ? 1 : 0), yc = y + 1 < h ? y + 1 : y;
+        for (int odd = 0; odd < 2; odd++) {
+            for (long p = 0; p < w; p++) {
+                const long px[3] = {p > 0 ? p - 1 : (w > 1 ? 1 : 0), p, p + 1 < w ? p + 1 : w - 1};
+                for (long ch = 0; ch < cn; ch++) {
+                    double q[3];
+                    for (int i = 0; i < 3; i++) {
+                        const long e = px[i] * cn + ch;
+                        const double a = ldd(src, ya, e), b = ldd(src, y, e), c = ldd(src, yc, e);
+                        if (f32) q[i] = odd ? (double)(((float)b + (float)c) * 4.0f) : (double)(((float)a + (float)c) + (float)b * 6.0f);
+                        else q[i] = odd ? (b + c) * 4 : (a + c) + b * 6;
+                    }
+                    double ev, od;
+                    if (f32) {
+                        ev = (double)(((float)q[0] + (float)q[2]) + (float)q[1] * 6.0f);
+                        od = (double)(((float)q[1] + (float)q[2]) * 4.0f);
+                    } else {
+                        ev = (q[0] + q[2]) + q[1] * 6;
+                        od = (q[1] + q[2]) * 4;
+                    }
+                    pyr_store(dst, 2 * y + odd, (2 * p) * cn + ch, ev, 6);
+                    pyr_store(dst, 2 * y + odd, (2 * p + 1) * cn + ch, od, 6);
+                }
+            }
+        }
+    }
+    return ORC_OK;
+}
+
+/* capi.zig:326-337 visar_adaptive_threshold -> threshold.zig:306-360:
+ * the mean of each pixel's block x block neighbourhood (boxFilter, or the
+ * Gaussian blur of an f32 copy saturated back to u8; replicated border),
+ * then dst = s + idelta > m ? maxval : 0 (BINARY) or the opposite
+ * (BINARY_INV), idelta = ceil(delta) or floor(delta). */
+int orc_adaptive_threshold(const orc_image *src, const orc_image *dst, double maxval, int method, int type, int block,
+                           double delta) {
+    if (method != 0 && method != 1) return ORC_INVALID_ARGUMENT;
+    if (type != 0 && type != 1) return ORC_INVALID_ARGUMENT;
+    if (block < 3) return ORC_INVALID_ARGUMENT;
+    int rc;
+    if ((rc = check(src, 0)) || (rc = check(dst, 0))) return rc;
+    if (!same_shape(src, dst)) return ORC_SIZE_MISMATCH;
+    if (src->channels != 1) return ORC_UNSUPPORTED;
+    if (block % 2 != 1) return ORC_INVALID_ARGUMENT;
+    const long W = src->width, H = src->height;
+    uint8_t *mb = (uint8_t *)malloc((size_t)W * H);
+    if (!mb) return ORC_OUT_OF_MEMORY;
+    orc_image mean = {(uint32_t)W, (uint32_t)H, 1, 0, (size_t)W, mb};
+    if (method == 0) {
+        rc = orc_box_filter(src, &mean, block, block, 1, 1);
+    } else {
+        float *fa = (float *)malloc(sizeof(float) * (size_t)W * H * 2);
+        if (!fa) { free(mb); return ORC_OUT_OF_MEMORY; }
+        orc_image f = {(uint32_t)W, (uint32_t)H, 1, 5, (size_t)W * 4, fa};
+        orc_image fb = {(uint32_t)W, (uint32_t)H, 1, 5, (size_t)W * 4, fa + (size_t)W * H};
+        for (long y = 0; y < H; y++)
+            for (long x = 0; x < W; x++) fa[y * W + x] = (float)((const uint8_t *)rowp(src, y))[x];
+        rc = orc_gaussian_blur(&f, &fb, block, block, 0, 0, 1);
+        if (!rc)
+            for (long y = 0; y < H; y++)
+                for (long x = 0; x < W; x++) mb[y * W + x] = (uint8_t)sat_cast_f((double)((float *)fb.data)[y * W + x], 0);
+        free(fa);
+    }
+    if (rc) { free(mb); return rc; }
+    const int64_t imax = sat_cast_f(maxval, 0);
+    long idelta = (long)(type == 0 ? ceil(delta) : floor(delta));
+    if (idelta < -1024) idelta = -1024;
+    if (idelta > 1024) idelta = 1024;
+    for (long y = 0; y < H; y++) {
+        const uint8_t *sr = (const uint8_t *)rowp(src, y);
+        uint8_t *dr = (uint8_t *)rowp(dst, y);
+        for (long x = 0; x < W; x++) {
+            const int gt = (long)sr[x] + idelta > (long)mb[y * W + x];
+            dr[x] = (uint8_t)((type == 0) == gt ? imax : 0);
+        }
+    }
+    free(mb);
     return ORC_OK;
 }
 

@@ -65,6 +65,11 @@ int orc_sep_filter_2d(const orc_image *src, const orc_image *dst, const float *k
 
 /* color / threshold */
 int orc_cvt_color(const orc_image *src, const orc_image *dst, int code);
+int orc_box_filter(const orc_image *src, const orc_image *dst, int kw, int kh, int normalize, int border);
+int orc_pyr_down(const orc_image *src, const orc_image *dst, int border);
+int orc_pyr_up(const orc_image *src, const orc_image *dst, int border);
+int orc_adaptive_threshold(const orc_image *src, const orc_image *dst, double maxval, int method, int type, int block,
+                           double delta);
 int orc_undistort(const orc_image *src, const orc_image *dst, const double K[9], const double *dist, size_t nd,
                   const double *new_k);
 int orc_init_undistort_rectify_map(const double K[9], const double *dist, size_t nd, const double *R, const double *new_k,

@@ -772,3 +772,72 @@ def test_init_undistort_rectify_map(va, orc, name):
             for g, dg, wv, ax in [(got[0], dm[0], want[0], "x"), (got[1], dm[1], want[1], "y")]:
                 assert_same(g, wv, f"map{ax} host {name} {w}x{h}")
                 assert_same(dg.cpu().numpy(), wv, f"map{ax} device {name} {w}x{h}")
+
+
+# ------------------------------------------ pyramids, box filter, adaptive
+
+PYR_SIZES = [(1, 1), (3, 2), (4, 5), (19, 37), (9, 70), (7, 129)]  # (h, w): pyramid.zig:615
+
+
+@pytest.mark.parametrize("dt", [np.uint8, np.uint16, np.int16, np.float32])
+def test_pyramids(va, orc, dt):
+    for (h, w) in PYR_SIZES:
+        for cn in (1, 2, 3, 4, 5):
+            a = natural(h, w, cn, h * 31 + w + cn, dt) if cn <= 4 else np.dstack([natural(h, w, 3, 1, dt), natural(h, w, 2, 2, dt)])
+            for b in (4, 2, 1, 3):
+                check(va, orc, "pyr_down", a, b)
+            check(va, orc, "pyr_up", a, 4)
+
+
+def test_pyramids_frames(va, orc):
+    from visar_amd import synth
+    for cn in (1, 3):
+        a = synth.uniform_bytes(cn, (1080, 1920, cn)) if cn > 1 else synth.uniform_bytes(cn, (1080, 1920))
+        check(va, orc, "pyr_down", a, 4)
+        check(va, orc, "pyr_up", a[:540, :960], 4)
+
+
+@pytest.mark.parametrize("cn", [1, 2, 3, 4])
+def test_box_filter(va, orc, cn):
+    for (h, w) in ((23, 67), (1, 1), (5, 3), (37, 53)):
+        a = natural(h, w, cn, h + w + cn)
+        for ks in ((3, 3), (5, 5), (7, 3), (9, 1), (15, 15), (31, 31), (1, 9), (127, 3), (3, 127)):
+            for b in range(5):
+                check(va, orc, "box_filter", a, ks, True, b)
+            check(va, orc, "box_filter", a, ks, False, 4)
+
+
+def test_box_filter_other_depths(va, orc):
+    for dt in (np.uint16, np.float32):
+        for cn in (1, 3):
+            a = natural(29, 41, cn, 7, dt)
+            for ks in ((3, 3), (5, 7), (21, 1), (127, 5)):
+                for b in (4, 1, 0):
+                    check(va, orc, "box_filter", a, ks, True, b)
+                check(va, orc, "box_filter", a, ks, False, 3)
+
+
+def test_box_filter_frame(va, orc):
+    from visar_amd import synth
+    a = synth.uniform_bytes(11, (1080, 1920))
+    check(va, orc, "box_filter", a, (5, 5), True, 4)
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_adaptive_threshold(va, orc, method):
+    for (h, w) in ((41, 57), (1, 1), (2, 300), (200, 3)):
+        a = natural(h, w, 1, h + 3 * w)
+        for kind in (0, 1):
+            for block in (3, 5, 7, 11, 31, 127):
+                for delta in (0, 5, -3.5, 2.25, 300):
+                    check(va, orc, "adaptive_threshold", a, 255, method, kind, block, delta)
+            check(va, orc, "adaptive_threshold", a, 77.6, method, kind, 9, -1)
+    from visar_amd import synth
+    a = synth.uniform_bytes(5, (1080, 1920))
+    check(va, orc, "adaptive_threshold", a, 255, method, 0, 11, 2)
+    # In place on the device.
+    a = natural(64, 96, 1, 8)
+    want = orc.adaptive_threshold(a, 255, method, 1, 15, 3)
+    da = dev(a)
+    va.adaptive_threshold(da, 255, method, 1, 15, 3, dst=da)
+    assert_same(da.cpu().numpy(), want, "adaptive in place")

@@ -768,3 +768,235 @@ def test_undistort_rejects(orc):
     out = np.full((10, 12, 3), 7, np.uint8)
     orc.undistort(np.ones((10, 12, 3), np.uint8), K, None, K0, out)
     assert (out == 7).all()
+
+
+# ------------------------------------------------ pyramids, box, adaptive
+
+def bmap(p, n, b):
+    """border.zig:22-39 for one coordinate: 1 replicate, 2 reflect, 3 wrap, 4 reflect101."""
+    if 0 <= p < n:
+        return p
+    if b == 1:
+        return 0 if p < 0 else n - 1
+    if b == 3:
+        return p % n
+    if n == 1:
+        return 0
+    d = 1 if b == 4 else 0
+    while p < 0 or p >= n:
+        p = -p - 1 + d if p < 0 else n - 1 - (p - n) - d
+    return p
+
+
+def fill_random(dtype, h, w, cn, seed):
+    """pyramid.zig:597-610 fillRandom: an LCG over the samples in row order."""
+    n = h * w * cn
+    out = np.empty(n, np.uint32)
+    s = seed & 0xFFFFFFFF
+    for i in range(n):
+        s = (s * 1103515245 + 12345) & 0xFFFFFFFF
+        out[i] = (s >> 12) & 0xFFFF
+    if dtype == np.uint8:
+        a = (out & 0xFF).astype(np.uint8)
+    elif dtype == np.uint16:
+        a = out.astype(np.uint16)
+    elif dtype == np.int16:
+        a = out.astype(np.uint16).view(np.int16)
+    else:
+        a = (out.astype(np.float32) / np.float32(65536.0)).astype(np.float32)
+    a = a.reshape(h, w, cn)
+    return a[..., 0] if cn == 1 else a
+
+
+PK = np.array([1, 4, 6, 4, 1], np.float64)
+
+
+def ref_down(a, b):
+    """pyramid.zig:570-580 refDown: the direct 5x5 sum of border-mapped samples."""
+    h, w = a.shape[:2]
+    a3 = a.reshape(h, w, -1).astype(np.float64)
+    dh, dw = (h + 1) // 2, (w + 1) // 2
+    rows = np.array([[bmap(2 * y + j - 2, h, b) for j in range(5)] for y in range(dh)])
+    cols = np.array([[bmap(2 * x + i - 2, w, b) for i in range(5)] for x in range(dw)])
+    s = np.zeros((dh, dw, a3.shape[2]))
+    for j in range(5):
+        for i in range(5):
+            s += PK[j] * PK[i] * a3[rows[:, j]][:, cols[:, i]]
+    return s
+
+
+def ref_up(a):
+    """pyramid.zig:584-595 refUp: the kernel over the zero-upsampled image,
+    reflect101 in upsampled coordinates, odd coordinates contributing 0."""
+    h, w = a.shape[:2]
+    a3 = a.reshape(h, w, -1).astype(np.float64)
+    s = np.zeros((2 * h, 2 * w, a3.shape[2]))
+    for j in range(5):
+        uy = np.array([bmap(y + j - 2, 2 * h, 4) for y in range(2 * h)])
+        for i in range(5):
+            ux = np.array([bmap(x + i - 2, 2 * w, 4) for x in range(2 * w)])
+            ok = (uy[:, None] % 2 == 0) & (ux[None, :] % 2 == 0)
+            s += PK[j] * PK[i] * a3[uy // 2][:, ux // 2] * ok[..., None]
+    return s
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.int16, np.float32])
+def test_pyr_down_up_direct_sum(orc, dtype):
+    # pyramid.zig:612-644, every size, channel count and border it uses.
+    for (w, h) in ((1, 1), (2, 3), (5, 4), (37, 19), (70, 9), (129, 7)):
+        for cn in (1, 2, 3, 4, 5):
+            a = fill_random(dtype, h, w, cn, w * 31 + cn)
+            for b in (4, 2, 1, 3):
+                got = orc.pyr_down(a, b).reshape((h + 1) // 2, (w + 1) // 2, cn).astype(np.float64)
+                s = ref_down(a, b)
+                if dtype == np.float32:
+                    assert np.abs(got - s / 256).max() <= 1e-5, (w, h, cn, b)
+                else:
+                    assert np.array_equal(got, np.floor((s + 128) / 256)), (w, h, cn, b, dtype)
+            got = orc.pyr_up(a).reshape(2 * h, 2 * w, cn).astype(np.float64)
+            s = ref_up(a)
+            if dtype == np.float32:
+                assert np.abs(got - s / 64).max() <= 1e-5, (w, h, cn)
+            else:
+                assert np.array_equal(got, np.floor((s + 32) / 64)), (w, h, cn, dtype)
+
+
+def test_pyr_down_f32_order(orc):
+    # pyramid.zig:646-672: the f32 sums in the scalar formula's order.
+    f = np.float32
+    for (w, h) in ((131, 12), (64, 9), (70, 7), (9, 5)):
+        for cn in (3, 4):
+            a = fill_random(np.float32, h, w, cn, w * 7 + cn)
+            for b in (4, 1):
+                got = orc.pyr_down(a, b)
+                dh, dw = (h + 1) // 2, (w + 1) // 2
+                want = np.empty_like(got)
+                for y in range(dh):
+                    rs = [bmap(2 * y + k - 2, h, b) for k in range(5)]
+                    for x in range(dw):
+                        cs = [bmap(2 * x + i - 2, w, b) for i in range(5)]
+                        r = a[rs][:, cs]  # (5 rows, 5 cols, cn)
+                        p = (r[0] + r[4]) + (r[1] + r[3]) * f(4) + r[2] * f(6)
+                        want[y, x] = ((p[0] + p[4]) + (p[1] + p[3]) * f(4) + p[2] * f(6)) * f(1.0 / 256.0)
+                assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (w, h, cn, b)
+
+
+def test_pyr_sizes_and_rejects(orc):
+    # pyramid.zig:674-686 buildPyramid sizes on a constant 101x50 image.
+    lv = np.full((50, 101), 7, np.uint8)
+    sizes = []
+    for _ in range(4):
+        lv = orc.pyr_down(lv)
+        sizes.append(lv.shape[::-1])
+    assert sizes[0] == (51, 25) and sizes[3] == (7, 4) and lv[2, 3] == 7 and (lv == 7).all()
+    with pytest.raises(orc.OracleError) as e:
+        orc.pyr_down(np.zeros((50, 101), np.uint8), 4, np.zeros((12, 26), np.uint8))
+    assert e.value.rc == -2
+    with pytest.raises(orc.OracleError) as e:
+        orc.pyr_down(np.zeros((8, 8), np.uint8), 0)
+    assert e.value.rc == -3
+    with pytest.raises(orc.OracleError) as e:
+        orc.pyr_up(np.zeros((8, 8), np.uint8), 1)
+    assert e.value.rc == -3
+
+
+def naive_box(a, kw, kh, b):
+    h, w = a.shape[:2]
+    a3 = a.reshape(h, w, -1).astype(np.int64)
+    s = np.zeros(a3.shape, np.int64)
+    for j in range(kh):
+        rows = [bmap(y + j - kh // 2, h, b) if b else y + j - kh // 2 for y in range(h)]
+        for i in range(kw):
+            cols = [bmap(x + i - kw // 2, w, b) if b else x + i - kw // 2 for x in range(w)]
+            ry = np.array(rows)
+            cx = np.array(cols)
+            ok = ((ry >= 0) & (ry < h))[:, None] & ((cx >= 0) & (cx < w))[None, :]
+            s += a3[np.clip(ry, 0, h - 1)][:, np.clip(cx, 0, w - 1)] * ok[..., None]
+    return s
+
+
+def sat_round(v, lo, hi):
+    return np.clip(np.rint(v), lo, hi)
+
+
+def test_box_filter_naive_sum(orc):
+    # filter.zig:960-985 (67 x 23, cn 1-4, its seven windows, reflect101),
+    # then every other border, u16, f32 and unnormalised sums.
+    for cn in (1, 2, 3, 4):
+        seed = 99
+        vals = np.empty(67 * 23 * cn, np.uint8)
+        for i in range(vals.size):
+            seed = (seed * 1103515245 + 12345) & 0xFFFFFFFF
+            vals[i] = (seed >> 16) & 0xFF
+        a = vals.reshape(23, 67, cn)
+        a = a[..., 0] if cn == 1 else a
+        for kw, kh in ((3, 3), (5, 5), (7, 3), (9, 1), (15, 15), (31, 31), (1, 9)):
+            scale = np.float32(1.0 / (kw * kh))
+            for b in (4, 1, 2, 3, 0):
+                s = naive_box(a, kw, kh, b)
+                want = sat_round(s.astype(np.float32) * scale, 0, 255).reshape(a.shape)
+                assert np.array_equal(orc.box_filter(a, (kw, kh), True, b), want), (cn, kw, kh, b)
+            s = naive_box(a, kw, kh, 4)
+            assert np.array_equal(orc.box_filter(a, (kw, kh), False), np.clip(s, 0, 255).reshape(a.shape))
+    u = fill_random(np.uint16, 19, 37, 3, 5)
+    for kw, kh in ((3, 3), (255, 1), (5, 255)):
+        s = naive_box(u, kw, kh, 1)
+        want = sat_round(s.astype(np.float32) * np.float32(1.0 / (kw * kh)), 0, 65535).reshape(u.shape)
+        assert np.array_equal(orc.box_filter(u, (kw, kh), True, 1), want), (kw, kh)
+    f = fill_random(np.float32, 19, 37, 1, 6)
+    for kw, kh in ((3, 3), (9, 5), (255, 3)):
+        h, w = f.shape
+        ref = np.zeros((h, w))
+        for j in range(kh):
+            for i in range(kw):
+                ry = np.array([bmap(y + j - kh // 2, h, 4) for y in range(h)])
+                cx = np.array([bmap(x + i - kw // 2, w, 4) for x in range(w)])
+                ref += f.astype(np.float64)[ry][:, cx]
+        got = orc.box_filter(f, (kw, kh))
+        assert np.abs(got - ref / (kw * kh)).max() < 1e-5, (kw, kh)
+
+
+def test_box_filter_rejects(orc):
+    a = np.zeros((5, 6), np.uint8)
+    for ks, border, rc in [((3, 3), 7, -3), ((0, 3), 4, -1), ((4, 3), 4, -1), ((257, 3), 4, -1)]:
+        with pytest.raises(orc.OracleError) as e:
+            orc.box_filter(a, ks, True, border)
+        assert e.value.rc == rc, (ks, border)
+    with pytest.raises(orc.OracleError) as e:
+        orc.box_filter(np.zeros((5, 6), np.int16), (3, 3))
+    assert e.value.rc == -3
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_adaptive_threshold(orc, kind):
+    """threshold.zig:306-360: MEAN_C against a naive replicate-border box mean,
+    GAUSSIAN_C against the Gaussian blur of an f32 copy; closed forms on a
+    constant image."""
+    a = fill_random(np.uint8, 41, 57, 1, 21)
+    for block in (3, 5, 11, 31):
+        for delta in (0, 5, -3.5, 2.25, 300, -300):
+            s = naive_box(a, block, block, 1)
+            mean = sat_round(s.astype(np.float32) * np.float32(1.0 / (block * block)), 0, 255).reshape(a.shape)
+            idelta = int(np.ceil(delta) if kind == 0 else np.floor(delta))
+            gt = a.astype(np.int64) + idelta > mean
+            want = np.where(gt == (kind == 0), 200, 0).astype(np.uint8)
+            assert np.array_equal(orc.adaptive_threshold(a, 200, 0, kind, block, delta), want), (block, delta)
+            g = orc.gaussian_blur(a.astype(np.float32), (block, block), 0, 0, 1)
+            gm = sat_round(g, 0, 255)
+            want = np.where((a.astype(np.int64) + idelta > gm) == (kind == 0), 255, 0).astype(np.uint8)
+            assert np.array_equal(orc.adaptive_threshold(a, 255.4, 1, kind, block, delta), want), (block, delta)
+    c = np.full((9, 13), 77, np.uint8)
+    for delta, want_on in ((0.5, True), (0, False), (-0.5, False), (-1, False), (1, True)):
+        on = orc.adaptive_threshold(c, 9, 0, 0, 3, delta)
+        assert (on == (9 if want_on else 0)).all(), delta
+
+
+def test_adaptive_threshold_rejects(orc):
+    a = np.zeros((5, 6), np.uint8)
+    for args, rc in [((255, 2, 0, 3, 0), -1), ((255, 0, 2, 3, 0), -1), ((255, 0, 0, 1, 0), -1), ((255, 0, 0, 4, 0), -1)]:
+        with pytest.raises(orc.OracleError) as e:
+            orc.adaptive_threshold(a, *args)
+        assert e.value.rc == rc, args
+    with pytest.raises(orc.OracleError) as e:
+        orc.adaptive_threshold(np.zeros((5, 6, 3), np.uint8), 255, 0, 0, 3, 0)
+    assert e.value.rc == -3

@@ -265,6 +265,36 @@ def remap(a, mapx, mapy, value=0.0, dst=None):
     return dst
 
 
+def box_filter(a, ksize, normalize=True, border=4, dst=None):
+    """boxFilter / blur (filter.zig:617 boxFilter; tests/py/visar_ctypes.py:121): u8, u16, f32."""
+    dst = _empty(a, tuple(a.shape)) if dst is None else dst
+    _check(lib.visar_box_filter(_ref(a), _ref(dst), ksize[0], ksize[1], int(bool(normalize)), border))
+    return dst
+
+
+def pyr_down(a, border=4, dst=None):
+    """pyramid.zig:65 pyrDown (tests/py/visar_ctypes.py:133): dst ((w + 1) // 2, (h + 1) // 2)."""
+    dst = _empty(a, _shape(a, w=(a.shape[1] + 1) // 2, h=(a.shape[0] + 1) // 2)) if dst is None else dst
+    _check(lib.visar_pyr_down(_ref(a), _ref(dst), border))
+    return dst
+
+
+def pyr_up(a, border=4, dst=None):
+    """pyramid.zig:403 pyrUp (tests/py/visar_ctypes.py:139): dst (2w, 2h), reflect-101 only."""
+    dst = _empty(a, _shape(a, w=a.shape[1] * 2, h=a.shape[0] * 2)) if dst is None else dst
+    _check(lib.visar_pyr_up(_ref(a), _ref(dst), border))
+    return dst
+
+
+def adaptive_threshold(a, maxval, method, kind, block, delta, dst=None):
+    """threshold.zig:306 adaptiveThreshold: u8, one channel; method 0 MEAN_C, 1 GAUSSIAN_C;
+    kind 0 BINARY, 1 BINARY_INV."""
+    dst = _empty(a, tuple(a.shape)) if dst is None else dst
+    _check(lib.visar_adaptive_threshold(_ref(a), _ref(dst), ctypes.c_double(maxval), method, kind, block,
+                                        ctypes.c_double(delta)))
+    return dst
+
+
 def gray_downsample(a, k, dst=None):
     """BT.601 gray of a BGR/BGRA u8 image shrunk by k (1..16) with box averaging
     (videoio/frame.zig:170 grayDownsample; tests/py/visar_ctypes.py:102); dst is (h // k, w // k)."""

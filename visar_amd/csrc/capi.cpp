@@ -456,26 +456,21 @@ static bool same_shape(const visar_image *a, const visar_image *b) {
 }
 
 // capi.zig:197-206 + filter.zig:498-607
-int visar_gaussian_blur(const visar_image *src, const visar_image *dst, int kw, int kh, double sx, double sy,
-                        int border) {
-    if (border < 0 || border > 4) return fail(UNSUPPORTED, "Unsupported");
-    if (kw < 0 || kh < 0) return fail(INVALID_ARGUMENT, "InvalidArgument");
-    if (!src) return fail(INVALID_ARGUMENT, "InvalidArgument");
-    uint32_t T = src->depth;
-    if (!valid_depth(T)) return fail(UNSUPPORTED, "Unsupported");
-    int rc;
-    if ((rc = check_image(src, T)) || (rc = check_image(dst, T))) return rc;
+// filter.zig:498-607 gaussianBlur's taps for images of depth `T` (8.8 / 16.16
+// fixed point for u8 / u16, f32 otherwise), sizes from sigma where 0, and a
+// 1-pixel side reduced to one tap for non-constant borders. The caller sets
+// the source and destination depths.
+static int gaussian_params(SepParams &p, uint32_t T, int kw, int kh, double sx, double sy, int border,
+                           const visar_image *src) {
     double sy_in = sy > 0 ? sy : sx;
     if (kw == 0) kw = gaussian_ksize(sx, T == 0);
     if (kh == 0) kh = gaussian_ksize(sy_in, T == 0);
     if (kw <= 0 || kh <= 0 || kw % 2 == 0 || kh % 2 == 0 || kw > 127 || kh > 127)
         return fail(INVALID_ARGUMENT, "InvalidKernel");
-    if (!same_shape(src, dst)) return fail(SIZE_MISMATCH, "SizeMismatch");
     if (border != BORDER_CONSTANT) {
         if (src->width == 1) kw = 1;
         if (src->height == 1) kh = 1;
     }
-    SepParams p;
     std::memset(&p, 0, sizeof p);
     p.sdepth = p.ddepth = (int)T;
     p.nkx = kw;
@@ -510,10 +505,135 @@ int visar_gaussian_blur(const visar_image *src, const visar_image *dst, int kw, 
         }
         p.body = ((long)src->width * src->channels / 16) * 16;
     }
+    return OK;
+}
+
+int visar_gaussian_blur(const visar_image *src, const visar_image *dst, int kw, int kh, double sx, double sy,
+                        int border) {
+    if (border < 0 || border > 4) return fail(UNSUPPORTED, "Unsupported");
+    if (kw < 0 || kh < 0) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (!src) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    uint32_t T = src->depth;
+    if (!valid_depth(T)) return fail(UNSUPPORTED, "Unsupported");
+    int rc;
+    if ((rc = check_image(src, T)) || (rc = check_image(dst, T))) return rc;
+    // Kernel checks before the shape check, as in filter.zig:498-520.
+    double sy_in = sy > 0 ? sy : sx;
+    const int kw0 = kw == 0 ? gaussian_ksize(sx, T == 0) : kw, kh0 = kh == 0 ? gaussian_ksize(sy_in, T == 0) : kh;
+    if (kw0 <= 0 || kh0 <= 0 || kw0 % 2 == 0 || kh0 % 2 == 0 || kw0 > 127 || kh0 > 127)
+        return fail(INVALID_ARGUMENT, "InvalidKernel");
+    if (!same_shape(src, dst)) return fail(SIZE_MISMATCH, "SizeMismatch");
+    SepParams p;
+    if ((rc = gaussian_params(p, T, kw, kh, sx, sy, border, src))) return rc;
     Call call;
     if ((rc = call.begin())) return rc;
     if ((rc = call.input(src, p.src)) || (rc = call.output(dst, p.dst))) return rc;
     return call.finish(launch_sep(p, call.stream()));
+}
+
+// capi.zig:208-217 visar_box_filter -> filter.zig:617-633 boxFilter: unit
+// taps, exact integer sums (u8, u16) or the f32 engine's order, then
+// f32(sum) * f32(scale) saturated (FinishScaled). i16 is rejected: the
+// reference widens i16 samples into u32 accumulators with @intCast, which is
+// undefined for negative samples. Kernels wider or taller than 127 (the
+// reference allows 255) are not supported on the GPU.
+int visar_box_filter(const visar_image *src, const visar_image *dst, int kw, int kh, int normalize, int border) {
+    if (border < 0 || border > 4) return fail(UNSUPPORTED, "Unsupported");
+    if (kw <= 0 || kh <= 0) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (!src) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    uint32_t T = src->depth;
+    if (!valid_depth(T)) return fail(UNSUPPORTED, "Unsupported");
+    int rc;
+    if ((rc = check_image(src, T)) || (rc = check_image(dst, T))) return rc;
+    if (kw > 255 || kh > 255 || kw % 2 == 0 || kh % 2 == 0) return fail(INVALID_ARGUMENT, "InvalidKernel");
+    if (T == 3 || kw > 127 || kh > 127) return fail(UNSUPPORTED, "Unsupported");
+    if (!same_shape(src, dst)) return fail(SIZE_MISMATCH, "SizeMismatch");
+    SepParams p;
+    std::memset(&p, 0, sizeof p);
+    p.sdepth = p.ddepth = (int)T;
+    p.nkx = kw;
+    p.nky = kh;
+    p.border = border;
+    p.fin = FIN_SCALED;
+    p.scale = (float)(normalize ? 1.0 / (double)(kw * kh) : 1.0);
+    for (int i = 0; i < kw; i++) p.kxi[i] = 1, p.kxf[i] = 1.0f;
+    for (int i = 0; i < kh; i++) p.kyi[i] = 1, p.kyf[i] = 1.0f;
+    if (T == 5) {
+        p.f32mode = 1;
+        p.vsym = p.hsym = SYM_EVEN;
+        p.body = ((long)src->width * src->channels / 16) * 16;
+    }
+    Call call;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, p.src)) || (rc = call.output(dst, p.dst))) return rc;
+    return call.finish(launch_sep(p, call.stream()));
+}
+
+// capi.zig:234-250 visar_pyr_down / visar_pyr_up -> imgproc/pyramid.zig:65-122
+// pyrDown (dst ((w + 1) / 2, (h + 1) / 2), any border but constant) and
+// :403-427 pyrUp (dst (2w, 2h), reflect-101 only); u8, u16, i16, f32.
+static int pyramid_call(const visar_image *src, const visar_image *dst, int border, bool down) {
+    if (border < 0 || border > 4) return fail(UNSUPPORTED, "Unsupported");
+    if (!src) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    uint32_t T = src->depth;
+    if (!valid_depth(T)) return fail(UNSUPPORTED, "Unsupported");
+    int rc;
+    if ((rc = check_image(src, T)) || (rc = check_image(dst, T))) return rc;
+    if (down ? border == BORDER_CONSTANT : border != BORDER_REFLECT101) return fail(UNSUPPORTED, "Unsupported");
+    const uint64_t w = down ? (src->width + 1) / 2 : 2 * (uint64_t)src->width;
+    const uint64_t h = down ? (src->height + 1) / 2 : 2 * (uint64_t)src->height;
+    if (dst->channels != src->channels || dst->width != w || dst->height != h) return fail(SIZE_MISMATCH, "SizeMismatch");
+    Call call;
+    DImg s, d;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, s)) || (rc = call.output(dst, d))) return rc;
+    return call.finish(launch_pyramid(s, d, (int)T, border, down, call.stream()));
+}
+
+int visar_pyr_down(const visar_image *src, const visar_image *dst, int border) { return pyramid_call(src, dst, border, true); }
+int visar_pyr_up(const visar_image *src, const visar_image *dst, int border) { return pyramid_call(src, dst, border, false); }
+
+// capi.zig:326-337 visar_adaptive_threshold -> threshold.zig:306-360: the
+// block x block mean with a replicated border (boxFilter, or the f32
+// Gaussian blur of the u8 source saturated back to u8: one pass of the f32
+// engine, whose u8 loads convert exactly), then dst = s + idelta > m ? on :
+// off with idelta = ceil(delta) (BINARY) or floor(delta) (BINARY_INV).
+// Blocks above 127 are not supported on the GPU.
+int visar_adaptive_threshold(const visar_image *src, const visar_image *dst, double maxval, int method, int type,
+                             int block, double delta) {
+    if (method != 0 && method != 1) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (type != 0 && type != 1) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    if (block < 3) return fail(INVALID_ARGUMENT, "InvalidArgument");
+    int rc;
+    if ((rc = check_image(src, 0)) || (rc = check_image(dst, 0))) return rc;
+    if (!same_shape(src, dst)) return fail(SIZE_MISMATCH, "SizeMismatch");
+    if (src->channels != 1) return fail(UNSUPPORTED, "UnsupportedChannels");
+    if (block % 2 != 1) return fail(INVALID_ARGUMENT, "InvalidKernel");
+    if (block > 127) return fail(UNSUPPORTED, "Unsupported");
+    SepParams p;
+    if (method == 0) {
+        std::memset(&p, 0, sizeof p);
+        p.nkx = p.nky = block;
+        p.border = BORDER_REPLICATE;
+        p.fin = FIN_SCALED;
+        p.scale = (float)(1.0 / (double)(block * block));
+        for (int i = 0; i < block; i++) p.kxi[i] = p.kyi[i] = 1;
+    } else if ((rc = gaussian_params(p, 5, block, block, 0, 0, BORDER_REPLICATE, src))) {
+        return rc;
+    }
+    p.sdepth = p.ddepth = 0;
+    const double dl = type == 0 ? std::ceil(delta) : std::floor(delta);
+    const int idelta = (int)std::max(-1024.0, std::min(1024.0, dl));
+    const int imax = (int)sat_cast<uint8_t>(maxval);
+    Call call;
+    DImg s, d, m;
+    if ((rc = call.begin())) return rc;
+    if ((rc = call.input(src, s)) || (rc = call.output(dst, d))) return rc;
+    if ((rc = call.temp_like(d, 1, m))) return rc;
+    p.src = s;
+    p.dst = m;
+    if ((rc = launch_sep(p, call.stream()))) return call.finish(rc);
+    return call.finish(launch_adaptive_u8(s, m, d, idelta, type == 0 ? imax : 0, type == 0 ? 0 : imax, call.stream()));
 }
 
 // capi.zig:218-232 + filter.zig:856-906

@@ -77,6 +77,13 @@ int launch_copy(const DImg &s, const DImg &d, size_t elem, hipStream_t st);
 int launch_half_u8(const DImg &s, const DImg &d, hipStream_t st);
 // BGR / BGRA u8 -> gray shrunk by k with box averaging (videoio/frame.zig:167-177).
 int launch_gray_down(const DImg &s, const DImg &d, int k, hipStream_t st);
+
+// imgproc/pyramid.zig pyrDown (`down`, any border but constant) and pyrUp
+// (reflect-101): u8, u16, i16 and f32 (depth codes 0, 2, 3, 5).
+int launch_pyramid(const DImg &s, const DImg &d, int depth, int border, bool down, hipStream_t st);
+
+// threshold.zig:340-360: dst = s + idelta > m ? on : off, u8, one channel.
+int launch_adaptive_u8(const DImg &s, const DImg &m, const DImg &d, int idelta, int on, int off, hipStream_t st);
 int launch_area_int_u8(const DImg &s, const DImg &d, int ix, int iy, hipStream_t st);
 int launch_nearest(const DImg &s, const DImg &d, size_t elem, hipStream_t st);
 

@@ -17,7 +17,10 @@ workload. `--config 1` times the plain 1080p GaussianBlur 5x5 instead;
 `--config 6` the tracking image of a 3840x2160 BGRA frame (grayDownsample,
 BT.601 gray shrunk by 6 with box averaging, to 640x360); `--config 7` lens
 undistortion of a 1280x720 BGR frame (the reference benchmark's camera and
-5 distortion coefficients: f64 map, 1/32 px fixed point, 15-bit bilinear).
+5 distortion coefficients: f64 map, 1/32 px fixed point, 15-bit bilinear);
+`--config 8` the reference benchmark's pyramid and box cases at 1920x1080
+(pyrDown BGR, gray and f32 BGR, pyrUp gray, blur 3x3 and 15x15 BGR) and
+adaptiveThreshold (block 11, mean and Gaussian) in one step.
 """
 import argparse
 import json
@@ -38,7 +41,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=240)
     ap.add_argument("--warmup", type=int, default=24)
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5, 6, 7])
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5, 6, 7, 8])
     ap.add_argument("--streams", type=int, default=64)
     ap.add_argument("--seconds", type=float, default=6.0, help="served workload: Poisson run length")
     ap.add_argument("--host-workers", type=int, default=8, help="served workload: connected-components threads")
@@ -501,6 +504,31 @@ def main():
         ops = {"undistort_bgr": (lambda i: va.undistort(frames[i % R], UD_K, UD_DIST, dst=outs[i % 2]), 2 * W * H * 3)}
         pix_per_step = W * H
         workload = "undistort 1280x720 BGR u8, K = [[900, 0, 640], [0, 900, 360], [0, 0, 1]], 5 coefficients"
+    elif args.config == 8:
+        W, H = 1920, 1080
+        bgr = [torch.from_numpy(synth.uniform_bytes(i, (H, W, 3))).to(dev) for i in range(R)]
+        gray = [torch.from_numpy(synth.uniform_bytes(100 + i, (H, W))).to(dev) for i in range(R)]
+        fbgr = [(b.float() / 255.0).contiguous() for b in bgr[:4]]
+        hd, wd = (H + 1) // 2, (W + 1) // 2
+        o_pd = torch.empty((hd, wd, 3), dtype=torch.uint8, device=dev)
+        o_pg = torch.empty((hd, wd), dtype=torch.uint8, device=dev)
+        o_pu = torch.empty((2 * H, 2 * W), dtype=torch.uint8, device=dev)
+        o_pf = torch.empty((hd, wd, 3), dtype=torch.float32, device=dev)
+        o_bx = torch.empty((H, W, 3), dtype=torch.uint8, device=dev)
+        o_at = torch.empty((H, W), dtype=torch.uint8, device=dev)
+        ops = {
+            "pyr_down_bgr": (lambda i: va.pyr_down(bgr[i % R], dst=o_pd), W * H * 3 + wd * hd * 3),
+            "pyr_down_gray": (lambda i: va.pyr_down(gray[i % R], dst=o_pg), W * H + wd * hd),
+            "pyr_up_gray": (lambda i: va.pyr_up(gray[i % R], dst=o_pu), W * H + 4 * W * H),
+            "pyr_down_f32": (lambda i: va.pyr_down(fbgr[i % 4], dst=o_pf), 4 * (W * H * 3 + wd * hd * 3)),
+            "blur_3x3_bgr": (lambda i: va.box_filter(bgr[i % R], (3, 3), dst=o_bx), 2 * W * H * 3),
+            "blur_15x15_bgr": (lambda i: va.box_filter(bgr[i % R], (15, 15), dst=o_bx), 2 * W * H * 3),
+            "adaptive_mean_11": (lambda i: va.adaptive_threshold(gray[i % R], 255, 0, 0, 11, 2, dst=o_at), 2 * W * H),
+            "adaptive_gauss_11": (lambda i: va.adaptive_threshold(gray[i % R], 255, 1, 0, 11, 2, dst=o_at), 2 * W * H),
+        }
+        pix_per_step = W * H
+        workload = ("1920x1080: pyrDown BGR, gray, f32 BGR; pyrUp gray; blur 3x3 and 15x15 BGR; "
+                    "adaptiveThreshold 11 mean and Gaussian (one frame each per step)")
     else:
         W, H = 1920, 1080
         frames = [torch.from_numpy(synth.uniform_bytes(i, (H, W))).to(dev) for i in range(R * 6)]
@@ -631,6 +659,27 @@ def cpu_baseline(args, synth):
 
         return cpu_measure(args.cpu_seconds, gd_worker, W * H / 1e9, "Gpix/s",
                            "grayDownsample calls on 3840x2160 BGRA, k = 6")
+    if args.config == 8:
+        W, H = 1920, 1080
+
+        def r8_worker(k):
+            b = synth.uniform_bytes(k, (H, W, 3))
+            g = synth.uniform_bytes(100 + k, (H, W))
+            f = (b.astype(np.float32) / 255.0).astype(np.float32)
+
+            def one(n):
+                oracle.pyr_down(b)
+                oracle.pyr_down(g)
+                oracle.pyr_up(g)
+                oracle.pyr_down(f)
+                oracle.box_filter(b, (3, 3))
+                oracle.box_filter(b, (15, 15))
+                oracle.adaptive_threshold(g, 255, 0, 0, 11, 2)
+                oracle.adaptive_threshold(g, 255, 1, 0, 11, 2)
+            return one
+
+        return cpu_measure(args.cpu_seconds, r8_worker, W * H / 1e9, "Gpix/s",
+                           "whole steps of the 8 pyramid / box / adaptive calls on 1920x1080")
     if args.config == 7:
         W, H = 1280, 720
 

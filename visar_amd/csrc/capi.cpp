@@ -625,15 +625,17 @@ int visar_adaptive_threshold(const visar_image *src, const visar_image *dst, dou
     const double dl = type == 0 ? std::ceil(delta) : std::floor(delta);
     const int idelta = (int)std::max(-1024.0, std::min(1024.0, dl));
     const int imax = (int)sat_cast<uint8_t>(maxval);
+    // The mean goes into the output itself and the compare then runs in
+    // place over it (each element reads its own mean first), so the call
+    // needs no temporary and can be captured in a graph.
     Call call;
-    DImg s, d, m;
+    DImg s, d;
     if ((rc = call.begin())) return rc;
     if ((rc = call.input(src, s)) || (rc = call.output(dst, d))) return rc;
-    if ((rc = call.temp_like(d, 1, m))) return rc;
     p.src = s;
-    p.dst = m;
+    p.dst = d;
     if ((rc = launch_sep(p, call.stream()))) return call.finish(rc);
-    return call.finish(launch_adaptive_u8(s, m, d, idelta, type == 0 ? imax : 0, type == 0 ? 0 : imax, call.stream()));
+    return call.finish(launch_adaptive_u8(s, d, d, idelta, type == 0 ? imax : 0, type == 0 ? 0 : imax, call.stream()));
 }
 
 // capi.zig:218-232 + filter.zig:856-906

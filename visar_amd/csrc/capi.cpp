@@ -566,6 +566,8 @@ int visar_box_filter(const visar_image *src, const visar_image *dst, int kw, int
     Call call;
     if ((rc = call.begin())) return rc;
     if ((rc = call.input(src, p.src)) || (rc = call.output(dst, p.dst))) return rc;
+    if (T == 0 && src->channels <= 4)
+        return call.finish(launch_box_u8(p.src, p.dst, kw, kh, border, p.scale, call.stream()));
     return call.finish(launch_sep(p, call.stream()));
 }
 
@@ -634,7 +636,9 @@ int visar_adaptive_threshold(const visar_image *src, const visar_image *dst, dou
     if ((rc = call.input(src, s)) || (rc = call.output(dst, d))) return rc;
     p.src = s;
     p.dst = d;
-    if ((rc = launch_sep(p, call.stream()))) return call.finish(rc);
+    if ((rc = method == 0 ? launch_box_u8(s, d, block, block, BORDER_REPLICATE, p.scale, call.stream())
+                          : launch_sep(p, call.stream())))
+        return call.finish(rc);
     return call.finish(launch_adaptive_u8(s, d, d, idelta, type == 0 ? imax : 0, type == 0 ? 0 : imax, call.stream()));
 }
 

@@ -1,0 +1,111 @@
+// k_box.hip -- 8-bit box filter for gfx950 (imgproc/filter.zig:617-806
+// boxFilter / blur): exact integer window sums, then f32(sum) * f32(scale)
+// rounded and saturated (FinishScaled), any border, 1-4 channels, odd
+// windows up to 127 x 127.
+//
+// A workgroup owns a tile of TH rows and TW pixels, TW chosen so that the
+// tile's padded columns ((TW + kw - 1) * cn samples) fill the 256 threads.
+// Vertical pass: one thread per padded sample column loads the first window
+// of kh rows and then the rows that enter and leave the window, all before
+// using them, and writes the column sums of the tile's rows to LDS (running
+// sums: exact in u32), one plane per channel. Horizontal pass: row by row,
+// one lane per output element sums its kw column sums from LDS and stores
+// its byte; consecutive lanes write consecutive bytes, and the planar layout
+// keeps neighbouring lanes' LDS reads in different banks.
+#include "kernels.hpp"
+#include "tiles.hpp"
+
+namespace visar {
+
+namespace {
+
+constexpr int kBoxThreads = 256;
+constexpr int kBoxTH = 16;  // output rows per tile
+
+template <int CN>
+__global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, int kh, int border, float scale, int tw,
+                                                      TileOrder o) {
+    extern __shared__ uint32_t colsum[];  // [kBoxTH][CN][pitch]
+    __shared__ long row_off[kBoxTH + 126];  // source rows y0 - ry .. y0 + kBoxTH - 1 + ry, -1 outside
+    int bx, by;
+    if (!tile_of(o, bx, by)) return;
+    const int x0 = bx * tw, y0 = by * kBoxTH;
+    const int rx = kw / 2, ry = kh / 2;
+    const int pitch = tw + kw - 1, ncols = pitch * CN, plane = ncols;
+    const int W = s.w, H = s.h;
+    // Rows are mapped once per tile (no per-load border logic, whose call
+    // path would make the compiler wait for each load before the next).
+    for (int i = threadIdx.x; i < kBoxTH + kh - 1; i += kBoxThreads) {
+        const int my = border_map_halo(y0 - ry + i, H, border);
+        row_off[i] = my < 0 ? -1l : (long)my * s.stride;
+    }
+    __syncthreads();
+    for (int col = threadIdx.x; col < ncols; col += kBoxThreads) {
+        const int mx = border_map_halo(x0 - rx + col / CN, W, border);
+        const long off = (long)mx * CN + col % CN;
+        const gptr<const uint8_t> base = gp((const uint8_t *)s.p);
+        auto at = [&](int i) -> uint32_t {  // window row i (source row y0 - ry + i)
+            const long r = row_off[i];
+            return (mx < 0 || r < 0) ? 0u : (uint32_t)base[r + off];
+        };
+        uint32_t v = 0;
+        for (int j0 = 0; j0 < kh; j0 += 8) {
+            uint32_t b[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) b[j] = j0 + j < kh ? at(j0 + j) : 0u;
+#pragma unroll
+            for (int j = 0; j < 8; j++) v += b[j];
+        }
+        uint32_t add[kBoxTH - 1], sub[kBoxTH - 1];
+#pragma unroll
+        for (int t = 0; t < kBoxTH - 1; t++) {
+            add[t] = at(t + kh);
+            sub[t] = at(t);
+        }
+        uint32_t *cs = colsum + (col % CN) * pitch + col / CN;
+        cs[0] = v;
+#pragma unroll
+        for (int t = 0; t < kBoxTH - 1; t++) {
+            v += add[t] - sub[t];
+            cs[(t + 1) * plane] = v;
+        }
+    }
+    __syncthreads();
+    // One lane per output element of a row: its kw column sums from LDS, the
+    // byte stored next to its neighbours' (consecutive lanes, consecutive
+    // bytes).
+    const int n_row = min(tw, W - x0) * CN, rows = min(kBoxTH, H - y0);
+    for (int t = 0; t < rows; t++) {
+        const uint32_t *rowt = colsum + t * plane;
+        const gptr<uint8_t> out = grow<uint8_t>(d, y0 + t) + (long)x0 * CN;
+        for (int e = threadIdx.x; e < n_row; e += kBoxThreads) {
+            const int ox = e / CN, c = e - ox * CN;
+            const uint32_t *r = rowt + c * pitch + ox;
+            uint32_t sum = 0;
+            for (int i = 0; i < kw; i++) sum += r[i];
+            out[e] = sat_v<uint8_t>((float)sum * scale);
+        }
+    }
+}
+
+inline int launched() { return hipGetLastError() == hipSuccess ? OK : INTERNAL; }
+
+}  // namespace
+
+int launch_box_u8(const DImg &s, const DImg &d, int kw, int kh, int border, float scale, hipStream_t st) {
+    const int cn = s.cn;
+    if (cn < 1 || cn > 4 || kw > 127 || kh > 127) return UNSUPPORTED;
+    const int tw = std::max(16, kBoxThreads / cn - (kw - 1));
+    const size_t lds = (size_t)kBoxTH * (tw + kw - 1) * cn * sizeof(uint32_t);
+    const TileOrder o = tile_order((s.w + tw - 1) / tw, (s.h + kBoxTH - 1) / kBoxTH);
+    const dim3 g(tile_grid(o.ntiles)), b(kBoxThreads);
+    switch (cn) {
+    case 1: hipLaunchKernelGGL(box_u8<1>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o); break;
+    case 2: hipLaunchKernelGGL(box_u8<2>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o); break;
+    case 3: hipLaunchKernelGGL(box_u8<3>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o); break;
+    default: hipLaunchKernelGGL(box_u8<4>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o); break;
+    }
+    return launched();
+}
+
+}  // namespace visar

@@ -6,8 +6,9 @@
 //                upsampled with zeros and reflected about its own edges,
 //                (v + 32) >> 6 (v / 64 for f32)
 //
-// One thread per output pixel (all channels), 64 x 4 pixels per tile. The
-// integer sums are exact in 32 bits (u8 at most 65280, u16 2^24, i16
+// 1-4 channels: LDS tiles of column sums (pyr_down_tile, pyr_up_tile); more
+// channels: one thread per output pixel (all channels), 64 x 4 pixels per
+// tile, taps read through the cache. The integer sums are exact in 32 bits (u8 at most 65280, u16 2^24, i16
 // +-2^23), so the order of the taps does not matter for them; f32 sums run
 // in the reference's order: each column's vertical sum (a + f) + (b + e) * 4
 // + c * 6, then the same form across the five column sums.
@@ -91,10 +92,121 @@ __global__ __launch_bounds__(kPyrThreads) void pyr_up_k(DImg s, DImg d, TileOrde
     }
 }
 
+// Tiled forms for 1-4 channels. A tile has kPyrTH rows of column sums: one
+// thread per padded sample column loads that column's source rows once (all
+// loads issued before use) and writes the vertical sums of the tile's rows
+// to LDS; then each output takes its horizontal taps from LDS.
+constexpr int kPyrTH = 8;
+
+// pyrDown: tile of kPyrTH output rows and tw output pixels; padded source
+// columns 2 x0 - 2 .. 2 (x0 + tw) (2 tw + 3 pixels), source rows
+// 2 y0 - 2 .. 2 (y0 + kPyrTH) (2 kPyrTH + 3).
+template <typename T, int CN>
+__global__ __launch_bounds__(kPyrThreads) void pyr_down_tile(DImg s, DImg d, int border, int tw, TileOrder o) {
+    using A = typename PyrAcc<T>::type;
+    extern __shared__ uint32_t lds_words[];
+    A *V = reinterpret_cast<A *>(lds_words);  // [kPyrTH][ncols]
+    __shared__ long row_off[2 * kPyrTH + 3];  // source rows 2 y0 - 2 .. 2 y0 + 2 kPyrTH
+    int bx, by;
+    if (!tile_of(o, bx, by)) return;
+    const int x0 = bx * tw, y0 = by * kPyrTH;
+    const int ncols = (2 * tw + 3) * CN;
+    // Rows mapped once per tile: no border logic between the loads below.
+    if (threadIdx.x < 2 * kPyrTH + 3) row_off[threadIdx.x] = (long)border_map_halo(2 * y0 - 2 + (int)threadIdx.x, s.h, border) * s.stride;
+    __syncthreads();
+    for (int col = threadIdx.x; col < ncols; col += kPyrThreads) {
+        const long off = ((long)border_map_halo(2 * x0 - 2 + col / CN, s.w, border) * CN + col % CN) * (long)sizeof(T);
+        const gptr<const uint8_t> base = gp((const uint8_t *)s.p) + off;
+        A r[2 * kPyrTH + 3];
+#pragma unroll
+        for (int k = 0; k < 2 * kPyrTH + 3; k++) r[k] = (A)*(gptr<const T>)(base + row_off[k]);
+#pragma unroll
+        for (int t = 0; t < kPyrTH; t++)
+            V[t * ncols + col] = tap5<A>(r[2 * t], r[2 * t + 1], r[2 * t + 2], r[2 * t + 3], r[2 * t + 4]);
+    }
+    __syncthreads();
+    const int n_out = min(tw, d.w - x0) * CN, rows = min(kPyrTH, d.h - y0);
+    for (int q = threadIdx.x; q < rows * n_out; q += kPyrThreads) {
+        const int t = q / n_out, e = q - t * n_out, ox = e / CN, c = e - ox * CN;
+        const A *row = V + t * ncols + 2 * ox * CN + c;
+        ((gptr<T>)grow<uint8_t>(d, y0 + t))[(long)x0 * CN + e] =
+            pyr_finish<T, 8>(tap5<A>(row[0], row[CN], row[2 * CN], row[3 * CN], row[4 * CN]));
+    }
+}
+
+// pyrUp's edge rule (pyramid.zig:440-450): -1 -> 1 (0 for n = 1), n -> n - 1.
+__device__ __attribute__((always_inline)) inline int up_map(int p, int n) {
+    return p < 0 ? (n > 1 ? 1 : 0) : p >= n ? n - 1 : p;
+}
+
+// pyrUp: tile of kPyrTH source rows (2 kPyrTH output rows) and tw source
+// pixels (2 tw output pixels); padded source columns x0 - 1 .. x0 + tw.
+template <typename T, int CN>
+__global__ __launch_bounds__(kPyrThreads) void pyr_up_tile(DImg s, DImg d, int tw, TileOrder o) {
+    using A = typename PyrAcc<T>::type;
+    extern __shared__ uint32_t lds_words[];
+    A *E = reinterpret_cast<A *>(lds_words);  // [kPyrTH][ncols] even output rows
+    int bx, by;
+    if (!tile_of(o, bx, by)) return;
+    const int x0 = bx * tw, y0 = by * kPyrTH;
+    const int ncols = (tw + 2) * CN;
+    A *O = E + kPyrTH * ncols;  // odd output rows
+    for (int col = threadIdx.x; col < ncols; col += kPyrThreads) {
+        const long off = (long)up_map(x0 - 1 + col / CN, s.w) * CN + col % CN;
+        A r[kPyrTH + 2];
+#pragma unroll
+        for (int k = 0; k < kPyrTH + 2; k++)
+            r[k] = (A)((gptr<const T>)grow<const uint8_t>(s, up_map(y0 - 1 + k, s.h)))[off];
+#pragma unroll
+        for (int t = 0; t < kPyrTH; t++) {
+            E[t * ncols + col] = (r[t] + r[t + 2]) + r[t + 1] * (A)6;
+            O[t * ncols + col] = (r[t + 1] + r[t + 2]) * (A)4;
+        }
+    }
+    __syncthreads();
+    // Output rows are 2 tw * CN (about 250) elements: row by row, one lane
+    // per element, no division.
+    const int n_out = min(2 * tw, d.w - 2 * x0) * CN, rows = min(2 * kPyrTH, d.h - 2 * y0);
+    for (int t2 = 0; t2 < rows; t2++) {
+        const A *rowt = ((t2 & 1) ? O : E) + (t2 >> 1) * ncols;
+        const gptr<T> out = (gptr<T>)grow<uint8_t>(d, 2 * y0 + t2) + (long)2 * x0 * CN;
+        for (int e = threadIdx.x; e < n_out; e += kPyrThreads) {
+            const int X = e / CN, c = e - X * CN, p = X >> 1;  // padded column p + 1 is source pixel x0 + p
+            const A *row = rowt + c;
+            const A v = (X & 1) ? (row[(p + 1) * CN] + row[(p + 2) * CN]) * (A)4
+                                : (row[p * CN] + row[(p + 2) * CN]) + row[(p + 1) * CN] * (A)6;
+            out[e] = pyr_finish<T, 6>(v);
+        }
+    }
+}
+
 inline int launched() { return hipGetLastError() == hipSuccess ? OK : INTERNAL; }
+
+template <typename T, int CN>
+void pyr_tiled(const DImg &s, const DImg &d, int border, bool down, hipStream_t st) {
+    const dim3 b(kPyrThreads);
+    if (down) {
+        const int tw = std::max(8, (kPyrThreads / CN - 3) / 2);
+        const TileOrder o = tile_order((d.w + tw - 1) / tw, (d.h + kPyrTH - 1) / kPyrTH);
+        const size_t lds = (size_t)kPyrTH * (2 * tw + 3) * CN * 4;
+        hipLaunchKernelGGL((pyr_down_tile<T, CN>), dim3(tile_grid(o.ntiles)), b, lds, st, s, d, border, tw, o);
+    } else {
+        const int tw = std::max(8, kPyrThreads / 2 / CN - 2);
+        const TileOrder o = tile_order((s.w + tw - 1) / tw, (s.h + kPyrTH - 1) / kPyrTH);
+        const size_t lds = (size_t)2 * kPyrTH * (tw + 2) * CN * 4;
+        hipLaunchKernelGGL((pyr_up_tile<T, CN>), dim3(tile_grid(o.ntiles)), b, lds, st, s, d, tw, o);
+    }
+}
 
 template <typename T>
 int pyr_launch(const DImg &s, const DImg &d, int border, bool down, hipStream_t st) {
+    switch (s.cn) {
+    case 1: pyr_tiled<T, 1>(s, d, border, down, st); return launched();
+    case 2: pyr_tiled<T, 2>(s, d, border, down, st); return launched();
+    case 3: pyr_tiled<T, 3>(s, d, border, down, st); return launched();
+    case 4: pyr_tiled<T, 4>(s, d, border, down, st); return launched();
+    default: break;
+    }
     const TileOrder o = tile_order((d.w + 63) / 64, (d.h + 3) / 4);
     const dim3 g(tile_grid(o.ntiles)), b(kPyrThreads);
     if (down) hipLaunchKernelGGL(pyr_down_k<T>, g, b, 0, st, s, d, border, o);

@@ -82,6 +82,10 @@ int launch_gray_down(const DImg &s, const DImg &d, int k, hipStream_t st);
 // (reflect-101): u8, u16, i16 and f32 (depth codes 0, 2, 3, 5).
 int launch_pyramid(const DImg &s, const DImg &d, int depth, int border, bool down, hipStream_t st);
 
+// filter.zig:617-806 boxFilter on u8, 1-4 channels, odd windows up to 127:
+// exact sums, sat(f32(sum) * scale); UNSUPPORTED outside that.
+int launch_box_u8(const DImg &s, const DImg &d, int kw, int kh, int border, float scale, hipStream_t st);
+
 // threshold.zig:340-360: dst = s + idelta > m ? on : off, u8, one channel.
 int launch_adaptive_u8(const DImg &s, const DImg &m, const DImg &d, int idelta, int on, int off, hipStream_t st);
 int launch_area_int_u8(const DImg &s, const DImg &d, int ix, int iy, hipStream_t st);

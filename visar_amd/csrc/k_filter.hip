@@ -83,20 +83,63 @@ __device__ inline float taps_seq(const float *k, int n, int from_zero, F v) {
 }
 
 // Vertical value of column `col` (flat sample index) for output row y, with
-// the tile's mapped source rows in `rmap` (-1 = constant border row).
+// the tile's mapped source rows in `rmap` (-1 = constant border row). The
+// loads are issued in chunks (4 symmetric pairs or 8 taps) before they are
+// accumulated, so a column's taps are in flight together; the accumulation
+// order is that of taps_fold / taps_seq above.
 template <typename S, typename VT, bool F32>
 __device__ inline VT vert(const SepParams &p, const int *rmap, long col) {
+    auto ld = [&](int j) -> VT {
+        const int sy = rmap[j];
+        if constexpr (F32) return sy < 0 ? 0.0f : to_f(row_ptr<const S>(p.src, sy)[col]);
+        else return sy < 0 ? (VT)0 : (VT)row_ptr<const S>(p.src, sy)[col];
+    };
+    const int n = p.nky;
     if constexpr (F32) {
-        auto v = [&](int j) -> float {
-            int sy = rmap[j];
-            return sy < 0 ? 0.0f : to_f(row_ptr<const S>(p.src, sy)[col]);
-        };
-        return col < p.body ? taps_fold(p.kyf, p.nky, p.vsym, v) : taps_seq(p.kyf, p.nky, p.const_kernel, v);
+        const float *k = p.kyf;
+        if (col < p.body && p.vsym != SYM_NONE) {
+            const int r = n / 2;
+            const bool even = p.vsym == SYM_EVEN;
+            float a = even ? ld(r) * k[r] : 0.0f;
+            for (int i0 = 0; i0 < r; i0 += 4) {
+                float t[4], b[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    t[u] = i0 + u < r ? ld(i0 + u) : 0.0f;
+                    b[u] = i0 + u < r ? ld(n - 1 - i0 - u) : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (i0 + u < r) a = a + (even ? t[u] + b[u] : b[u] - t[u]) * (even ? k[i0 + u] : k[n - 1 - i0 - u]);
+            }
+            return a;
+        }
+        // Taps in order: the unfolded body (0 + v k), the compile-time
+        // kernel's tail (0 + k v) or the scalar tail (k0 v0 + k v).
+        int j0 = 0;
+        float a = 0.0f;
+        if (col >= p.body && !p.const_kernel) {
+            a = k[0] * ld(0);
+            j0 = 1;
+        }
+        for (int c0 = j0; c0 < n; c0 += 8) {
+            float v8[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v8[u] = c0 + u < n ? ld(c0 + u) : 0.0f;
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (c0 + u < n) a = a + k[c0 + u] * v8[u];
+        }
+        return a;
     } else {
         VT a = 0;
-        for (int j = 0; j < p.nky; j++) {
-            int sy = rmap[j];
-            if (sy >= 0) a += (VT)p.kyi[j] * (VT)row_ptr<const S>(p.src, sy)[col];
+        for (int c0 = 0; c0 < n; c0 += 8) {
+            VT v8[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v8[u] = c0 + u < n ? ld(c0 + u) : (VT)0;
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (c0 + u < n) a += (VT)p.kyi[c0 + u] * v8[u];
         }
         return a;
     }
@@ -119,7 +162,7 @@ __global__ __launch_bounds__(kThreads) void sep_tile(SepParams p, int twpx, int 
     for (int idx = threadIdx.x; idx < rows * ldw; idx += kThreads) {
         int r = idx / ldw, j = idx - r * ldw;
         int q = j / cn, ch = j - q * cn;
-        long mx = border_map(x0 - rx + q, W, p.border);
+        long mx = border_map_halo((int)(x0 - rx + q), W, p.border);
         VT v = 0;
         if (mx >= 0) v = vert<S, VT, F32>(p, rmap + r, mx * cn + ch);
         V[r * ldw + j] = v;
@@ -159,7 +202,7 @@ __global__ __launch_bounds__(kThreads) void sep_direct(SepParams p) {
     for (int j = 0; j < p.nky; j++) rmap[j] = (int)border_map(y + j - ry, H, p.border);
     long x = gid / cn, ch = gid - x * cn;
     auto vcol = [&](int i) -> VT {
-        long mx = border_map(x + i - rx, W, p.border);
+        long mx = border_map_halo((int)(x + i - rx), W, p.border);
         return mx < 0 ? (VT)0 : vert<S, VT, F32>(p, rmap, mx * cn + ch);
     };
     D o;
@@ -181,7 +224,7 @@ int run_sep(const SepParams &p, hipStream_t st) {
     const int rx = p.nkx / 2, ry = p.nky / 2;
     const size_t lds_cap = 60 * 1024;
     int twpx = cn == 1 ? 256 : cn == 2 ? 128 : cn <= 4 ? 64 : 16;
-    int th = 16;
+    int th = 4;  // short tiles: more workgroups to overlap the column loads' latency (16: 53 us, 4: 35 us, 1: 42 us for an 11x11 f32 blur of 1080p)
     auto need = [&](int tw, int t) {
         return (((size_t)(t + 2 * ry) * sizeof(int) + 15) & ~(size_t)15) + (size_t)t * (tw + 2 * rx) * cn * sizeof(VT);
     };

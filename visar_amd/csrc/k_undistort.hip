@@ -65,10 +65,15 @@ __device__ __attribute__((always_inline)) inline int32_t fix32(double v) {
 // undistort.zig:120-157 sample: the inside, outside and per-tap border
 // branches are one formula, sum over taps of (tap inside ? pixel : 0) * w,
 // so every lane takes the same path: tap addresses are clamped into the
-// image (always in bounds) and the taps outside contribute 0. ALIGNED (rows
-// 4-byte aligned): each wave stages its row of the tile in LDS and stores it
-// as 4 * CN contiguous bytes per lane (dword stores) instead of byte stores.
-template <int CN, bool ALIGNED>
+// image (always in bounds) and the taps outside contribute 0. ALIGNED
+// (destination rows 4-byte aligned): each wave stages its row of the tile in
+// LDS and stores it as 4 * CN contiguous bytes per lane (dword stores)
+// instead of byte stores. WIN (2-4 channels, source rows 4-byte aligned and
+// at least 12 bytes): a tap row's two pixels (2 CN bytes) come from two or
+// three aligned dword loads, unpacked with alignbyte, instead of 2 CN byte
+// loads; pixels whose taps reach past the left or right edge, or whose
+// window would cross the row's end, are loaded byte by byte afterwards.
+template <int CN, bool ALIGNED, bool WIN>
 __global__ __launch_bounds__(kUdThreads) void undistort_u8(UndistortArgs p, TileOrder o) {
     __shared__ uint32_t stage[kUdTileH][kUdTileW * CN / 4];
     int bx, by;
@@ -96,6 +101,9 @@ __global__ __launch_bounds__(kUdThreads) void undistort_u8(UndistortArgs p, Tile
     }
     uint8_t t[kUdPx][4][CN];
     int32_t wq[kUdPx][4];
+    long cxs[kUdPx][2];
+    gptr<const uint8_t> rs[kUdPx][2];
+    bool slow[kUdPx];
 #pragma unroll
     for (int q = 0; q < kUdPx; q++) {
         const int32_t sx = iu[q] >> 5, sy = iv[q] >> 5, a = iu[q] & 31, b = iv[q] & 31;
@@ -105,15 +113,48 @@ __global__ __launch_bounds__(kUdThreads) void undistort_u8(UndistortArgs p, Tile
         wq[q][1] = inx1 && iny0 ? a * (32 - b) : 0;
         wq[q][2] = inx0 && iny1 ? (32 - a) * b : 0;
         wq[q][3] = inx1 && iny1 ? a * b : 0;
-        const long cx0 = min(max(sx, 0), W - 1), cx1 = min(max(sx + 1, 0), W - 1);
+        cxs[q][0] = min(max(sx, 0), W - 1);
+        cxs[q][1] = min(max(sx + 1, 0), W - 1);
         const long cy0 = min(max(sy, 0), H - 1), cy1 = min(max(sy + 1, 0), H - 1);
-        const gptr<const uint8_t> r0 = grow<const uint8_t>(p.s, cy0), r1 = grow<const uint8_t>(p.s, cy1);
+        rs[q][0] = grow<const uint8_t>(p.s, cy0);
+        rs[q][1] = grow<const uint8_t>(p.s, cy1);
+        slow[q] = true;
+        if constexpr (WIN) {
+            // Window: dwords from the aligned-down start of tap 0's bytes.
+            constexpr int ND = CN <= 2 ? 2 : 3;
+            const long b0 = cxs[q][0] * CN, a0 = b0 & ~3l;
+            const long lim = (long)W * CN - 4 * ND;
+            slow[q] = !(inx0 && inx1) || a0 > lim;
+            const long aw = min(a0, lim & ~3l);  // in bounds for every lane
+            const int sh = (int)(b0 - a0);
 #pragma unroll
-        for (int c = 0; c < CN; c++) {
-            t[q][0][c] = r0[cx0 * CN + c];
-            t[q][1][c] = r0[cx1 * CN + c];
-            t[q][2][c] = r1[cx0 * CN + c];
-            t[q][3][c] = r1[cx1 * CN + c];
+            for (int r = 0; r < 2; r++) {
+                const gptr<const uint32_t> wp = (gptr<const uint32_t>)(rs[q][r] + aw);
+                const uint32_t w0 = wp[0], w1 = wp[1], w2 = ND == 3 ? wp[2] : 0u;
+                const uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, sh), hi = __builtin_amdgcn_alignbyte(w2, w1, sh);
+                const uint64_t v = (uint64_t)hi << 32 | lo;
+#pragma unroll
+                for (int c = 0; c < CN; c++) {
+                    t[q][2 * r][c] = (uint8_t)(v >> (8 * c));
+                    t[q][2 * r + 1][c] = (uint8_t)(v >> (8 * (CN + c)));
+                }
+            }
+        }
+    }
+    bool any_slow = false;
+#pragma unroll
+    for (int q = 0; q < kUdPx; q++) any_slow |= slow[q];
+    if (!WIN || __any(any_slow)) {
+#pragma unroll
+        for (int q = 0; q < kUdPx; q++) {
+            if (!slow[q]) continue;
+#pragma unroll
+            for (int c = 0; c < CN; c++) {
+                t[q][0][c] = rs[q][0][cxs[q][0] * CN + c];
+                t[q][1][c] = rs[q][0][cxs[q][1] * CN + c];
+                t[q][2][c] = rs[q][1][cxs[q][0] * CN + c];
+                t[q][3][c] = rs[q][1][cxs[q][1] * CN + c];
+            }
         }
     }
     uint8_t res[kUdPx][CN];
@@ -180,15 +221,22 @@ int launch_undistort(const UndistortArgs &p, hipStream_t st) {
     const TileOrder o = tile_order((p.d.w + kUdTileW - 1) / kUdTileW, (p.d.h + kUdTileH - 1) / kUdTileH, true);
     const dim3 g(tile_grid(o.ntiles)), b(kUdThreads);
     const bool al = (uintptr_t)p.d.p % 4 == 0 && p.d.stride % 4 == 0;
-    switch (p.s.cn * 2 + al) {
-    case 2: hipLaunchKernelGGL((undistort_u8<1, false>), g, b, 0, st, p, o); break;
-    case 3: hipLaunchKernelGGL((undistort_u8<1, true>), g, b, 0, st, p, o); break;
-    case 4: hipLaunchKernelGGL((undistort_u8<2, false>), g, b, 0, st, p, o); break;
-    case 5: hipLaunchKernelGGL((undistort_u8<2, true>), g, b, 0, st, p, o); break;
-    case 6: hipLaunchKernelGGL((undistort_u8<3, false>), g, b, 0, st, p, o); break;
-    case 7: hipLaunchKernelGGL((undistort_u8<3, true>), g, b, 0, st, p, o); break;
-    case 8: hipLaunchKernelGGL((undistort_u8<4, false>), g, b, 0, st, p, o); break;
-    case 9: hipLaunchKernelGGL((undistort_u8<4, true>), g, b, 0, st, p, o); break;
+    const bool win = (uintptr_t)p.s.p % 4 == 0 && p.s.stride % 4 == 0 && (long)p.s.w * p.s.cn >= 12;
+    switch (p.s.cn * 4 + al * 2 + win) {
+    case 4: case 5: hipLaunchKernelGGL((undistort_u8<1, false, false>), g, b, 0, st, p, o); break;
+    case 6: case 7: hipLaunchKernelGGL((undistort_u8<1, true, false>), g, b, 0, st, p, o); break;
+    case 8: hipLaunchKernelGGL((undistort_u8<2, false, false>), g, b, 0, st, p, o); break;
+    case 9: hipLaunchKernelGGL((undistort_u8<2, false, true>), g, b, 0, st, p, o); break;
+    case 10: hipLaunchKernelGGL((undistort_u8<2, true, false>), g, b, 0, st, p, o); break;
+    case 11: hipLaunchKernelGGL((undistort_u8<2, true, true>), g, b, 0, st, p, o); break;
+    case 12: hipLaunchKernelGGL((undistort_u8<3, false, false>), g, b, 0, st, p, o); break;
+    case 13: hipLaunchKernelGGL((undistort_u8<3, false, true>), g, b, 0, st, p, o); break;
+    case 14: hipLaunchKernelGGL((undistort_u8<3, true, false>), g, b, 0, st, p, o); break;
+    case 15: hipLaunchKernelGGL((undistort_u8<3, true, true>), g, b, 0, st, p, o); break;
+    case 16: hipLaunchKernelGGL((undistort_u8<4, false, false>), g, b, 0, st, p, o); break;
+    case 17: hipLaunchKernelGGL((undistort_u8<4, false, true>), g, b, 0, st, p, o); break;
+    case 18: hipLaunchKernelGGL((undistort_u8<4, true, false>), g, b, 0, st, p, o); break;
+    case 19: hipLaunchKernelGGL((undistort_u8<4, true, true>), g, b, 0, st, p, o); break;
     default: return UNSUPPORTED;
     }
     return launched();

@@ -8,10 +8,10 @@
 // Vertical pass: one thread per padded sample column loads the first window
 // of kh rows and then the rows that enter and leave the window, all before
 // using them, and writes the column sums of the tile's rows to LDS (running
-// sums: exact in u32), one plane per channel. Horizontal pass: row by row,
-// one lane per output element sums its kw column sums from LDS and stores
-// its byte; consecutive lanes write consecutive bytes, and the planar layout
-// keeps neighbouring lanes' LDS reads in different banks.
+// sums: exact in u32), one plane per channel. Horizontal pass, windows up
+// to 11: row by row, one lane per output element sums its kw column sums
+// from LDS and stores its byte (consecutive lanes, consecutive bytes); wider
+// windows: one lane per run of 7 outputs, a running sum along the run.
 #include "kernels.hpp"
 #include "tiles.hpp"
 
@@ -21,6 +21,7 @@ namespace {
 
 constexpr int kBoxThreads = 256;
 constexpr int kBoxTH = 16;  // output rows per tile
+constexpr int kBoxRun = 7;  // outputs per lane for wide windows (odd: conflict-free LDS reads)
 
 template <int CN>
 __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, int kh, int border, float scale, int tw,
@@ -71,19 +72,40 @@ __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, in
         }
     }
     __syncthreads();
-    // One lane per output element of a row: its kw column sums from LDS, the
-    // byte stored next to its neighbours' (consecutive lanes, consecutive
-    // bytes).
     const int n_row = min(tw, W - x0) * CN, rows = min(kBoxTH, H - y0);
-    for (int t = 0; t < rows; t++) {
-        const uint32_t *rowt = colsum + t * plane;
-        const gptr<uint8_t> out = grow<uint8_t>(d, y0 + t) + (long)x0 * CN;
-        for (int e = threadIdx.x; e < n_row; e += kBoxThreads) {
-            const int ox = e / CN, c = e - ox * CN;
-            const uint32_t *r = rowt + c * pitch + ox;
-            uint32_t sum = 0;
-            for (int i = 0; i < kw; i++) sum += r[i];
-            out[e] = sat_v<uint8_t>((float)sum * scale);
+    if (kw <= 11) {
+        // One lane per output element of a row: its kw column sums from
+        // LDS, the byte stored next to its neighbours' (consecutive lanes,
+        // consecutive bytes).
+        for (int t = 0; t < rows; t++) {
+            const uint32_t *rowt = colsum + t * plane;
+            const gptr<uint8_t> out = grow<uint8_t>(d, y0 + t) + (long)x0 * CN;
+            for (int e = threadIdx.x; e < n_row; e += kBoxThreads) {
+                const int ox = e / CN, c = e - ox * CN;
+                const uint32_t *r = rowt + c * pitch + ox;
+                uint32_t sum = 0;
+                for (int i = 0; i < kw; i++) sum += r[i];
+                out[e] = sat_v<uint8_t>((float)sum * scale);
+            }
+        }
+        return;
+    }
+    // Wide windows: one lane per run of kBoxRun outputs of one channel, the
+    // first window summed, then one column in and one out per output. Runs
+    // of neighbouring lanes start kBoxRun (odd) words apart in a plane, so
+    // their LDS reads fall in different banks.
+    const int tw_row = min(tw, W - x0), runs_x = (tw_row + kBoxRun - 1) / kBoxRun, per_row = runs_x * CN;
+    for (int q = threadIdx.x; q < rows * per_row; q += kBoxThreads) {
+        const int t = q / per_row, r = q - t * per_row, c = r / runs_x, run = r - c * runs_x;
+        const uint32_t *row = colsum + t * plane + c * pitch;
+        const int ox0 = run * kBoxRun;
+        uint32_t sum = 0;
+        for (int i = 0; i < kw; i++) sum += row[ox0 + i];
+        const gptr<uint8_t> out = grow<uint8_t>(d, y0 + t) + (long)x0 * CN + c;
+        for (int k = 0; k < kBoxRun && ox0 + k < tw_row; k++) {
+            const int ox = ox0 + k;
+            if (k) sum += row[ox + kw - 1] - row[ox - 1];
+            out[(long)ox * CN] = sat_v<uint8_t>((float)sum * scale);
         }
     }
 }

@@ -145,6 +145,21 @@ __device__ inline VT vert(const SepParams &p, const int *rmap, long col) {
     }
 }
 
+// The same vertical value from the tile's staged source rows: tap j of the
+// column is stg[j * ldw] (0 already stored for constant-border samples).
+template <typename S, typename VT, bool F32>
+__device__ inline VT vert_staged(const SepParams &p, const S *stg, int ldw, long col) {
+    const int n = p.nky;
+    if constexpr (F32) {
+        auto v = [&](int j) -> float { return to_f(stg[j * ldw]); };
+        return col < p.body ? taps_fold(p.kyf, n, p.vsym, v) : taps_seq(p.kyf, n, p.const_kernel, v);
+    } else {
+        VT a = 0;
+        for (int j = 0; j < n; j++) a += (VT)p.kyi[j] * (VT)stg[j * ldw];
+        return a;
+    }
+}
+
 template <typename S, typename D, typename VT, typename AT, bool F32>
 __global__ __launch_bounds__(kThreads) void sep_tile(SepParams p, int twpx, int th) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -156,22 +171,43 @@ __global__ __launch_bounds__(kThreads) void sep_tile(SepParams p, int twpx, int 
     const int rows = (int)min((long)th, (long)H - y0);
     int *rmap = reinterpret_cast<int *>(smem);
     VT *V = reinterpret_cast<VT *>(smem + (((th + 2 * ry) * sizeof(int) + 15) & ~(size_t)15));
+    S *stg = reinterpret_cast<S *>(V + (size_t)th * ldw);  // [rows + 2 ry][ldw] source samples
     for (int i = threadIdx.x; i < rows + 2 * ry; i += kThreads) rmap[i] = (int)border_map(y0 + i - ry, H, p.border);
     __syncthreads();
-    // Pass 1: vertical taps into LDS (halo columns border-mapped).
-    for (int idx = threadIdx.x; idx < rows * ldw; idx += kThreads) {
-        int r = idx / ldw, j = idx - r * ldw;
-        int q = j / cn, ch = j - q * cn;
-        long mx = border_map_halo((int)(x0 - rx + q), W, p.border);
-        VT v = 0;
-        if (mx >= 0) v = vert<S, VT, F32>(p, rmap + r, mx * cn + ch);
-        V[r * ldw + j] = v;
+    // Pass 0: the tile's source rows (halo columns border-mapped, constant
+    // border 0) into LDS; one column per thread, its mapping computed once,
+    // four rows' loads in flight before they are stored.
+    const int srows = rows + 2 * ry;
+    for (int j = threadIdx.x; j < ldw; j += kThreads) {
+        const int q = j / cn, ch = j - q * cn;
+        const int mx = border_map_halo((int)(x0 - rx + q), W, p.border);
+        const long off = (long)mx * cn + ch;
+        for (int r0 = 0; r0 < srows; r0 += 4) {
+            S v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int sy = r0 + u < srows ? rmap[r0 + u] : -1;
+                v[u] = (mx >= 0 && sy >= 0) ? row_ptr<const S>(p.src, sy)[off] : (S)0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (r0 + u < srows) stg[(r0 + u) * ldw + j] = v[u];
+        }
+    }
+    __syncthreads();
+    // Pass 1: vertical taps from the staged rows into LDS, one column per
+    // thread.
+    for (int j = threadIdx.x; j < ldw; j += kThreads) {
+        const int q = j / cn, ch = j - q * cn;
+        const long mx = border_map_halo((int)(x0 - rx + q), W, p.border);
+        for (int r = 0; r < rows; r++)
+            V[r * ldw + j] = mx >= 0 ? vert_staged<S, VT, F32>(p, stg + r * ldw + j, ldw, mx * cn + ch) : (VT)0;
     }
     __syncthreads();
     // Pass 2: horizontal taps and finish.
     const int outw = (int)min((long)twpx, (long)W - x0) * cn;
-    for (int idx = threadIdx.x; idx < rows * outw; idx += kThreads) {
-        int r = idx / outw, e = idx - r * outw;
+    for (int r = 0; r < rows; r++)
+    for (int e = threadIdx.x; e < outw; e += kThreads) {
         const VT *vr = V + r * ldw + e;
         long ge = x0 * cn + e;
         D o;
@@ -224,9 +260,10 @@ int run_sep(const SepParams &p, hipStream_t st) {
     const int rx = p.nkx / 2, ry = p.nky / 2;
     const size_t lds_cap = 60 * 1024;
     int twpx = cn == 1 ? 256 : cn == 2 ? 128 : cn <= 4 ? 64 : 16;
-    int th = 4;  // short tiles: more workgroups to overlap the column loads' latency (16: 53 us, 4: 35 us, 1: 42 us for an 11x11 f32 blur of 1080p)
+    int th = 4;  // short tiles: more workgroups in flight (an 11x11 f32 blur of 1080p: 16 rows 40 us, 4 rows 29 us)
     auto need = [&](int tw, int t) {
-        return (((size_t)(t + 2 * ry) * sizeof(int) + 15) & ~(size_t)15) + (size_t)t * (tw + 2 * rx) * cn * sizeof(VT);
+        return (((size_t)(t + 2 * ry) * sizeof(int) + 15) & ~(size_t)15) + (size_t)t * (tw + 2 * rx) * cn * sizeof(VT) +
+               (size_t)(t + 2 * ry) * (tw + 2 * rx) * cn * sizeof(S);
     };
     while (need(twpx, th) > lds_cap && th > 1) th /= 2;
     while (need(twpx, th) > lds_cap && twpx > 1) twpx /= 2;

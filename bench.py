@@ -8,7 +8,10 @@ scale U(0.8, 1.2), shift U(+-10 %)), and warpPerspective with the corners
 jittered by +-5 %. One step = the whole chain on one frame, every output
 written. Inputs are resident in HBM and rotate through 24 distinct frames
 (597 MB, more than twice the 256 MB MALL). The chain is enqueued through the
-C ABI on torch's stream and replayed from HIP graphs of 24 steps each.
+C ABI and replayed from HIP graphs of 24 steps each; inside a graph, step j
+runs on stream j % L (L frames in flight, each step writing its own output
+set), so the short 1080p kernels of one frame overlap those of the next.
+Every step still does all of its work; `--lanes 1` gives one frame at a time.
 
 Prints one JSON line: value = input Gpix/s of the whole chain, a per-kernel
 table (HIP-event durations, algorithmic bytes, GB/s), the roofline of the
@@ -43,6 +46,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=24)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5, 6, 7, 8])
     ap.add_argument("--streams", type=int, default=64)
+    ap.add_argument("--lanes", type=int, default=0, choices=[0, 1, 2, 3, 4],
+                    help="frames in flight in the graph-replayed workloads (step j runs on stream j %% lanes); "
+                         "0: the measured best per workload (chain 3, pyramid/box/adaptive step 4, others 2)")
     ap.add_argument("--seconds", type=float, default=6.0, help="served workload: Poisson run length")
     ap.add_argument("--host-workers", type=int, default=8, help="served workload: connected-components threads")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -468,23 +474,26 @@ def main():
         return bench_serve(args, torch, va)
 
     R = 24
+    if args.lanes == 0:
+        args.lanes = {2: 3, 8: 4}.get(args.config, 2)
+    NB = max(2, args.lanes)  # output buffer sets: step i writes set i % NB (one per frame in flight)
     if args.config == 2:
         W, H = 3840, 2160
         frames = [torch.from_numpy(synth.uniform_bytes(i, (H, W, 3))).to(dev) for i in range(R)]
-        gray = [torch.empty((H, W), dtype=torch.uint8, device=dev) for _ in range(2)]
-        small = [torch.empty((H // 2, W // 2), dtype=torch.uint8, device=dev) for _ in range(2)]
-        sob = [torch.empty((H // 2, W // 2), dtype=torch.int16, device=dev) for _ in range(2)]
-        wa = [torch.empty((H // 2, W // 2), dtype=torch.uint8, device=dev) for _ in range(2)]
-        wp = [torch.empty((H // 2, W // 2), dtype=torch.uint8, device=dev) for _ in range(2)]
+        gray = [torch.empty((H, W), dtype=torch.uint8, device=dev) for _ in range(NB)]
+        small = [torch.empty((H // 2, W // 2), dtype=torch.uint8, device=dev) for _ in range(NB)]
+        sob = [torch.empty((H // 2, W // 2), dtype=torch.int16, device=dev) for _ in range(NB)]
+        wa = [torch.empty((H // 2, W // 2), dtype=torch.uint8, device=dev) for _ in range(NB)]
+        wp = [torch.empty((H // 2, W // 2), dtype=torch.uint8, device=dev) for _ in range(NB)]
         ma = synth.affine_matrix(0, W // 2, H // 2)
         mp = synth.perspective_matrix(0, W // 2, H // 2)
         sz = (W // 2, H // 2)
         ops = {
-            "cvt_color_bgr2gray": (lambda i: va.cvt_color(frames[i % R], 6, dst=gray[i % 2]), W * H * 3 + W * H),
-            "resize_linear_half": (lambda i: va.resize(gray[i % 2], sz, 1, dst=small[i % 2]), W * H + W * H // 4),
-            "sobel_dx_i16": (lambda i: va.sobel(small[i % 2], np.int16, 1, 0, 3, dst=sob[i % 2]), W * H // 4 * 3),
-            "warp_affine_linear": (lambda i: va.warp_affine(small[i % 2], ma, sz, 1, 0, 0, dst=wa[i % 2]), W * H // 2),
-            "warp_perspective_linear": (lambda i: va.warp_perspective(small[i % 2], mp, sz, 1, 0, 0, dst=wp[i % 2]),
+            "cvt_color_bgr2gray": (lambda i: va.cvt_color(frames[i % R], 6, dst=gray[i % NB]), W * H * 3 + W * H),
+            "resize_linear_half": (lambda i: va.resize(gray[i % NB], sz, 1, dst=small[i % NB]), W * H + W * H // 4),
+            "sobel_dx_i16": (lambda i: va.sobel(small[i % NB], np.int16, 1, 0, 3, dst=sob[i % NB]), W * H // 4 * 3),
+            "warp_affine_linear": (lambda i: va.warp_affine(small[i % NB], ma, sz, 1, 0, 0, dst=wa[i % NB]), W * H // 2),
+            "warp_perspective_linear": (lambda i: va.warp_perspective(small[i % NB], mp, sz, 1, 0, 0, dst=wp[i % NB]),
                                         W * H // 2),
         }
         pix_per_step = W * H
@@ -492,16 +501,16 @@ def main():
     elif args.config == 6:
         W, H, K6 = 3840, 2160, 6
         frames = [torch.from_numpy(synth.uniform_bytes(i, (H, W, 4))).to(dev) for i in range(R)]
-        outs = [torch.empty((H // K6, W // K6), dtype=torch.uint8, device=dev) for _ in range(2)]
-        ops = {"gray_downsample_k6": (lambda i: va.gray_downsample(frames[i % R], K6, dst=outs[i % 2]),
+        outs = [torch.empty((H // K6, W // K6), dtype=torch.uint8, device=dev) for _ in range(NB)]
+        ops = {"gray_downsample_k6": (lambda i: va.gray_downsample(frames[i % R], K6, dst=outs[i % NB]),
                                       W * H * 4 + (W // K6) * (H // K6))}
         pix_per_step = W * H
         workload = "grayDownsample 3840x2160 BGRA u8 by 6 to 640x360 (tracking image)"
     elif args.config == 7:
         W, H = 1280, 720
         frames = [torch.from_numpy(synth.uniform_bytes(i, (H, W, 3))).to(dev) for i in range(R)]
-        outs = [torch.empty((H, W, 3), dtype=torch.uint8, device=dev) for _ in range(2)]
-        ops = {"undistort_bgr": (lambda i: va.undistort(frames[i % R], UD_K, UD_DIST, dst=outs[i % 2]), 2 * W * H * 3)}
+        outs = [torch.empty((H, W, 3), dtype=torch.uint8, device=dev) for _ in range(NB)]
+        ops = {"undistort_bgr": (lambda i: va.undistort(frames[i % R], UD_K, UD_DIST, dst=outs[i % NB]), 2 * W * H * 3)}
         pix_per_step = W * H
         workload = "undistort 1280x720 BGR u8, K = [[900, 0, 640], [0, 900, 360], [0, 0, 1]], 5 coefficients"
     elif args.config == 8:
@@ -510,21 +519,20 @@ def main():
         gray = [torch.from_numpy(synth.uniform_bytes(100 + i, (H, W))).to(dev) for i in range(R)]
         fbgr = [(b.float() / 255.0).contiguous() for b in bgr[:4]]
         hd, wd = (H + 1) // 2, (W + 1) // 2
-        o_pd = torch.empty((hd, wd, 3), dtype=torch.uint8, device=dev)
-        o_pg = torch.empty((hd, wd), dtype=torch.uint8, device=dev)
-        o_pu = torch.empty((2 * H, 2 * W), dtype=torch.uint8, device=dev)
-        o_pf = torch.empty((hd, wd, 3), dtype=torch.float32, device=dev)
-        o_bx = torch.empty((H, W, 3), dtype=torch.uint8, device=dev)
-        o_at = torch.empty((H, W), dtype=torch.uint8, device=dev)
+        def two(shape, dtype=torch.uint8):
+            return [torch.empty(shape, dtype=dtype, device=dev) for _ in range(NB)]
+        o_pd, o_pg, o_pu = two((hd, wd, 3)), two((hd, wd)), two((2 * H, 2 * W))
+        o_pf, o_b3, o_b15 = two((hd, wd, 3), torch.float32), two((H, W, 3)), two((H, W, 3))
+        o_am, o_ag = two((H, W)), two((H, W))
         ops = {
-            "pyr_down_bgr": (lambda i: va.pyr_down(bgr[i % R], dst=o_pd), W * H * 3 + wd * hd * 3),
-            "pyr_down_gray": (lambda i: va.pyr_down(gray[i % R], dst=o_pg), W * H + wd * hd),
-            "pyr_up_gray": (lambda i: va.pyr_up(gray[i % R], dst=o_pu), W * H + 4 * W * H),
-            "pyr_down_f32": (lambda i: va.pyr_down(fbgr[i % 4], dst=o_pf), 4 * (W * H * 3 + wd * hd * 3)),
-            "blur_3x3_bgr": (lambda i: va.box_filter(bgr[i % R], (3, 3), dst=o_bx), 2 * W * H * 3),
-            "blur_15x15_bgr": (lambda i: va.box_filter(bgr[i % R], (15, 15), dst=o_bx), 2 * W * H * 3),
-            "adaptive_mean_11": (lambda i: va.adaptive_threshold(gray[i % R], 255, 0, 0, 11, 2, dst=o_at), 2 * W * H),
-            "adaptive_gauss_11": (lambda i: va.adaptive_threshold(gray[i % R], 255, 1, 0, 11, 2, dst=o_at), 2 * W * H),
+            "pyr_down_bgr": (lambda i: va.pyr_down(bgr[i % R], dst=o_pd[i % NB]), W * H * 3 + wd * hd * 3),
+            "pyr_down_gray": (lambda i: va.pyr_down(gray[i % R], dst=o_pg[i % NB]), W * H + wd * hd),
+            "pyr_up_gray": (lambda i: va.pyr_up(gray[i % R], dst=o_pu[i % NB]), W * H + 4 * W * H),
+            "pyr_down_f32": (lambda i: va.pyr_down(fbgr[i % 4], dst=o_pf[i % NB]), 4 * (W * H * 3 + wd * hd * 3)),
+            "blur_3x3_bgr": (lambda i: va.box_filter(bgr[i % R], (3, 3), dst=o_b3[i % NB]), 2 * W * H * 3),
+            "blur_15x15_bgr": (lambda i: va.box_filter(bgr[i % R], (15, 15), dst=o_b15[i % NB]), 2 * W * H * 3),
+            "adaptive_mean_11": (lambda i: va.adaptive_threshold(gray[i % R], 255, 0, 0, 11, 2, dst=o_am[i % NB]), 2 * W * H),
+            "adaptive_gauss_11": (lambda i: va.adaptive_threshold(gray[i % R], 255, 1, 0, 11, 2, dst=o_ag[i % NB]), 2 * W * H),
         }
         pix_per_step = W * H
         workload = ("1920x1080: pyrDown BGR, gray, f32 BGR; pyrUp gray; blur 3x3 and 15x15 BGR; "
@@ -532,8 +540,8 @@ def main():
     else:
         W, H = 1920, 1080
         frames = [torch.from_numpy(synth.uniform_bytes(i, (H, W))).to(dev) for i in range(R * 6)]
-        outs = [torch.empty((H, W), dtype=torch.uint8, device=dev) for _ in range(2)]
-        ops = {"gaussian_blur_5x5": (lambda i: va.gaussian_blur(frames[i % len(frames)], (5, 5), 0, dst=outs[i % 2]),
+        outs = [torch.empty((H, W), dtype=torch.uint8, device=dev) for _ in range(NB)]
+        ops = {"gaussian_blur_5x5": (lambda i: va.gaussian_blur(frames[i % len(frames)], (5, 5), 0, dst=outs[i % NB]),
                                      2 * W * H)}
         pix_per_step = W * H
         workload = "GaussianBlur 5x5 reflect-101 1920x1080 u8"
@@ -550,23 +558,44 @@ def main():
     # Graphs of G consecutive steps (input slots taken mod R, R a multiple of
     # G), so the GPU and not the host's launch rate sets the pace: K steps
     # replay K // G chunk graphs and then K % G single-step graphs.
+    # With L lanes, step j runs on lane j % L: lane 0 is the capture stream,
+    # the others are streams forked from it inside the graph and joined at
+    # its end, so L frames are in flight. Step j writes output set j % NB
+    # (NB = L, at least 2), so the lanes never share a buffer.
     G = 24
     n_chunks = R // G
     chunks, singles = [], []
     cap_stream = torch.cuda.Stream()
     cap_stream.wait_stream(stream)
+    side = [torch.cuda.Stream() for _ in range(args.lanes - 1)]
+
+    def capture(indices):
+        lanes = [cap_stream] + side if len(indices) > 1 else [cap_stream]
+        if len(lanes) > 1:
+            fork = torch.cuda.Event()
+            fork.record(cap_stream)
+            for st in lanes[1:]:
+                st.wait_event(fork)
+        for j in indices:
+            st = lanes[j % len(lanes)]
+            va.set_stream(st)
+            with torch.cuda.stream(st):
+                step(j)
+        va.set_stream(cap_stream)
+        for st in lanes[1:]:
+            cap_stream.wait_stream(st)
+
     with torch.cuda.stream(cap_stream):
         va.set_stream(cap_stream)
         for c in range(n_chunks):
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g, stream=cap_stream):
-                for j in range(G):
-                    step(c * G + j)
+                capture(list(range(c * G, c * G + G)))
             chunks.append(g)
         for i in range(G):
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g, stream=cap_stream):
-                step(i)
+                capture([i])
             singles.append(g)
     va.set_stream(stream)
     stream.wait_stream(cap_stream)
@@ -620,7 +649,7 @@ def main():
         "dtype": "u8",
         "data": "synthetic (splitmix64 uniform bytes, 24 rotating device frames)",
         "config": {"workload": workload, "width": W, "height": H, "rotating_inputs": R, "graph_replay": True,
-                   "steps_per_graph": G},
+                   "steps_per_graph": G, "frames_in_flight": args.lanes},
         "kernels": kernels,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

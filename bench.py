@@ -442,8 +442,8 @@ def serve_kernel_roofline(torch, va, W, H, N, fused_bytes):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); images and streams shard "
-              "across GPUs without exchange, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): images and camera streams shard "
+              "across GPUs without exchange, so one GPU's throughput is the measured unit", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -39,7 +39,7 @@ class VisarError(RuntimeError):
 
 
 if not os.path.exists(LIB_PATH):
-    raise ImportError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    raise ImportError(f"{LIB_PATH} is not built: run `make -C visar_amd/csrc`")
 
 lib = ctypes.CDLL(LIB_PATH)
 lib.visar_version.restype = ctypes.c_char_p

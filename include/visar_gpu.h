@@ -40,7 +40,8 @@ int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D
  * stencils (1, 2, 4); "batch_stencil_rows" the same inside batches (1, 2);
  * "batch_unit_kb" KiB of input + output per batch workgroup (1-1024);
  * "tile_order" the order of a plain call's tiles over the XCDs (0 each
- * kernel's default, 1 row-major, 2 one band of neighbouring tiles per XCD).
+ * kernel's default, 1 row-major, 2 one band of neighbouring tiles per XCD);
+ * "sep_rows" output rows per tile of the separable filter engine (1-16).
  * Unknown keys or values out of range return VISAR_INVALID_ARGUMENT. */
 int visar_gpu_set_tuning(const char *key, int value);
 int visar_gpu_get_tuning(const char *key, int *value);

@@ -206,6 +206,28 @@ def test_sep_filter_2d(va, orc):
                     check(va, orc, "sep_filter_2d", a, ddt, kxx, kyy, border=border)
 
 
+def test_sep_filter_window_lengths(va, orc):
+    """Every window length the engine unrolls (odd 3..15) and the runtime
+    loop on either side of them (1, 17), across and down, with plain,
+    symmetric and antisymmetric taps; 300 columns so that both the folded
+    body and the in-order tail of a row are used."""
+    rng = np.random.default_rng(21)
+    for n in range(1, 19, 2):
+        m = 18 - n
+        p = rng.normal(size=n).astype(np.float32)
+        q = rng.normal(size=m).astype(np.float32)
+        kinds_x = (p, p + p[::-1], p - p[::-1])
+        kinds_y = (q, q - q[::-1], q + q[::-1])
+        for sdt, ddt in ((np.uint8, np.uint8), (np.uint8, np.float32), (np.float32, np.float32), (np.uint16, np.int16)):
+            a = natural(23, 300, 1, n + 40, sdt)
+            for kx, ky in zip(kinds_x, kinds_y):
+                check(va, orc, "sep_filter_2d", a, ddt, kx, ky, border=4)
+        a = natural(23, 300, 3, n + 41)
+        check(va, orc, "box_filter", a, (n, m), True, 4)
+        check(va, orc, "box_filter", a, (m, n), True, 1)
+        check(va, orc, "gaussian_blur", a, (n, m), 0, border=4)
+
+
 # --------------------------------------------------- color / threshold
 
 def test_cvt_gray(va, orc):
@@ -828,7 +850,7 @@ def test_adaptive_threshold(va, orc, method):
     for (h, w) in ((41, 57), (1, 1), (2, 300), (200, 3)):
         a = natural(h, w, 1, h + 3 * w)
         for kind in (0, 1):
-            for block in (3, 5, 7, 11, 31, 127):
+            for block in (3, 5, 7, 11, 13, 15, 31, 127):
                 for delta in (0, 5, -3.5, 2.25, 300):
                     check(va, orc, "adaptive_threshold", a, 255, method, kind, block, delta)
             check(va, orc, "adaptive_threshold", a, 77.6, method, kind, 9, -1)

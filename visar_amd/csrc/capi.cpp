@@ -599,7 +599,8 @@ int visar_pyr_up(const visar_image *src, const visar_image *dst, int border) { r
 // block x block mean with a replicated border (boxFilter, or the f32
 // Gaussian blur of the u8 source saturated back to u8: one pass of the f32
 // engine, whose u8 loads convert exactly), then dst = s + idelta > m ? on :
-// off with idelta = ceil(delta) (BINARY) or floor(delta) (BINARY_INV).
+// off with idelta = ceil(delta) (BINARY) or floor(delta) (BINARY_INV), the
+// compare done by the kernel that computes m.
 // Blocks above 127 are not supported on the GPU.
 int visar_adaptive_threshold(const visar_image *src, const visar_image *dst, double maxval, int method, int type,
                              int block, double delta) {
@@ -627,19 +628,21 @@ int visar_adaptive_threshold(const visar_image *src, const visar_image *dst, dou
     const double dl = type == 0 ? std::ceil(delta) : std::floor(delta);
     const int idelta = (int)std::max(-1024.0, std::min(1024.0, dl));
     const int imax = (int)sat_cast<uint8_t>(maxval);
-    // The mean goes into the output itself and the compare then runs in
-    // place over it (each element reads its own mean first), so the call
-    // needs no temporary and can be captured in a graph.
+    // One kernel: the mean's producer compares each mean with its source
+    // sample before the store, so the call needs no temporary and can be
+    // captured in a graph.
+    p.adapt.enabled = 1;
+    p.adapt.idelta = idelta;
+    p.adapt.on = type == 0 ? imax : 0;
+    p.adapt.off = type == 0 ? 0 : imax;
     Call call;
     DImg s, d;
     if ((rc = call.begin())) return rc;
     if ((rc = call.input(src, s)) || (rc = call.output(dst, d))) return rc;
     p.src = s;
     p.dst = d;
-    if ((rc = method == 0 ? launch_box_u8(s, d, block, block, BORDER_REPLICATE, p.scale, call.stream())
-                          : launch_sep(p, call.stream())))
-        return call.finish(rc);
-    return call.finish(launch_adaptive_u8(s, d, d, idelta, type == 0 ? imax : 0, type == 0 ? 0 : imax, call.stream()));
+    return call.finish(method == 0 ? launch_box_u8(s, d, block, block, BORDER_REPLICATE, p.scale, call.stream(), p.adapt)
+                                   : launch_sep(p, call.stream()));
 }
 
 // capi.zig:218-232 + filter.zig:856-906

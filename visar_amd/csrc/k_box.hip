@@ -12,6 +12,9 @@
 // to 11: row by row, one lane per output element sums its kw column sums
 // from LDS and stores its byte (consecutive lanes, consecutive bytes); wider
 // windows: one lane per run of 7 outputs, a running sum along the run.
+// Window lengths up to 15 rows and 11 columns run unrolled (with_odd_len).
+// With `adapt` on (adaptiveThreshold), each mean is compared with its source
+// byte before it is stored.
 #include "kernels.hpp"
 #include "tiles.hpp"
 
@@ -25,7 +28,7 @@ constexpr int kBoxRun = 7;  // outputs per lane for wide windows (odd: conflict-
 
 template <int CN>
 __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, int kh, int border, float scale, int tw,
-                                                      TileOrder o) {
+                                                      TileOrder o, AdaptPost adapt) {
     extern __shared__ uint32_t colsum[];  // [kBoxTH][CN][pitch]
     __shared__ long row_off[kBoxTH + 126];  // source rows y0 - ry .. y0 + kBoxTH - 1 + ry, -1 outside
     int bx, by;
@@ -50,13 +53,24 @@ __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, in
             return (mx < 0 || r < 0) ? 0u : (uint32_t)base[r + off];
         };
         uint32_t v = 0;
-        for (int j0 = 0; j0 < kh; j0 += 8) {
-            uint32_t b[8];
+        with_odd_len<15>(kh, [&](auto len) {
+            constexpr int KH = decltype(len)::value;
+            if constexpr (KH > 0) {
+                uint32_t b[KH];
 #pragma unroll
-            for (int j = 0; j < 8; j++) b[j] = j0 + j < kh ? at(j0 + j) : 0u;
+                for (int j = 0; j < KH; j++) b[j] = at(j);
 #pragma unroll
-            for (int j = 0; j < 8; j++) v += b[j];
-        }
+                for (int j = 0; j < KH; j++) v += b[j];
+            } else {
+                for (int j0 = 0; j0 < kh; j0 += 8) {
+                    uint32_t b[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) b[j] = j0 + j < kh ? at(j0 + j) : 0u;
+#pragma unroll
+                    for (int j = 0; j < 8; j++) v += b[j];
+                }
+            }
+        });
         uint32_t add[kBoxTH - 1], sub[kBoxTH - 1];
 #pragma unroll
         for (int t = 0; t < kBoxTH - 1; t++) {
@@ -77,17 +91,27 @@ __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, in
         // One lane per output element of a row: its kw column sums from
         // LDS, the byte stored next to its neighbours' (consecutive lanes,
         // consecutive bytes).
-        for (int t = 0; t < rows; t++) {
-            const uint32_t *rowt = colsum + t * plane;
-            const gptr<uint8_t> out = grow<uint8_t>(d, y0 + t) + (long)x0 * CN;
-            for (int e = threadIdx.x; e < n_row; e += kBoxThreads) {
-                const int ox = e / CN, c = e - ox * CN;
-                const uint32_t *r = rowt + c * pitch + ox;
-                uint32_t sum = 0;
-                for (int i = 0; i < kw; i++) sum += r[i];
-                out[e] = sat_v<uint8_t>((float)sum * scale);
+        with_odd_len<11>(kw, [&](auto len) {
+            constexpr int KW = decltype(len)::value;
+            for (int t = 0; t < rows; t++) {
+                const uint32_t *rowt = colsum + t * plane;
+                const gptr<uint8_t> out = grow<uint8_t>(d, y0 + t) + (long)x0 * CN;
+                for (int e = threadIdx.x; e < n_row; e += kBoxThreads) {
+                    const int ox = e / CN, c = e - ox * CN;
+                    const uint32_t *r = rowt + c * pitch + ox;
+                    uint32_t sum = 0;
+                    if constexpr (KW > 0) {
+#pragma unroll
+                        for (int i = 0; i < KW; i++) sum += r[i];
+                    } else {
+                        for (int i = 0; i < kw; i++) sum += r[i];
+                    }
+                    uint8_t m = sat_v<uint8_t>((float)sum * scale);
+                    if (adapt.enabled) m = adapt_out(adapt.idelta, adapt.on, adapt.off, grow<const uint8_t>(s, y0 + t)[(long)x0 * CN + e], m);
+                    out[e] = m;
+                }
             }
-        }
+        });
         return;
     }
     // Wide windows: one lane per run of kBoxRun outputs of one channel, the
@@ -105,7 +129,9 @@ __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, in
         for (int k = 0; k < kBoxRun && ox0 + k < tw_row; k++) {
             const int ox = ox0 + k;
             if (k) sum += row[ox + kw - 1] - row[ox - 1];
-            out[(long)ox * CN] = sat_v<uint8_t>((float)sum * scale);
+            uint8_t m = sat_v<uint8_t>((float)sum * scale);
+            if (adapt.enabled) m = adapt_out(adapt.idelta, adapt.on, adapt.off, grow<const uint8_t>(s, y0 + t)[(long)(x0 + ox) * CN + c], m);
+            out[(long)ox * CN] = m;
         }
     }
 }
@@ -114,7 +140,8 @@ inline int launched() { return hipGetLastError() == hipSuccess ? OK : INTERNAL; 
 
 }  // namespace
 
-int launch_box_u8(const DImg &s, const DImg &d, int kw, int kh, int border, float scale, hipStream_t st) {
+int launch_box_u8(const DImg &s, const DImg &d, int kw, int kh, int border, float scale, hipStream_t st,
+                  const AdaptPost &adapt) {
     const int cn = s.cn;
     if (cn < 1 || cn > 4 || kw > 127 || kh > 127) return UNSUPPORTED;
     const int tw = std::max(16, kBoxThreads / cn - (kw - 1));
@@ -122,10 +149,10 @@ int launch_box_u8(const DImg &s, const DImg &d, int kw, int kh, int border, floa
     const TileOrder o = tile_order((s.w + tw - 1) / tw, (s.h + kBoxTH - 1) / kBoxTH);
     const dim3 g(tile_grid(o.ntiles)), b(kBoxThreads);
     switch (cn) {
-    case 1: hipLaunchKernelGGL(box_u8<1>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o); break;
-    case 2: hipLaunchKernelGGL(box_u8<2>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o); break;
-    case 3: hipLaunchKernelGGL(box_u8<3>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o); break;
-    default: hipLaunchKernelGGL(box_u8<4>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o); break;
+    case 1: hipLaunchKernelGGL(box_u8<1>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o, adapt); break;
+    case 2: hipLaunchKernelGGL(box_u8<2>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o, adapt); break;
+    case 3: hipLaunchKernelGGL(box_u8<3>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o, adapt); break;
+    default: hipLaunchKernelGGL(box_u8<4>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o, adapt); break;
     }
     return launched();
 }

@@ -9,12 +9,18 @@
 // horizontal taps out of LDS and finishes. Integer modes are exact, so pass
 // order is free; the f32 mode keeps the reference's per-lane order.
 #include "kernels.hpp"
+#include "tiles.hpp"
 
 namespace visar {
 
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kStageRows = 8;  // source rows whose loads are in flight together
+
+// Calls that can carry adaptiveThreshold's compare (SepParams::adapt).
+template <typename S, typename D>
+constexpr bool kAdaptable = std::is_same<S, uint8_t>::value && std::is_same<D, uint8_t>::value;
 
 template <typename S>
 __device__ inline float to_f(S v) {
@@ -79,6 +85,52 @@ __device__ inline float taps_seq(const float *k, int n, int from_zero, F v) {
         j0 = 1;
     }
     for (int j = j0; j < n; j++) a = a + k[j] * v(j);
+    return a;
+}
+
+// taps_fold / taps_seq / integer taps for a window length N fixed at compile
+// time (with_odd_len): fully unrolled, so the N reads issue together and the
+// weights sit in scalar registers. The operations and their order are those
+// of the runtime versions.
+template <int N, typename F>
+__device__ inline float taps_fold_n(const float *k, int sym, F v) {
+    float t[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) t[j] = v(j);
+    if (sym == SYM_NONE) {
+        float a = 0.0f;
+#pragma unroll
+        for (int j = 0; j < N; j++) a = a + t[j] * k[j];
+        return a;
+    }
+    constexpr int r = N / 2;
+    const bool even = sym == SYM_EVEN;
+    float a = even ? t[r] * k[r] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < r; i++) a = a + (even ? t[i] + t[N - 1 - i] : t[N - 1 - i] - t[i]) * (even ? k[i] : k[N - 1 - i]);
+    return a;
+}
+
+template <int N, typename F>
+__device__ inline float taps_seq_n(const float *k, int from_zero, F v) {
+    float t[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) t[j] = v(j);
+    float a = k[0] * t[0];
+    if (from_zero) a = 0.0f + a;  // 0 + k0 v0, as the loop from zero computes it
+#pragma unroll
+    for (int j = 1; j < N; j++) a = a + k[j] * t[j];
+    return a;
+}
+
+template <int N, typename A, typename F>
+__device__ inline A taps_int_n(const int32_t *k, F v) {
+    A t[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) t[j] = (A)v(j);
+    A a = 0;
+#pragma unroll
+    for (int j = 0; j < N; j++) a += (A)k[j] * t[j];
     return a;
 }
 
@@ -147,12 +199,15 @@ __device__ inline VT vert(const SepParams &p, const int *rmap, long col) {
 
 // The same vertical value from the tile's staged source rows: tap j of the
 // column is stg[j * ldw] (0 already stored for constant-border samples).
-template <typename S, typename VT, bool F32>
+template <int N, typename S, typename VT, bool F32>
 __device__ inline VT vert_staged(const SepParams &p, const S *stg, int ldw, long col) {
     const int n = p.nky;
     if constexpr (F32) {
         auto v = [&](int j) -> float { return to_f(stg[j * ldw]); };
-        return col < p.body ? taps_fold(p.kyf, n, p.vsym, v) : taps_seq(p.kyf, n, p.const_kernel, v);
+        if constexpr (N > 0) return col < p.body ? taps_fold_n<N>(p.kyf, p.vsym, v) : taps_seq_n<N>(p.kyf, p.const_kernel, v);
+        else return col < p.body ? taps_fold(p.kyf, n, p.vsym, v) : taps_seq(p.kyf, n, p.const_kernel, v);
+    } else if constexpr (N > 0) {
+        return taps_int_n<N, VT>(p.kyi, [&](int j) { return stg[j * ldw]; });
     } else {
         VT a = 0;
         for (int j = 0; j < n; j++) a += (VT)p.kyi[j] * (VT)stg[j * ldw];
@@ -169,6 +224,10 @@ __global__ __launch_bounds__(kThreads) void sep_tile(SepParams p, int twpx, int 
     const long x0 = (long)blockIdx.x * twpx;
     const long y0 = (long)blockIdx.y * th;
     const int rows = (int)min((long)th, (long)H - y0);
+    // adaptiveThreshold's compare, read here once: read inside the lambdas
+    // below, the fields make the compiler copy the parameter block to
+    // scratch memory.
+    const int a_en = p.adapt.enabled, a_d = p.adapt.idelta, a_on = p.adapt.on, a_off = p.adapt.off;
     int *rmap = reinterpret_cast<int *>(smem);
     VT *V = reinterpret_cast<VT *>(smem + (((th + 2 * ry) * sizeof(int) + 15) & ~(size_t)15));
     S *stg = reinterpret_cast<S *>(V + (size_t)th * ldw);  // [rows + 2 ry][ldw] source samples
@@ -176,52 +235,69 @@ __global__ __launch_bounds__(kThreads) void sep_tile(SepParams p, int twpx, int 
     __syncthreads();
     // Pass 0: the tile's source rows (halo columns border-mapped, constant
     // border 0) into LDS; one column per thread, its mapping computed once,
-    // four rows' loads in flight before they are stored.
+    // eight rows' loads in flight before they are stored.
     const int srows = rows + 2 * ry;
     for (int j = threadIdx.x; j < ldw; j += kThreads) {
         const int q = j / cn, ch = j - q * cn;
         const int mx = border_map_halo((int)(x0 - rx + q), W, p.border);
         const long off = (long)mx * cn + ch;
-        for (int r0 = 0; r0 < srows; r0 += 4) {
-            S v[4];
+        for (int r0 = 0; r0 < srows; r0 += kStageRows) {
+            S v[kStageRows];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < kStageRows; u++) {
                 const int sy = r0 + u < srows ? rmap[r0 + u] : -1;
                 v[u] = (mx >= 0 && sy >= 0) ? row_ptr<const S>(p.src, sy)[off] : (S)0;
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < kStageRows; u++)
                 if (r0 + u < srows) stg[(r0 + u) * ldw + j] = v[u];
         }
     }
     __syncthreads();
     // Pass 1: vertical taps from the staged rows into LDS, one column per
     // thread.
-    for (int j = threadIdx.x; j < ldw; j += kThreads) {
-        const int q = j / cn, ch = j - q * cn;
-        const long mx = border_map_halo((int)(x0 - rx + q), W, p.border);
-        for (int r = 0; r < rows; r++)
-            V[r * ldw + j] = mx >= 0 ? vert_staged<S, VT, F32>(p, stg + r * ldw + j, ldw, mx * cn + ch) : (VT)0;
-    }
+    with_odd_len<15>(p.nky, [&](auto len) {
+        constexpr int N = decltype(len)::value;
+        for (int j = threadIdx.x; j < ldw; j += kThreads) {
+            const int q = j / cn, ch = j - q * cn;
+            const long mx = border_map_halo((int)(x0 - rx + q), W, p.border);
+            for (int r = 0; r < rows; r++)
+                V[r * ldw + j] = mx >= 0 ? vert_staged<N, S, VT, F32>(p, stg + r * ldw + j, ldw, mx * cn + ch) : (VT)0;
+        }
+    });
     __syncthreads();
     // Pass 2: horizontal taps and finish.
     const int outw = (int)min((long)twpx, (long)W - x0) * cn;
-    for (int r = 0; r < rows; r++)
-    for (int e = threadIdx.x; e < outw; e += kThreads) {
-        const VT *vr = V + r * ldw + e;
-        long ge = x0 * cn + e;
-        D o;
-        if constexpr (F32) {
-            auto v = [&](int i) -> float { return vr[i * cn]; };
-            float a = ge < p.body ? taps_fold(p.kxf, p.nkx, p.hsym, v) : taps_seq(p.kxf, p.nkx, p.const_kernel, v);
-            o = finish_f<D>(a, p.fin, p.scale);
-        } else {
-            AT a = 0;
-            for (int i = 0; i < p.nkx; i++) a += (AT)p.kxi[i] * (AT)vr[i * cn];
-            o = finish_int<D, AT>(a, p.fin, p.scale);
+    with_odd_len<15>(p.nkx, [&](auto len) {
+        constexpr int N = decltype(len)::value;
+        for (int r = 0; r < rows; r++)
+        for (int e = threadIdx.x; e < outw; e += kThreads) {
+            const VT *vr = V + r * ldw + e;
+            long ge = x0 * cn + e;
+            D o;
+            if constexpr (F32) {
+                auto v = [&](int i) -> float { return vr[i * cn]; };
+                float a;
+                if constexpr (N > 0) a = ge < p.body ? taps_fold_n<N>(p.kxf, p.hsym, v) : taps_seq_n<N>(p.kxf, p.const_kernel, v);
+                else a = ge < p.body ? taps_fold(p.kxf, p.nkx, p.hsym, v) : taps_seq(p.kxf, p.nkx, p.const_kernel, v);
+                o = finish_f<D>(a, p.fin, p.scale);
+            } else {
+                AT a;
+                if constexpr (N > 0) {
+                    a = taps_int_n<N, AT>(p.kxi, [&](int i) { return vr[i * cn]; });
+                } else {
+                    a = 0;
+                    for (int i = 0; i < p.nkx; i++) a += (AT)p.kxi[i] * (AT)vr[i * cn];
+                }
+                o = finish_int<D, AT>(a, p.fin, p.scale);
+            }
+            // adaptiveThreshold: the source sample is staged row r + ry,
+            // column e + rx cn.
+            if constexpr (kAdaptable<S, D>)
+                if (a_en) o = adapt_out(a_d, a_on, a_off, stg[(r + ry) * ldw + e + rx * cn], o);
+            row_ptr<D>(p.dst, y0 + r)[ge] = o;
         }
-        row_ptr<D>(p.dst, y0 + r)[ge] = o;
-    }
+    });
 }
 
 // Fallback when a tile's halo does not fit LDS (very wide kernels on many
@@ -251,6 +327,8 @@ __global__ __launch_bounds__(kThreads) void sep_direct(SepParams p) {
         for (int i = 0; i < p.nkx; i++) a += (AT)p.kxi[i] * (AT)vcol(i);
         o = finish_int<D, AT>(a, p.fin, p.scale);
     }
+    if constexpr (kAdaptable<S, D>)
+        if (p.adapt.enabled) o = adapt_out(p.adapt.idelta, p.adapt.on, p.adapt.off, row_ptr<const S>(p.src, y)[gid], o);
     row_ptr<D>(p.dst, y)[gid] = o;
 }
 
@@ -259,8 +337,14 @@ int run_sep(const SepParams &p, hipStream_t st) {
     const int cn = p.src.cn;
     const int rx = p.nkx / 2, ry = p.nky / 2;
     const size_t lds_cap = 60 * 1024;
-    int twpx = cn == 1 ? 256 : cn == 2 ? 128 : cn <= 4 ? 64 : 16;
-    int th = 4;  // short tiles: more workgroups in flight (an 11x11 f32 blur of 1080p: 16 rows 40 us, 4 rows 29 us)
+    // The padded tile row ((twpx + 2 rx) * cn samples) fits the 256 threads,
+    // so passes 0 and 1 take one column per thread and no wave does a second
+    // one while the others wait at the barrier.
+    int twpx = std::max(16, kThreads / std::min(cn, 16) - 2 * rx);
+    // Rows per tile, measured on 1080p: 6 for one channel (an 11x11 f32
+    // blur: 4 rows 17.5 us, 6 rows 15.3, 8 rows 16.3), 8 for three or more
+    // (an 11x11 u8 BGR blur: 4 rows 37.8 us, 6 rows 31.9, 8 rows 30.3).
+    int th = tuning().sep_rows ? tuning().sep_rows : cn >= 3 ? 8 : 6;
     auto need = [&](int tw, int t) {
         return (((size_t)(t + 2 * ry) * sizeof(int) + 15) & ~(size_t)15) + (size_t)t * (tw + 2 * rx) * cn * sizeof(VT) +
                (size_t)(t + 2 * ry) * (tw + 2 * rx) * cn * sizeof(S);
@@ -293,7 +377,7 @@ int run_sep_f32_dst(const SepParams &p, hipStream_t st) {
 
 int launch_sep(const SepParams &p, hipStream_t st) {
     const int th = tuning().stencil_rows ? tuning().stencil_rows : stencil_rows_per_thread((long)p.src.w * p.src.cn, p.src.h);
-    if (launch_sep_small_u8(p, st, th)) return OK;
+    if (!p.adapt.enabled && launch_sep_small_u8(p, st, th)) return OK;
     if (p.f32mode) {
         switch (p.sdepth) {
         case 0: return run_sep_f32_dst<uint8_t>(p, st);

@@ -100,17 +100,6 @@ __device__ __attribute__((always_inline)) inline uint8_t gray_down_out(uint32_t 
     return (uint8_t)(uint32_t)(v < 255.0f ? v : 255.0f);
 }
 
-// threshold.zig:340-360 adaptiveThreshold's last step: the table
-// tab[s - m + 255] and the vector compare both give on exactly when
-// s + idelta > m (idelta clamped to +-1024), off otherwise.
-__global__ __launch_bounds__(kThreads) void adaptive_u8(DImg s, DImg m, DImg d, long n, int idelta, int on, int off) {
-    for (long g = (long)blockIdx.x * kThreads + threadIdx.x; g < n; g += (long)gridDim.x * kThreads) {
-        const long y = g / d.w, x = g - y * d.w;
-        const int sv = grow<const uint8_t>(s, y)[x], mv = grow<const uint8_t>(m, y)[x];
-        grow<uint8_t>(d, y)[x] = (uint8_t)(sv + idelta > mv ? on : off);
-    }
-}
-
 // Any layout: one output pixel per thread, byte loads; 64 x 4 outputs per
 // tile, tiles walked grid-stride.
 template <int CN>
@@ -537,13 +526,6 @@ int launch_gray_down(const DImg &s, const DImg &d, int k, hipStream_t st) {
     const unsigned grid = (unsigned)std::min<long>(tiles, 1l << 20);
     if (s.cn == 3) hipLaunchKernelGGL((gray_down_u8<3>), dim3(grid), dim3(kThreads), 0, st, s, d, k, inv, tiles_x, tiles);
     else hipLaunchKernelGGL((gray_down_u8<4>), dim3(grid), dim3(kThreads), 0, st, s, d, k, inv, tiles_x, tiles);
-    return launched();
-}
-
-int launch_adaptive_u8(const DImg &s, const DImg &m, const DImg &d, int idelta, int on, int off, hipStream_t st) {
-    const long n = (long)d.w * d.h;
-    hipLaunchKernelGGL(adaptive_u8, dim3((unsigned)std::min<long>((n + kThreads - 1) / kThreads, 1l << 20)), dim3(kThreads), 0,
-                       st, s, m, d, n, idelta, on, off);
     return launched();
 }
 

@@ -11,6 +11,7 @@ struct Tuning {
     int batch_stencil_rows = 0;  // batched stencils: output rows per thread (1, 2)
     int batch_unit_kb = 0;       // batched calls: KiB of input + output per workgroup
     int tile_order = 0;          // plain tile kernels: 0 each kernel's default, 1 row-major, 2 one band of tiles per XCD
+    int sep_rows = 0;            // separable engine: output rows per tile (1-16)
 };
 Tuning &tuning();
 bool set_tuning(const char *key, int value);
@@ -23,6 +24,17 @@ enum Finish : int { FIN_FIX8 = 0, FIN_FIX16 = 1, FIN_CAST = 2, FIN_SCALED = 3 };
 enum Sym : int { SYM_NONE = 0, SYM_EVEN = 1, SYM_ODD = 2 };
 
 constexpr int kMaxTaps = 128;
+
+// adaptiveThreshold's last step (threshold.zig:340-360) folded into the
+// kernel that computes the local mean m of a u8 image s: the destination
+// gets on when s + idelta > m, off otherwise (idelta clamped to +-1024), in
+// place of m. Off (enabled 0) for every other call.
+struct AdaptPost {
+    int enabled, idelta, on, off;
+};
+__host__ __device__ inline uint8_t adapt_out(int idelta, int on, int off, int s, uint8_t m) {
+    return (uint8_t)(s + idelta > (int)m ? on : off);
+}
 
 // One separable filter call (filter.zig:120-205). Integer modes use the
 // integer taps; the f32 mode uses the float taps and reproduces the
@@ -40,6 +52,7 @@ struct SepParams {
     long body;
     int32_t kxi[kMaxTaps], kyi[kMaxTaps];
     float kxf[kMaxTaps], kyf[kMaxTaps];
+    AdaptPost adapt;  // u8 -> u8 only
 };
 
 int launch_sep(const SepParams &p, hipStream_t st);
@@ -84,10 +97,9 @@ int launch_pyramid(const DImg &s, const DImg &d, int depth, int border, bool dow
 
 // filter.zig:617-806 boxFilter on u8, 1-4 channels, odd windows up to 127:
 // exact sums, sat(f32(sum) * scale); UNSUPPORTED outside that.
-int launch_box_u8(const DImg &s, const DImg &d, int kw, int kh, int border, float scale, hipStream_t st);
+int launch_box_u8(const DImg &s, const DImg &d, int kw, int kh, int border, float scale, hipStream_t st,
+                  const AdaptPost &adapt = AdaptPost{});
 
-// threshold.zig:340-360: dst = s + idelta > m ? on : off, u8, one channel.
-int launch_adaptive_u8(const DImg &s, const DImg &m, const DImg &d, int idelta, int on, int off, hipStream_t st);
 int launch_area_int_u8(const DImg &s, const DImg &d, int ix, int iy, hipStream_t st);
 int launch_nearest(const DImg &s, const DImg &d, size_t elem, hipStream_t st);
 

@@ -15,6 +15,27 @@ namespace visar {
 
 constexpr int kTileThreads = 256;
 
+// Runs fn(integral_constant<int, N>) with N = n when n is odd and 3 <= n <=
+// MaxN, else with N = 0, so that a window loop whose length is known at
+// compile time can be unrolled in full: its reads then issue together and
+// its weights or counters stay out of the scalar unit (a runtime-length
+// loop spends about four scalar instructions per tap, and the one scalar
+// unit of a CU then bounds the kernel). n is uniform: one scalar compare
+// per candidate length, once per call site.
+template <int N, int MaxN, typename Fn>
+__device__ inline void with_odd_len_from(int n, Fn &fn) {
+    if constexpr (N > MaxN) {
+        fn(std::integral_constant<int, 0>{});
+    } else {
+        if (n == N) fn(std::integral_constant<int, N>{});
+        else with_odd_len_from<N + 2, MaxN>(n, fn);
+    }
+}
+template <int MaxN, typename Fn>
+__device__ inline void with_odd_len(int n, Fn fn) {
+    with_odd_len_from<3, MaxN>(n, fn);
+}
+
 // Global-memory views. Image pointers that reach a kernel through memory
 // (the batch's job table) are generic to the compiler, which would then
 // emit flat loads and stores; casting them to the global address space

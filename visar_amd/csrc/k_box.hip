@@ -52,16 +52,26 @@ __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, in
             const long r = row_off[i];
             return (mx < 0 || r < 0) ? 0u : (uint32_t)base[r + off];
         };
-        uint32_t v = 0;
+        uint32_t *cs = colsum + (col % CN) * pitch + col / CN;
         with_odd_len<15>(kh, [&](auto len) {
             constexpr int KH = decltype(len)::value;
             if constexpr (KH > 0) {
-                uint32_t b[KH];
+                // The KH + TH - 1 rows the column's sums use, each loaded
+                // once, all in flight together; then the window slides.
+                uint32_t r[KH + kBoxTH - 1];
 #pragma unroll
-                for (int j = 0; j < KH; j++) b[j] = at(j);
+                for (int i = 0; i < KH + kBoxTH - 1; i++) r[i] = at(i);
+                uint32_t v = 0;
 #pragma unroll
-                for (int j = 0; j < KH; j++) v += b[j];
+                for (int j = 0; j < KH; j++) v += r[j];
+                cs[0] = v;
+#pragma unroll
+                for (int t = 0; t < kBoxTH - 1; t++) {
+                    v += r[t + KH] - r[t];
+                    cs[(t + 1) * plane] = v;
+                }
             } else {
+                uint32_t v = 0;
                 for (int j0 = 0; j0 < kh; j0 += 8) {
                     uint32_t b[8];
 #pragma unroll
@@ -69,21 +79,20 @@ __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, in
 #pragma unroll
                     for (int j = 0; j < 8; j++) v += b[j];
                 }
+                uint32_t add[kBoxTH - 1], sub[kBoxTH - 1];
+#pragma unroll
+                for (int t = 0; t < kBoxTH - 1; t++) {
+                    add[t] = at(t + kh);
+                    sub[t] = at(t);
+                }
+                cs[0] = v;
+#pragma unroll
+                for (int t = 0; t < kBoxTH - 1; t++) {
+                    v += add[t] - sub[t];
+                    cs[(t + 1) * plane] = v;
+                }
             }
         });
-        uint32_t add[kBoxTH - 1], sub[kBoxTH - 1];
-#pragma unroll
-        for (int t = 0; t < kBoxTH - 1; t++) {
-            add[t] = at(t + kh);
-            sub[t] = at(t);
-        }
-        uint32_t *cs = colsum + (col % CN) * pitch + col / CN;
-        cs[0] = v;
-#pragma unroll
-        for (int t = 0; t < kBoxTH - 1; t++) {
-            v += add[t] - sub[t];
-            cs[(t + 1) * plane] = v;
-        }
     }
     __syncthreads();
     const int n_row = min(tw, W - x0) * CN, rows = min(kBoxTH, H - y0);

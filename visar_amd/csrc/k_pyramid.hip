@@ -139,43 +139,51 @@ __device__ __attribute__((always_inline)) inline int up_map(int p, int n) {
     return p < 0 ? (n > 1 ? 1 : 0) : p >= n ? n - 1 : p;
 }
 
-// pyrUp: tile of kPyrTH source rows (2 kPyrTH output rows) and tw source
-// pixels (2 tw output pixels); padded source columns x0 - 1 .. x0 + tw.
+// pyrUp: tile of kPyrUpTH source rows (2 kPyrUpTH output rows) and tw
+// source pixels (tw + 2 padded columns, about 128 samples). Column pass: the
+// two halves of the workgroup take the upper and lower half of the tile's
+// rows, each thread one padded column, and write the even (a + c + 6 b) and
+// odd (4 (b + c)) output rows' column sums to LDS. Row pass: the halves take
+// the even and the odd output row of a pair; a lane reads the three column
+// sums of one source pixel and stores both of its output pixels.
+constexpr int kPyrUpTH = 16;
 template <typename T, int CN>
 __global__ __launch_bounds__(kPyrThreads) void pyr_up_tile(DImg s, DImg d, int tw, TileOrder o) {
     using A = typename PyrAcc<T>::type;
+    constexpr int kHalf = kPyrThreads / 2, kRows = kPyrUpTH / 2;
     extern __shared__ uint32_t lds_words[];
-    A *E = reinterpret_cast<A *>(lds_words);  // [kPyrTH][ncols] even output rows
+    A *E = reinterpret_cast<A *>(lds_words);  // [kPyrUpTH][ncols] even output rows
     int bx, by;
     if (!tile_of(o, bx, by)) return;
-    const int x0 = bx * tw, y0 = by * kPyrTH;
+    const int x0 = bx * tw, y0 = by * kPyrUpTH;
     const int ncols = (tw + 2) * CN;
-    A *O = E + kPyrTH * ncols;  // odd output rows
-    for (int col = threadIdx.x; col < ncols; col += kPyrThreads) {
+    A *O = E + kPyrUpTH * ncols;  // odd output rows
+    const int half = threadIdx.x >= kHalf, lane = threadIdx.x & (kHalf - 1);
+    const int t0 = half * kRows;
+    for (int col = lane; col < ncols; col += kHalf) {
         const long off = (long)up_map(x0 - 1 + col / CN, s.w) * CN + col % CN;
-        A r[kPyrTH + 2];
+        A r[kRows + 2];
 #pragma unroll
-        for (int k = 0; k < kPyrTH + 2; k++)
-            r[k] = (A)((gptr<const T>)grow<const uint8_t>(s, up_map(y0 - 1 + k, s.h)))[off];
+        for (int k = 0; k < kRows + 2; k++)
+            r[k] = (A)((gptr<const T>)grow<const uint8_t>(s, up_map(y0 - 1 + t0 + k, s.h)))[off];
 #pragma unroll
-        for (int t = 0; t < kPyrTH; t++) {
-            E[t * ncols + col] = (r[t] + r[t + 2]) + r[t + 1] * (A)6;
-            O[t * ncols + col] = (r[t + 1] + r[t + 2]) * (A)4;
+        for (int t = 0; t < kRows; t++) {
+            E[(t0 + t) * ncols + col] = (r[t] + r[t + 2]) + r[t + 1] * (A)6;
+            O[(t0 + t) * ncols + col] = (r[t + 1] + r[t + 2]) * (A)4;
         }
     }
     __syncthreads();
-    // Output rows are 2 tw * CN (about 250) elements: row by row, one lane
-    // per element, no division.
-    const int n_out = min(2 * tw, d.w - 2 * x0) * CN, rows = min(2 * kPyrTH, d.h - 2 * y0);
-    for (int t2 = 0; t2 < rows; t2++) {
-        const A *rowt = ((t2 & 1) ? O : E) + (t2 >> 1) * ncols;
-        const gptr<T> out = (gptr<T>)grow<uint8_t>(d, 2 * y0 + t2) + (long)2 * x0 * CN;
-        for (int e = threadIdx.x; e < n_out; e += kPyrThreads) {
-            const int X = e / CN, c = e - X * CN, p = X >> 1;  // padded column p + 1 is source pixel x0 + p
+    const int n_src = min(tw, s.w - x0) * CN, src_rows = min(kPyrUpTH, s.h - y0);
+    const A *plane = half ? O : E;
+    for (int i = 0; i < src_rows; i++) {
+        const A *rowt = plane + i * ncols;
+        const gptr<T> out = (gptr<T>)grow<uint8_t>(d, 2 * (y0 + i) + half) + (long)2 * x0 * CN;
+        for (int e = lane; e < n_src; e += kHalf) {
+            const int p = e / CN, c = e - p * CN;  // padded column p + 1 is source pixel x0 + p
             const A *row = rowt + c;
-            const A v = (X & 1) ? (row[(p + 1) * CN] + row[(p + 2) * CN]) * (A)4
-                                : (row[p * CN] + row[(p + 2) * CN]) + row[(p + 1) * CN] * (A)6;
-            out[e] = pyr_finish<T, 6>(v);
+            const A a = row[p * CN], b = row[(p + 1) * CN], cc = row[(p + 2) * CN];
+            out[2 * p * CN + c] = pyr_finish<T, 6>((a + cc) + b * (A)6);
+            out[(2 * p + 1) * CN + c] = pyr_finish<T, 6>((b + cc) * (A)4);
         }
     }
 }
@@ -192,8 +200,8 @@ void pyr_tiled(const DImg &s, const DImg &d, int border, bool down, hipStream_t 
         hipLaunchKernelGGL((pyr_down_tile<T, CN>), dim3(tile_grid(o.ntiles)), b, lds, st, s, d, border, tw, o);
     } else {
         const int tw = std::max(8, kPyrThreads / 2 / CN - 2);
-        const TileOrder o = tile_order((s.w + tw - 1) / tw, (s.h + kPyrTH - 1) / kPyrTH);
-        const size_t lds = (size_t)2 * kPyrTH * (tw + 2) * CN * 4;
+        const TileOrder o = tile_order((s.w + tw - 1) / tw, (s.h + kPyrUpTH - 1) / kPyrUpTH);
+        const size_t lds = (size_t)2 * kPyrUpTH * (tw + 2) * CN * 4;
         hipLaunchKernelGGL((pyr_up_tile<T, CN>), dim3(tile_grid(o.ntiles)), b, lds, st, s, d, tw, o);
     }
 }

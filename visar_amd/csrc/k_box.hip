@@ -9,10 +9,11 @@
 // of kh rows and then the rows that enter and leave the window, all before
 // using them, and writes the column sums of the tile's rows to LDS (running
 // sums: exact in u32), one plane per channel. Horizontal pass, windows up
-// to 11: row by row, one lane per output element sums its kw column sums
-// from LDS and stores its byte (consecutive lanes, consecutive bytes); wider
-// windows: one lane per run of 7 outputs, a running sum along the run.
-// Window lengths up to 15 rows and 11 columns run unrolled (with_odd_len).
+// to 11 (one channel) or 7 (more): row by row, one lane per output element
+// sums its kw column sums from LDS and stores its byte (consecutive lanes,
+// consecutive bytes); wider windows: one lane per run of 7 outputs, a
+// running sum along the run. Window lengths up to 15 rows and up to the
+// direct limit in columns run unrolled (with_odd_len).
 // With `adapt` on (adaptiveThreshold), each mean is compared with its source
 // byte before it is stored.
 #include "kernels.hpp"
@@ -25,6 +26,12 @@ namespace {
 constexpr int kBoxThreads = 256;
 constexpr int kBoxTH = 16;  // output rows per tile
 constexpr int kBoxRun = 7;  // outputs per lane for wide windows (odd: conflict-free LDS reads)
+// Widest window summed directly, one lane per output; wider ones in runs.
+// Measured on 1080p: gray direct up to 11 (adaptive mean 11: 9.7 us direct,
+// 10.3 runs), BGR up to 7 (7x7: 12.4 direct, 14.7 runs; 11x11: 15.3 direct,
+// 14.0 runs).
+template <int CN>
+constexpr int kBoxDirect = CN == 1 ? 11 : 7;
 
 template <int CN>
 __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, int kh, int border, float scale, int tw,
@@ -96,11 +103,11 @@ __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, in
     }
     __syncthreads();
     const int n_row = min(tw, W - x0) * CN, rows = min(kBoxTH, H - y0);
-    if (kw <= 11) {
+    if (kw <= kBoxDirect<CN>) {
         // One lane per output element of a row: its kw column sums from
         // LDS, the byte stored next to its neighbours' (consecutive lanes,
         // consecutive bytes).
-        with_odd_len<11>(kw, [&](auto len) {
+        with_odd_len<kBoxDirect<CN>>(kw, [&](auto len) {
             constexpr int KW = decltype(len)::value;
             for (int t = 0; t < rows; t++) {
                 const uint32_t *rowt = colsum + t * plane;
@@ -155,7 +162,10 @@ int launch_box_u8(const DImg &s, const DImg &d, int kw, int kh, int border, floa
     if (cn < 1 || cn > 4 || kw > 127 || kh > 127) return UNSUPPORTED;
     const int tw = std::max(16, kBoxThreads / cn - (kw - 1));
     const size_t lds = (size_t)kBoxTH * (tw + kw - 1) * cn * sizeof(uint32_t);
-    const TileOrder o = tile_order((s.w + tw - 1) / tw, (s.h + kBoxTH - 1) / kBoxTH);
+    // XCD bands: a tile and the one below it share kh - 1 source rows, and in
+    // a band they run on the same XCD, whose L2 then serves the second read
+    // (1080p BGR 15x15: 16.1 -> 15.5 us).
+    const TileOrder o = tile_order((s.w + tw - 1) / tw, (s.h + kBoxTH - 1) / kBoxTH, true);
     const dim3 g(tile_grid(o.ntiles)), b(kBoxThreads);
     switch (cn) {
     case 1: hipLaunchKernelGGL(box_u8<1>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o, adapt); break;

@@ -195,12 +195,14 @@ void pyr_tiled(const DImg &s, const DImg &d, int border, bool down, hipStream_t 
     const dim3 b(kPyrThreads);
     if (down) {
         const int tw = std::max(8, (kPyrThreads / CN - 3) / 2);
-        const TileOrder o = tile_order((d.w + tw - 1) / tw, (d.h + kPyrTH - 1) / kPyrTH);
+        // XCD bands, so vertically neighbouring tiles' shared source rows are
+        // read through one L2 (1080p f32 BGR: 6.7 -> 6.1 us).
+        const TileOrder o = tile_order((d.w + tw - 1) / tw, (d.h + kPyrTH - 1) / kPyrTH, true);
         const size_t lds = (size_t)kPyrTH * (2 * tw + 3) * CN * 4;
         hipLaunchKernelGGL((pyr_down_tile<T, CN>), dim3(tile_grid(o.ntiles)), b, lds, st, s, d, border, tw, o);
     } else {
         const int tw = std::max(8, kPyrThreads / 2 / CN - 2);
-        const TileOrder o = tile_order((s.w + tw - 1) / tw, (s.h + kPyrUpTH - 1) / kPyrUpTH);
+        const TileOrder o = tile_order((s.w + tw - 1) / tw, (s.h + kPyrUpTH - 1) / kPyrUpTH, true);  // 8.35 -> 7.9 us (1080p gray)
         const size_t lds = (size_t)2 * kPyrUpTH * (tw + 2) * CN * 4;
         hipLaunchKernelGGL((pyr_up_tile<T, CN>), dim3(tile_grid(o.ntiles)), b, lds, st, s, d, tw, o);
     }

@@ -63,10 +63,11 @@ struct TileOrder {
     int tiles_x, ntiles, xcd;
 };
 inline unsigned tile_grid(int ntiles) { return (unsigned)((ntiles + 7) / 8 * 8); }
-// xcd_default: the kernel's default order. XCD bands for the warps: the
+// xcd_default: the kernel's default order. XCD bands for the warps (the
 // perspective warp runs faster in them, and the affine box warp, whose
 // neighbouring tiles' boxes overlap, reads a quarter of the bytes from
-// memory at the same speed; row-major for the others.
+// memory at the same speed), the box filter and the pyramids (neighbouring
+// tiles share halo rows); row-major for the others.
 inline TileOrder tile_order(int tiles_x, int tiles_y, bool xcd_default = false) {
     const int k = tuning().tile_order;
     return TileOrder{tiles_x, tiles_x * tiles_y, k == 2 || (k == 0 && xcd_default)};

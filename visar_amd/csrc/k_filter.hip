@@ -342,9 +342,10 @@ int run_sep(const SepParams &p, hipStream_t st) {
     // one while the others wait at the barrier.
     int twpx = std::max(16, kThreads / std::min(cn, 16) - 2 * rx);
     // Rows per tile, measured on 1080p: 6 for one channel (an 11x11 f32
-    // blur: 4 rows 17.5 us, 6 rows 15.3, 8 rows 16.3), 8 for three or more
-    // (an 11x11 u8 BGR blur: 4 rows 37.8 us, 6 rows 31.9, 8 rows 30.3).
-    int th = tuning().sep_rows ? tuning().sep_rows : cn >= 3 ? 8 : 6;
+    // blur: 4 rows 17.4 us, 6 rows 15.2, 8 rows 16.2; adaptive Gaussian 11:
+    // 18.1, 16.0, 17.1), 10 for three or more (an 11x11 u8 BGR blur: 6 rows
+    // 31.7 us, 8 rows 30.0, 10 rows 28.1, 12 rows 29.1).
+    int th = tuning().sep_rows ? tuning().sep_rows : cn >= 3 ? 10 : 6;
     auto need = [&](int tw, int t) {
         return (((size_t)(t + 2 * ry) * sizeof(int) + 15) & ~(size_t)15) + (size_t)t * (tw + 2 * rx) * cn * sizeof(VT) +
                (size_t)(t + 2 * ry) * (tw + 2 * rx) * cn * sizeof(S);

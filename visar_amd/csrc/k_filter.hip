@@ -215,12 +215,19 @@ __device__ inline VT vert_staged(const SepParams &p, const S *stg, int ldw, long
     }
 }
 
-template <typename S, typename D, typename VT, typename AT, bool F32>
+// ONE: one channel whose padded tile row fits kThreads samples. The LDS row
+// pitch is then the constant kThreads and the channel stride 1, so every
+// tap's LDS offset is an immediate of its read instead of an address the
+// vector unit computes; f32-mode calls also stage the samples converted to
+// f32, once, instead of converting each at every tap that reads it.
+template <typename S, typename D, typename VT, typename AT, bool F32, bool ONE>
 __global__ __launch_bounds__(kThreads) void sep_tile(SepParams p, int twpx, int th) {
+    using ST = typename std::conditional<F32 && ONE, float, S>::type;  // staged sample
     extern __shared__ __align__(16) unsigned char smem[];
-    const int cn = p.src.cn, W = p.src.w, H = p.src.h;
+    const int cn = ONE ? 1 : p.src.cn, W = p.src.w, H = p.src.h;
     const int rx = p.nkx / 2, ry = p.nky / 2;
-    const int ldw = (twpx + 2 * rx) * cn;
+    const int ncols = (twpx + 2 * rx) * cn;  // padded tile row, samples
+    const int ldw = ONE ? kThreads : ncols;  // LDS row pitch
     const long x0 = (long)blockIdx.x * twpx;
     const long y0 = (long)blockIdx.y * th;
     const int rows = (int)min((long)th, (long)H - y0);
@@ -230,14 +237,14 @@ __global__ __launch_bounds__(kThreads) void sep_tile(SepParams p, int twpx, int 
     const int a_en = p.adapt.enabled, a_d = p.adapt.idelta, a_on = p.adapt.on, a_off = p.adapt.off;
     int *rmap = reinterpret_cast<int *>(smem);
     VT *V = reinterpret_cast<VT *>(smem + (((th + 2 * ry) * sizeof(int) + 15) & ~(size_t)15));
-    S *stg = reinterpret_cast<S *>(V + (size_t)th * ldw);  // [rows + 2 ry][ldw] source samples
+    ST *stg = reinterpret_cast<ST *>(V + (size_t)th * ldw);  // [rows + 2 ry][ldw] source samples
     for (int i = threadIdx.x; i < rows + 2 * ry; i += kThreads) rmap[i] = (int)border_map(y0 + i - ry, H, p.border);
     __syncthreads();
     // Pass 0: the tile's source rows (halo columns border-mapped, constant
     // border 0) into LDS; one column per thread, its mapping computed once,
     // eight rows' loads in flight before they are stored.
     const int srows = rows + 2 * ry;
-    for (int j = threadIdx.x; j < ldw; j += kThreads) {
+    for (int j = threadIdx.x; j < ncols; j += kThreads) {
         const int q = j / cn, ch = j - q * cn;
         const int mx = border_map_halo((int)(x0 - rx + q), W, p.border);
         const long off = (long)mx * cn + ch;
@@ -250,7 +257,7 @@ __global__ __launch_bounds__(kThreads) void sep_tile(SepParams p, int twpx, int 
             }
 #pragma unroll
             for (int u = 0; u < kStageRows; u++)
-                if (r0 + u < srows) stg[(r0 + u) * ldw + j] = v[u];
+                if (r0 + u < srows) stg[(r0 + u) * ldw + j] = (ST)v[u];
         }
     }
     __syncthreads();
@@ -258,11 +265,11 @@ __global__ __launch_bounds__(kThreads) void sep_tile(SepParams p, int twpx, int 
     // thread.
     with_odd_len<15>(p.nky, [&](auto len) {
         constexpr int N = decltype(len)::value;
-        for (int j = threadIdx.x; j < ldw; j += kThreads) {
+        for (int j = threadIdx.x; j < ncols; j += kThreads) {
             const int q = j / cn, ch = j - q * cn;
             const long mx = border_map_halo((int)(x0 - rx + q), W, p.border);
             for (int r = 0; r < rows; r++)
-                V[r * ldw + j] = mx >= 0 ? vert_staged<N, S, VT, F32>(p, stg + r * ldw + j, ldw, mx * cn + ch) : (VT)0;
+                V[r * ldw + j] = mx >= 0 ? vert_staged<N, ST, VT, F32>(p, stg + r * ldw + j, ldw, mx * cn + ch) : (VT)0;
         }
     });
     __syncthreads();
@@ -294,7 +301,7 @@ __global__ __launch_bounds__(kThreads) void sep_tile(SepParams p, int twpx, int 
             // adaptiveThreshold: the source sample is staged row r + ry,
             // column e + rx cn.
             if constexpr (kAdaptable<S, D>)
-                if (a_en) o = adapt_out(a_d, a_on, a_off, stg[(r + ry) * ldw + e + rx * cn], o);
+                if (a_en) o = adapt_out(a_d, a_on, a_off, (int)stg[(r + ry) * ldw + e + rx * cn], o);
             row_ptr<D>(p.dst, y0 + r)[ge] = o;
         }
     });
@@ -346,15 +353,24 @@ int run_sep(const SepParams &p, hipStream_t st) {
     // 18.1, 16.0, 17.1), 10 for three or more (an 11x11 u8 BGR blur: 6 rows
     // 31.7 us, 8 rows 30.0, 10 rows 28.1, 12 rows 29.1).
     int th = tuning().sep_rows ? tuning().sep_rows : cn >= 3 ? 10 : 6;
-    auto need = [&](int tw, int t) {
-        return (((size_t)(t + 2 * ry) * sizeof(int) + 15) & ~(size_t)15) + (size_t)t * (tw + 2 * rx) * cn * sizeof(VT) +
-               (size_t)(t + 2 * ry) * (tw + 2 * rx) * cn * sizeof(S);
+    // LDS bytes of a tile (the row table, the vertical values, the staged
+    // rows), as sep_tile lays them out: `one` when its ONE form applies.
+    auto need_as = [&](int tw, int t, bool one) {
+        const size_t pitch = one ? (size_t)kThreads : (size_t)(tw + 2 * rx) * cn;
+        const size_t st_size = one && F32 ? sizeof(float) : sizeof(S);
+        return (((size_t)(t + 2 * ry) * sizeof(int) + 15) & ~(size_t)15) + (size_t)t * pitch * sizeof(VT) +
+               (size_t)(t + 2 * ry) * pitch * st_size;
     };
+    auto need = [&](int tw, int t) { return need_as(tw, t, false); };
     while (need(twpx, th) > lds_cap && th > 1) th /= 2;
     while (need(twpx, th) > lds_cap && twpx > 1) twpx /= 2;
     if (need(twpx, th) <= lds_cap) {
         dim3 grid((unsigned)((p.src.w + twpx - 1) / twpx), (unsigned)((p.src.h + th - 1) / th));
-        hipLaunchKernelGGL((sep_tile<S, D, VT, AT, F32>), grid, dim3(kThreads), need(twpx, th), st, p, twpx, th);
+        const bool one = cn == 1 && twpx + 2 * rx <= kThreads && need_as(twpx, th, true) <= lds_cap;
+        if (one)
+            hipLaunchKernelGGL((sep_tile<S, D, VT, AT, F32, true>), grid, dim3(kThreads), need_as(twpx, th, true), st, p, twpx, th);
+        else
+            hipLaunchKernelGGL((sep_tile<S, D, VT, AT, F32, false>), grid, dim3(kThreads), need(twpx, th), st, p, twpx, th);
     } else {
         long n = (long)p.src.w * cn;
         dim3 grid((unsigned)((n + kThreads - 1) / kThreads), (unsigned)p.src.h);

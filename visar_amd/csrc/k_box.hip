@@ -135,21 +135,33 @@ __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, in
     // of neighbouring lanes start kBoxRun (odd) words apart in a plane, so
     // their LDS reads fall in different banks.
     const int tw_row = min(tw, W - x0), runs_x = (tw_row + kBoxRun - 1) / kBoxRun, per_row = runs_x * CN;
-    for (int q = threadIdx.x; q < rows * per_row; q += kBoxThreads) {
-        const int t = q / per_row, r = q - t * per_row, c = r / runs_x, run = r - c * runs_x;
-        const uint32_t *row = colsum + t * plane + c * pitch;
-        const int ox0 = run * kBoxRun;
-        uint32_t sum = 0;
-        for (int i = 0; i < kw; i++) sum += row[ox0 + i];
-        const gptr<uint8_t> out = grow<uint8_t>(d, y0 + t) + (long)x0 * CN + c;
-        for (int k = 0; k < kBoxRun && ox0 + k < tw_row; k++) {
-            const int ox = ox0 + k;
-            if (k) sum += row[ox + kw - 1] - row[ox - 1];
-            uint8_t m = sat_v<uint8_t>((float)sum * scale);
-            if (adapt.enabled) m = adapt_out(adapt.idelta, adapt.on, adapt.off, grow<const uint8_t>(s, y0 + t)[(long)(x0 + ox) * CN + c], m);
-            out[(long)ox * CN] = m;
+    // Windows up to 15 columns unrolled (constant LDS offsets).
+    with_odd_len<15>(kw, [&](auto len) {
+        constexpr int KW = decltype(len)::value;
+        const int n = KW > 0 ? KW : kw;
+        for (int q = threadIdx.x; q < rows * per_row; q += kBoxThreads) {
+            const int t = q / per_row, r = q - t * per_row, c = r / runs_x, run = r - c * runs_x;
+            const int ox0 = run * kBoxRun;
+            const uint32_t *row = colsum + t * plane + c * pitch + ox0;
+            uint32_t sum = 0;
+            if constexpr (KW > 0) {
+#pragma unroll
+                for (int i = 0; i < KW; i++) sum += row[i];
+            } else {
+                for (int i = 0; i < n; i++) sum += row[i];
+            }
+            const gptr<uint8_t> out = grow<uint8_t>(d, y0 + t) + (long)(x0 + ox0) * CN + c;
+#pragma unroll
+            for (int k = 0; k < kBoxRun; k++) {
+                if (ox0 + k >= tw_row) break;
+                if (k) sum += row[k + n - 1] - row[k - 1];
+                uint8_t m = sat_v<uint8_t>((float)sum * scale);
+                if (adapt.enabled)
+                    m = adapt_out(adapt.idelta, adapt.on, adapt.off, grow<const uint8_t>(s, y0 + t)[(long)(x0 + ox0 + k) * CN + c], m);
+                out[(long)k * CN] = m;
+            }
         }
-    }
+    });
 }
 
 inline int launched() { return hipGetLastError() == hipSuccess ? OK : INTERNAL; }

@@ -9,7 +9,7 @@
 // of kh rows and then the rows that enter and leave the window, all before
 // using them, and writes the column sums of the tile's rows to LDS (running
 // sums: exact in u32), one plane per channel. Horizontal pass, windows up
-// to 11 (one channel) or 7 (more): row by row, one lane per output element
+// to 7 (11 for adaptiveThreshold on one channel): row by row, one lane per output element
 // sums its kw column sums from LDS and stores its byte (consecutive lanes,
 // consecutive bytes); wider windows: one lane per run of 7 outputs, a
 // running sum along the run. Window lengths up to 15 rows and up to the
@@ -27,11 +27,13 @@ constexpr int kBoxThreads = 256;
 constexpr int kBoxTH = 16;  // output rows per tile
 constexpr int kBoxRun = 7;  // outputs per lane for wide windows (odd: conflict-free LDS reads)
 // Widest window summed directly, one lane per output; wider ones in runs.
-// Measured on 1080p: gray direct up to 11 (adaptive mean 11: 9.7 us direct,
-// 10.3 runs), BGR up to 7 (7x7: 12.4 direct, 14.7 runs; 11x11: 15.3 direct,
-// 14.0 runs).
+// Measured on 1080p with both paths unrolled: gray up to 7 for a plain box
+// (11x11: 7.7 us direct, 7.2 runs) and up to 11 with adaptiveThreshold's
+// compare, whose source reads are strided along a run (adaptive mean 11:
+// 9.2 direct, 10.0 runs); BGR up to 7 (7x7: 12.7 direct, 13.5 runs; 11x11:
+// 15.3 direct, 13.1 runs).
 template <int CN>
-constexpr int kBoxDirect = CN == 1 ? 11 : 7;
+constexpr int kBoxDirect = CN == 1 ? 11 : 7;  // widest compiled direct path
 
 template <int CN>
 __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, int kh, int border, float scale, int tw,
@@ -103,7 +105,8 @@ __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, in
     }
     __syncthreads();
     const int n_row = min(tw, W - x0) * CN, rows = min(kBoxTH, H - y0);
-    if (kw <= kBoxDirect<CN>) {
+    const int direct_max = CN == 1 && !adapt.enabled ? 7 : kBoxDirect<CN>;
+    if (kw <= direct_max) {
         // One lane per output element of a row: its kw column sums from
         // LDS, the byte stored next to its neighbours' (consecutive lanes,
         // consecutive bytes).

@@ -197,8 +197,9 @@ def bench_batch(args, torch, va):
     }
     if not args.no_cpu:
         import oracle
+        from oracle import pipelines
         line["cpu_baseline"] = cpu_measure(
-            args.cpu_seconds, lambda k: (lambda n: batch.run_oracle(oracle, jobs[(k * 37 + n) % len(jobs)])), 1,
+            args.cpu_seconds, lambda k: (lambda n: pipelines.batch_item(oracle, jobs[(k * 37 + n) % len(jobs)])), 1,
             "images/s", "images of the batch")
     print(json.dumps(line))
 
@@ -557,20 +558,22 @@ def main():
 
     # Graphs of G consecutive steps (input slots taken mod R, R a multiple of
     # G), so the GPU and not the host's launch rate sets the pace: K steps
-    # replay K // G chunk graphs and then K % G single-step graphs.
+    # replay K // G chunk graphs and then one graph of the K % G remaining
+    # steps, captured the same way, so every timed step runs with the same
+    # frames in flight whatever K is.
     # With L lanes, step j runs on lane j % L: lane 0 is the capture stream,
     # the others are streams forked from it inside the graph and joined at
     # its end, so L frames are in flight. Step j writes output set j % NB
     # (NB = L, at least 2), so the lanes never share a buffer.
+    K = args.steps
     G = 24
     n_chunks = R // G
-    chunks, singles = [], []
     cap_stream = torch.cuda.Stream()
     cap_stream.wait_stream(stream)
     side = [torch.cuda.Stream() for _ in range(args.lanes - 1)]
 
     def capture(indices):
-        lanes = [cap_stream] + side if len(indices) > 1 else [cap_stream]
+        lanes = ([cap_stream] + side)[:max(1, min(len(indices), args.lanes))]
         if len(lanes) > 1:
             fork = torch.cuda.Event()
             fork.record(cap_stream)
@@ -584,40 +587,49 @@ def main():
         va.set_stream(cap_stream)
         for st in lanes[1:]:
             cap_stream.wait_stream(st)
+        return len(lanes)
 
+    # Chunk graph c holds steps c*G .. c*G + G - 1; the remainder graph
+    # continues from step (K // G) * G, so consecutive graphs take
+    # consecutive input slots.
+    plan =[list(range(c * G, c * G + G)) for c in range(min(n_chunks, max(1, K // G)))] if K >= G else []
+    rem = K % G
+    rem_graph = None
+    lanes_used = set()
     with torch.cuda.stream(cap_stream):
         va.set_stream(cap_stream)
-        for c in range(n_chunks):
+        chunks = []
+        for idx in plan:
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g, stream=cap_stream):
-                capture(list(range(c * G, c * G + G)))
+                lanes_used.add(capture(idx))
             chunks.append(g)
-        for i in range(G):
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g, stream=cap_stream):
-                capture([i])
-            singles.append(g)
+        if rem:
+            first = ((K // G) * G) % R
+            rem_graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(rem_graph, stream=cap_stream):
+                lanes_used.add(capture(list(range(first, first + rem))))
     va.set_stream(stream)
     stream.wait_stream(cap_stream)
     torch.cuda.synchronize()
-    for g in chunks + singles:  # graph warmup
+    for g in chunks + ([rem_graph] if rem_graph else []):  # graph warmup
         g.replay()
     torch.cuda.synchronize()
 
-    K = args.steps
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0.record(stream)
     for c in range(K // G):
-        chunks[c % n_chunks].replay()
-    for i in range(K % G):
-        singles[i].replay()
+        chunks[c % len(chunks)].replay()
+    if rem_graph is not None:
+        rem_graph.replay()
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = e0.elapsed_time(e1)
     ms_per_step = max(gpu_ms, wall * 1e3) / K
+    graph_steps = [G] * (K // G) + ([rem] if rem else [])
     value = pix_per_step / (ms_per_step * 1e-3) / 1e9
 
     # Per-kernel durations, live, with HIP events on the launch stream.
@@ -649,7 +661,8 @@ def main():
         "dtype": "u8",
         "data": "synthetic (splitmix64 uniform bytes, 24 rotating device frames)",
         "config": {"workload": workload, "width": W, "height": H, "rotating_inputs": R, "graph_replay": True,
-                   "steps_per_graph": G, "frames_in_flight": args.lanes},
+                   "graph_steps": sorted(set(graph_steps)), "graphs_replayed": len(graph_steps),
+                   "frames_in_flight": max(lanes_used)},
         "kernels": kernels,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

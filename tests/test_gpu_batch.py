@@ -24,6 +24,7 @@ def va():
 
 
 def _compare_batch(va, orc, n, lo, hi, seed, oracle_every, use_graph):
+    from oracle import pipelines
     from visar_amd import batch
     jobs = batch.workload(seed, n, lo, hi)
     b = batch.Batch().plan(batch.items(jobs))
@@ -39,7 +40,7 @@ def _compare_batch(va, orc, n, lo, hi, seed, oracle_every, use_graph):
         got = j["dst"].cpu().numpy()
         assert np.array_equal(got, plain.cpu().numpy()), f"item {i} {j['spec']} differs from the plain call"
         if i % oracle_every == 0:
-            assert np.array_equal(got, batch.run_oracle(orc, j)), f"item {i} {j['spec']} differs from the oracle"
+            assert np.array_equal(got, pipelines.batch_item(orc, j)), f"item {i} {j['spec']} differs from the oracle"
 
 
 def test_batch_small_mixed_graph(va, orc):

@@ -62,12 +62,13 @@ def run_streams(srv, streams, interleave=True):
 
 @pytest.mark.parametrize("w,h", [(301, 197), (64, 33), (5, 3), (1, 1), (257, 17)])
 def test_serve_matches_oracle_pipeline(va, orc, w, h):
-    from visar_amd.serve import Server, pipeline_oracle
+    from oracle.pipelines import serve_front_end
+    from visar_amd.serve import Server
     streams = [frames_for(s, 6, w, h) for s in range(3)]
     with Server(width=w, height=h, max_batch=2, queue_depth=3, min_area=5) as srv:
         out = run_streams(srv, streams)
     for fl, (masks, rs) in zip(streams, out):
-        want = pipeline_oracle(orc, fl)
+        want = serve_front_end(orc, fl)
         for k, (m, wm) in enumerate(zip(masks, want)):
             assert np.array_equal(m, wm), f"{w}x{h} frame {k}: {(m != wm).sum()} pixels differ"
             assert rs[k]["components"] == orc.connected_components_with_stats(wm)[0]
@@ -123,3 +124,61 @@ def test_serve_admission_by_budget(va):
         srv.submit(c, f[0])
         srv.drain()
         assert len(srv.poll(100)) == 5
+
+
+def test_serve_threaded_admit_submit_release(va):
+    """Four threads share a server of three stream slots. Each repeatedly
+    admits a stream (waiting while every slot is taken), submits the same four
+    frames and releases it, while the others do the same: every frame
+    completes, each admission gives the same per-frame checksums (a reused
+    slot starts a fresh model), and no slot or byte is left behind."""
+    import threading
+    import time
+
+    from visar_amd.serve import Server
+    frames = frames_for(3, 4, 160, 90)
+    errors = []
+    reps, nthreads = 5, 4
+    with Server(width=160, height=90, max_streams=3, queue_depth=2, max_batch=4, min_area=5) as srv:
+        def worker(t):
+            try:
+                for rep in range(reps):
+                    sid = srv.try_admit()
+                    while sid is None:
+                        time.sleep(0.001)
+                        sid = srv.try_admit()
+                    for k, f in enumerate(frames):
+                        srv.submit(sid, f, tag=(t * 100 + rep) * 10 + k)
+                    srv.release(sid)
+            except Exception as e:  # reported below
+                errors.append(repr(e))
+
+        ts = [threading.Thread(target=worker, args=(t,)) for t in range(nthreads)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join(timeout=60)
+        assert not any(t.is_alive() for t in ts), "a thread did not finish"
+        assert not errors, errors
+        srv.drain()
+        res = []
+        while True:
+            got = srv.poll(1024)
+            if not got:
+                break
+            res += got
+        st = srv.stats()
+    assert len(res) == reps * nthreads * len(frames)
+    assert all(r["status"] == 0 for r in res)
+    by_run = {}
+    for r in res:
+        by_run.setdefault(r["tag"] // 10, {})[r["tag"] % 10] = (r["frame_index"], r["checksum"])
+    want = None
+    for run in by_run.values():
+        assert sorted(run) == [0, 1, 2, 3]
+        assert [run[k][0] for k in range(4)] == [0, 1, 2, 3]
+        seq = [run[k][1] for k in range(4)]
+        want = seq if want is None else want
+        assert seq == want, "an admission on a reused slot gave other results"
+    assert st["active_streams"] == 0 and st["device_bytes"] == 0
+    assert st["admitted"] == reps * nthreads

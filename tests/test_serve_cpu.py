@@ -62,7 +62,8 @@ def test_oracle_pipeline_semantics(serve, orc):
     bg = np.full((h, w), 100, np.uint8)
     frames = [bg.copy() for _ in range(3)]
     frames[2][10:20, 20:35] = 220
-    masks = serve.pipeline_oracle(orc, frames)
+    from oracle.pipelines import serve_front_end
+    masks = serve_front_end(orc, frames)
     assert not masks[0].any()                               # first frame primes the model
     assert not masks[1].any()                               # static scene
     m = masks[2]

@@ -77,30 +77,33 @@ def _tdepth(t):
     return _TORCH_DEPTH[t.dtype]
 
 
-def desc(a):
-    """visar_image describing a NumPy array or torch tensor (rows may be strided; pixels must be packed)."""
-    if a.ndim == 2:
-        h, w = a.shape
-        c = 1
-    elif a.ndim == 3:
-        h, w, c = a.shape
-    else:
-        raise ValueError("expected 2-D or 3-D array")
+def _geometry(a):
+    """(shape, element bytes, byte strides, depth code, address) of a NumPy
+    array or a torch tensor."""
     if _is_tensor(a):
         es = a.element_size()
-        st = a.stride()
-        if st[1] != c or (a.ndim == 3 and st[2] != 1):
-            raise ValueError("pixels must be packed")
-        stride = st[0] * es if h > 1 else w * c * es
-        return Image(w, h, c, _tdepth(a), stride, a.data_ptr())
-    t = a.dtype.type
-    if t not in _DEPTH:
+        return tuple(a.shape), es, tuple(k * es for k in a.stride()), _tdepth(a), a.data_ptr()
+    code = _DEPTH.get(a.dtype.type)
+    if code is None:
         raise ValueError(f"unsupported dtype {a.dtype}")
-    if a.strides[-1] != a.itemsize or (a.ndim == 3 and a.strides[1] != a.itemsize * c):
-        raise ValueError("pixels must be packed")
-    # NumPy reports stride 0 for a single row; the row length is the stride then.
-    stride = a.strides[0] if h > 1 else w * c * a.itemsize
-    return Image(w, h, c, _DEPTH[t], stride, a.ctypes.data)
+    return a.shape, a.itemsize, a.strides, code, a.ctypes.data
+
+
+def desc(a):
+    """The visar_image of an (H, W) or (H, W, C) array or tensor. Rows may be
+    padded (any byte stride); within a row, pixels and channels are packed."""
+    shape, es, steps, depth, addr = _geometry(a)
+    if len(shape) not in (2, 3):
+        raise ValueError("expected 2-D or 3-D array")
+    h, w = shape[0], shape[1]
+    c = shape[2] if len(shape) == 3 else 1
+    # Packed: the pixel step is one pixel and the channel step one element;
+    # a step along an axis of extent 1 is never taken and not checked.
+    for extent, step, packed in zip(shape[1:], steps[1:], (c * es, es)):
+        if extent > 1 and step != packed:
+            raise ValueError("pixels must be packed")
+    row = w * c * es
+    return Image(w, h, c, depth, steps[0] if h > 1 else row, addr)
 
 
 _tls = threading.local()
@@ -133,13 +136,12 @@ def _empty(like, shape, dtype=None):
     return np.empty(shape, dtype=like.dtype if dtype is None else dtype)
 
 
-def _shape(a, w=None, h=None, c=None):
-    H, W = a.shape[:2]
-    C = a.shape[2] if a.ndim == 3 else 1
-    w = W if w is None else w
-    h = H if h is None else h
-    c = C if c is None else c
-    return (h, w) if c == 1 else (h, w, c)
+def _like(a, size=None, channels=None):
+    """Shape of an output like `a`: `size` (width, height) and `channels`
+    replace a's own; one channel gives a 2-D shape."""
+    w, h = (a.shape[1], a.shape[0]) if size is None else size
+    c = (a.shape[2] if a.ndim == 3 else 1) if channels is None else channels
+    return (h, w, c) if c > 1 else (h, w)
 
 
 # ------------------------------------------------------------ library
@@ -198,13 +200,13 @@ _COLOR_CN = {6: (3, 1), 7: (3, 1), 10: (4, 1), 11: (4, 1)}
 
 
 def cvt_color(a, code, dst=None):
-    dst = _empty(a, _shape(a, c=_COLOR_CN[code][1])) if dst is None else dst
+    dst = _empty(a, _like(a, channels=_COLOR_CN[code][1])) if dst is None else dst
     _check(lib.visar_cvt_color(_ref(a), _ref(dst), code))
     return dst
 
 
 def resize(a, size, interpolation=1, dst=None):
-    dst = _empty(a, _shape(a, w=size[0], h=size[1])) if dst is None else dst
+    dst = _empty(a, _like(a, size)) if dst is None else dst
     _check(lib.visar_resize(_ref(a), _ref(dst), interpolation))
     return dst
 
@@ -246,21 +248,21 @@ def morphology(a, op, shape, ksize, iterations=1, dst=None):
 
 
 def warp_affine(a, m, size, interpolation=1, border=0, value=0.0, dst=None):
-    dst = _empty(a, _shape(a, w=size[0], h=size[1])) if dst is None else dst
+    dst = _empty(a, _like(a, size)) if dst is None else dst
     arr = (ctypes.c_double * 6)(*np.asarray(m, np.float64).ravel()[:6])
     _check(lib.visar_warp_affine(_ref(a), _ref(dst), arr, interpolation, border, ctypes.c_double(value)))
     return dst
 
 
 def warp_perspective(a, m, size, interpolation=1, border=0, value=0.0, dst=None):
-    dst = _empty(a, _shape(a, w=size[0], h=size[1])) if dst is None else dst
+    dst = _empty(a, _like(a, size)) if dst is None else dst
     arr = (ctypes.c_double * 9)(*np.asarray(m, np.float64).ravel()[:9])
     _check(lib.visar_warp_perspective(_ref(a), _ref(dst), arr, interpolation, border, ctypes.c_double(value)))
     return dst
 
 
 def remap(a, mapx, mapy, value=0.0, dst=None):
-    dst = _empty(a, _shape(a, w=mapx.shape[1], h=mapx.shape[0])) if dst is None else dst
+    dst = _empty(a, _like(a, (mapx.shape[1], mapx.shape[0]))) if dst is None else dst
     _check(lib.visar_remap(_ref(a), _ref(dst), _ref(mapx), _ref(mapy), ctypes.c_double(value)))
     return dst
 
@@ -274,14 +276,14 @@ def box_filter(a, ksize, normalize=True, border=4, dst=None):
 
 def pyr_down(a, border=4, dst=None):
     """pyramid.zig:65 pyrDown (tests/py/visar_ctypes.py:133): dst ((w + 1) // 2, (h + 1) // 2)."""
-    dst = _empty(a, _shape(a, w=(a.shape[1] + 1) // 2, h=(a.shape[0] + 1) // 2)) if dst is None else dst
+    dst = _empty(a, _like(a, ((a.shape[1] + 1) // 2, (a.shape[0] + 1) // 2))) if dst is None else dst
     _check(lib.visar_pyr_down(_ref(a), _ref(dst), border))
     return dst
 
 
 def pyr_up(a, border=4, dst=None):
     """pyramid.zig:403 pyrUp (tests/py/visar_ctypes.py:139): dst (2w, 2h), reflect-101 only."""
-    dst = _empty(a, _shape(a, w=a.shape[1] * 2, h=a.shape[0] * 2)) if dst is None else dst
+    dst = _empty(a, _like(a, (a.shape[1] * 2, a.shape[0] * 2))) if dst is None else dst
     _check(lib.visar_pyr_up(_ref(a), _ref(dst), border))
     return dst
 

@@ -130,19 +130,6 @@ def run_plain(va, j, dst=None):
     return va.morphology(src, kw["morph_op"], kw["shape"], (kw["ksize"], kw["ksize"]), kw["iterations"], dst=dst)
 
 
-def run_oracle(orc, j):
-    kw, src = j["kw"], j["src"].cpu().numpy()
-    op = kw["op"]
-    dh, dw = j["dst"].shape[:2]
-    if op == GAUSSIAN_BLUR:
-        return orc.gaussian_blur(src, (kw["ksize"], kw["ksize"]), kw["sigma"], border=kw["border"])
-    if op == RESIZE:
-        return orc.resize(src, (dw, dh), kw["interpolation"])
-    if op == WARP_AFFINE:
-        return orc.warp_affine(src, kw["m"], (dw, dh), kw["interpolation"], kw["border"], kw["value"])
-    return orc.morphology(src, kw["morph_op"], kw["shape"], (kw["ksize"], kw["ksize"]), kw["iterations"])
-
-
 def algorithmic_bytes(jobs):
     """Sum of every image's input and output bytes."""
     return int(sum(j["src"].numel() + j["dst"].numel() for j in jobs))

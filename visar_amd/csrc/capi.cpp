@@ -346,6 +346,104 @@ int Call::finish(int launch_status) {
     return OK;
 }
 
+// threshold.zig:184-207 getThreshVal_Otsu_8u
+double threshold_otsu(const uint32_t *hist, double total) {
+    const double scale = 1.0 / total;
+    const double eps = 2.220446049250313e-16;
+    double mu = 0;
+    for (int i = 0; i < 256; i++) mu += (double)i * (double)hist[i];
+    mu *= scale;
+    double mu1 = 0, q1 = 0, max_sigma = 0, max_val = 0;
+    for (int i = 0; i < 256; i++) {
+        double p_i = (double)hist[i] * scale;
+        mu1 *= q1;
+        q1 += p_i;
+        double q2 = 1.0 - q1;
+        if (std::min(q1, q2) < eps || std::max(q1, q2) > 1.0 - eps) continue;
+        mu1 = (mu1 + (double)i * p_i) / q1;
+        double mu2 = (mu - q1 * mu1) / q2;
+        double sigma = q1 * q2 * (mu1 - mu2) * (mu1 - mu2);
+        if (sigma > max_sigma) max_sigma = sigma, max_val = (double)i;
+    }
+    return max_val;
+}
+
+// threshold.zig:210-253 getThreshVal_Triangle_8u
+double threshold_triangle(const uint32_t *hist_in) {
+    int left = 0, right = 0, max_ind = 0;
+    uint32_t mx = 0;
+    for (int i = 0; i < 256; i++)
+        if (hist_in[i] > 0) {
+            left = i;
+            break;
+        }
+    if (left > 0) left -= 1;
+    for (int i = 255; i > 0; i--)
+        if (hist_in[i] > 0) {
+            right = i;
+            break;
+        }
+    if (right < 255) right += 1;
+    for (int i = 0; i < 256; i++)
+        if (hist_in[i] > mx) mx = hist_in[i], max_ind = i;
+    uint32_t h[256];
+    std::memcpy(h, hist_in, sizeof h);
+    bool flip = false;
+    if (max_ind - left < right - max_ind) {
+        flip = true;
+        std::reverse(h, h + 256);
+        left = 255 - right;
+        max_ind = 255 - max_ind;
+    }
+    int thresh = left;
+    double a = (double)mx, b = (double)(left - max_ind), dist = 0;
+    for (int j = left + 1; j <= max_ind; j++) {
+        double t = a * (double)j + b * (double)h[j];
+        if (t > dist) dist = t, thresh = j;
+    }
+    thresh -= 1;
+    if (flip) thresh = 255 - thresh;
+    return (double)thresh;
+}
+
+// threshold.zig:25-82: the per-depth form of (thresh, maxval) the kernel
+// applies; for u8 a 256-entry table of every type's result.
+void threshold_setup(ThreshParams &p, uint32_t T, int kind, double thresh, double maxval) {
+    p.depth = (int)T;
+    p.kind = kind;
+    if (T == 0) {
+        // threshold.zig:40-60: floored threshold; the LUT covers every case.
+        double fl = std::floor(thresh);
+        long ti = fl < -2147483648.0 ? -2147483648L : fl > 2147483647.0 ? 2147483647L : (long)fl;
+        int m = sat_cast<uint8_t>(maxval);
+        int tc = ti < 0 ? 0 : ti > 255 ? 255 : (int)ti;
+        p.ti = (int)(ti < -1 ? -1 : ti > 256 ? 256 : ti);
+        p.mi = m;
+        for (int i = 0; i < 256; i++) {
+            bool gt = i > ti;
+            int v;
+            switch (kind) {
+            case 0: v = gt ? m : 0; break;
+            case 1: v = gt ? 0 : m; break;
+            case 2: v = gt ? tc : i; break;
+            case 3: v = gt ? i : 0; break;
+            default: v = gt ? 0 : i; break;
+            }
+            p.lut[i] = (uint8_t)v;
+        }
+    } else if (T == 5) {
+        p.t = (double)(float)thresh;
+        p.m = (double)(float)maxval;
+    } else {
+        // threshold.zig:62-81: thresholds below the range make every value
+        // "above" (tozero copies, binary fills maxval, the rest give 0).
+        double lo = T == 2 ? 0 : -32768.0;
+        p.below = std::floor(thresh) < lo;
+        p.t = T == 2 ? (double)sat_cast<uint16_t>(std::floor(thresh)) : (double)sat_cast<int16_t>(std::floor(thresh));
+        p.m = T == 2 ? (double)sat_cast<uint16_t>(maxval) : (double)sat_cast<int16_t>(maxval);
+    }
+}
+
 }  // namespace visar
 
 using namespace visar;
@@ -876,66 +974,6 @@ int visar_init_undistort_rectify_map(const double k[9], const double *dist, size
     return call.finish(launch_undistort_map(p, call.stream()));
 }
 
-// threshold.zig:184-207 getThreshVal_Otsu_8u
-static double otsu(const uint32_t *hist, double total) {
-    const double scale = 1.0 / total;
-    const double eps = 2.220446049250313e-16;
-    double mu = 0;
-    for (int i = 0; i < 256; i++) mu += (double)i * (double)hist[i];
-    mu *= scale;
-    double mu1 = 0, q1 = 0, max_sigma = 0, max_val = 0;
-    for (int i = 0; i < 256; i++) {
-        double p_i = (double)hist[i] * scale;
-        mu1 *= q1;
-        q1 += p_i;
-        double q2 = 1.0 - q1;
-        if (std::min(q1, q2) < eps || std::max(q1, q2) > 1.0 - eps) continue;
-        mu1 = (mu1 + (double)i * p_i) / q1;
-        double mu2 = (mu - q1 * mu1) / q2;
-        double sigma = q1 * q2 * (mu1 - mu2) * (mu1 - mu2);
-        if (sigma > max_sigma) max_sigma = sigma, max_val = (double)i;
-    }
-    return max_val;
-}
-
-// threshold.zig:210-253 getThreshVal_Triangle_8u
-static double triangle(const uint32_t *hist_in) {
-    int left = 0, right = 0, max_ind = 0;
-    uint32_t mx = 0;
-    for (int i = 0; i < 256; i++)
-        if (hist_in[i] > 0) {
-            left = i;
-            break;
-        }
-    if (left > 0) left -= 1;
-    for (int i = 255; i > 0; i--)
-        if (hist_in[i] > 0) {
-            right = i;
-            break;
-        }
-    if (right < 255) right += 1;
-    for (int i = 0; i < 256; i++)
-        if (hist_in[i] > mx) mx = hist_in[i], max_ind = i;
-    uint32_t h[256];
-    std::memcpy(h, hist_in, sizeof h);
-    bool flip = false;
-    if (max_ind - left < right - max_ind) {
-        flip = true;
-        std::reverse(h, h + 256);
-        left = 255 - right;
-        max_ind = 255 - max_ind;
-    }
-    int thresh = left;
-    double a = (double)mx, b = (double)(left - max_ind), dist = 0;
-    for (int j = left + 1; j <= max_ind; j++) {
-        double t = a * (double)j + b * (double)h[j];
-        if (t > dist) dist = t, thresh = j;
-    }
-    thresh -= 1;
-    if (flip) thresh = 255 - thresh;
-    return (double)thresh;
-}
-
 // capi.zig:310-322 + threshold.zig:25-82
 int visar_threshold(const visar_image *src, const visar_image *dst, double thresh, double maxval, int type,
                     double *used) {
@@ -963,42 +1001,10 @@ int visar_threshold(const visar_image *src, const visar_image *dst, double thres
             hipMemcpyAsync(hist, hd, sizeof hist, hipMemcpyDeviceToHost, call.stream()) != hipSuccess ||
             hipStreamSynchronize(call.stream()) != hipSuccess)
             return fail(INTERNAL, "KernelFailed");
-        thresh = autom == 1 ? otsu(hist, (double)src->width * src->height) : triangle(hist);
+        thresh = autom == 1 ? threshold_otsu(hist, (double)src->width * src->height) : threshold_triangle(hist);
     }
     if (used) *used = thresh;
-    p.depth = (int)T;
-    p.kind = kind;
-    if (T == 0) {
-        // threshold.zig:40-60: floored threshold; the LUT covers every case.
-        double fl = std::floor(thresh);
-        long ti = fl < -2147483648.0 ? -2147483648L : fl > 2147483647.0 ? 2147483647L : (long)fl;
-        int m = sat_cast<uint8_t>(maxval);
-        int tc = ti < 0 ? 0 : ti > 255 ? 255 : (int)ti;
-        p.ti = (int)(ti < -1 ? -1 : ti > 256 ? 256 : ti);
-        p.mi = m;
-        for (int i = 0; i < 256; i++) {
-            bool gt = i > ti;
-            int v;
-            switch (kind) {
-            case 0: v = gt ? m : 0; break;
-            case 1: v = gt ? 0 : m; break;
-            case 2: v = gt ? tc : i; break;
-            case 3: v = gt ? i : 0; break;
-            default: v = gt ? 0 : i; break;
-            }
-            p.lut[i] = (uint8_t)v;
-        }
-    } else if (T == 5) {
-        p.t = (double)(float)thresh;
-        p.m = (double)(float)maxval;
-    } else {
-        // threshold.zig:62-81: thresholds below the range make every value
-        // "above" (tozero copies, binary fills maxval, the rest give 0).
-        double lo = T == 2 ? 0 : -32768.0;
-        p.below = std::floor(thresh) < lo;
-        p.t = T == 2 ? (double)sat_cast<uint16_t>(std::floor(thresh)) : (double)sat_cast<int16_t>(std::floor(thresh));
-        p.m = T == 2 ? (double)sat_cast<uint16_t>(maxval) : (double)sat_cast<int16_t>(maxval);
-    }
+    threshold_setup(p, T, kind, thresh, maxval);
     DImg d;
     if ((rc = call.output(dst, d))) return rc;
     p.d = d;

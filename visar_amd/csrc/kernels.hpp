@@ -79,6 +79,12 @@ struct ThreshParams {
     double t, m;  // threshold / maxval already converted to the sample type
 };
 int launch_threshold(const ThreshParams &p, hipStream_t st);
+// Host side of threshold (capi.cpp): the automatic thresholds of a 256-bin
+// histogram (threshold.zig:184-253) and the kernel's parameters for a
+// threshold and maxval of depth T (threshold.zig:25-82).
+double threshold_otsu(const uint32_t *hist, double total);
+double threshold_triangle(const uint32_t *hist);
+void threshold_setup(ThreshParams &p, uint32_t T, int kind, double thresh, double maxval);
 
 // video/track.zig:33-81 (primed update) and the priming pass.
 int launch_background(const DImg &frame, const DImg &mask, uint16_t *model, long model_stride, int shift, int thr,

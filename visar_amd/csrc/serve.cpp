@@ -54,13 +54,24 @@ struct Req {
     uint32_t batch_size = 0;
 };
 
+// A stream slot's life: kFree -> kAdmitting (admit makes its buffers
+// outside the lock) -> kActive -> kReleasing (release waits for its frames,
+// then frees the buffers outside the lock) -> kFree. Only kFree slots are
+// admitted and only kActive ones take frames, so an admit never reuses a
+// slot whose buffers are still being made or freed.
+enum SlotState { kFree, kAdmitting, kActive, kReleasing };
+
 struct Stream {
-    bool active = false;
+    SlotState state = kFree;
+    bool active = false;  // the dispatcher takes its queued frames
     void *dev = nullptr;  // frame[2] | bg[2] | mask[2]
     ServeStream d{};
     std::vector<uint8_t *> stage, hmask;  // pinned, queue_depth each
     std::vector<int> free_slots;
     std::deque<Req> queued;
+    // submitted counts a frame from the moment it takes a staging slot (its
+    // copy into the slot may still be running), so release and drain wait
+    // for it.
     uint64_t submitted = 0, dispatched = 0, finished = 0;
 };
 
@@ -377,12 +388,13 @@ int visar_serve_admit(visar_serve *s, int32_t *stream_id) {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->sticky) return fail(s->sticky, "ServerFailed");
         for (slot = 0; slot < s->streams.size(); slot++)
-            if (!s->streams[slot].active && !s->streams[slot].dev) break;
+            if (s->streams[slot].state == kFree) break;
         if (slot == s->streams.size() || (s->cfg.hbm_budget && s->used_bytes + s->per_stream > s->cfg.hbm_budget)) {
             s->rejected++;
             return fail(VISAR_OUT_OF_MEMORY, "HbmBudget");
         }
         s->used_bytes += s->per_stream;  // reserved while the buffers are made
+        s->streams[slot].state = kAdmitting;
     }
     Stream &sm = s->streams[slot];
     const uint32_t w = s->cfg.width, h = s->cfg.height;
@@ -408,14 +420,17 @@ int visar_serve_admit(visar_serve *s, int32_t *stream_id) {
             sm.free_slots.push_back((int)q);
         }
     }
-    std::lock_guard<std::mutex> lk(s->mu);
     if (!ok) {
         (void)hipGetLastError();
         free_stream_memory(sm);
+        std::lock_guard<std::mutex> lk(s->mu);
         s->used_bytes -= s->per_stream;
         s->rejected++;
+        sm.state = kFree;
         return fail(VISAR_OUT_OF_MEMORY, "OutOfMemory");
     }
+    std::lock_guard<std::mutex> lk(s->mu);
+    sm.state = kActive;
     sm.active = true;
     sm.submitted = sm.dispatched = sm.finished = 0;
     sm.queued.clear();
@@ -433,14 +448,18 @@ int visar_serve_submit(visar_serve *s, int32_t id, const uint8_t *frame, size_t 
     int slot;
     {
         std::unique_lock<std::mutex> lk(s->mu);
-        if (id < 0 || (size_t)id >= s->streams.size() || !s->streams[(size_t)id].active)
+        if (id < 0 || (size_t)id >= s->streams.size() || s->streams[(size_t)id].state != kActive)
             return fail(VISAR_INVALID_ARGUMENT, "UnknownStream");
         Stream &sm = s->streams[(size_t)id];
-        s->cv_slots.wait(lk, [&] { return s->stop || !sm.free_slots.empty(); });
+        s->cv_slots.wait(lk, [&] { return s->stop || sm.state != kActive || !sm.free_slots.empty(); });
         if (s->sticky) return fail(s->sticky, "ServerFailed");
         if (s->stop) return fail(VISAR_INVALID_ARGUMENT, "ServerStopped");
+        if (sm.state != kActive) return fail(VISAR_INVALID_ARGUMENT, "UnknownStream");
         slot = sm.free_slots.back();
         sm.free_slots.pop_back();
+        // Pending from here: release and drain wait for this frame.
+        sm.submitted++;
+        s->submitted++;
     }
     Stream &sm = s->streams[(size_t)id];
     uint8_t *dst = sm.stage[(size_t)slot];
@@ -454,8 +473,6 @@ int visar_serve_submit(visar_serve *s, int32_t id, const uint8_t *frame, size_t 
     r.mask_out = mask_out;
     r.mask_stride = mask_stride;
     sm.queued.push_back(r);
-    sm.submitted++;
-    s->submitted++;
     s->cv_dispatch.notify_all();
     return VISAR_OK;
 }
@@ -485,17 +502,25 @@ int visar_serve_drain(visar_serve *s) {
 
 int visar_serve_release(visar_serve *s, int32_t id) {
     if (!s) return fail(VISAR_INVALID_ARGUMENT, "InvalidArgument");
+    Stream gone;  // the slot's buffers, detached under the lock and freed outside it
     {
         std::unique_lock<std::mutex> lk(s->mu);
-        if (id < 0 || (size_t)id >= s->streams.size() || !s->streams[(size_t)id].active)
+        if (id < 0 || (size_t)id >= s->streams.size() || s->streams[(size_t)id].state != kActive)
             return fail(VISAR_INVALID_ARGUMENT, "UnknownStream");
         Stream &sm = s->streams[(size_t)id];
-        s->cv_results.wait(lk, [&] { return s->stop || sm.finished == sm.submitted; });
+        sm.state = kReleasing;   // no new frames; the queued ones still run
+        s->cv_slots.notify_all();
+        s->cv_results.wait(lk, [&] { return s->stop || s->sticky != VISAR_OK || sm.finished == sm.submitted; });
         sm.active = false;
+        std::swap(gone.dev, sm.dev);
+        gone.stage.swap(sm.stage);
+        gone.hmask.swap(sm.hmask);
+        gone.free_slots.swap(sm.free_slots);
+        sm.queued.clear();
     }
-    Stream &sm = s->streams[(size_t)id];
-    free_stream_memory(sm);
+    free_stream_memory(gone);
     std::lock_guard<std::mutex> lk(s->mu);
+    s->streams[(size_t)id].state = kFree;
     s->used_bytes -= s->per_stream;
     if (s->sticky) return fail(s->sticky, "ServerFailed");
     return VISAR_OK;

@@ -130,20 +130,6 @@ class Server:
 
 # ------------------------------------------------------ reference pipelines
 
-def pipeline_oracle(orc, frames, shift=4, threshold=28, mask_threshold=127):
-    """The detector front end through the oracle, frame by frame: masks."""
-    h, w = frames[0].shape
-    bg = orc.Background(w, h, shift)
-    out = []
-    for f in frames:
-        b = orc.gaussian_blur(f, (5, 5), 0)
-        m = np.empty_like(f)
-        bg.apply(b, m, threshold)
-        t = orc.threshold(m, mask_threshold, 255, 0)[1]
-        out.append(orc.morphology(t, 2, 0, (3, 3)))
-    return out
-
-
 class PlainPipeline:
     """The same front end through the plain C-ABI calls, one synchronous call
     after the other (each returns with its output written): the frame is

@@ -956,6 +956,40 @@ int orc_resize(const orc_image *src, const orc_image *dst, int interp) {
     return area_frac_u8(src, dst);
 }
 
+/* Test hooks: the generic u8 kernels whatever the dispatch would choose, so
+ * tests can hold them to the closed forms the reference's own tests tie them
+ * to (resize.zig:654-680 fast paths == generic LINEAR, resize.zig:703-722
+ * area kernels == areaGeneric), and the tap tables themselves. */
+int orc_resize_linear_generic(const orc_image *src, const orc_image *dst) {
+    if (!src || !dst || src->depth != 0 || dst->depth != 0 || src->channels != dst->channels) return ORC_INVALID_ARGUMENT;
+    int rc = check(src, 0);
+    if (rc || (rc = check(dst, 0))) return rc;
+    linear_u8(src, dst, 0);
+    return ORC_OK;
+}
+
+int orc_resize_area_generic(const orc_image *src, const orc_image *dst) {
+    if (!src || !dst || src->depth != 0 || dst->depth != 0 || src->channels != dst->channels) return ORC_INVALID_ARGUMENT;
+    if (dst->width > src->width || dst->height > src->height) return ORC_INVALID_ARGUMENT;
+    int rc = check(src, 0);
+    if (rc || (rc = check(dst, 0))) return rc;
+    return area_frac_u8(src, dst);
+}
+
+long orc_area_tab(uint32_t ssize, uint32_t dsz, uint32_t *di, uint32_t *si, float *alpha, long cap) {
+    if (dsz == 0 || ssize == 0 || dsz > ssize) return -1;
+    area_tab *t = (area_tab *)malloc(sizeof(area_tab) * (2 * (size_t)ssize + 2 * (size_t)dsz + 4));
+    if (!t) return -1;
+    long n = area_tab_build(t, ssize, dsz, (double)ssize / (double)dsz);
+    for (long k = 0; k < n && k < cap; k++) { di[k] = t[k].di; si[k] = t[k].si; alpha[k] = t[k].alpha; }
+    free(t);
+    return n;
+}
+
+void orc_linear_tap(long d, uint32_t ssize, uint32_t dsz, int area, int pin, long o[2], int w[2]) {
+    linear_tap(d, ssize, dsz, area, pin, o, w);
+}
+
 /* -------------------------------------------------------------- warps */
 
 static float toF_any(const orc_image *s, long y, long i) { return ldf(s, y, i); }

@@ -81,6 +81,14 @@ int orc_threshold(const orc_image *src, const orc_image *dst, double thresh, dou
 
 /* resize */
 int orc_resize(const orc_image *src, const orc_image *dst, int interp);
+/* Test hooks: generic u8 LINEAR (resize.zig:216-242) and fractional AREA
+ * (resize.zig:527-557) whatever the size ratio, the AREA table
+ * (resize.zig:470-493; returns the entry count, -1 on bad sizes) and one
+ * LINEAR tap (resize.zig:186-214). */
+int orc_resize_linear_generic(const orc_image *src, const orc_image *dst);
+int orc_resize_area_generic(const orc_image *src, const orc_image *dst);
+long orc_area_tab(uint32_t ssize, uint32_t dsize, uint32_t *di, uint32_t *si, float *alpha, long cap);
+void orc_linear_tap(long d, uint32_t ssize, uint32_t dsize, int area, int pin, long o[2], int w[2]);
 
 /* warps */
 int orc_warp_affine(const orc_image *src, const orc_image *dst, const double m[6], int interp, int border, double value);

@@ -410,6 +410,47 @@ def test_warp_affine_u8_box_cases(va, orc):
                 assert_same(va.warp_affine(dv, m, size, 1, 0, 17.0).cpu().numpy(), want, f"device {a.shape} {size}")
 
 
+def test_warp_u8_layouts_agree_on_views(va, orc):
+    """The one-channel u8 bilinear constant-border warps in both kernel
+    layouts (block layout with 8-byte tap loads, and the row / LDS-box
+    kernels): device views at odd offsets and strides, a view that ends at
+    its buffer's last byte (the 8-byte loads must stop there), and
+    perspective maps whose horizon crosses the image (w <= 0, the division's
+    slow path)."""
+    from visar_amd import autotune, synth
+    big = natural(260, 420, 1, 93)
+    dbig = dev(big)
+    cases = [(0, 0, 197, 301), (3, 1, 200, 305), (5, 7, 1, 1), (0, 0, 2, 3), (60, 113, 200, 307)]
+    persp = lambda w, h: ([synth.perspective_matrix(4, w, h)] if w > 1 and h > 1 else []) + [
+        np.array([[1, 0.1, 2], [0.05, 1, -3], [0.004, -0.003, 1]]), np.array([[1.2, 0, 0], [0, 1.2, 0], [0, 0.01, 0.4]])]
+    try:
+        for layout in (0, 1):
+            autotune.set_knob("warp_layout", layout)
+            for y0, x0, h, w in cases:
+                a = np.ascontiguousarray(big[y0:y0 + h, x0:x0 + w])
+                dv = dbig[y0:y0 + h, x0:x0 + w]
+                for m in [synth.affine_matrix(3, w, h), synth.rotation_matrix(w / 2, h / 2, -25, 0.85)]:
+                    want = orc.warp_affine(a, m, (w, h), 1, 0, 5.0)
+                    assert_same(va.warp_affine(dv, m, (w, h), 1, 0, 5.0).cpu().numpy(), want,
+                                f"affine layout {layout} view {(y0, x0, h, w)}")
+                for m in persp(w, h):
+                    for size in [(w, h), (w + 33, h + 9)]:
+                        want = orc.warp_perspective(a, m, size, 1, 0, 5.0)
+                        assert_same(va.warp_perspective(dv, m, size, 1, 0, 5.0).cpu().numpy(), want,
+                                    f"persp layout {layout} view {(y0, x0, h, w)} {m.tolist()} {size}")
+            # A packed image filling its whole buffer: the last row's 8-byte loads would pass its end.
+            t = dev(natural(37, 61, 1, 95))
+            tn = t.cpu().numpy()
+            for m in [np.array([[1, 0, 0.4], [0, 1, 0.3]]), np.array([[1, 0, 0.4], [0, 1, 0.3], [0, 0, 1]])]:
+                if m.shape[0] == 2:
+                    got, want = va.warp_affine(t, m, (61, 37), 1, 0, 3.0), orc.warp_affine(tn, m, (61, 37), 1, 0, 3.0)
+                else:
+                    got, want = va.warp_perspective(t, m, (61, 37), 1, 0, 3.0), orc.warp_perspective(tn, m, (61, 37), 1, 0, 3.0)
+                assert_same(got.cpu().numpy(), want, f"layout {layout} packed {m.tolist()}")
+    finally:
+        autotune.set_knob("warp_layout", 0)
+
+
 def test_warp_affine_1080p_config(va, orc):
     from visar_amd import synth
     a = synth.uniform_bytes(7, (1080, 1920))

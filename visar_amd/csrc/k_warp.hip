@@ -8,6 +8,8 @@
 // One thread computes one output pixel (all channels): stores are
 // coalesced along the row, the four taps are gathers that hit L2/MALL
 // because neighbouring threads read neighbouring source pixels.
+#include <cmath>
+
 #include "tiles.hpp"
 
 namespace visar {
@@ -174,6 +176,217 @@ __global__ __launch_bounds__(kThreads) void warp_affine_k(AffineParams p) {
     affine_px_tile<T>(p, blockIdx.x, blockIdx.y);
 }
 
+// ------------------------------------------------ block-layout warps
+//
+// One-channel u8 bilinear warpAffine / warpPerspective with BORDER_CONSTANT.
+// A wave owns 64 x 4 output pixels as four blocks of 16 x 4; lane l computes
+// pixel (16 j + l % 16, l / 16) of block j, so the 64 lanes of one gather
+// instruction cover a compact 16 x 4 patch and their taps a compact source
+// region under any rotation (one output row per wave spreads a rotated
+// instruction over many more source rows). A tap row's two samples come
+// from one 8-byte load at the dword-aligned address at or below the first
+// sample, aligned into place with v_alignbyte: half the load instructions of
+// byte gathers, whose per-instruction address work in the texture unit is
+// what bounds gather kernels. Results go through LDS so each lane stores 4
+// adjacent output bytes. A wave whose footprints are not all inside the
+// image (or whose 8-byte loads would pass the image's last byte) gathers
+// byte by byte with the constant-border rules instead.
+
+constexpr int kBlkW = 16, kBlkH = 4;  // pixels of one gather instruction
+
+struct BlkCoord {
+    int ix, iy;
+    float ax, ay;
+};
+
+// warpAffine bilinear blend (geometry.zig:508-533): fused, in this order.
+__device__ __attribute__((always_inline)) inline float blend_affine(float p00, float p01, float p10, float p11, float ax,
+                                                                    float ay) {
+    const float v0 = fmaf(ax, p01 - p00, p00);
+    const float v1 = fmaf(ax, p11 - p10, p10);
+    return fmaf(ay, v1 - v0, v0);
+}
+
+// warpPerspective bilinear blend (geometry.zig:660-668): f32 weights, the
+// sum ((0 + v0 w0) + v1 w1) + ... unfused. 0 + v0 w0 is v0 w0 exactly (a
+// product of non-negative values is +0 or positive), so the sum starts there.
+__device__ __attribute__((always_inline)) inline float blend_persp(float p00, float p01, float p10, float p11, float ax,
+                                                                   float ay) {
+    const float w0 = (1 - ax) * (1 - ay), w1 = ax * (1 - ay), w2 = (1 - ax) * ay, w3 = ax * ay;
+    float acc = p00 * w0;
+    acc += p01 * w1;
+    acc += p10 * w2;
+    acc += p11 * w3;
+    return acc;
+}
+
+typedef uint64_t u64_a4 __attribute__((aligned(4)));
+
+// Blends of the lane's 4 pixels. `lim` = (image base & 3) + (H - 1) * stride
+// + W: the bytes from the dword-aligned address below the image base to the
+// end of its last row (fits in 32 bits, checked by the host).
+template <bool PERSP>
+__device__ __attribute__((always_inline)) inline void blk_sample(const DImg &s, float cv, uint32_t lim,
+                                                                 const BlkCoord (&c)[4], float (&out)[4]) {
+    const int W = s.w, H = s.h;
+    const uint32_t st = (uint32_t)s.stride;
+    const uint32_t b3 = (uint32_t)((uintptr_t)s.p & 3);
+    bool inner = true, none = true;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t o1 = (uint32_t)(c[j].iy + 1) * st + (uint32_t)c[j].ix + b3;
+        inner = inner && (unsigned)c[j].ix < (unsigned)(W - 1) && (unsigned)c[j].iy < (unsigned)(H - 1) &&
+                (o1 & ~3u) + 8 <= lim;
+        none = none && (c[j].ix < -1 || c[j].iy < -1 || c[j].ix >= W || c[j].iy >= H);
+    }
+    if (__all(none)) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) out[j] = cv;
+        return;
+    }
+    if (__all(inner)) {
+        const gptr<const uint8_t> base = gp(s.p - b3);
+        uint64_t r0[4], r1[4];
+        uint32_t sh[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t o0 = (uint32_t)c[j].iy * st + (uint32_t)c[j].ix + b3;
+            sh[j] = o0 & 3;
+            r0[j] = *(const gptr<const u64_a4>)(base + (o0 & ~3u));
+            r1[j] = *(const gptr<const u64_a4>)(base + ((o0 + st) & ~3u));
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            // Row y + 1 starts stride bytes on, so its first sample sits at
+            // byte (o0 + stride) & 3 of its own load.
+            const uint32_t s1 = (sh[j] + st) & 3;
+            const uint32_t t = __builtin_amdgcn_alignbyte((uint32_t)(r0[j] >> 32), (uint32_t)r0[j], sh[j]);
+            const uint32_t u = __builtin_amdgcn_alignbyte((uint32_t)(r1[j] >> 32), (uint32_t)r1[j], s1);
+            const float p00 = (float)(t & 0xff), p01 = (float)((t >> 8) & 0xff);
+            const float p10 = (float)(u & 0xff), p11 = (float)((u >> 8) & 0xff);
+            out[j] = PERSP ? blend_persp(p00, p01, p10, p11, c[j].ax, c[j].ay)
+                           : blend_affine(p00, p01, p10, p11, c[j].ax, c[j].ay);
+        }
+        return;
+    }
+    const gptr<const uint8_t> base = gp(s.p);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int x0 = c[j].ix, y0 = c[j].iy;
+        const bool vx0 = (unsigned)x0 < (unsigned)W, vx1 = (unsigned)(x0 + 1) < (unsigned)W;
+        const bool vy0 = (unsigned)y0 < (unsigned)H, vy1 = (unsigned)(y0 + 1) < (unsigned)H;
+        const uint32_t xa = (uint32_t)min(max(x0, 0), W - 1), xb = (uint32_t)min(max(x0 + 1, 0), W - 1);
+        const uint32_t ya = (uint32_t)min(max(y0, 0), H - 1) * st, yb = (uint32_t)min(max(y0 + 1, 0), H - 1) * st;
+        const float p00 = vx0 && vy0 ? (float)base[ya + xa] : cv;
+        const float p01 = vx1 && vy0 ? (float)base[ya + xb] : cv;
+        const float p10 = vx0 && vy1 ? (float)base[yb + xa] : cv;
+        const float p11 = vx1 && vy1 ? (float)base[yb + xb] : cv;
+        const float v = PERSP ? blend_persp(p00, p01, p10, p11, c[j].ax, c[j].ay)
+                              : blend_affine(p00, p01, p10, p11, c[j].ax, c[j].ay);
+        out[j] = x0 < -1 || y0 < -1 || x0 >= W || y0 >= H ? cv : v;
+    }
+}
+
+// Rounds the lane's 4 results, transposes them through the wave's 256 LDS
+// bytes (4 rows x 64 pixels) and stores 4 adjacent bytes per lane.
+__device__ __attribute__((always_inline)) inline void blk_store(const DImg &d, int bx0, int by0, const float (&v)[4],
+                                                                uint32_t *lds) {
+    const int l = (int)(threadIdx.x & 63);
+    uint8_t *row = reinterpret_cast<uint8_t *>(lds) + (l / kBlkW) * 64;
+#pragma unroll
+    for (int j = 0; j < 4; j++) row[kBlkW * j + (l % kBlkW)] = (uint8_t)pack_u8(v[j], 0, 0);
+    __syncthreads();
+    const uint32_t word = lds[l];
+    const int y = by0 + (l >> 4), x = bx0 + 4 * (l & 15);
+    if (y < d.h && x < d.w) store4_u8(d, x, y, word);
+}
+
+// Affine coordinates of pixel (x, y) (geometry.zig:432-494). The host runs
+// this kernel only when |sx| and |sy| stay below 1e9 for every pixel, where
+// toIdx's clamp changes nothing.
+__device__ __attribute__((always_inline)) inline BlkCoord blk_affine_coord(const float *m, int x, float bx, float by) {
+    const float xf = (float)x;
+    const float sx = fmaf(m[0], xf, bx), sy = fmaf(m[3], xf, by);
+    const float fx = floorf(sx), fy = floorf(sy);
+    return BlkCoord{(int)fx, (int)fy, sx - fx, sy - fy};
+}
+
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void warp_affine_blk_u8(AffineParams p, TileOrder o, uint32_t lim) {
+    __shared__ uint32_t lds[NW][64];
+    int bx, by;
+    if (!tile_of(o, bx, by)) return;
+    const int wv = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
+    const int bx0 = bx * 4 * kBlkW, by0 = (by * NW + wv) * kBlkH;
+    const int y = by0 + l / kBlkW;
+    const float yf = (float)y;
+    const float bxv = p.m[1] * yf + p.m[2], byv = p.m[4] * yf + p.m[5];
+    BlkCoord c[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) c[j] = blk_affine_coord(p.m, bx0 + kBlkW * j + l % kBlkW, bxv, byv);
+    float v[4];
+    blk_sample<false>(p.s, p.cv, lim, c, v);
+    blk_store(p.d, bx0, by0, v, lds[wv]);
+}
+
+// 1 / w in f64, bit for bit as the compiler's correctly rounded division
+// computes it where v_div_scale leaves its operands unscaled and
+// v_div_fixup passes the quotient through, which holds for 1 / w with
+// |w| in [2^-64, 2^64]: the reciprocal estimate, two Newton-Raphson steps
+// and one correction of the quotient, fused.
+__device__ __attribute__((always_inline)) inline double recip_f64(double w) {
+    double r = __builtin_amdgcn_rcp(w);
+    double e = fma(-w, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-w, r, 1.0);
+    r = fma(r, e, r);
+    const double q = fma(-w, r, 1.0);
+    return fma(q, r, r);
+}
+
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void warp_persp_blk_u8(PerspParams p, TileOrder o, uint32_t lim) {
+    __shared__ uint32_t lds[NW][64];
+    int bx, by;
+    if (!tile_of(o, bx, by)) return;
+    const int wv = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
+    const int bx0 = bx * 4 * kBlkW, by0 = (by * NW + wv) * kBlkH;
+    const double fy = (double)(by0 + l / kBlkW);
+    const double c1 = p.m[1] * fy, c4 = p.m[4] * fy, c7 = p.m[7] * fy;
+    double wq[4];
+    bool in_range = true;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const double fx = (double)(bx0 + kBlkW * j + l % kBlkW);
+        wq[j] = p.m[6] * fx + c7 + p.m[8];
+        const double a = fabs(wq[j]);
+        in_range = in_range && a >= 0x1p-64 && a <= 0x1p64;
+    }
+    const bool fast = __all(in_range);
+    BlkCoord c[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        // geometry.zig:625-645: f64 map, x0 = floor(clamp(sx, +-1e9)),
+        // ax = f32(sx - x0). Without the clamp where 1 / w is in range: a
+        // coordinate beyond 1e9 then only ever lands outside the image.
+        const double fx = (double)(bx0 + kBlkW * j + l % kBlkW);
+        double sx = p.m[0] * fx + c1 + p.m[2];
+        double sy = p.m[3] * fx + c4 + p.m[5];
+        const double iw = fast ? recip_f64(wq[j]) : (wq[j] != 0 ? 1.0 / wq[j] : 0.0);
+        sx *= iw;
+        sy *= iw;
+        if (!fast) {
+            sx = clampd(sx, -1e9, 1e9);
+            sy = clampd(sy, -1e9, 1e9);
+        }
+        const double x0f = floor(sx), y0f = floor(sy);
+        c[j] = BlkCoord{(int)x0f, (int)y0f, (float)(sx - x0f), (float)(sy - y0f)};
+    }
+    float v[4];
+    blk_sample<true>(p.s, (float)p.cvd, lim, c, v);
+    blk_store(p.d, bx0, by0, v, lds[wv]);
+}
+
 inline unsigned blocks_for(long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 inline int launched() { return hipGetLastError() == hipSuccess ? OK : INTERNAL; }
 
@@ -181,8 +394,40 @@ inline dim3 grid2d(int w, int h, int px) { return dim3((unsigned)((w + 64 * px -
 
 }  // namespace
 
+// The block kernels' 8-byte gathers address the source with 32-bit offsets
+// from the dword below its base: 0 if those do not fit.
+inline uint32_t blk_limit(const DImg &s) {
+    const uint64_t lim = ((uintptr_t)s.p & 3) + (uint64_t)(s.h - 1) * (uint64_t)s.stride + (uint64_t)s.w;
+    return lim + 8 < (1ull << 32) ? (uint32_t)lim : 0;
+}
+
+// Whether |m0 x + m1 y + m2| and |m3 x + m4 y + m5| stay below 1e9 over the
+// destination with every entry finite, so the affine coordinates need no clamp.
+inline bool affine_bounded(const AffineParams &p) {
+    for (int i = 0; i < 6; i++)
+        if (!std::isfinite(p.m[i])) return false;
+    const double w = p.d.w, h = p.d.h;
+    const double bx = std::fabs(p.m[0]) * w + std::fabs(p.m[1]) * h + std::fabs(p.m[2]);
+    const double by = std::fabs(p.m[3]) * w + std::fabs(p.m[4]) * h + std::fabs(p.m[5]);
+    return bx < 1e8 && by < 1e8;
+}
+
+inline bool persp_finite(const PerspParams &p) {
+    for (int i = 0; i < 9; i++)
+        if (!std::isfinite(p.m[i])) return false;
+    return std::isfinite(p.cvd);
+}
+
 int launch_warp_affine(const AffineParams &p, hipStream_t st) {
     const dim3 b(kTileThreads);
+    const uint32_t lim = blk_limit(p.s);
+    if (p.depth == 0 && p.d.cn == 1 && p.interp == 1 && p.border == BORDER_CONSTANT && lim && affine_bounded(p) &&
+        tuning().warp_layout != 1) {
+        constexpr int NW = 2;
+        const TileOrder o = tile_order((p.d.w + 63) / 64, (p.d.h + 4 * NW - 1) / (4 * NW), true);
+        hipLaunchKernelGGL(warp_affine_blk_u8<NW>, dim3(tile_grid(o.ntiles)), dim3(64 * NW), 0, st, p, o, lim);
+        return launched();
+    }
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1 && p.border == BORDER_CONSTANT) {
         const TileOrder o = tile_order((p.d.w + kWarpTW - 1) / kWarpTW, (p.d.h + kWarpTH - 1) / kWarpTH, true);
         hipLaunchKernelGGL(warp_affine_box_u8, dim3(tile_grid(o.ntiles)), b, 0, st, p, o);
@@ -206,6 +451,14 @@ int launch_warp_affine(const AffineParams &p, hipStream_t st) {
 
 int launch_warp_perspective(const PerspParams &p, hipStream_t st) {
     const dim3 b(kTileThreads);
+    const uint32_t lim = blk_limit(p.s);
+    if (p.depth == 0 && p.d.cn == 1 && p.interp == 1 && p.border == BORDER_CONSTANT && lim && persp_finite(p) &&
+        tuning().warp_layout != 1) {
+        constexpr int NW = 4;
+        const TileOrder o = tile_order((p.d.w + 63) / 64, (p.d.h + 4 * NW - 1) / (4 * NW), false);
+        hipLaunchKernelGGL(warp_persp_blk_u8<NW>, dim3(tile_grid(o.ntiles)), dim3(64 * NW), 0, st, p, o, lim);
+        return launched();
+    }
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1) {
         const TileOrder o = tile_order((p.d.w + 255) / 256, (p.d.h + 3) / 4, true);
         hipLaunchKernelGGL(warp_persp_lin4_u8, dim3(tile_grid(o.ntiles)), b, 0, st, p, o);

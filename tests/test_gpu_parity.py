@@ -411,9 +411,9 @@ def test_warp_affine_u8_box_cases(va, orc):
 
 
 def test_warp_u8_layouts_agree_on_views(va, orc):
-    """The one-channel u8 bilinear constant-border warps in both kernel
-    layouts (block layout with 8-byte tap loads, and the row / LDS-box
-    kernels): device views at odd offsets and strides, a view that ends at
+    """The one-channel u8 bilinear constant-border warps in every kernel
+    layout (block layout with 8-byte tap loads issued per pixel or after a
+    wave-wide footprint check, and the row / LDS-box kernels): device views at odd offsets and strides, a view that ends at
     its buffer's last byte (the 8-byte loads must stop there), and
     perspective maps whose horizon crosses the image (w <= 0, the division's
     slow path)."""
@@ -424,7 +424,7 @@ def test_warp_u8_layouts_agree_on_views(va, orc):
     persp = lambda w, h: ([synth.perspective_matrix(4, w, h)] if w > 1 and h > 1 else []) + [
         np.array([[1, 0.1, 2], [0.05, 1, -3], [0.004, -0.003, 1]]), np.array([[1.2, 0, 0], [0, 1.2, 0], [0, 0.01, 0.4]])]
     try:
-        for layout in (0, 1):
+        for layout in (0, 1, 2):
             autotune.set_knob("warp_layout", layout)
             for y0, x0, h, w in cases:
                 a = np.ascontiguousarray(big[y0:y0 + h, x0:x0 + w])

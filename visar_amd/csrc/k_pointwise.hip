@@ -17,9 +17,11 @@ constexpr int kThreads = 256;
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// 128-bit streaming load (read once: the nontemporal hint keeps it out of the caches' way).
+// 128-bit load of streamed input. A plain (temporal) load: measured on a
+// 3840x2160 BGR frame, the nontemporal hint made the gray kernel 8.5 us
+// against 6.7 us with plain loads.
 __device__ inline uint4 ld_stream(const void *p) {
-    u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+    const u32x4 v = *(const __attribute__((address_space(1))) u32x4 *)(p);
     return make_uint4(v.x, v.y, v.z, v.w);
 }
 
@@ -120,8 +122,9 @@ __global__ __launch_bounds__(kThreads) void gray_down_u8(DImg s, DImg d, int k, 
 
 // Rows 4-byte (BGR) or 16-byte (BGRA) aligned: a workgroup takes one output
 // row and up to kGdPix of its source pixels. Each thread reads one 4-pixel
-// chunk of each of the K rows with nontemporal vector loads (the frame is
-// read once) and sums every pixel's
+// chunk of each of the K rows with nontemporal vector loads (measured a
+// little faster here than plain loads, unlike the gray kernel) and sums
+// every pixel's
 // weighted luma down the rows with v_dot4_u32_u8 (the weights split into
 // high and low bytes: 1868 = 7*256 + 76, 9617 = 37*256 + 145,
 // 4899 = 19*256 + 35); the column sums go to LDS, and each output adds K

@@ -231,12 +231,12 @@ __device__ __attribute__((always_inline)) inline void blk_sample(const DImg &s, 
     const int W = s.w, H = s.h;
     const uint32_t st = (uint32_t)s.stride;
     const uint32_t b3 = (uint32_t)((uintptr_t)s.p & 3);
+    const uint32_t lim8 = lim - 8;  // o1 <= lim8: the 8-byte load at o1 & ~3 ends inside the image
     bool inner = true, none = true;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const uint32_t o1 = (uint32_t)(c[j].iy + 1) * st + (uint32_t)c[j].ix + b3;
-        inner = inner && (unsigned)c[j].ix < (unsigned)(W - 1) && (unsigned)c[j].iy < (unsigned)(H - 1) &&
-                (o1 & ~3u) + 8 <= lim;
+        inner = inner && (unsigned)c[j].ix < (unsigned)(W - 1) && (unsigned)c[j].iy < (unsigned)(H - 1) && o1 <= lim8;
         none = none && (c[j].ix < -1 || c[j].iy < -1 || c[j].ix >= W || c[j].iy >= H);
     }
     if (__all(none)) {
@@ -287,6 +287,69 @@ __device__ __attribute__((always_inline)) inline void blk_sample(const DImg &s, 
     }
 }
 
+// The same blends with every pixel's loads issued as soon as its own
+// coordinates are known, so they fly while the next pixel's coordinates are
+// computed (the coordinates of a projective map are most of the kernel's
+// arithmetic, and gating the loads on a vote over all four pixels would
+// hold every load until the last coordinate). Each pixel loads its two tap
+// rows at addresses clamped into the image; a pixel whose footprint is not
+// wholly inside (or whose second row's 8-byte load would pass the image's
+// last byte) is redone afterwards by the byte-gather rules, by the lanes
+// that have one. Needs W >= 2 and H >= 2.
+template <bool PERSP, typename CoordFn>
+__device__ __attribute__((always_inline)) inline void blk_sample_eager(const DImg &s, float cv, uint32_t lim,
+                                                                       CoordFn coord, BlkCoord (&c)[4],
+                                                                       float (&out)[4]) {
+    const int W = s.w, H = s.h;
+    const uint32_t st = (uint32_t)s.stride;
+    const uint32_t b3 = (uint32_t)((uintptr_t)s.p & 3);
+    const uint32_t lim8 = lim - 8;
+    const gptr<const uint8_t> base = gp(s.p - b3);
+    uint64_t r0[4], r1[4];
+    uint32_t o0[4];
+    bool ok[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        c[j] = coord(j);
+        const int cx = min(max(c[j].ix, 0), W - 2), cy = min(max(c[j].iy, 0), H - 2);
+        o0[j] = (uint32_t)cy * st + (uint32_t)cx + b3;
+        const uint32_t o1 = o0[j] + st;
+        ok[j] = cx == c[j].ix && cy == c[j].iy && o1 <= lim8;
+        r0[j] = *(const gptr<const u64_a4>)(base + (min(o0[j], lim8) & ~3u));
+        r1[j] = *(const gptr<const u64_a4>)(base + (min(o1, lim8) & ~3u));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t t = __builtin_amdgcn_alignbyte((uint32_t)(r0[j] >> 32), (uint32_t)r0[j], o0[j] & 3);
+        const uint32_t u = __builtin_amdgcn_alignbyte((uint32_t)(r1[j] >> 32), (uint32_t)r1[j], (o0[j] + st) & 3);
+        const float p00 = (float)(t & 0xff), p01 = (float)((t >> 8) & 0xff);
+        const float p10 = (float)(u & 0xff), p11 = (float)((u >> 8) & 0xff);
+        out[j] = PERSP ? blend_persp(p00, p01, p10, p11, c[j].ax, c[j].ay)
+                       : blend_affine(p00, p01, p10, p11, c[j].ax, c[j].ay);
+    }
+    const gptr<const uint8_t> img = gp(s.p);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (!ok[j]) {
+            const int x0 = c[j].ix, y0 = c[j].iy;
+            if (x0 < -1 || y0 < -1 || x0 >= W || y0 >= H) {
+                out[j] = cv;
+            } else {
+                const bool vx0 = (unsigned)x0 < (unsigned)W, vx1 = (unsigned)(x0 + 1) < (unsigned)W;
+                const bool vy0 = (unsigned)y0 < (unsigned)H, vy1 = (unsigned)(y0 + 1) < (unsigned)H;
+                const uint32_t xa = (uint32_t)min(max(x0, 0), W - 1), xb = (uint32_t)min(max(x0 + 1, 0), W - 1);
+                const uint32_t ya = (uint32_t)min(max(y0, 0), H - 1) * st, yb = (uint32_t)min(max(y0 + 1, 0), H - 1) * st;
+                const float p00 = vx0 && vy0 ? (float)img[ya + xa] : cv;
+                const float p01 = vx1 && vy0 ? (float)img[ya + xb] : cv;
+                const float p10 = vx0 && vy1 ? (float)img[yb + xa] : cv;
+                const float p11 = vx1 && vy1 ? (float)img[yb + xb] : cv;
+                out[j] = PERSP ? blend_persp(p00, p01, p10, p11, c[j].ax, c[j].ay)
+                               : blend_affine(p00, p01, p10, p11, c[j].ax, c[j].ay);
+            }
+        }
+    }
+}
+
 // Rounds the lane's 4 results, transposes them through the wave's 256 LDS
 // bytes (4 rows x 64 pixels) and stores 4 adjacent bytes per lane.
 __device__ __attribute__((always_inline)) inline void blk_store(const DImg &d, int bx0, int by0, const float (&v)[4],
@@ -311,7 +374,7 @@ __device__ __attribute__((always_inline)) inline BlkCoord blk_affine_coord(const
     return BlkCoord{(int)fx, (int)fy, sx - fx, sy - fy};
 }
 
-template <int NW>
+template <int NW, bool EAGER>
 __global__ __launch_bounds__(64 * NW) void warp_affine_blk_u8(AffineParams p, TileOrder o, uint32_t lim) {
     __shared__ uint32_t lds[NW][64];
     int bx, by;
@@ -322,17 +385,22 @@ __global__ __launch_bounds__(64 * NW) void warp_affine_blk_u8(AffineParams p, Ti
     const float yf = (float)y;
     const float bxv = p.m[1] * yf + p.m[2], byv = p.m[4] * yf + p.m[5];
     BlkCoord c[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) c[j] = blk_affine_coord(p.m, bx0 + kBlkW * j + l % kBlkW, bxv, byv);
     float v[4];
-    blk_sample<false>(p.s, p.cv, lim, c, v);
+    if (EAGER) {
+        blk_sample_eager<false>(p.s, p.cv, lim, [&](int j) { return blk_affine_coord(p.m, bx0 + kBlkW * j + l % kBlkW, bxv, byv); },
+                                c, v);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) c[j] = blk_affine_coord(p.m, bx0 + kBlkW * j + l % kBlkW, bxv, byv);
+        blk_sample<false>(p.s, p.cv, lim, c, v);
+    }
     blk_store(p.d, bx0, by0, v, lds[wv]);
 }
 
 // 1 / w in f64, bit for bit as the compiler's correctly rounded division
 // computes it where v_div_scale leaves its operands unscaled and
 // v_div_fixup passes the quotient through, which holds for 1 / w with
-// |w| in [2^-64, 2^64]: the reciprocal estimate, two Newton-Raphson steps
+// |w| in [2^-60, 2^60]: the reciprocal estimate, two Newton-Raphson steps
 // and one correction of the quotient, fused.
 __device__ __attribute__((always_inline)) inline double recip_f64(double w) {
     double r = __builtin_amdgcn_rcp(w);
@@ -344,7 +412,11 @@ __device__ __attribute__((always_inline)) inline double recip_f64(double w) {
     return fma(q, r, r);
 }
 
-template <int NW>
+// FAST: the host has found every |w| of the launch's pixels in [2^-60,
+// 2^60] with one sign (w is affine in x and y, so its corners bound it),
+// where recip_f64 equals the division; otherwise 1 / w (0 for w = 0) and
+// the reference's clamp of the coordinates to +-1e9.
+template <int NW, bool FAST, bool EAGER>
 __global__ __launch_bounds__(64 * NW) void warp_persp_blk_u8(PerspParams p, TileOrder o, uint32_t lim) {
     __shared__ uint32_t lds[NW][64];
     int bx, by;
@@ -353,37 +425,33 @@ __global__ __launch_bounds__(64 * NW) void warp_persp_blk_u8(PerspParams p, Tile
     const int bx0 = bx * 4 * kBlkW, by0 = (by * NW + wv) * kBlkH;
     const double fy = (double)(by0 + l / kBlkW);
     const double c1 = p.m[1] * fy, c4 = p.m[4] * fy, c7 = p.m[7] * fy;
-    double wq[4];
-    bool in_range = true;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const double fx = (double)(bx0 + kBlkW * j + l % kBlkW);
-        wq[j] = p.m[6] * fx + c7 + p.m[8];
-        const double a = fabs(wq[j]);
-        in_range = in_range && a >= 0x1p-64 && a <= 0x1p64;
-    }
-    const bool fast = __all(in_range);
-    BlkCoord c[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        // geometry.zig:625-645: f64 map, x0 = floor(clamp(sx, +-1e9)),
-        // ax = f32(sx - x0). Without the clamp where 1 / w is in range: a
-        // coordinate beyond 1e9 then only ever lands outside the image.
+    // geometry.zig:625-645: f64 map, x0 = floor(clamp(sx, +-1e9)),
+    // ax = f32(sx - x0). Without the clamp where 1 / w is in range: a
+    // coordinate beyond 1e9 then only ever lands outside the image.
+    auto coord = [&](int j) {
         const double fx = (double)(bx0 + kBlkW * j + l % kBlkW);
         double sx = p.m[0] * fx + c1 + p.m[2];
         double sy = p.m[3] * fx + c4 + p.m[5];
-        const double iw = fast ? recip_f64(wq[j]) : (wq[j] != 0 ? 1.0 / wq[j] : 0.0);
+        const double w = p.m[6] * fx + c7 + p.m[8];
+        const double iw = FAST ? recip_f64(w) : (w != 0 ? 1.0 / w : 0.0);
         sx *= iw;
         sy *= iw;
-        if (!fast) {
+        if (!FAST) {
             sx = clampd(sx, -1e9, 1e9);
             sy = clampd(sy, -1e9, 1e9);
         }
         const double x0f = floor(sx), y0f = floor(sy);
-        c[j] = BlkCoord{(int)x0f, (int)y0f, (float)(sx - x0f), (float)(sy - y0f)};
-    }
+        return BlkCoord{(int)x0f, (int)y0f, (float)(sx - x0f), (float)(sy - y0f)};
+    };
+    BlkCoord c[4];
     float v[4];
-    blk_sample<true>(p.s, (float)p.cvd, lim, c, v);
+    if (EAGER) {
+        blk_sample_eager<true>(p.s, (float)p.cvd, lim, coord, c, v);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) c[j] = coord(j);
+        blk_sample<true>(p.s, (float)p.cvd, lim, c, v);
+    }
     blk_store(p.d, bx0, by0, v, lds[wv]);
 }
 
@@ -398,7 +466,7 @@ inline dim3 grid2d(int w, int h, int px) { return dim3((unsigned)((w + 64 * px -
 // from the dword below its base: 0 if those do not fit.
 inline uint32_t blk_limit(const DImg &s) {
     const uint64_t lim = ((uintptr_t)s.p & 3) + (uint64_t)(s.h - 1) * (uint64_t)s.stride + (uint64_t)s.w;
-    return lim + 8 < (1ull << 32) ? (uint32_t)lim : 0;
+    return lim >= 8 && lim + 8 < (1ull << 32) ? (uint32_t)lim : 0;
 }
 
 // Whether |m0 x + m1 y + m2| and |m3 x + m4 y + m5| stay below 1e9 over the
@@ -425,7 +493,10 @@ int launch_warp_affine(const AffineParams &p, hipStream_t st) {
         tuning().warp_layout != 1) {
         constexpr int NW = 2;
         const TileOrder o = tile_order((p.d.w + 63) / 64, (p.d.h + 4 * NW - 1) / (4 * NW), true);
-        hipLaunchKernelGGL(warp_affine_blk_u8<NW>, dim3(tile_grid(o.ntiles)), dim3(64 * NW), 0, st, p, o, lim);
+        if (p.s.w >= 2 && p.s.h >= 2 && tuning().warp_layout != 2)
+            hipLaunchKernelGGL((warp_affine_blk_u8<NW, true>), dim3(tile_grid(o.ntiles)), dim3(64 * NW), 0, st, p, o, lim);
+        else
+            hipLaunchKernelGGL((warp_affine_blk_u8<NW, false>), dim3(tile_grid(o.ntiles)), dim3(64 * NW), 0, st, p, o, lim);
         return launched();
     }
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1 && p.border == BORDER_CONSTANT) {
@@ -456,7 +527,27 @@ int launch_warp_perspective(const PerspParams &p, hipStream_t st) {
         tuning().warp_layout != 1) {
         constexpr int NW = 4;
         const TileOrder o = tile_order((p.d.w + 63) / 64, (p.d.h + 4 * NW - 1) / (4 * NW), false);
-        hipLaunchKernelGGL(warp_persp_blk_u8<NW>, dim3(tile_grid(o.ntiles)), dim3(64 * NW), 0, st, p, o, lim);
+        // w over the launched pixels (the tiles cover x < 64 tiles_x, y < 16 tiles_y).
+        const double xs[2] = {0.0, 64.0 * o.tiles_x - 1}, ys[2] = {0.0, 16.0 * (o.ntiles / o.tiles_x) - 1};
+        // Every corner well away from 0 relative to the terms' size, so that
+        // the device's rounded w keeps the sign and range of the exact one.
+        const double scale = std::fabs(p.m[6]) * xs[1] + std::fabs(p.m[7]) * ys[1] + std::fabs(p.m[8]);
+        bool fast = true;
+        int sign = 0;
+        for (double x : xs)
+            for (double y : ys) {
+                const double w = p.m[6] * x + p.m[7] * y + p.m[8];
+                const int sg = w > 0 ? 1 : w < 0 ? -1 : 0;
+                fast = fast && sg != 0 && (sign == 0 || sg == sign) && std::fabs(w) >= 0x1p-60 &&
+                       std::fabs(w) <= 0x1p60 && std::fabs(w) >= 1e-9 * scale;
+                sign = sg;
+            }
+        const bool eager = p.s.w >= 2 && p.s.h >= 2 && tuning().warp_layout != 2;
+        const dim3 g(tile_grid(o.ntiles)), t(64 * NW);
+        if (fast && eager) hipLaunchKernelGGL((warp_persp_blk_u8<NW, true, true>), g, t, 0, st, p, o, lim);
+        else if (fast) hipLaunchKernelGGL((warp_persp_blk_u8<NW, true, false>), g, t, 0, st, p, o, lim);
+        else if (eager) hipLaunchKernelGGL((warp_persp_blk_u8<NW, false, true>), g, t, 0, st, p, o, lim);
+        else hipLaunchKernelGGL((warp_persp_blk_u8<NW, false, false>), g, t, 0, st, p, o, lim);
         return launched();
     }
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1) {

@@ -12,7 +12,7 @@ struct Tuning {
     int batch_unit_kb = 0;       // batched calls: KiB of input + output per workgroup
     int tile_order = 0;          // plain tile kernels: 0 each kernel's default, 1 row-major, 2 one band of tiles per XCD
     int sep_rows = 0;            // separable engine: output rows per tile (1-16)
-    int warp_layout = 0;         // u8 bilinear constant-border warps: 0 block layout, 1 the row / LDS-box kernels
+    int warp_layout = 0;         // u8 bilinear constant-border warps: 0 block layout, loads per pixel; 1 row / LDS-box kernels; 2 block layout, loads after a wave vote
 };
 Tuning &tuning();
 bool set_tuning(const char *key, int value);

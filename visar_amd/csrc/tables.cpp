@@ -24,7 +24,7 @@ const Knob kKnobs[] = {
     {"batch_unit_kb", &Tuning::batch_unit_kb, 0, 1024},
     {"tile_order", &Tuning::tile_order, 0, 2},
     {"sep_rows", &Tuning::sep_rows, 0, 16},
-    {"warp_layout", &Tuning::warp_layout, 0, 1},
+    {"warp_layout", &Tuning::warp_layout, 0, 2},
 };
 }  // namespace
 

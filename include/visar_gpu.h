@@ -41,7 +41,11 @@ int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D
  * "batch_unit_kb" KiB of input + output per batch workgroup (1-1024);
  * "tile_order" the order of a plain call's tiles over the XCDs (0 each
  * kernel's default, 1 row-major, 2 one band of neighbouring tiles per XCD);
- * "sep_rows" output rows per tile of the separable filter engine (1-16).
+ * "sep_rows" output rows per tile of the separable filter engine (1-16);
+ * "warp_layout" the one-channel u8 bilinear constant-border warps' kernels
+ * (0 block layout with 8-byte tap loads issued per pixel, 1 the row / LDS
+ * source-box kernels, 2 the block layout with the loads after a wave-wide
+ * check of the footprints).
  * Unknown keys or values out of range return VISAR_INVALID_ARGUMENT. */
 int visar_gpu_set_tuning(const char *key, int value);
 int visar_gpu_get_tuning(const char *key, int *value);

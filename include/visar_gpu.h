@@ -13,6 +13,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "visar.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -49,6 +51,21 @@ int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D
  * Unknown keys or values out of range return VISAR_INVALID_ARGUMENT. */
 int visar_gpu_set_tuning(const char *key, int value);
 int visar_gpu_get_tuning(const char *key, int *value);
+
+/* visar_threshold without device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments, results and errors are those of visar_threshold, except that src
+ * and dst must be device memory (else VISAR_INVALID_ARGUMENT "NotDeviceMemory")
+ * and dst must be src itself or not overlap it ("Overlap"). The automatic
+ * modes count their histogram in a 1 KiB device array that the library holds
+ * per GPU until the process ends, so they take at most 2^32 - 1 pixels
+ * ("TooManyPixels"), and two such calls that overlap in time on one GPU race.
+ * Nothing is staged: the automatic modes wait once for the calling thread's
+ * stream (the histogram comes back to the host); the fixed-threshold form
+ * only enqueues its kernel unless the thread's sync mode is on. */
+int visar_threshold_noalloc(const visar_image *src, const visar_image *dst, double thresh, double maxval, int type,
+                            double *used);
 
 /* Name and compute units of the current device, e.g. "gfx950 256 CUs". */
 int visar_gpu_device_info(char *buf, size_t n);

@@ -26,6 +26,7 @@ undistortion of a 1280x720 BGR frame (the reference benchmark's camera and
 adaptiveThreshold (block 11, mean and Gaussian) in one step.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -51,7 +52,6 @@ def parse():
                          "0: the measured best per workload (chain 3, pyramid/box/adaptive step 4, others 2)")
     ap.add_argument("--seconds", type=float, default=6.0, help="served workload: Poisson run length")
     ap.add_argument("--host-workers", type=int, default=8, help="served workload: connected-components threads")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (scales the HBM byte counters of a profile)")
@@ -75,43 +75,47 @@ UD_K = np.array([[900.0, 0, 640], [0, 900, 360], [0, 0, 1]])
 UD_DIST = np.array([-0.28, 0.09, 0.001, -0.0005, -0.01])
 
 
-def cpu_measure(seconds, make_worker, per_unit, unit, what):
-    """The oracle on a bounded sample: `make_worker(k)` returns a callable
-    doing one unit of the workload per call with worker k's own inputs. Runs
-    one worker for half of `seconds`, then CPU_THREADS workers on threads
-    (the oracle's C calls release the GIL) for the other half. `value` is the
-    all-cores rate; `value_1_thread` the single-thread one."""
-    import threading
+def cpu_lib():
+    """The oracle module driving its -O3 -march=native build, made here on
+    the host that times it."""
+    import oracle
+    from oracle import bands
+    oracle.use_library(bands.build_baseline())
+    return oracle
 
-    def timed(threads, secs):
-        workers = [make_worker(k) for k in range(threads)]
-        counts = [0] * threads
-        t0 = time.perf_counter()
-        stop = t0 + secs
 
-        def run(k):
-            n = 0
-            while True:
-                workers[k](n)
-                n += 1
-                if time.perf_counter() >= stop:
-                    break
-            counts[k] = n
+def cpu_threads():
+    from oracle import bands
+    return max(1, min(CPU_THREADS, bands.cpu_info()["usable_cpus"]))
 
-        ts = [threading.Thread(target=run, args=(k,)) for k in range(threads)]
-        for t in ts:
-            t.start()
-        for t in ts:
-            t.join()
-        return sum(counts), time.perf_counter() - t0
 
-    n1, dt1 = timed(1, seconds / 2)
-    threads = max(1, min(CPU_THREADS, os.cpu_count() or 1))
-    nn, dtn = timed(threads, seconds / 2)
-    return {"value": round(nn * per_unit / dtn, 6), "unit": unit, "cores": threads, "kind": "port",
-            "value_1_thread": round(n1 * per_unit / dt1, 6),
-            "sample": f"{what}: {nn} units on {threads} threads in {dtn:.1f} s, {n1} on one thread in {dt1:.1f} s "
-                      f"(scalar C oracle, -O2, one call per unit per thread)"}
+def oc(oracle, name, *args):
+    """A call of oracle C function `name` on pre-built arguments: images
+    become visar_image-shaped descriptors once, so a band's call costs no
+    Python work beyond the call itself."""
+    f = getattr(oracle.lib(), name)
+    conv = []
+    keep = []
+    for a in args:
+        if isinstance(a, np.ndarray) and a.ndim in (2, 3) and a.dtype != np.float64:
+            d = oracle.desc(a)
+            keep.append((a, d))
+            conv.append(ctypes.byref(d))
+        elif isinstance(a, np.ndarray):
+            arr = np.ascontiguousarray(a, np.float64).ravel()
+            keep.append(arr)
+            conv.append(arr.ctypes.data_as(ctypes.c_void_p))
+        elif isinstance(a, float):
+            conv.append(ctypes.c_double(a))
+        else:
+            conv.append(a)
+
+    def call():
+        rc = f(*conv)
+        if rc != 0:
+            raise RuntimeError(f"{name}: oracle status {rc}")
+    call.keep = keep
+    return call
 
 
 def per_launch_ms(torch, fn, n, stream):
@@ -196,11 +200,15 @@ def bench_batch(args, torch, va):
         "autotune_ms": {k: round(v, 4) for k, v in timings.items()},
     }
     if not args.no_cpu:
-        import oracle
-        from oracle import pipelines
-        line["cpu_baseline"] = cpu_measure(
-            args.cpu_seconds, lambda k: (lambda n: pipelines.batch_item(oracle, jobs[(k * 37 + n) % len(jobs)])), 1,
-            "images/s", "images of the batch")
+        from oracle import bands, pipelines
+        oracle = cpu_lib()
+        # Every 16th image of the batch, whole calls shared by the threads
+        # (the images are independent); host copies of the sources.
+        sample = [dict(j, src=j["src"].cpu().numpy()) for j in jobs[::16]]
+        line["cpu_baseline"] = bands.baseline(
+            oracle.lib(), lambda i: [bands.Items([lambda j=j: pipelines.batch_item(oracle, j) for j in sample])],
+            len(sample), "images/s", f"{len(sample)} images of the batch (every 16th), one whole call per image",
+            cpu_threads())
     print(json.dumps(line))
 
 
@@ -257,17 +265,7 @@ def bench_morph(args, torch, va):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
     }
     if not args.no_cpu:
-        import oracle
-
-        def morph_worker(k):
-            def one(n):
-                n += k
-                shape, ks, _ = MORPH_ELEMENTS[(n // 7) % 3]
-                oracle.morphology(base, n % 7, shape, (ks, ks), 1)
-            return one
-
-        line["cpu_baseline"] = cpu_measure(args.cpu_seconds, morph_worker, S * S / 1e9, "Gpix/s",
-                                           "morphology calls (ops x elements, iteration 1) on the 4096x4096 mask")
+        line["cpu_baseline"] = morph_cpu_baseline(base)
     print(json.dumps(line))
 
 
@@ -402,24 +400,27 @@ def bench_serve(args, torch, va):
     if per_frame is not None:
         line["roofline"]["traffic"] = int(per_frame * line["roofline"]["frames_per_launch"])
     if not args.no_cpu:
-        import oracle
+        from oracle import bands
+        oracle = cpu_lib()
+        nt = cpu_threads()
+        # One frame of each of nt streams per step, each stream's frame on
+        # one thread (the streams are independent), through the detector
+        # front end: blur 5x5 -> Background -> threshold -> open 3x3 -> CC.
+        models = [oracle.Background(W, H, 4) for _ in range(nt)]
+        bufs = [[np.empty((H, W), np.uint8) for _ in range(4)] for _ in range(nt)]
 
-        def stream_worker(k):
-            bg = oracle.Background(W, H, 4)
-            fl = frames[k % len(frames)]
+        def frame(k, i):
+            b, m, t, o = bufs[k]
+            f = frames[k % len(frames)][i % cycle]
+            oracle.gaussian_blur(f, (5, 5), 0, dst=b)
+            models[k].apply(b, m, 28)
+            oracle.threshold(m, 127, 255, 0, dst=t)
+            oracle.morphology(t, 2, 0, (3, 3), dst=o)
+            oracle.connected_components_with_stats(o)
 
-            def one(n):
-                f = fl[n % cycle]
-                b = oracle.gaussian_blur(f, (5, 5), 0)
-                m = np.empty_like(f)
-                bg.apply(b, m, 28)
-                tt = oracle.threshold(m, 127, 255, 0)[1]
-                o = oracle.morphology(tt, 2, 0, (3, 3))
-                oracle.connected_components_with_stats(o)
-            return one
-
-        line["cpu_baseline"] = cpu_measure(args.cpu_seconds, stream_worker, 1, "frames/s",
-                                           "frames through the oracle pipeline, one stream per thread")
+        line["cpu_baseline"] = bands.baseline(
+            oracle.lib(), lambda i: [bands.Items([lambda k=k: frame(k, i) for k in range(nt)])], nt, "frames/s",
+            f"one 1280x720 frame of each of {nt} streams per step, one stream per thread", nt)
     print(json.dumps(line))
 
 
@@ -671,73 +672,133 @@ def main():
     print(json.dumps(line))
 
 
+MORPH_SAMPLE = ((4, 0, 3, 1), (2, 2, 5, 2), (5, 1, 7, 3))  # (op, shape, size, iterations)
+
+
+def morph_steps(oracle, src, case, out, tmp):
+    """The ops of one morphologyEx case as the reference runs it: each
+    erode or dilate pass one banded op over the rows (morph.zig:532-593),
+    the saturating subtraction another. tmp: four scratch images."""
+    from oracle import bands
+    op, shape, ks, it = case
+    mask = oracle.structuring_element(shape, ks, ks)
+    S = src.shape[0]
+    grain = max(16, 2 * ks)
+    ops = []
+
+    def passes(a, op_max, n, final, scratch):
+        cur = a
+        for k in range(n):
+            dst = final if k == n - 1 else scratch[k % 2]
+            ops.append(bands.Banded(oc(oracle, "orc_morph_pass", cur, dst, int(op_max), mask.ctypes.data_as(ctypes.c_void_p),
+                                       ks, ks, ks // 2, ks // 2), S, grain))
+            cur = dst
+        ops[-1].call.keep.append(mask)
+
+    t0, t1, t2, t3 = tmp
+    if op in (0, 1):
+        passes(src, op == 1, it, out, (t0, t1))
+    elif op in (2, 3):
+        passes(src, op == 3, it, t2, (t0, t1))
+        passes(t2, op == 2, it, out, (t0, t1))
+    elif op == 4:
+        passes(src, True, it, t2, (t0, t1))
+        passes(src, False, it, t3, (t0, t1))
+        ops.append(bands.Banded(oc(oracle, "orc_sub_sat", t2, t3, out), S, grain))
+    else:
+        passes(src, op == 6, it, t2, (t0, t1))
+        passes(t2, op == 5, it, t3, (t0, t1))
+        a, b = (src, t3) if op == 5 else (t3, src)
+        ops.append(bands.Banded(oc(oracle, "orc_sub_sat", a, b, out), S, grain))
+    return ops
+
+
+def morph_cpu_baseline(base):
+    from oracle import bands
+    oracle = cpu_lib()
+    S = base.shape[0]
+    out = np.empty_like(base)
+    tmp = [np.empty_like(base) for _ in range(4)]
+    steps = [op for case in MORPH_SAMPLE for op in morph_steps(oracle, base, case, out, tmp)]
+    names = ", ".join(f"{MORPH_OPS[o]} {MORPH_ELEMENTS[[e[0] for e in MORPH_ELEMENTS].index(sh)][2]} it{it}"
+                      for o, sh, _, it in MORPH_SAMPLE)
+    return bands.baseline(oracle.lib(), lambda i: steps, len(MORPH_SAMPLE) * S * S / 1e9, "Gpix/s",
+                          f"3 of the 63 cases on the 4096x4096 mask ({names}), every pass in row bands",
+                          cpu_threads())
+
+
 def cpu_baseline(args, synth):
-    """The oracle on a bounded sample of the same workload (whole steps, each
-    worker on its own input)."""
-    import oracle
+    """The restatement, -O3 -march=native on this host, on the same
+    workload: each step's ops one after the other, every op's destination
+    rows split into one band per thread at the op's reference grain."""
+    from oracle import bands
+    oracle = cpu_lib()
+    nt = cpu_threads()
+    F = 4  # rotating host frames
     if args.config == 2:
         W, H = 3840, 2160
         ma = synth.affine_matrix(0, W // 2, H // 2)
         mp = synth.perspective_matrix(0, W // 2, H // 2)
-
-        def chain_worker(k):
-            a = synth.uniform_bytes(k, (H, W, 3))
-
-            def one(n):
-                g = oracle.cvt_color(a, 6)
-                s = oracle.resize(g, (W // 2, H // 2), 1)
-                oracle.sobel(s, np.int16, 1, 0, 3)
-                oracle.warp_affine(s, ma, (W // 2, H // 2), 1, 0, 0)
-                oracle.warp_perspective(s, mp, (W // 2, H // 2), 1, 0, 0)
-            return one
-
-        return cpu_measure(args.cpu_seconds, chain_worker, W * H / 1e9, "Gpix/s", "full chains of 3840x2160x3")
+        srcs = [synth.uniform_bytes(k, (H, W, 3)) for k in range(F)]
+        g, sm = np.empty((H, W), np.uint8), np.empty((H // 2, W // 2), np.uint8)
+        sob = np.empty((H // 2, W // 2), np.int16)
+        wa, wp = np.empty_like(sm), np.empty_like(sm)
+        h2 = H // 2
+        steps = [[bands.Banded(oc(oracle, "orc_cvt_color", a, g, 6), H, 16),
+                  bands.Banded(oc(oracle, "orc_resize", g, sm, 1), h2, 16),
+                  bands.Banded(oc(oracle, "orc_sobel", sm, sob, 1, 0, 3, 1.0, 4), h2, max(8, 65536 // (W // 2))),
+                  bands.Banded(oc(oracle, "orc_warp_affine", sm, wa, ma, 1, 0, 0.0), h2, 4),
+                  bands.Banded(oc(oracle, "orc_warp_perspective", sm, wp, mp, 1, 0, 0.0), h2, 8)] for a in srcs]
+        return bands.baseline(oracle.lib(), lambda i: steps[i % F], W * H / 1e9, "Gpix/s",
+                              "whole chains (gray, resize, Sobel, warpAffine, warpPerspective) of 3840x2160x3", nt)
     if args.config == 6:
         W, H = 3840, 2160
-
-        def gd_worker(k):
-            a = synth.uniform_bytes(k, (H, W, 4))
-            return lambda n: oracle.gray_downsample(a, 6)
-
-        return cpu_measure(args.cpu_seconds, gd_worker, W * H / 1e9, "Gpix/s",
-                           "grayDownsample calls on 3840x2160 BGRA, k = 6")
+        srcs = [synth.uniform_bytes(k, (H, W, 4)) for k in range(F)]
+        o = np.empty((H // 6, W // 6), np.uint8)
+        steps = [[bands.Banded(oc(oracle, "orc_gray_downsample", a, o, ctypes.c_uint32(6)), H // 6, 4)] for a in srcs]
+        return bands.baseline(oracle.lib(), lambda i: steps[i % F], W * H / 1e9, "Gpix/s",
+                              "grayDownsample of 3840x2160 BGRA by 6", nt)
     if args.config == 8:
         W, H = 1920, 1080
+        hd, wd = (H + 1) // 2, (W + 1) // 2
+        b = [synth.uniform_bytes(k, (H, W, 3)) for k in range(F)]
+        gr = [synth.uniform_bytes(100 + k, (H, W)) for k in range(F)]
+        f = [(x.astype(np.float32) / 255.0).astype(np.float32) for x in b]
+        o_pd, o_pg, o_pu = np.empty((hd, wd, 3), np.uint8), np.empty((hd, wd), np.uint8), np.empty((2 * H, 2 * W), np.uint8)
+        o_pf, o_b3, o_b15 = np.empty((hd, wd, 3), np.float32), np.empty((H, W, 3), np.uint8), np.empty((H, W, 3), np.uint8)
+        o_am, o_ag = np.empty((H, W), np.uint8), np.empty((H, W), np.uint8)
+        gbox = max(8, 65536 // (W * 3))
 
-        def r8_worker(k):
-            b = synth.uniform_bytes(k, (H, W, 3))
-            g = synth.uniform_bytes(100 + k, (H, W))
-            f = (b.astype(np.float32) / 255.0).astype(np.float32)
-
-            def one(n):
-                oracle.pyr_down(b)
-                oracle.pyr_down(g)
-                oracle.pyr_up(g)
-                oracle.pyr_down(f)
-                oracle.box_filter(b, (3, 3))
-                oracle.box_filter(b, (15, 15))
-                oracle.adaptive_threshold(g, 255, 0, 0, 11, 2)
-                oracle.adaptive_threshold(g, 255, 1, 0, 11, 2)
-            return one
-
-        return cpu_measure(args.cpu_seconds, r8_worker, W * H / 1e9, "Gpix/s",
-                           "whole steps of the 8 pyramid / box / adaptive calls on 1920x1080")
+        def step(k):
+            return [bands.Banded(oc(oracle, "orc_pyr_down", b[k], o_pd, 4), hd, 8),
+                    bands.Banded(oc(oracle, "orc_pyr_down", gr[k], o_pg, 4), hd, 8),
+                    bands.Banded(oc(oracle, "orc_pyr_up", gr[k], o_pu, 4), H, 8),
+                    bands.Banded(oc(oracle, "orc_pyr_down", f[k], o_pf, 4), hd, 8),
+                    bands.Banded(oc(oracle, "orc_box_filter", b[k], o_b3, 3, 3, 1, 4), H, gbox),
+                    bands.Banded(oc(oracle, "orc_box_filter", b[k], o_b15, 15, 15, 1, 4), H, gbox),
+                    # adaptiveThreshold builds a whole mean image first: one call, one thread.
+                    bands.Items([oc(oracle, "orc_adaptive_threshold", gr[k], o_am, 255.0, 0, 0, 11, 2.0)]),
+                    bands.Items([oc(oracle, "orc_adaptive_threshold", gr[k], o_ag, 255.0, 1, 0, 11, 2.0)])]
+        steps = [step(k) for k in range(F)]
+        return bands.baseline(oracle.lib(), lambda i: steps[i % F], W * H / 1e9, "Gpix/s",
+                              "whole steps of the 8 pyramid / box / adaptive calls on 1920x1080 (the two "
+                              "adaptiveThreshold calls on one thread each)", nt)
     if args.config == 7:
         W, H = 1280, 720
-
-        def ud_worker(k):
-            a = synth.uniform_bytes(k, (H, W, 3))
-            return lambda n: oracle.undistort(a, UD_K, UD_DIST)
-
-        return cpu_measure(args.cpu_seconds, ud_worker, W * H / 1e9, "Gpix/s",
-                           "undistort calls on 1280x720 BGR (the reference benchmark's camera)")
+        srcs = [synth.uniform_bytes(k, (H, W, 3)) for k in range(F)]
+        o = np.empty((H, W, 3), np.uint8)
+        dist = np.asarray(UD_DIST, np.float64)
+        steps = [[bands.Banded(oc(oracle, "orc_undistort", a, o, np.asarray(UD_K, np.float64), dist,
+                                  ctypes.c_size_t(dist.size), None), H, 8)] for a in srcs]
+        return bands.baseline(oracle.lib(), lambda i: steps[i % F], W * H / 1e9, "Gpix/s",
+                              "undistort of 1280x720 BGR (the reference benchmark's camera)", nt)
     W, H = 1920, 1080
-
-    def blur_worker(k):
-        a = synth.uniform_bytes(k, (H, W))
-        return lambda n: oracle.gaussian_blur(a, (5, 5), 0)
-
-    return cpu_measure(args.cpu_seconds, blur_worker, W * H / 1e9, "Gpix/s", "GaussianBlur 5x5 calls on 1920x1080")
+    srcs = [synth.uniform_bytes(k, (H, W)) for k in range(F)]
+    o = np.empty((H, W), np.uint8)
+    steps = [[bands.Banded(oc(oracle, "orc_gaussian_blur", a, o, 5, 5, 0.0, 0.0, 4), H, max(8, 65536 // W))]
+             for a in srcs]
+    return bands.baseline(oracle.lib(), lambda i: steps[i % F], W * H / 1e9, "Gpix/s",
+                          "GaussianBlur 5x5 of 1920x1080", nt)
 
 
 if __name__ == "__main__":

@@ -22,6 +22,18 @@
 
 /* ------------------------------------------------------------ helpers */
 
+/* Row band of the calling thread, for timing the restatement with one image
+ * split across threads in row bands (the reference's Exec.parallelFor,
+ * pool.zig:31-37): functions whose destination rows are independent visit
+ * only rows [lo, hi) of them. The checker never sets it (the whole image). */
+static __thread long g_band_lo = 0, g_band_hi = -1;
+void orc_set_band(long lo, long hi) {
+    g_band_lo = lo;
+    g_band_hi = hi;
+}
+static inline long band_lo(void) { return g_band_lo; }
+static inline long band_hi(long n) { return g_band_hi < 0 || g_band_hi > n ? n : g_band_hi; }
+
 static size_t dsize(uint32_t depth) {
     switch (depth) {
     case 0: return 1;
@@ -315,7 +327,7 @@ static int sep_int(const orc_image *s, const orc_image *d, const int64_t *kx, in
     int rx = nkx / 2, ry = nky / 2;
     int64_t *V = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
     if (!V) return ORC_OUT_OF_MEMORY;
-    for (long y = 0; y < H; y++) {
+    for (long y = band_lo(); y < band_hi(H); y++) {
         for (long c = 0; c < n; c++) {
             int64_t a = 0;
             for (int j = 0; j < nky; j++) {
@@ -417,7 +429,7 @@ static int sep_f32(const orc_image *s, const orc_image *d, const float *kx, int 
     float const_v = 0.0f;
     for (int j = 0; j < nky; j++) const_v = const_v + ky[j] * 0.0f;
     long rows[256];
-    for (long y = 0; y < H; y++) {
+    for (long y = band_lo(); y < band_hi(H); y++) {
         for (int j = 0; j < nky; j++) rows[j] = orc_border_map(y + j - ry, H, border);
         for (long c = 0; c < n; c++) {
             for (int j = 0; j < nky; j++) tv[j] = rows[j] < 0 ? 0.0f : ldf(s, rows[j], c);
@@ -565,7 +577,7 @@ int orc_cvt_color(const orc_image *src, const orc_image *dst, int code) {
     if ((int)src->channels != scn || dst->channels != 1) return ORC_UNSUPPORTED;
     if (T == 3) return ORC_UNSUPPORTED;
     int bi = sb, ri = 2 - sb;
-    for (long y = 0; y < src->height; y++)
+    for (long y = band_lo(); y < band_hi(src->height); y++)
         for (long x = 0; x < src->width; x++) {
             if (T == 5) {
                 float b = ldf(src, y, x * scn + bi), g = ldf(src, y, x * scn + 1), r = ldf(src, y, x * scn + ri);
@@ -640,7 +652,7 @@ int orc_gray_downsample(const orc_image *src, const orc_image *dst, uint32_t k) 
     if (dst->width != src->width / k || dst->height != src->height / k) return ORC_SIZE_MISMATCH;
     const uint32_t cn = src->channels;
     const float inv = 1.0f / (float)(k * k * (1u << 14));
-    for (uint32_t oy = 0; oy < dst->height; oy++)
+    for (uint32_t oy = (uint32_t)band_lo(); oy < (uint32_t)band_hi(dst->height); oy++)
         for (uint32_t ox = 0; ox < dst->width; ox++) {
             uint32_t sb = 0, sg = 0, sr = 0;
             for (uint32_t dy = 0; dy < k; dy++) {
@@ -805,7 +817,7 @@ static void linear_tap(long d, uint32_t ssize, uint32_t dsize_, int area, int pi
 /* resize.zig:216-242 + resize_u8.zig:641-711: fixed-point bilinear */
 static void linear_u8(const orc_image *s, const orc_image *d, int area) {
     long cn = s->channels;
-    for (long y = 0; y < d->height; y++) {
+    for (long y = band_lo(); y < band_hi(d->height); y++) {
         long yo[2];
         int b[2];
         linear_tap(y, s->height, d->height, area, 0, yo, b);
@@ -830,7 +842,7 @@ static void linear_u8(const orc_image *s, const orc_image *d, int area) {
 /* resize_u8.zig:381-447 exact half */
 static void half_u8(const orc_image *s, const orc_image *d) {
     long cn = s->channels;
-    for (long y = 0; y < d->height; y++)
+    for (long y = band_lo(); y < band_hi(d->height); y++)
         for (long x = 0; x < d->width; x++)
             for (long ch = 0; ch < cn; ch++) {
                 long i = 2 * x * cn + ch;
@@ -843,7 +855,7 @@ static void half_u8(const orc_image *s, const orc_image *d) {
 static void area_int_u8(const orc_image *s, const orc_image *d, long ix, long iy) {
     long cn = s->channels;
     float scale = 1.0f / (float)(ix * iy);
-    for (long y = 0; y < d->height; y++)
+    for (long y = band_lo(); y < band_hi(d->height); y++)
         for (long x = 0; x < d->width; x++)
             for (long ch = 0; ch < cn; ch++) {
                 uint32_t sum = 0;
@@ -895,7 +907,8 @@ static int area_frac_u8(const orc_image *s, const orc_image *d) {
     long nx = area_tab_build(xt, s->width, d->width, sxs);
     long ny = area_tab_build(yt, s->height, d->height, sys);
     long k = 0;
-    for (long dy = 0; dy < d->height; dy++) {
+    while (k < ny && (long)yt[k].di < band_lo()) k++;
+    for (long dy = band_lo(); dy < band_hi(d->height); dy++) {
         for (long i = 0; i < nd; i++) sum[i] = 0.0f;
         for (; k < ny && yt[k].di == (uint32_t)dy; k++) {
             for (long i = 0; i < nd; i++) hbuf[i] = 0.0f;
@@ -915,7 +928,7 @@ static void nearest_any(const orc_image *s, const orc_image *d) {
     long cn = s->channels;
     double ifx = (double)s->width / (double)d->width, ify = (double)s->height / (double)d->height;
     size_t es = dsize(s->depth);
-    for (long y = 0; y < d->height; y++) {
+    for (long y = band_lo(); y < band_hi(d->height); y++) {
         long sy = (long)floor((double)y * ify);
         if (sy > (long)s->height - 1) sy = (long)s->height - 1;
         for (long x = 0; x < d->width; x++) {
@@ -1050,7 +1063,7 @@ int orc_warp_affine(const orc_image *src, const orc_image *dst, const double m[6
     int T = (int)src->depth;
     double cvd = T == 5 ? (double)(float)value : (double)sat_cast_f(value, (uint32_t)T);
     float cv = (float)cvd;
-    for (long y = 0; y < dst->height; y++) {
+    for (long y = band_lo(); y < band_hi(dst->height); y++) {
         float yf = (float)y;
         float bx = mf[1] * yf + mf[2];
         float by = mf[4] * yf + mf[5];
@@ -1111,7 +1124,7 @@ int orc_warp_perspective(const orc_image *src, const orc_image *dst, const doubl
     long W = src->width, H = src->height, cn = src->channels;
     int T = (int)src->depth;
     double cvd = T == 5 ? (double)(float)value : (double)sat_cast_f(value, (uint32_t)T);
-    for (long y = 0; y < dst->height; y++) {
+    for (long y = band_lo(); y < band_hi(dst->height); y++) {
         double fy = (double)y;
         for (long x = 0; x < dst->width; x++) {
             double fx = (double)x;
@@ -1179,7 +1192,7 @@ int orc_remap(const orc_image *src, const orc_image *dst, const orc_image *mapx,
     double rv = round(clampd(value, 0, 255));
     int bv = (int)rv;
     long W = src->width, H = src->height, cn = src->channels;
-    for (long y = 0; y < dst->height; y++)
+    for (long y = band_lo(); y < band_hi(dst->height); y++)
         for (long x = 0; x < dst->width; x++) {
             float sx = ldf(mapx, y, x), sy = ldf(mapy, y, x);
             float fx = floorf(sx), fy = floorf(sy);
@@ -1239,7 +1252,7 @@ int orc_morph_pass(const orc_image *s, const orc_image *d, int op_max, const uin
     double lo, hi;
     depth_range((uint32_t)T, &lo, &hi);
     double init = T == 5 ? (op_max ? -INFINITY : INFINITY) : (op_max ? lo : hi);
-    for (long y = 0; y < H; y++)
+    for (long y = band_lo(); y < band_hi(H); y++)
         for (long x = 0; x < W; x++)
             for (long ch = 0; ch < cn; ch++) {
                 double acc = init;
@@ -1325,7 +1338,7 @@ int orc_pyr_down(const orc_image *src, const orc_image *dst, int border) {
     if (dst->channels != src->channels || dst->width != (src->width + 1) / 2 || dst->height != (src->height + 1) / 2)
         return ORC_SIZE_MISMATCH;
     const int f32 = T == 5;
-    for (long y = 0; y < (long)dst->height; y++) {
+    for (long y = band_lo(); y < band_hi((long)dst->height); y++) {
         long ry[5];
         for (int k = 0; k < 5; k++) ry[k] = orc_border_map(2 * y + k - 2, H, border);
         for (long x = 0; x < (long)dst->width; x++) {
@@ -1364,7 +1377,7 @@ int orc_pyr_up(const orc_image *src, const orc_image *dst, int border) {
     if (dst->channels != src->channels || (uint64_t)dst->width != 2 * (uint64_t)w || (uint64_t)dst->height != 2 * (uint64_t)h)
         return ORC_SIZE_MISMATCH;
     const int f32 = T == 5;
-    for (long y = 0; y < h; y++) {
+    for (long y = band_lo(); y < band_hi(h); y++) {
         const long ya = y > 0 ? y - 1 : (h > 1 ? 1 : 0), yc = y + 1 < h ? y + 1 : y;
         for (int odd = 0; odd < 2; odd++) {
             for (long p = 0; p < w; p++) {
@@ -1567,7 +1580,9 @@ int orc_undistort(const orc_image *src, const orc_image *dst, const double K[9],
         ar[5] = ar0[5] - (double)y0;
         if (!inv3(ar, ir)) continue;
         const uint32_t y1 = y0 + stripe < H ? y0 + stripe : H;
-        for (uint32_t y = y0; y < y1; y++) {
+        const uint32_t ylo = y0 > (uint32_t)band_lo() ? y0 : (uint32_t)band_lo();
+        const uint32_t yhi = y1 < (uint32_t)band_hi(H) ? y1 : (uint32_t)band_hi(H);
+        for (uint32_t y = ylo; y < yhi; y++) {
             uint8_t *d = (uint8_t *)rowp(dst, y);
             for (uint32_t x = 0; x < W; x++) {
                 double u, v;
@@ -1644,12 +1659,20 @@ static int morph_iter(const orc_image *s, const orc_image *d, int op_max, const 
 /* morph.zig:574-593 d = saturate(a - b) */
 static void sub_sat(const orc_image *a, const orc_image *b, const orc_image *d) {
     long n = (long)a->width * a->channels;
-    for (long y = 0; y < a->height; y++)
+    for (long y = band_lo(); y < band_hi(a->height); y++)
         for (long i = 0; i < n; i++) {
             if (a->depth == 5) { st_f(d, y, i, ldf(a, y, i) - ldf(b, y, i)); continue; }
             int64_t x = ldi(a, y, i), z = ldi(b, y, i);
             st_i(d, y, i, x > z ? x - z : 0);
         }
+}
+
+/* morph.zig:574-593 subInPlace as an entry point: d = a - b, saturating at 0. */
+int orc_sub_sat(const orc_image *a, const orc_image *b, const orc_image *d) {
+    if (!a || !b || !d || !same_shape(a, b) || !same_shape(a, d) || a->depth != b->depth || a->depth != d->depth)
+        return ORC_INVALID_ARGUMENT;
+    sub_sat(a, b, d);
+    return ORC_OK;
 }
 
 /* morph.zig:532-571 morphologyEx, C ABI checks of capi.zig:339-350 */

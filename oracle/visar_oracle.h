@@ -81,6 +81,12 @@ int orc_threshold(const orc_image *src, const orc_image *dst, double thresh, dou
 
 /* resize */
 int orc_resize(const orc_image *src, const orc_image *dst, int interp);
+/* Timing hook: restrict this thread's row-independent functions to
+ * destination rows [lo, hi) (hi < 0: all rows), so one image can be split
+ * across threads in row bands. */
+void orc_set_band(long lo, long hi);
+/* d = a - b saturating at 0 (morph.zig:574-593). */
+int orc_sub_sat(const orc_image *a, const orc_image *b, const orc_image *d);
 /* Test hooks: generic u8 LINEAR (resize.zig:216-242) and fractional AREA
  * (resize.zig:527-557) whatever the size ratio, the AREA table
  * (resize.zig:470-493; returns the entry count, -1 on bad sizes) and one

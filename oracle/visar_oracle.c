@@ -788,6 +788,9 @@ int orc_threshold(const orc_image *src, const orc_image *dst, double thresh, dou
 
 /* ------------------------------------------------------------- resize */
 
+static void store_round(const orc_image *d, long y, long i, float v);
+
+
 /* resize.zig:186-214 linearTapU8 */
 static void linear_tap(long d, uint32_t ssize, uint32_t dsize_, int area, int pin, long o[2], int w[2]) {
     double inv = (double)dsize_ / (double)ssize;
@@ -894,9 +897,10 @@ static long area_tab_build(area_tab *out, uint32_t ssize, uint32_t dsz, double s
     return k;
 }
 
-/* resize.zig:527-557 areaGeneric (f32 table-order sums, no FMA), which the
- * u8 kernels of resize_u8.zig:1058-1208 reproduce bit for bit. */
-static int area_frac_u8(const orc_image *s, const orc_image *d) {
+/* resize.zig:527-565 areaGeneric (f32 table-order sums, no FMA, then
+ * sat.cast to the sample type), any depth; the u8 kernels of
+ * resize_u8.zig:1058-1208 reproduce it bit for bit. */
+static int area_generic(const orc_image *s, const orc_image *d) {
     long cn = s->channels;
     double sxs = (double)s->width / (double)d->width, sys = (double)s->height / (double)d->height;
     area_tab *xt = (area_tab *)malloc(sizeof(area_tab) * (2 * (size_t)s->width + 2 * (size_t)d->width + 4));
@@ -917,7 +921,7 @@ static int area_frac_u8(const orc_image *s, const orc_image *d) {
                     hbuf[xt[e].di * cn + ch] = hbuf[xt[e].di * cn + ch] + ldf(s, yt[k].si, xt[e].si * cn + ch) * xt[e].alpha;
             for (long i = 0; i < nd; i++) sum[i] = sum[i] + hbuf[i] * yt[k].alpha;
         }
-        for (long i = 0; i < nd; i++) st_i(d, dy, i, sat_cast_f((double)sum[i], 0));
+        for (long i = 0; i < nd; i++) store_round(d, dy, i, sum[i]);
     }
     free(xt); free(yt); free(hbuf); free(sum);
     return ORC_OK;
@@ -939,7 +943,100 @@ static void nearest_any(const orc_image *s, const orc_image *d) {
     }
 }
 
-/* resize.zig:28-62 dispatch (u8 with 1-4 channels for LINEAR/AREA; NEAREST any depth) */
+/* resize.zig:154-180 linearCoord: source index and fractional weight, half-
+ * pixel centres (or the area-mode coordinate when enlarging), clamped to
+ * the edges with weight 0. */
+static void linear_coord(long d, double scale, uint32_t len, int area, long *idx, double *frac) {
+    long sx;
+    double f;
+    if (area) {
+        const double df = (double)d;
+        sx = (long)floor(df * scale);
+        f = (df + 1) - (double)(sx + 1) / scale;
+        f = f <= 0 ? 0 : f - floor(f);
+    } else {
+        const double fx = ((double)d + 0.5) * scale - 0.5;
+        sx = (long)floor(fx);
+        f = fx - (double)sx;
+    }
+    if (sx < 0) { sx = 0; f = 0; }
+    if (sx >= (long)len - 1) { sx = (long)len - 1; f = 0; }
+    *idx = sx;
+    *frac = f;
+}
+
+/* resize.zig:284-370 linear for u16 / i16 / f32 (Weights of :401-418): f32
+ * weights (f32(1 - f), f32(f)), each sample converted to f32, horizontal
+ * sums s0 a0 + s1 a1 and the vertical r0 b0 + r1 b1 unfused, then
+ * sat.cast to the sample type. x and y both clamp to the edges. */
+static int linear_any(const orc_image *s, const orc_image *d, int area) {
+    const long cn = s->channels, W = s->width, H = s->height;
+    const double sxs = (double)s->width / (double)d->width, sys = (double)s->height / (double)d->height;
+    long *xi = (long *)malloc(sizeof(long) * (size_t)d->width);
+    float *xa = (float *)malloc(sizeof(float) * 2 * (size_t)d->width);
+    if (!xi || !xa) { free(xi); free(xa); return ORC_OUT_OF_MEMORY; }
+    for (long x = 0; x < d->width; x++) {
+        long i;
+        double f;
+        linear_coord(x, sxs, s->width, area, &i, &f);
+        xi[x] = i;
+        xa[2 * x] = (float)(1.0 - f);
+        xa[2 * x + 1] = (float)f;
+    }
+    const long last = (W - 1) * cn;
+    for (long y = band_lo(); y < band_hi(d->height); y++) {
+        long iy;
+        double fy;
+        linear_coord(y, sys, s->height, area, &iy, &fy);
+        const long y0 = iy, y1 = iy + 1 < H ? iy + 1 : H - 1;
+        const float b0 = (float)(1.0 - fy), b1 = (float)fy;
+        for (long x = 0; x < d->width; x++)
+            for (long ch = 0; ch < cn; ch++) {
+                const long i0 = xi[x] * cn + ch, i1 = i0 >= last ? i0 : i0 + cn;
+                const float a0 = xa[2 * x], a1 = xa[2 * x + 1];
+                const float h0 = ldf(s, y0, i0) * a0 + ldf(s, y0, i1) * a1;
+                const float h1 = ldf(s, y1, i0) * a0 + ldf(s, y1, i1) * a1;
+                store_round(d, y, x * cn + ch, h0 * b0 + h1 * b1);
+            }
+    }
+    free(xi);
+    free(xa);
+    return ORC_OK;
+}
+
+/* resize.zig:420-457 areaInt for any depth: u8 2x2 is (a + b + c + d + 2) >> 2;
+ * other unsigned samples the integer block mean rounded half up; signed
+ * and float samples the f64 mean, sat.cast to the sample type. */
+static void area_int_any(const orc_image *s, const orc_image *d, long ix, long iy) {
+    const long cn = s->channels;
+    const uint32_t T = s->depth;
+    const long area = ix * iy;
+    for (long y = band_lo(); y < band_hi(d->height); y++)
+        for (long x = 0; x < d->width; x++)
+            for (long ch = 0; ch < cn; ch++) {
+                const long o = x * cn + ch;
+                if (T == 0 && ix == 2 && iy == 2) {
+                    const long i = 2 * x * cn + ch;
+                    const int64_t v = ldi(s, 2 * y, i) + ldi(s, 2 * y, i + cn) + ldi(s, 2 * y + 1, i) + ldi(s, 2 * y + 1, i + cn);
+                    st_i(d, y, o, (v + 2) >> 2);
+                } else if (T == 3 || T == 5) {
+                    double sum = 0;
+                    for (long dy = 0; dy < iy; dy++)
+                        for (long dx = 0; dx < ix; dx++) sum += (double)ldf(s, y * iy + dy, (x * ix + dx) * cn + ch);
+                    const double m = sum / (double)area;
+                    if (T == 5) st_f(d, y, o, (float)m);
+                    else st_i(d, y, o, sat_cast_f(m, T));
+                } else {
+                    uint64_t sum = 0;
+                    for (long dy = 0; dy < iy; dy++)
+                        for (long dx = 0; dx < ix; dx++) sum += (uint64_t)ldi(s, y * iy + dy, (x * ix + dx) * cn + ch);
+                    st_i(d, y, o, (int64_t)((sum + (uint64_t)area / 2) / (uint64_t)area));
+                }
+            }
+}
+
+/* resize.zig:28-62 dispatch: u8 with 1-4 channels takes the fixed-point fast
+ * paths; other depths and channel counts the generic kernels of :47-60. */
 int orc_resize(const orc_image *src, const orc_image *dst, int interp) {
     if (interp < 0 || interp > 4) return ORC_UNSUPPORTED;
     if (!src || !dst) return ORC_INVALID_ARGUMENT;
@@ -955,7 +1052,18 @@ int orc_resize(const orc_image *src, const orc_image *dst, int interp) {
         return ORC_OK;
     }
     if (interp == 0) { nearest_any(src, dst); return ORC_OK; }
-    if (T != 0 || src->channels > 4 || (interp != 1 && interp != 3)) return ORC_UNSUPPORTED;
+    if (interp != 1 && interp != 3) return ORC_UNSUPPORTED;
+    if (T != 0 || src->channels > 4) {
+        /* resize.zig:47-60: the generic kernels (u8 keeps its 11-bit taps). */
+        const int up = dst->width > src->width || dst->height > src->height;
+        if (interp == 1 || up) {
+            if (T == 0) { linear_u8(src, dst, interp == 3); return ORC_OK; }
+            return linear_any(src, dst, interp == 3);
+        }
+        long ix = src->width / dst->width, iy = src->height / dst->height;
+        if (ix * dst->width == src->width && iy * dst->height == src->height) { area_int_any(src, dst, ix, iy); return ORC_OK; }
+        return area_generic(src, dst);
+    }
     int exact_half = src->width == 2 * dst->width && src->height == 2 * dst->height;
     if (interp == 1) {
         if (exact_half) half_u8(src, dst);
@@ -966,7 +1074,7 @@ int orc_resize(const orc_image *src, const orc_image *dst, int interp) {
     if (exact_half) { half_u8(src, dst); return ORC_OK; }
     long ix = src->width / dst->width, iy = src->height / dst->height;
     if (ix * dst->width == src->width && iy * dst->height == src->height) { area_int_u8(src, dst, ix, iy); return ORC_OK; }
-    return area_frac_u8(src, dst);
+    return area_generic(src, dst);
 }
 
 /* Test hooks: the generic u8 kernels whatever the dispatch would choose, so
@@ -986,7 +1094,7 @@ int orc_resize_area_generic(const orc_image *src, const orc_image *dst) {
     if (dst->width > src->width || dst->height > src->height) return ORC_INVALID_ARGUMENT;
     int rc = check(src, 0);
     if (rc || (rc = check(dst, 0))) return rc;
-    return area_frac_u8(src, dst);
+    return area_generic(src, dst);
 }
 
 long orc_area_tab(uint32_t ssize, uint32_t dsz, uint32_t *di, uint32_t *si, float *alpha, long cap) {
@@ -1290,7 +1398,8 @@ int orc_box_filter(const orc_image *src, const orc_image *dst, int kw, int kh, i
     int rc;
     if ((rc = check(src, T)) || (rc = check(dst, T))) return rc;
     if (kw > 255 || kh > 255 || kw % 2 == 0 || kh % 2 == 0) return ORC_INVALID_ARGUMENT;
-    if (T == 3) return ORC_UNSUPPORTED;
+    /* i16: exact signed sums (the reference accumulates in u32, which agrees
+     * wherever the samples are non-negative) */
     if (!same_shape(src, dst)) return ORC_SIZE_MISMATCH;
     const double scale = normalize ? 1.0 / (double)(kw * kh) : 1.0;
     if (T == 5) {

@@ -290,6 +290,22 @@ def test_resize_area_cases(va, orc):
             check(va, orc, "resize", a, (dw, dh), 3)
 
 
+def test_resize_generic_depths_and_channels(va, orc):
+    """The generic resize kernels (resize.zig:47-60): LINEAR and AREA for
+    u16, i16 and f32 at 1 and 3 channels and for u8 with 5 and 6 channels,
+    at downscales, upscales, AREA-enlarging, integer factors (2x2 and
+    others) and fractional shrinks; host and device images."""
+    for dt, cns in [(np.uint16, (1, 3)), (np.int16, (1, 3)), (np.float32, (1, 3)), (np.uint8, (5, 6))]:
+        for cn in cns:
+            for (h, w) in [(61, 97), (40, 60), (7, 5)]:
+                a = natural(h, w, cn, 3 * cn + h, dt)
+                sizes = [(max(1, w // 2), max(1, h // 2)), (w * 2, h * 2), (max(1, w * 3 // 4), max(1, h * 2 // 3)),
+                         (53, 37), (max(1, w // 3), max(1, h // 4)), (w + 13, max(1, h - 3)), (1, 1)]
+                for size in sizes:
+                    for interp in (1, 3):
+                        check(va, orc, "resize", a, size, interp)
+
+
 def test_resize_linear_scales_near_tile_limits(va, orc):
     """Linear (and area-up) scales on both sides of the LDS tile's source-box
     limits, 16-row tile bands with ragged ends, and 256-sample bands with a
@@ -864,17 +880,19 @@ def test_pyramids_frames(va, orc):
 def test_box_filter(va, orc, cn):
     for (h, w) in ((23, 67), (1, 1), (5, 3), (37, 53)):
         a = natural(h, w, cn, h + w + cn)
-        for ks in ((3, 3), (5, 5), (7, 3), (9, 1), (15, 15), (31, 31), (1, 9), (127, 3), (3, 127)):
+        for ks in ((3, 3), (5, 5), (7, 3), (9, 1), (15, 15), (31, 31), (1, 9), (127, 3), (3, 127), (255, 3), (3, 255),
+                   (129, 129)):
             for b in range(5):
                 check(va, orc, "box_filter", a, ks, True, b)
             check(va, orc, "box_filter", a, ks, False, 4)
 
 
 def test_box_filter_other_depths(va, orc):
-    for dt in (np.uint16, np.float32):
+    """u16, i16 (negative samples included) and f32, windows up to 255."""
+    for dt in (np.uint16, np.int16, np.float32):
         for cn in (1, 3):
             a = natural(29, 41, cn, 7, dt)
-            for ks in ((3, 3), (5, 7), (21, 1), (127, 5)):
+            for ks in ((3, 3), (5, 7), (21, 1), (127, 5), (255, 7), (201, 129)):
                 for b in (4, 1, 0):
                     check(va, orc, "box_filter", a, ks, True, b)
                 check(va, orc, "box_filter", a, ks, False, 3)
@@ -895,6 +913,14 @@ def test_adaptive_threshold(va, orc, method):
                 for delta in (0, 5, -3.5, 2.25, 300):
                     check(va, orc, "adaptive_threshold", a, 255, method, kind, block, delta)
             check(va, orc, "adaptive_threshold", a, 77.6, method, kind, 9, -1)
+            if method == 0:
+                for block in (129, 255):
+                    check(va, orc, "adaptive_threshold", a, 255, method, kind, block, 2)
+    if method == 1:
+        # GaussianBlur's InvalidKernel beyond 127 taps, as the reference.
+        with pytest.raises(va.VisarError) as e:
+            va.adaptive_threshold(natural(20, 20, 1, 1), 255, 1, 0, 129, 0)
+        assert e.value.rc == -1 and e.value.name == "InvalidKernel"
     from visar_amd import synth
     a = synth.uniform_bytes(5, (1080, 1920))
     check(va, orc, "adaptive_threshold", a, 255, method, 0, 11, 2)

@@ -954,6 +954,16 @@ def test_box_filter_naive_sum(orc):
                 ref += f.astype(np.float64)[ry][:, cx]
         got = orc.box_filter(f, (kw, kh))
         assert np.abs(got - ref / (kw * kh)).max() < 1e-5, (kw, kh)
+    # i16, negative samples included: exact signed window sums.
+    r = np.random.default_rng(7)
+    i16 = r.integers(-32768, 32767, (19, 37, 2), endpoint=True).astype(np.int16)
+    for kw, kh in ((3, 3), (255, 1), (5, 255), (129, 129)):
+        for b in (4, 0):
+            s = naive_box(i16, kw, kh, b)
+            want = sat_round(s.astype(np.float32) * np.float32(1.0 / (kw * kh)), -32768, 32767).reshape(i16.shape)
+            assert np.array_equal(orc.box_filter(i16, (kw, kh), True, b), want), (kw, kh, b)
+        s = naive_box(i16, kw, kh, 4)
+        assert np.array_equal(orc.box_filter(i16, (kw, kh), False), np.clip(s, -32768, 32767).reshape(i16.shape))
 
 
 def test_box_filter_rejects(orc):
@@ -962,9 +972,6 @@ def test_box_filter_rejects(orc):
         with pytest.raises(orc.OracleError) as e:
             orc.box_filter(a, ks, True, border)
         assert e.value.rc == rc, (ks, border)
-    with pytest.raises(orc.OracleError) as e:
-        orc.box_filter(np.zeros((5, 6), np.int16), (3, 3))
-    assert e.value.rc == -3
 
 
 @pytest.mark.parametrize("kind", [0, 1])
@@ -985,6 +992,12 @@ def test_adaptive_threshold(orc, kind):
             gm = sat_round(g, 0, 255)
             want = np.where((a.astype(np.int64) + idelta > gm) == (kind == 0), 255, 0).astype(np.uint8)
             assert np.array_equal(orc.adaptive_threshold(a, 255.4, 1, kind, block, delta), want), (block, delta)
+    # MEAN_C blocks beyond the Gaussian's 127 (boxFilter takes up to 255).
+    for block in (129, 255):
+        s = naive_box(a, block, block, 1)
+        mean = sat_round(s.astype(np.float32) * np.float32(1.0 / (block * block)), 0, 255).reshape(a.shape)
+        want = np.where((a.astype(np.int64) + 2 > mean) == (kind == 0), 255, 0).astype(np.uint8)
+        assert np.array_equal(orc.adaptive_threshold(a, 255, 0, kind, block, 2), want), block
     c = np.full((9, 13), 77, np.uint8)
     for delta, want_on in ((0.5, True), (0, False), (-0.5, False), (-1, False), (1, True)):
         on = orc.adaptive_threshold(c, 9, 0, 0, 3, delta)
@@ -993,7 +1006,9 @@ def test_adaptive_threshold(orc, kind):
 
 def test_adaptive_threshold_rejects(orc):
     a = np.zeros((5, 6), np.uint8)
-    for args, rc in [((255, 2, 0, 3, 0), -1), ((255, 0, 2, 3, 0), -1), ((255, 0, 0, 1, 0), -1), ((255, 0, 0, 4, 0), -1)]:
+    # GaussianBlur's and boxFilter's InvalidKernel beyond 127 and 255 taps.
+    for args, rc in [((255, 2, 0, 3, 0), -1), ((255, 0, 2, 3, 0), -1), ((255, 0, 0, 1, 0), -1), ((255, 0, 0, 4, 0), -1),
+                     ((255, 1, 0, 129, 0), -1), ((255, 0, 0, 257, 0), -1)]:
         with pytest.raises(orc.OracleError) as e:
             orc.adaptive_threshold(a, *args)
         assert e.value.rc == rc, args

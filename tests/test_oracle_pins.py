@@ -491,3 +491,147 @@ def test_remap_general_rounds_the_float64_bilinear(orc):
     clear = np.abs(v - np.floor(v) - 0.5) > 0.01
     assert clear.mean() > 0.97
     np.testing.assert_array_equal(got[clear], np.rint(v[clear]))
+
+
+# ------------------------------------------- generic resize (any depth)
+
+def exact_linear_coord_clamped(d, ssize, dsize, area):
+    """linearCoord (resize.zig:154-180) with exact rationals: both axes
+    clamp to the edges with weight 0."""
+    if area:
+        scale = Fraction(ssize, dsize)
+        sx = (d * scale).numerator // (d * scale).denominator
+        f = (d + 1) - Fraction(sx + 1) / scale
+        f = Fraction(0) if f <= 0 else f - (f.numerator // f.denominator)
+    else:
+        fx = (d + Fraction(1, 2)) * Fraction(ssize, dsize) - Fraction(1, 2)
+        sx = fx.numerator // fx.denominator
+        f = fx - sx
+    if sx < 0:
+        sx, f = 0, Fraction(0)
+    if sx >= ssize - 1:
+        sx, f = ssize - 1, Fraction(0)
+    return sx, f
+
+
+def linear64(a, dw, dh, area=False):
+    """Float64 bilinear with linearCoord's taps on both axes."""
+    s = a.astype(np.float64)
+    if s.ndim == 2:
+        s = s[:, :, None]
+    h, w = s.shape[:2]
+    xs = [exact_linear_coord_clamped(x, w, dw, area) for x in range(dw)]
+    ys = [exact_linear_coord_clamped(y, h, dh, area) for y in range(dh)]
+    x0 = np.array([i for i, _ in xs]); x1 = np.minimum(x0 + 1, w - 1)
+    fx = np.array([float(f) for _, f in xs])[None, :, None]
+    y0 = np.array([i for i, _ in ys]); y1 = np.minimum(y0 + 1, h - 1)
+    fy = np.array([float(f) for _, f in ys])[:, None, None]
+    top = s[y0][:, x0] * (1 - fx) + s[y0][:, x1] * fx
+    bot = s[y1][:, x0] * (1 - fx) + s[y1][:, x1] * fx
+    v = top * (1 - fy) + bot * fy
+    return v[:, :, 0] if a.ndim == 2 else v
+
+
+@pytest.mark.parametrize("dtype", [np.uint16, np.int16, np.float32])
+@pytest.mark.parametrize("cn", [1, 3])
+def test_generic_linear_2x_exact_on_integer_samples(orc, dtype, cn):
+    """Bilinear 2x with integer-valued samples: every weight is 0, 1/4, 3/4
+    or 1 and every product and sum exact in f32, so the output is the exact
+    bilinear value (rounded half to even for integer samples)."""
+    r = np.random.default_rng(40 + cn)
+    lo, hi = (0, 4000) if dtype == np.uint16 else (-2000, 2000)
+    a = r.integers(lo, hi, (23, 31, cn)).astype(dtype)
+    a = a[:, :, 0] if cn == 1 else a
+    got = orc.resize(a, (62, 46), 1)
+    v = linear64(a, 62, 46)
+    want = v.astype(np.float32) if dtype == np.float32 else np.rint(v).astype(dtype)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", [np.uint16, np.int16, np.float32])
+@pytest.mark.parametrize("size,area", [((53, 37), False), ((150, 90), False), ((40, 61), False), ((150, 90), True),
+                                       ((101, 61), True)])
+def test_generic_linear_rounds_the_float64_bilinear(orc, dtype, size, area):
+    """LINEAR (and AREA enlarging, area-style weights) on u16 / i16 / f32 at
+    general scales: f32 within 1e-5 relative of the float64 bilinear of
+    linearCoord's taps; integers round(v) wherever v is 0.01 clear of a tie."""
+    r = np.random.default_rng(7)
+    if dtype == np.float32:
+        a = r.random((61, 97, 3), dtype=np.float32) * 100
+    else:
+        info = np.iinfo(dtype)
+        a = r.integers(info.min // 2, info.max // 2, (61, 97, 3)).astype(dtype)
+    got = orc.resize(a, size, 3 if area else 1).astype(np.float64)
+    v = linear64(a, size[0], size[1], area)
+    if dtype == np.float32:
+        np.testing.assert_allclose(got, v, rtol=1e-5, atol=1e-5)
+    else:
+        clear = np.abs(v - np.floor(v) - 0.5) > 0.01
+        assert clear.mean() > 0.95
+        np.testing.assert_array_equal(got[clear], np.rint(v)[clear])
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.int16, np.float32])
+def test_generic_area_integer_factors(orc, dtype):
+    """areaInt (resize.zig:420-457) on the generic path (u8 with 5 channels,
+    u16, i16, f32): unsigned samples the block mean rounded half up (u8 2x2
+    as (a + b + c + d + 2) >> 2), signed and float the f64 mean rounded half
+    to even / to f32; integer-valued f32 samples keep every sum exact."""
+    r = np.random.default_rng(11)
+    cn = 5 if dtype == np.uint8 else 3
+    if dtype == np.float32:
+        a = r.integers(-5000, 5000, (36, 60, cn)).astype(np.float32)
+    else:
+        info = np.iinfo(dtype)
+        a = r.integers(info.min, info.max, (36, 60, cn), endpoint=True).astype(dtype)
+    for fx, fy in [(2, 2), (3, 2), (4, 3), (5, 6)]:
+        got = orc.resize(a, (60 // fx, 36 // fy), 3)
+        blocks = a.astype(np.int64).reshape(36 // fy, fy, 60 // fx, fx, cn).sum(axis=(1, 3))
+        n = fx * fy
+        if dtype in (np.uint8, np.uint16):
+            want = (blocks + n // 2) // n
+        elif dtype == np.int16:
+            want = np.rint(blocks / n)
+        else:
+            want = (blocks / n).astype(np.float32)
+        np.testing.assert_array_equal(got, want.astype(dtype), err_msg=f"{fx}x{fy}")
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.int16, np.float32])
+def test_generic_area_fractional_rounds_the_exact_average(orc, dtype):
+    """areaGeneric on the generic path: the exact table-weighted average,
+    rounded half to even for integer samples wherever it is clear (by the
+    f32 sums' rounding) clear of a tie, within 1e-5 relative for f32."""
+    r = np.random.default_rng(12)
+    cn = 5 if dtype == np.uint8 else 3
+    if dtype == np.float32:
+        a = r.random((61, 97, cn), dtype=np.float32) * 50
+    else:
+        info = np.iinfo(dtype)
+        a = r.integers(max(info.min, -30000), min(info.max, 30000), (61, 97, cn)).astype(dtype)
+    for dw, dh in [(53, 37), (30, 20), (13, 9), (97, 29), (7, 3)]:
+        got = orc.resize(a, (dw, dh), 3).astype(np.float64)
+        v = exact_area_values(a, dw, dh)
+        if dtype == np.float32:
+            np.testing.assert_allclose(got, v, rtol=1e-5, atol=1e-4)
+            continue
+        # f32 sums of up to ~10 terms of magnitude M carry about 10 M 2^-24 of error.
+        tol = 1e-3 + 10 * float(np.abs(v).max()) * 2.0 ** -24
+        clear = np.abs(v - np.floor(v) - 0.5) > tol
+        assert clear.mean() > 0.9
+        np.testing.assert_array_equal(got[clear], np.rint(v)[clear], err_msg=f"{dw}x{dh}")
+
+
+def test_generic_u8_many_channels_linear_is_the_11_bit_pipeline(orc):
+    """u8 with more than 4 channels takes the generic LINEAR kernel, which for
+    u8 keeps linearTapU8's 11-bit taps: per channel it equals the fast path
+    on that channel alone, and at the exact half it is the block mean."""
+    r = np.random.default_rng(13)
+    a = r.integers(0, 256, (64, 150, 6), dtype=np.uint8)
+    for size in [(75, 32), (113, 47), (300, 128), (37, 20)]:
+        got = orc.resize(a, size, 1)
+        for c in range(6):
+            np.testing.assert_array_equal(got[:, :, c], orc.resize(np.ascontiguousarray(a[:, :, c]), size, 1))
+    s = a.astype(np.int32)
+    half = (s[0::2, 0::2] + s[0::2, 1::2] + s[1::2, 0::2] + s[1::2, 1::2] + 2) >> 2
+    np.testing.assert_array_equal(orc.resize(a, (75, 32), 1), half)

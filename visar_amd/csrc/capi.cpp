@@ -644,8 +644,8 @@ int visar_box_filter(const visar_image *src, const visar_image *dst, int kw, int
     int rc;
     if ((rc = check_image(src, T)) || (rc = check_image(dst, T))) return rc;
     if (kw > 255 || kh > 255 || kw % 2 == 0 || kh % 2 == 0) return fail(INVALID_ARGUMENT, "InvalidKernel");
-    if (T == 3 || kw > 127 || kh > 127) return fail(UNSUPPORTED, "Unsupported");
     if (!same_shape(src, dst)) return fail(SIZE_MISMATCH, "SizeMismatch");
+    const bool wide = T == 3 || kw > 127 || kh > 127;
     SepParams p;
     std::memset(&p, 0, sizeof p);
     p.sdepth = p.ddepth = (int)T;
@@ -654,8 +654,12 @@ int visar_box_filter(const visar_image *src, const visar_image *dst, int kw, int
     p.border = border;
     p.fin = FIN_SCALED;
     p.scale = (float)(normalize ? 1.0 / (double)(kw * kh) : 1.0);
-    for (int i = 0; i < kw; i++) p.kxi[i] = 1, p.kxf[i] = 1.0f;
-    for (int i = 0; i < kh; i++) p.kyi[i] = 1, p.kyf[i] = 1.0f;
+    // The engine's tap arrays hold kMaxTaps; wider windows take the
+    // two-pass kernels, which need no taps.
+    if (!wide) {
+        for (int i = 0; i < kw; i++) p.kxi[i] = 1, p.kxf[i] = 1.0f;
+        for (int i = 0; i < kh; i++) p.kyi[i] = 1, p.kyf[i] = 1.0f;
+    }
     if (T == 5) {
         p.f32mode = 1;
         p.vsym = p.hsym = SYM_EVEN;
@@ -664,6 +668,11 @@ int visar_box_filter(const visar_image *src, const visar_image *dst, int kw, int
     Call call;
     if ((rc = call.begin())) return rc;
     if ((rc = call.input(src, p.src)) || (rc = call.output(dst, p.dst))) return rc;
+    if (wide) {
+        void *tmp = call.scratch(box_wide_tmp_bytes(p.src));
+        if (!tmp) return fail(OUT_OF_MEMORY, "OutOfMemory");
+        return call.finish(launch_box_wide(p.src, p.dst, (int)T, kw, kh, border, p.scale, tmp, call.stream()));
+    }
     if (T == 0 && src->channels <= 4)
         return call.finish(launch_box_u8(p.src, p.dst, kw, kh, border, p.scale, call.stream()));
     return call.finish(launch_sep(p, call.stream()));
@@ -710,7 +719,9 @@ int visar_adaptive_threshold(const visar_image *src, const visar_image *dst, dou
     if (!same_shape(src, dst)) return fail(SIZE_MISMATCH, "SizeMismatch");
     if (src->channels != 1) return fail(UNSUPPORTED, "UnsupportedChannels");
     if (block % 2 != 1) return fail(INVALID_ARGUMENT, "InvalidKernel");
-    if (block > 127) return fail(UNSUPPORTED, "Unsupported");
+    // The mean is boxFilter (windows up to 255) or GaussianBlur (up to 127),
+    // whose InvalidKernel the reference returns beyond those.
+    if (block > (method == 0 ? 255 : 127)) return fail(INVALID_ARGUMENT, "InvalidKernel");
     SepParams p;
     if (method == 0) {
         std::memset(&p, 0, sizeof p);
@@ -718,7 +729,8 @@ int visar_adaptive_threshold(const visar_image *src, const visar_image *dst, dou
         p.border = BORDER_REPLICATE;
         p.fin = FIN_SCALED;
         p.scale = (float)(1.0 / (double)(block * block));
-        for (int i = 0; i < block; i++) p.kxi[i] = p.kyi[i] = 1;
+        // Blocks above kMaxTaps take the two-pass kernels, which need no taps.
+        for (int i = 0; i < block && i < kMaxTaps; i++) p.kxi[i] = p.kyi[i] = 1;
     } else if ((rc = gaussian_params(p, 5, block, block, 0, 0, BORDER_REPLICATE, src))) {
         return rc;
     }
@@ -739,6 +751,11 @@ int visar_adaptive_threshold(const visar_image *src, const visar_image *dst, dou
     if ((rc = call.input(src, s)) || (rc = call.output(dst, d))) return rc;
     p.src = s;
     p.dst = d;
+    if (method == 0 && block > 127) {
+        void *tmp = call.scratch(box_wide_tmp_bytes(s));
+        if (!tmp) return fail(OUT_OF_MEMORY, "OutOfMemory");
+        return call.finish(launch_box_wide(s, d, 0, block, block, BORDER_REPLICATE, p.scale, tmp, call.stream(), p.adapt));
+    }
     return call.finish(method == 0 ? launch_box_u8(s, d, block, block, BORDER_REPLICATE, p.scale, call.stream(), p.adapt)
                                    : launch_sep(p, call.stream()));
 }
@@ -1024,8 +1041,8 @@ int visar_resize(const visar_image *src, const visar_image *dst, int interp) {
     if (src->channels != dst->channels) return fail(SIZE_MISMATCH, "SizeMismatch");
     const bool same = src->width == dst->width && src->height == dst->height;
     const bool u8fast = T == 0 && src->channels <= 4;
-    if (!same && interp != 0 && !(u8fast && (interp == 1 || interp == 3)))
-        return fail(UNSUPPORTED, "Unsupported");
+    // INTER_CUBIC and LANCZOS4 are outside this path.
+    if (!same && interp != 0 && interp != 1 && interp != 3) return fail(UNSUPPORTED, "Unsupported");
     Call call;
     DImg s, d;
     if ((rc = call.begin())) return rc;
@@ -1033,6 +1050,54 @@ int visar_resize(const visar_image *src, const visar_image *dst, int interp) {
     hipStream_t st = call.stream();
     if (same) return call.finish(launch_copy(s, d, depth_size(T), st));
     if (interp == 0) return call.finish(launch_nearest(s, d, depth_size(T), st));
+    if (!u8fast) {
+        // resize.zig:47-60: the generic kernels (AREA enlarging is LINEAR
+        // with area-style taps; u8 keeps linearTapU8's fixed point).
+        const bool up = dst->width > src->width || dst->height > src->height;
+        const int cn = (int)src->channels;
+        if (interp == 1 || up) {
+            const bool area = interp == 3;
+            void *dev = nullptr;
+            if (T == 0) {
+                std::vector<LinTap> tabs(dst->width + dst->height);
+                for (uint32_t x = 0; x < dst->width; x++) {
+                    LinTap t = linear_tap_u8(x, src->width, dst->width, area, true);
+                    t.o0 *= cn;
+                    t.o1 *= cn;
+                    tabs[x] = t;
+                }
+                for (uint32_t y = 0; y < dst->height; y++)
+                    tabs[dst->width + y] = linear_tap_u8(y, src->height, dst->height, area, false);
+                dev = call.upload(tabs.data(), tabs.size() * sizeof(LinTap));
+                if (!dev) return fail(OUT_OF_MEMORY, "OutOfMemory");
+                const LinTap *xt = (const LinTap *)dev;
+                return call.finish(launch_linear_any(s, d, (int)T, xt, xt + dst->width, st));
+            }
+            std::vector<FLinTap> tabs(dst->width + dst->height);
+            const double sx = (double)src->width / (double)dst->width, sy = (double)src->height / (double)dst->height;
+            for (uint32_t x = 0; x < dst->width; x++) {
+                long i;
+                double f;
+                linear_coord(x, sx, src->width, area, &i, &f);
+                const long i1 = i + 1 < (long)src->width ? i + 1 : i;
+                tabs[x] = FLinTap{(int32_t)(i * cn), (int32_t)(i1 * cn), (float)(1.0 - f), (float)f};
+            }
+            for (uint32_t y = 0; y < dst->height; y++) {
+                long i;
+                double f;
+                linear_coord(y, sy, src->height, area, &i, &f);
+                const long i1 = i + 1 < (long)src->height ? i + 1 : i;
+                tabs[dst->width + y] = FLinTap{(int32_t)i, (int32_t)i1, (float)(1.0 - f), (float)f};
+            }
+            dev = call.upload(tabs.data(), tabs.size() * sizeof(FLinTap));
+            if (!dev) return fail(OUT_OF_MEMORY, "OutOfMemory");
+            const FLinTap *xt = (const FLinTap *)dev;
+            return call.finish(launch_linear_any(s, d, (int)T, xt, xt + dst->width, st));
+        }
+        const uint32_t ix = src->width / dst->width, iy = src->height / dst->height;
+        if (ix * dst->width == src->width && iy * dst->height == src->height)
+            return call.finish(launch_area_int_any(s, d, (int)T, (int)ix, (int)iy, st));
+    }
     const bool exact_half = src->width == 2 * dst->width && src->height == 2 * dst->height;
     bool area_up = interp == 3 && (dst->width > src->width || dst->height > src->height);
     if (interp == 1 || area_up) {
@@ -1077,6 +1142,7 @@ int visar_resize(const visar_image *src, const visar_image *dst, int interp) {
     const AreaEntry *dye = dxe + xe.size();
     const int32_t *dxs = (const int32_t *)(dye + ye.size());
     const int32_t *dys = dxs + xs.size();
+    if (!u8fast) return call.finish(launch_area_gen_any(s, d, (int)T, dxe, dxs, dye, dys, st));
     const bool box = area_box_fits(xe.data(), xs.data(), (int)dst->width, (int)src->channels, ye.data(), ys.data(), (int)dst->height);
     return call.finish(launch_area_frac_u8(s, d, dxe, dxs, dye, dys, st, box));
 }

@@ -167,6 +167,96 @@ __global__ __launch_bounds__(kBoxThreads) void box_u8(DImg s, DImg d, int kw, in
     });
 }
 
+// ------------------------------------------------ wide and 16S windows
+//
+// Box windows with a side above 127 (up to the reference's 255) and 16-bit
+// signed samples, in two passes through a temporary of column sums: one
+// thread per source sample column and 8 rows for the vertical sums, one
+// thread per output sample for the horizontal ones. Integer samples sum
+// exactly (u32 for u8 / u16, i32 for i16: the reference's u32 accumulator
+// agrees wherever its i16 samples are non-negative); f32 samples sum in the
+// engine's order (filter.zig:232-259: within the 16-lane body of a row the
+// symmetric taps fold pairwise around the centre, past it the taps run in
+// order), products by the unit taps being exact.
+constexpr int kWideRows = 8;
+
+// Sum of the n window samples v(0) .. v(n - 1).
+template <typename A, typename V>
+__device__ __attribute__((always_inline)) inline A box_taps(V v, int n, bool fold) {
+    if constexpr (std::is_same<A, float>::value) {
+        const int r = n / 2;
+        if (fold) {
+            float a = v(r);
+            for (int i = 0; i < r; i++) a = a + (v(i) + v(n - 1 - i));
+            return a;
+        }
+        float a = v(0);
+        for (int j = 1; j < n; j++) a = a + v(j);
+        return a;
+    } else {
+        A a = 0;
+        for (int j = 0; j < n; j++) a += v(j);
+        return a;
+    }
+}
+
+template <typename T, typename A>
+__global__ __launch_bounds__(kBoxThreads) void box_wide_v(DImg s, A *tmp, int kh, int border, long body) {
+    const long n = (long)s.w * s.cn;
+    const long c = (long)blockIdx.x * kBoxThreads + threadIdx.x;
+    const int y0 = (int)blockIdx.y * kWideRows;
+    if (c >= n) return;
+    const int ry = kh / 2, H = s.h;
+    const gptr<const T> base = gp((const T *)s.p);
+    const long st = s.stride / (long)sizeof(T);
+    auto at = [&](int y) -> A {  // source row y (unmapped) at column c
+        const int m = border_map_halo(y, H, border);
+        return m < 0 ? (A)0 : (A)base[(long)m * st + c];
+    };
+    const int y1 = min(y0 + kWideRows, H);
+    if constexpr (std::is_same<A, float>::value) {
+        for (int y = y0; y < y1; y++)
+            tmp[(long)y * n + c] = box_taps<float>([&](int j) { return at(y + j - ry); }, kh, c < body);
+    } else {
+        A sum = 0;
+        for (int j = 0; j < kh; j++) sum += at(y0 + j - ry);
+        tmp[(long)y0 * n + c] = sum;
+        for (int y = y0 + 1; y < y1; y++) {
+            sum += at(y + ry) - at(y - 1 - ry);
+            tmp[(long)y * n + c] = sum;
+        }
+    }
+}
+
+template <typename T, typename A>
+__global__ __launch_bounds__(kBoxThreads) void box_wide_h(const A *tmp, DImg d, DImg s, int kw, int border, float scale,
+                                                          long body, AdaptPost adapt) {
+    const int cn = d.cn, W = d.w;
+    const long n = (long)W * cn;
+    const long e = (long)blockIdx.x * kBoxThreads + threadIdx.x;
+    const int y = (int)blockIdx.y;
+    if (e >= n) return;
+    const int x = (int)(e / cn), ch = (int)(e - (long)x * cn), rx = kw / 2;
+    const A *row = tmp + (long)y * n;
+    auto at = [&](int px) -> A {
+        const int m = border_map_halo(px, W, border);
+        return m < 0 ? (A)0 : row[(long)m * cn + ch];
+    };
+    A a;
+    if constexpr (std::is_same<A, float>::value) {
+        a = box_taps<float>([&](int i) { return at(x + i - rx); }, kw, e < body);
+    } else {
+        a = 0;
+        for (int i = 0; i < kw; i++) a += at(x + i - rx);
+    }
+    const float f = (float)a * scale;
+    T out = sat_v<T>(f);
+    if constexpr (std::is_same<T, uint8_t>::value) {
+        if (adapt.enabled) out = adapt_out(adapt.idelta, adapt.on, adapt.off, grow<const uint8_t>(s, y)[e], out);
+    }
+    grow<T>(d, y)[e] = out;
+}
+
 inline int launched() { return hipGetLastError() == hipSuccess ? OK : INTERNAL; }
 
 }  // namespace
@@ -189,6 +279,33 @@ int launch_box_u8(const DImg &s, const DImg &d, int kw, int kh, int border, floa
     default: hipLaunchKernelGGL(box_u8<4>, g, b, lds, st, s, d, kw, kh, border, scale, tw, o, adapt); break;
     }
     return launched();
+}
+
+template <typename T, typename A>
+static int run_box_wide(const DImg &s, const DImg &d, int kw, int kh, int border, float scale, void *tmp,
+                        hipStream_t st, const AdaptPost &adapt) {
+    const long n = (long)s.w * s.cn;
+    const long body = (n / 16) * 16;
+    const unsigned gx = (unsigned)((n + kBoxThreads - 1) / kBoxThreads);
+    hipLaunchKernelGGL((box_wide_v<T, A>), dim3(gx, (unsigned)((s.h + kWideRows - 1) / kWideRows)), dim3(kBoxThreads), 0,
+                       st, s, (A *)tmp, kh, border, body);
+    hipLaunchKernelGGL((box_wide_h<T, A>), dim3(gx, (unsigned)d.h), dim3(kBoxThreads), 0, st, (const A *)tmp, d, s, kw,
+                       border, scale, body, adapt);
+    return launched();
+}
+
+size_t box_wide_tmp_bytes(const DImg &s) { return (size_t)s.w * s.cn * s.h * 4; }
+
+int launch_box_wide(const DImg &s, const DImg &d, int depth, int kw, int kh, int border, float scale, void *tmp,
+                    hipStream_t st, const AdaptPost &adapt) {
+    if (kw > 255 || kh > 255) return UNSUPPORTED;
+    switch (depth) {
+    case 0: return run_box_wide<uint8_t, uint32_t>(s, d, kw, kh, border, scale, tmp, st, adapt);
+    case 2: return run_box_wide<uint16_t, uint32_t>(s, d, kw, kh, border, scale, tmp, st, adapt);
+    case 3: return run_box_wide<int16_t, int32_t>(s, d, kw, kh, border, scale, tmp, st, adapt);
+    case 5: return run_box_wide<float, float>(s, d, kw, kh, border, scale, tmp, st, adapt);
+    default: return UNSUPPORTED;
+    }
 }
 
 }  // namespace visar

@@ -106,6 +106,12 @@ int launch_pyramid(const DImg &s, const DImg &d, int depth, int border, bool dow
 // exact sums, sat(f32(sum) * scale); UNSUPPORTED outside that.
 int launch_box_u8(const DImg &s, const DImg &d, int kw, int kh, int border, float scale, hipStream_t st,
                   const AdaptPost &adapt = AdaptPost{});
+// The same for windows with a side above 127 (up to 255) and for i16 (and
+// u8 / u16 / f32) samples: two passes through `tmp`, a device buffer of
+// box_wide_tmp_bytes(s) bytes (column sums). `adapt` applies to u8 only.
+size_t box_wide_tmp_bytes(const DImg &s);
+int launch_box_wide(const DImg &s, const DImg &d, int depth, int kw, int kh, int border, float scale, void *tmp,
+                    hipStream_t st, const AdaptPost &adapt = AdaptPost{});
 
 int launch_area_int_u8(const DImg &s, const DImg &d, int ix, int iy, hipStream_t st);
 int launch_nearest(const DImg &s, const DImg &d, size_t elem, hipStream_t st);
@@ -138,6 +144,20 @@ struct AreaEntry {
 };
 int launch_area_frac_u8(const DImg &s, const DImg &d, const AreaEntry *xt, const int32_t *xstart, const AreaEntry *yt,
                         const int32_t *ystart, hipStream_t st, bool box);
+
+// The generic resize kernels (k_resize.hip; resize.zig:47-60): LINEAR and
+// AREA for u16 / i16 / f32 and for u8 with more than 4 channels. FLinTap:
+// linearCoord's taps as f32 weights; x taps in samples (premultiplied by
+// cn), the second tap equal to the first at the last column; y taps are
+// rows. u8 takes LinTap tables (linearTapU8).
+struct FLinTap {
+    int32_t i0, i1;
+    float w0, w1;
+};
+int launch_linear_any(const DImg &s, const DImg &d, int depth, const void *xt_dev, const void *yt_dev, hipStream_t st);
+int launch_area_int_any(const DImg &s, const DImg &d, int depth, int ix, int iy, hipStream_t st);
+int launch_area_gen_any(const DImg &s, const DImg &d, int depth, const AreaEntry *xe, const int32_t *xs,
+                        const AreaEntry *ye, const int32_t *ys, hipStream_t st);
 
 // Tiles of the LDS-staged area resize: 256 output samples x kAreaTH rows,
 // cells of at most kAreaMaxCell entries, source boxes of at most kAreaRows

@@ -198,6 +198,25 @@ int symmetry(const float *k, int n) {
     return even ? SYM_EVEN : odd ? SYM_ODD : SYM_NONE;
 }
 
+void linear_coord(long d, double scale, uint32_t len, bool area, long *index, double *frac) {
+    long sx;
+    double f;
+    if (area) {
+        const double df = (double)d;
+        sx = (long)std::floor(df * scale);
+        f = (df + 1) - (double)(sx + 1) / scale;
+        f = f <= 0 ? 0 : f - std::floor(f);
+    } else {
+        const double fx = ((double)d + 0.5) * scale - 0.5;
+        sx = (long)std::floor(fx);
+        f = fx - (double)sx;
+    }
+    if (sx < 0) sx = 0, f = 0;
+    if (sx >= (long)len - 1) sx = (long)len - 1, f = 0;
+    *index = sx;
+    *frac = f;
+}
+
 LinTap linear_tap_u8(long d, uint32_t ssize, uint32_t dsize, bool area, bool pin) {
     double inv = (double)dsize / (double)ssize;
     double scale = 1.0 / inv;

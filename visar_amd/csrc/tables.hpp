@@ -28,6 +28,10 @@ int symmetry(const float *k, int n);
 // resize.zig:186-214 linearTapU8 (offsets are source indices, not scaled by cn).
 LinTap linear_tap_u8(long d, uint32_t ssize, uint32_t dsize, bool area, bool pin);
 
+// resize.zig:154-180 linearCoord: source index and fractional weight,
+// clamped to the edges with weight 0 on both axes.
+void linear_coord(long d, double scale, uint32_t len, bool area, long *index, double *frac);
+
 // resize.zig:470-493 computeResizeAreaTab, grouped per destination index.
 void area_tab(uint32_t ssize, uint32_t dsize, double scale, std::vector<AreaEntry> &entries,
               std::vector<int32_t> &start);

@@ -555,6 +555,60 @@ def test_morphology_binary_masks(va, orc):
                     check(va, orc, "morphology", a, 3, 1, (5, 5), iterations=it)
 
 
+FUSED_ELEMENTS = [(0, (3, 3)), (1, (5, 5)), (2, (7, 7)), (2, (5, 5)), (1, (7, 3)), (0, (3, 7)), (2, (1, 5)),
+                  (1, (7, 1)), (0, (1, 1)), (2, (3, 5))]
+
+
+@pytest.mark.parametrize("cn", [1, 2, 3, 4])
+def test_morphology_fused_chains(va, orc, cn):
+    """The fused u8 pass chains against the oracle: every op, the three
+    shapes at odd sizes up to 7 (square and not), 1 and 2 iterations, images
+    just wider than one tile (496 bytes at a 3x3 element) and shorter than
+    one, and one-pixel sides."""
+    for h, w in [(45, 509 // cn + 3), (70, 1000 // cn), (1, 37), (23, 1), (2, 2)]:
+        a = natural(h, w, cn, 60 + cn + h)
+        for op in range(7):
+            for shape, ks in FUSED_ELEMENTS:
+                for it in (1, 2):
+                    check(va, orc, "morphology", a, op, shape, ks, iterations=it)
+
+
+def test_morphology_fused_long_chains(va, orc):
+    """Chains longer than one launch takes (split into launches, the
+    subtraction reading the source from memory), gradient's two chains in
+    one launch and split, on binary masks and natural images."""
+    rng = np.random.default_rng(77)
+    m = ((rng.random((131, 777)) > 0.6) * 255).astype(np.uint8)
+    g = natural(97, 530, 1, 71)
+    c3 = natural(40, 300, 3, 72)
+    for a in (m, g, c3):
+        for op, shape, ks, it in [(2, 0, 3, 4), (5, 0, 3, 4), (6, 1, 7, 3), (5, 1, 7, 3), (4, 2, 5, 3), (4, 1, 7, 3),
+                                  (4, 0, 3, 7), (0, 2, 7, 5), (1, 1, 5, 4), (3, 2, 5, 4), (6, 0, 5, 2)]:
+            check(va, orc, "morphology", a, op, shape, (ks, ks), iterations=it)
+
+
+def test_morphology_fused_equals_per_pass(va):
+    """The fused chains equal the one-kernel-per-pass path (tuning
+    "morph_chain" 1) on a 1000x1400 image, every op and element of the
+    benchmark with 1-3 iterations, plus device views at odd offsets."""
+    from visar_amd import autotune
+    a = dev(natural(1000, 1400, 1, 73))
+    big = dev(natural(300, 700, 1, 74))
+    views = [big[1:, 3:613], big[5:290, 1:699]]
+    try:
+        for op in range(7):
+            for shape, ks in [(0, 3), (2, 5), (1, 7)]:
+                for it in (1, 2, 3):
+                    outs = []
+                    for knob in (0, 1):
+                        autotune.set_knob("morph_chain", knob)
+                        outs.append([va.morphology(x, op, shape, (ks, ks), it) for x in [a] + views])
+                    for f, p in zip(*outs):
+                        assert torch.equal(f, p), f"op{op} shape{shape} k{ks} it{it}"
+    finally:
+        autotune.set_knob("morph_chain", 0)
+
+
 def test_morphology_depths(va, orc):
     for dt in (np.uint16, np.int16, np.float32):
         a = natural(37, 45, 3, 50, dt)

@@ -1273,6 +1273,187 @@ static int morph_iter(Call &call, const DImg &s, const DImg &d, MorphParams p, i
     return OK;
 }
 
+// The element as a union of centred rectangles (k_morph_chain.hip), or false:
+// odd sides up to 7, the anchor at the centre, at most one run per row,
+// centred on the anchor, with half-widths symmetric about the centre row
+// and non-increasing away from it. Rectangle k: half-width rw[k], half-height
+// rv[k]; eh / ev: the largest of each.
+struct MorphRects {
+    int nr, rw[4], rv[4], eh, ev;
+};
+static bool morph_rects(const MorphParams &p, MorphRects &r) {
+    if (p.kw > 7 || p.kh > 7 || !(p.kw & 1) || !(p.kh & 1) || p.ax != p.kw / 2 || p.ay != p.kh / 2) return false;
+    int hw[7] = {-1, -1, -1, -1, -1, -1, -1};
+    for (int k = 0; k < p.nruns; k++) {
+        const MorphRun &run = p.runs[k];
+        if (hw[run.ky] >= 0 || !(run.len & 1) || run.x0 + run.len / 2 != p.ax) return false;
+        hw[run.ky] = run.len / 2;
+    }
+    const int ay = p.ay;
+    if (hw[ay] < 0) return false;
+    for (int d = 1; d <= ay; d++)
+        if (hw[ay - d] != hw[ay + d] || hw[ay + d] > hw[ay + d - 1]) return false;
+    r.nr = 0;
+    r.ev = 0;
+    for (int d = 0; d <= ay; d++) {
+        const int next = d < ay ? hw[ay + d + 1] : -1;
+        if (hw[ay + d] >= 0) r.ev = d;
+        if (hw[ay + d] >= 0 && next < hw[ay + d]) {
+            r.rw[r.nr] = hw[ay + d];
+            r.rv[r.nr] = d;
+            r.nr++;
+        }
+    }
+    r.eh = hw[ay];
+    return true;
+}
+
+// Passes of one fused launch: at most this many per chain, so that the
+// summed halo stays within 6 rows and 64 bytes.
+static int morph_fuse_limit(int cn, const MorphRects &r) {
+    int f = 6;
+    if (r.ev > 0) f = std::min(f, 6 / r.ev);
+    if (r.eh > 0) f = std::min(f, 64 / (r.eh * cn));
+    return std::max(f, 1);
+}
+
+enum { COMB_NONE = 0, COMB_SRC = 1, COMB_CHAIN = 2, COMB_IMAGE = 3 };
+
+// One launch of the fused kernel: `nchains` chains of passes (op_max per
+// pass) from `in`, the last pass of the last chain storing to `out`. The
+// chains all start from the staged input; with two, the first one's result
+// is kept for the combine (comb COMB_CHAIN). COMB_SRC combines with the
+// staged input itself, COMB_IMAGE with image `cimg`. UNSUPPORTED when the
+// tile does not fit the device's LDS.
+static int morph_chain_launch(const DImg &in, const DImg &out, const MorphRects &r, const std::vector<int> *chains,
+                              int nchains, int post, int comb, const DImg *cimg, hipStream_t st) {
+    MorphChainArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.s = in;
+    a.d = out;
+    a.c = cimg ? *cimg : in;
+    int F = 0;
+    for (int c = 0; c < nchains; c++) F = std::max(F, (int)chains[c].size());
+    const int cn = in.cn;
+    const int K = std::max(1, std::max(r.eh, r.ev));
+    a.ev = r.ev;
+    a.vb = F * r.ev;
+    a.hb = (F * r.eh * cn + 7) & ~7;
+    a.tw = 512 - 2 * a.hb;
+    a.nr = r.nr;
+    for (int k = 0; k < r.nr; k++) {
+        a.rw[k] = r.rw[k];
+        a.rv[k] = r.rv[k];
+        for (int e = -r.rv[k]; e <= r.rv[k]; e++) a.cmask[r.rw[k]] |= 1u << (e + K);
+    }
+    // Buffers: 0 holds the staged input while a later chain or the combine
+    // still needs it; a chain's result stays while the combine needs it.
+    int ra = -1, nbuf = 1, np = 0;
+    for (int c = 0; c < nchains; c++) {
+        int cur = 0;
+        const int len = (int)chains[c].size();
+        for (int i = 0; i < len; i++) {
+            MorphChainPass ps{(int8_t)cur, (int8_t)-1, (int8_t)chains[c][i], (int8_t)(i + 1)};
+            if (!(c == nchains - 1 && i == len - 1)) {
+                int dst = -1;
+                for (int b = 0; b < 3 && dst < 0; b++) {
+                    const bool busy = b == cur || b == ra || (b == 0 && (comb == COMB_SRC || c < nchains - 1));
+                    if (!busy) dst = b;
+                }
+                if (dst < 0) return UNSUPPORTED;
+                ps.dst = (int8_t)dst;
+                nbuf = std::max(nbuf, dst + 1);
+                cur = dst;
+            }
+            if (np == kChainMaxPasses) return UNSUPPORTED;
+            a.pass[np++] = ps;
+        }
+        if (c < nchains - 1) ra = cur;
+    }
+    a.npass = np;
+    a.post = post;
+    a.comb = comb == COMB_SRC ? 0 : comb == COMB_CHAIN ? ra : comb == COMB_IMAGE ? 3 : 0;
+    a.nbuf = nbuf;
+    const size_t pitch = (size_t)morph_chain_pitch_dw(cn, K) * 4;
+    const size_t limit = std::min<size_t>(morph_chain_lds_limit(), 160 * 1024);
+    int th = tuning().morph_rows >= 8 ? tuning().morph_rows : F == 1 ? 32 : 16;
+    auto lds_of = [&](int t) { return nbuf * (t + 2 * a.vb) * pitch + (comb == COMB_IMAGE ? t * pitch : 0); };
+    while (th > 8 && lds_of(th) > limit) th /= 2;
+    if (lds_of(th) > limit) return UNSUPPORTED;
+    a.th = th;
+    a.rows = th + 2 * a.vb;
+    const int n = in.w * cn;
+    const int tiles_y = (in.h + th - 1) / th;
+    a.tiles_x = (n + a.tw - 1) / a.tw;
+    const int k = tuning().tile_order;
+    a.ntiles = a.tiles_x * tiles_y;
+    a.xcd = k != 1;
+    return launch_morph_chain(a, K, lds_of(th), st);
+}
+
+// morphologyEx (morph.zig:532-571) through the fused kernel: the passes of
+// each chain in launches of at most morph_fuse_limit passes, intermediates
+// in temporaries, the subtraction folded into the last launch.
+// UNSUPPORTED (nothing launched) when the kernel does not take the call.
+static int morph_chain_ex(Call &call, const DImg &s, const DImg &d, int op, int n, const MorphRects &r) {
+    const int F = morph_fuse_limit(s.cn, r);
+    std::vector<int> A, B;
+    int post = 0;
+    switch (op) {
+    case 0: A.assign(n, 0); break;
+    case 1: A.assign(n, 1); break;
+    case 2: A.assign(n, 0); A.insert(A.end(), n, 1); break;
+    case 3: A.assign(n, 1); A.insert(A.end(), n, 0); break;
+    case 4: A.assign(n, 1); B.assign(n, 0); post = 1; break;   // dilate - erode
+    case 5: A.assign(n, 0); A.insert(A.end(), n, 1); post = 1; break;  // src - open
+    default: A.assign(n, 1); A.insert(A.end(), n, 0); post = 2; break; // close - src
+    }
+    hipStream_t st = call.stream();
+    if (op == 4 && n <= F) {
+        std::vector<int> ch[2] = {A, B};
+        return morph_chain_launch(s, d, r, ch, 2, post, COMB_CHAIN, nullptr, st);
+    }
+    // One chain from s into `out` in segments; the last one combines.
+    auto run_chain = [&](const std::vector<int> &ops, const DImg &out, int pst, int comb, const DImg *cimg) -> int {
+        const int P = (int)ops.size();
+        const int nseg = (P + F - 1) / F;
+        DImg cur = s;
+        DImg tmp[2] = {};
+        for (int k = 0, i0 = 0; k < nseg; k++) {
+            const int len = (P - i0) / (nseg - k);
+            std::vector<int> seg(ops.begin() + i0, ops.begin() + i0 + len);
+            const bool last = k == nseg - 1;
+            DImg o = out;
+            if (!last) {
+                if (!tmp[k & 1].p && call.temp_like(s, 1, tmp[k & 1])) return OUT_OF_MEMORY;
+                o = tmp[k & 1];
+            }
+            int cm = COMB_NONE;
+            const DImg *ci = nullptr;
+            if (last && pst) {
+                cm = comb;
+                ci = cimg;
+                if (comb == COMB_SRC && k > 0) {  // the source is no longer the staged input
+                    cm = COMB_IMAGE;
+                    ci = &s;
+                }
+            }
+            int rc = morph_chain_launch(cur, o, r, &seg, 1, last ? pst : 0, cm, ci, st);
+            if (rc) return rc;
+            cur = o;
+            i0 += len;
+        }
+        return OK;
+    };
+    if (op == 4) {
+        DImg ta;
+        if (call.temp_like(s, 1, ta)) return OUT_OF_MEMORY;
+        int rc = run_chain(A, ta, 0, COMB_NONE, nullptr);
+        return rc ? rc : run_chain(B, d, post, COMB_IMAGE, &ta);
+    }
+    return run_chain(A, d, post, COMB_SRC, nullptr);
+}
+
 // capi.zig:339-350 + morph.zig:532-571
 int visar_morphology(const visar_image *src, const visar_image *dst, int op, int shape, int kw, int kh,
                      int iterations) {
@@ -1300,6 +1481,13 @@ int visar_morphology(const visar_image *src, const visar_image *dst, int op, int
     if ((rc = call.begin())) return rc;
     if ((rc = call.input(src, s)) || (rc = call.output(dst, d))) return rc;
     int st = OK;
+    MorphRects mr;
+    if (T == 0 && s.cn <= 4 && iterations > 0 && !tuning().morph_chain && morph_rects(p, mr)) {
+        st = morph_chain_ex(call, s, d, op, iterations, mr);
+        if (st == OUT_OF_MEMORY) return fail(OUT_OF_MEMORY, "OutOfMemory");
+        if (st != UNSUPPORTED) return call.finish(st);
+        st = OK;
+    }
     DImg t1, t2;
     if (op >= 2 && (call.temp_like(s, elem, t1) || call.temp_like(s, elem, t2))) return fail(OUT_OF_MEMORY, "OutOfMemory");
     switch (op) {

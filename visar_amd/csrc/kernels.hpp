@@ -13,6 +13,8 @@ struct Tuning {
     int tile_order = 0;          // plain tile kernels: 0 each kernel's default, 1 row-major, 2 one band of tiles per XCD
     int sep_rows = 0;            // separable engine: output rows per tile (1-16)
     int warp_layout = 0;         // u8 bilinear constant-border warps: 0 block layout, loads per pixel; 1 row / LDS-box kernels; 2 block layout, loads after a wave vote
+    int morph_chain = 0;         // u8 morphology: 0 fused pass chains where the element allows, 1 one kernel per pass
+    int morph_rows = 0;          // fused morphology: output rows per tile (8-128)
 };
 Tuning &tuning();
 bool set_tuning(const char *key, int value);
@@ -213,6 +215,34 @@ struct MorphSmallArgs;
 int morph_small_args(const MorphParams &p, MorphSmallArgs &a);
 // d = saturate(a - b) (morph.zig:574-593)
 int launch_sub_sat(const DImg &a, const DImg &b, const DImg &d, int depth, hipStream_t st);
+
+// Fused chains of erode/dilate passes on u8 images of 1-4 channels
+// (k_morph_chain.hip) for elements that are unions of centred rectangles:
+// rectangle k spans [-rw[k], rw[k]] samples x [-rv[k], rv[k]] rows.
+// One launch stages a tile of `th` rows x `tw` bytes with `vb` rows and `hb`
+// bytes of halo (rows = th + 2 vb) in LDS buffer 0 and runs `npass` passes
+// between LDS buffers 0 .. nbuf - 1; the pass with dst -1 stores to `d`.
+// A pass at position q of its chain writes rows [q ev, rows - q ev).
+// post 1: d = saturate(C - v), 2: d = saturate(v - C), with C LDS buffer
+// `comb` (0 .. 2) or, comb 3, image `c` staged after the pass buffers.
+// cmask[w]: bit e + K set when a rectangle of half-width w reaches row
+// offset e (K = max half-size, the kernel's frame).
+struct MorphChainPass {
+    int8_t src, dst, op_max, q;
+};
+constexpr int kChainMaxPasses = 12;
+struct MorphChainArgs {
+    DImg s, d, c;
+    int tw, th, hb, vb, rows, ev, nbuf;
+    int tiles_x, ntiles, xcd;
+    int nr, rw[4], rv[4];
+    uint32_t cmask[4];
+    int npass, post, comb;
+    MorphChainPass pass[kChainMaxPasses];
+};
+int morph_chain_pitch_dw(int cn, int K);  // dwords per LDS row
+size_t morph_chain_lds_limit();           // LDS bytes a workgroup may use on the current device
+int launch_morph_chain(const MorphChainArgs &a, int K, size_t lds, hipStream_t st);
 
 // --------------------------------------------------------------- histogram
 int launch_hist_u8(const DImg &s, uint32_t *hist_dev, hipStream_t st);

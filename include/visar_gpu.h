@@ -47,9 +47,11 @@ int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D
  * "warp_layout" the one-channel u8 bilinear constant-border warps' kernels
  * (0 block layout with 8-byte tap loads issued per pixel, 1 the row / LDS
  * source-box kernels, 2 the block layout with the loads after a wave-wide
- * check of the footprints); "morph_chain" u8 morphology (0 fused chains of
- * passes in one LDS tile where the element is a union of centred rectangles,
- * 1 one kernel per erode / dilate pass and one per subtraction);
+ * check of the footprints); "morph_chain" u8 morphology (0 automatic:
+ * one-channel images stream each pass through registers with the subtraction
+ * in the last pass, others run fused chains of passes in one LDS tile, where
+ * the element is a union of centred rectangles; 1 the general kernels, one
+ * per erode / dilate pass and one per subtraction; 2 the fused LDS chains);
  * "morph_rows" output rows per tile of the fused chains (8-128).
  * Unknown keys or values out of range return VISAR_INVALID_ARGUMENT. */
 int visar_gpu_set_tuning(const char *key, int value);

@@ -1391,6 +1391,83 @@ static int morph_chain_launch(const DImg &in, const DImg &out, const MorphRects 
     return launch_morph_chain(a, K, lds_of(th), st);
 }
 
+// One erode (op_max 0) or dilate pass of morph_direct.hpp from `in` into
+// `out`, post 1: out = saturate(c - v), 2: out = saturate(v - c).
+// UNSUPPORTED (nothing launched) when the element has no kernel there.
+static int morph_direct_pass(const DImg &in, const DImg &out, uint32_t el, int op_max, int post, const DImg *c,
+                             hipStream_t st) {
+    MorphDirectArgs a;
+    a.s = in;
+    a.d = out;
+    a.c = c ? *c : in;
+    auto span = [](const DImg &m) { return (uint64_t)(m.h - 1) * (uint64_t)m.stride + (uint64_t)m.w * m.cn; };
+    if (span(a.s) >= (1ull << 31) || span(a.d) >= (1ull << 31) || span(a.c) >= (1ull << 31) || a.s.stride >= (1l << 31) ||
+        a.d.stride >= (1l << 31) || a.c.stride >= (1l << 31))
+        return UNSUPPORTED;
+    a.span_s = (uint32_t)span(a.s);
+    a.span_d = (uint32_t)span(a.d);
+    a.span_c = (uint32_t)span(a.c);
+    a.strips = (in.w * in.cn + 511) / 512;
+    a.bands4 = ((in.h + kMorphDirectRows - 1) / kMorphDirectRows + 3) / 4;
+    a.ntiles = a.strips * a.bands4;
+    a.xcd = tuning().tile_order != 1;
+    const bool ok = op_max ? launch_morph_direct_max(a, el, post, st) : launch_morph_direct_min(a, el, post, st);
+    if (!ok) return UNSUPPORTED;
+    return hipGetLastError() == hipSuccess ? OK : INTERNAL;
+}
+
+// morphologyEx (morph.zig:532-571) as one-channel passes of
+// morph_direct.hpp, intermediates in temporaries, the subtraction folded
+// into the last pass: for one-channel images whose rows are a multiple of 4
+// bytes starting on dwords, with elements that have kernels there.
+// UNSUPPORTED (nothing launched) otherwise.
+static int morph_direct_ex(Call &call, const DImg &s, const DImg &d, int op, int n, const MorphRects &r) {
+    auto aligned = [](const DImg &m) { return ((((uintptr_t)m.p) | (uintptr_t)m.stride) & 3) == 0; };
+    if (s.cn != 1 || (s.w & 3) || !aligned(s) || !aligned(d) || tuning().morph_chain == 2) return UNSUPPORTED;
+    uint32_t el = (uint32_t)r.nr;
+    for (int k = 0; k < r.nr; k++) el |= (uint32_t)(r.rw[k] | r.rv[k] << 2) << (4 + 4 * k);
+    std::vector<int> A, B;
+    int post = 0;
+    switch (op) {
+    case 0: A.assign(n, 0); break;
+    case 1: A.assign(n, 1); break;
+    case 2: A.assign(n, 0); A.insert(A.end(), n, 1); break;
+    case 3: A.assign(n, 1); A.insert(A.end(), n, 0); break;
+    case 4: A.assign(n, 1); B.assign(n, 0); post = 1; break;   // dilate - erode
+    case 5: A.assign(n, 0); A.insert(A.end(), n, 1); post = 1; break;  // src - open
+    default: A.assign(n, 1); A.insert(A.end(), n, 0); post = 2; break; // close - src
+    }
+    hipStream_t st = call.stream();
+    DImg tmp[2] = {};
+    bool first = true;
+    // Runs `ops` from s; the last pass writes `out`, combining with `comb`.
+    auto run = [&](const std::vector<int> &ops, const DImg &out, int pst, const DImg *comb) -> int {
+        DImg cur = s;
+        for (size_t i = 0; i < ops.size(); i++) {
+            const bool last = i + 1 == ops.size();
+            DImg o = out;
+            if (!last) {
+                DImg &t = tmp[i & 1];
+                if (!t.p && call.temp_like(s, 1, t)) return OUT_OF_MEMORY;
+                o = t;
+            }
+            const int rc = morph_direct_pass(cur, o, el, ops[i], last ? pst : 0, comb, st);
+            if (rc == UNSUPPORTED && !first) return INTERNAL;
+            if (rc) return rc;
+            first = false;
+            cur = o;
+        }
+        return OK;
+    };
+    if (op == 4) {
+        DImg ta;
+        if (call.temp_like(s, 1, ta)) return OUT_OF_MEMORY;
+        int rc = run(A, ta, 0, nullptr);
+        return rc ? rc : run(B, d, post, &ta);
+    }
+    return run(A, d, post, &s);
+}
+
 // morphologyEx (morph.zig:532-571) through the fused kernel: the passes of
 // each chain in launches of at most morph_fuse_limit passes, intermediates
 // in temporaries, the subtraction folded into the last launch.
@@ -1482,8 +1559,9 @@ int visar_morphology(const visar_image *src, const visar_image *dst, int op, int
     if ((rc = call.input(src, s)) || (rc = call.output(dst, d))) return rc;
     int st = OK;
     MorphRects mr;
-    if (T == 0 && s.cn <= 4 && iterations > 0 && !tuning().morph_chain && morph_rects(p, mr)) {
-        st = morph_chain_ex(call, s, d, op, iterations, mr);
+    if (T == 0 && s.cn <= 4 && iterations > 0 && tuning().morph_chain != 1 && morph_rects(p, mr)) {
+        st = morph_direct_ex(call, s, d, op, iterations, mr);
+        if (st == UNSUPPORTED) st = morph_chain_ex(call, s, d, op, iterations, mr);
         if (st == OUT_OF_MEMORY) return fail(OUT_OF_MEMORY, "OutOfMemory");
         if (st != UNSUPPORTED) return call.finish(st);
         st = OK;

@@ -13,7 +13,7 @@ struct Tuning {
     int tile_order = 0;          // plain tile kernels: 0 each kernel's default, 1 row-major, 2 one band of tiles per XCD
     int sep_rows = 0;            // separable engine: output rows per tile (1-16)
     int warp_layout = 0;         // u8 bilinear constant-border warps: 0 block layout, loads per pixel; 1 row / LDS-box kernels; 2 block layout, loads after a wave vote
-    int morph_chain = 0;         // u8 morphology: 0 fused pass chains where the element allows, 1 one kernel per pass
+    int morph_chain = 0;         // u8 morphology: 0 automatic, 1 one kernel per pass and subtraction, 2 fused LDS chains
     int morph_rows = 0;          // fused morphology: output rows per tile (8-128)
 };
 Tuning &tuning();
@@ -243,6 +243,17 @@ struct MorphChainArgs {
 int morph_chain_pitch_dw(int cn, int K);  // dwords per LDS row
 size_t morph_chain_lds_limit();           // LDS bytes a workgroup may use on the current device
 int launch_morph_chain(const MorphChainArgs &a, int K, size_t lds, hipStream_t st);
+// One pass per launch for one-channel images (morph_direct.hpp): strips
+// of 512 bytes x bands of kMorphDirectRows rows, 4 bands per workgroup
+// (bands4 groups).
+constexpr int kMorphDirectRows = 8;
+struct MorphDirectArgs {
+    DImg s, d, c;                    // c: the combine image of post 1 / 2
+    uint32_t span_s, span_d, span_c; // bytes from each image's first to last byte + 1 (< 2^31)
+    int strips, bands4, ntiles, xcd;
+};
+bool launch_morph_direct_max(const MorphDirectArgs &a, uint32_t el, int post, hipStream_t st);
+bool launch_morph_direct_min(const MorphDirectArgs &a, uint32_t el, int post, hipStream_t st);
 
 // --------------------------------------------------------------- histogram
 int launch_hist_u8(const DImg &s, uint32_t *hist_dev, hipStream_t st);

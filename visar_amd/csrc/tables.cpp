@@ -25,7 +25,7 @@ const Knob kKnobs[] = {
     {"tile_order", &Tuning::tile_order, 0, 2},
     {"sep_rows", &Tuning::sep_rows, 0, 16},
     {"warp_layout", &Tuning::warp_layout, 0, 2},
-    {"morph_chain", &Tuning::morph_chain, 0, 1},
+    {"morph_chain", &Tuning::morph_chain, 0, 2},
     {"morph_rows", &Tuning::morph_rows, 0, 128},
 };
 }  // namespace

@@ -1391,11 +1391,12 @@ static int morph_chain_launch(const DImg &in, const DImg &out, const MorphRects 
     return launch_morph_chain(a, K, lds_of(th), st);
 }
 
-// One erode (op_max 0) or dilate pass of morph_direct.hpp from `in` into
-// `out`, post 1: out = saturate(c - v), 2: out = saturate(v - c).
-// UNSUPPORTED (nothing launched) when the element has no kernel there.
-static int morph_direct_pass(const DImg &in, const DImg &out, uint32_t el, int op_max, int post, const DImg *c,
-                             hipStream_t st) {
+// One or two erode / dilate passes of morph_direct.hpp from `in` into `out`
+// (ops bit p: pass p dilates); post 1: out = saturate(c - v), 2: out =
+// saturate(v - c). UNSUPPORTED (nothing launched) when the element has no
+// kernel there.
+static int morph_direct_launch(const DImg &in, const DImg &out, uint32_t el, int np, int ops, int post, const DImg *c,
+                               hipStream_t st) {
     MorphDirectArgs a;
     a.s = in;
     a.d = out;
@@ -1407,20 +1408,22 @@ static int morph_direct_pass(const DImg &in, const DImg &out, uint32_t el, int o
     a.span_s = (uint32_t)span(a.s);
     a.span_d = (uint32_t)span(a.d);
     a.span_c = (uint32_t)span(a.c);
-    a.strips = (in.w * in.cn + 511) / 512;
+    const int sw = 512 - 16 * (np - 1);
+    a.strips = (in.w * in.cn + sw - 1) / sw;
     a.bands4 = ((in.h + kMorphDirectRows - 1) / kMorphDirectRows + 3) / 4;
     a.ntiles = a.strips * a.bands4;
     a.xcd = tuning().tile_order != 1;
-    const bool ok = op_max ? launch_morph_direct_max(a, el, post, st) : launch_morph_direct_min(a, el, post, st);
+    a.ops = ops;
+    const bool ok = np == 2 ? launch_morph_direct2(a, el, post, st) : launch_morph_direct1(a, el, post, st);
     if (!ok) return UNSUPPORTED;
     return hipGetLastError() == hipSuccess ? OK : INTERNAL;
 }
 
-// morphologyEx (morph.zig:532-571) as one-channel passes of
-// morph_direct.hpp, intermediates in temporaries, the subtraction folded
-// into the last pass: for one-channel images whose rows are a multiple of 4
-// bytes starting on dwords, with elements that have kernels there.
-// UNSUPPORTED (nothing launched) otherwise.
+// morphologyEx (morph.zig:532-571) as one-channel launches of
+// morph_direct.hpp, one or two passes each, intermediates in temporaries, the
+// subtraction folded into the last launch: for one-channel images whose
+// rows are a multiple of 4 bytes starting on dwords, with elements that
+// have kernels there. UNSUPPORTED (nothing launched) otherwise.
 static int morph_direct_ex(Call &call, const DImg &s, const DImg &d, int op, int n, const MorphRects &r) {
     auto aligned = [](const DImg &m) { return ((((uintptr_t)m.p) | (uintptr_t)m.stride) & 3) == 0; };
     if (s.cn != 1 || (s.w & 3) || !aligned(s) || !aligned(d) || tuning().morph_chain == 2) return UNSUPPORTED;
@@ -1440,22 +1443,29 @@ static int morph_direct_ex(Call &call, const DImg &s, const DImg &d, int op, int
     hipStream_t st = call.stream();
     DImg tmp[2] = {};
     bool first = true;
-    // Runs `ops` from s; the last pass writes `out`, combining with `comb`.
+    int k = 0;
+    // Two passes per launch where the element is at most 5 x 5 (a 7 x 7
+    // element's second pass costs more than the memory round trip it saves).
+    const int per_launch = std::max(r.eh, r.ev) <= 2 ? 2 : 1;
+    // Runs `ops` from s; the last launch writes `out`, combining with `comb`.
     auto run = [&](const std::vector<int> &ops, const DImg &out, int pst, const DImg *comb) -> int {
         DImg cur = s;
-        for (size_t i = 0; i < ops.size(); i++) {
-            const bool last = i + 1 == ops.size();
+        for (size_t i = 0; i < ops.size();) {
+            const int np = per_launch == 2 && i + 1 < ops.size() ? 2 : 1;
+            const bool last = i + np == ops.size();
             DImg o = out;
             if (!last) {
-                DImg &t = tmp[i & 1];
+                DImg &t = tmp[k++ & 1];
                 if (!t.p && call.temp_like(s, 1, t)) return OUT_OF_MEMORY;
                 o = t;
             }
-            const int rc = morph_direct_pass(cur, o, el, ops[i], last ? pst : 0, comb, st);
+            const int bits = ops[i] | (np == 2 ? ops[i + 1] << 1 : 0);
+            const int rc = morph_direct_launch(cur, o, el, np, bits, last ? pst : 0, comb, st);
             if (rc == UNSUPPORTED && !first) return INTERNAL;
             if (rc) return rc;
             first = false;
             cur = o;
+            i += np;
         }
         return OK;
     };

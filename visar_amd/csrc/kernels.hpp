@@ -247,13 +247,16 @@ int launch_morph_chain(const MorphChainArgs &a, int K, size_t lds, hipStream_t s
 // of 512 bytes x bands of kMorphDirectRows rows, 4 bands per workgroup
 // (bands4 groups).
 constexpr int kMorphDirectRows = 8;
+// With two passes, strips advance by 496 bytes (the lanes at both ends
+// only feed the second pass). ops bit p: pass p dilates.
 struct MorphDirectArgs {
     DImg s, d, c;                    // c: the combine image of post 1 / 2
     uint32_t span_s, span_d, span_c; // bytes from each image's first to last byte + 1 (< 2^31)
     int strips, bands4, ntiles, xcd;
+    int ops;
 };
-bool launch_morph_direct_max(const MorphDirectArgs &a, uint32_t el, int post, hipStream_t st);
-bool launch_morph_direct_min(const MorphDirectArgs &a, uint32_t el, int post, hipStream_t st);
+bool launch_morph_direct1(const MorphDirectArgs &a, uint32_t el, int post, hipStream_t st);
+bool launch_morph_direct2(const MorphDirectArgs &a, uint32_t el, int post, hipStream_t st);
 
 // --------------------------------------------------------------- histogram
 int launch_hist_u8(const DImg &s, uint32_t *hist_dev, hipStream_t st);

@@ -329,39 +329,73 @@ def bench_serve(args, torch, va):
     cap_lat = np.array([r["t_done"] - r["t_submit"] for r in res_cap])
 
     # Poisson arrivals, 30 frames/s per stream.
-    rng = np.random.default_rng(1)
-    events = []
-    for i in range(N):
-        t = rng.exponential(1 / 30.0)
-        while t < args.seconds:
-            events.append((t, i))
-            t += rng.exponential(1 / 30.0)
-    events.sort()
-    sched = {}
-    res_p = []
-    base = srv.now() + 0.05
-    nxt = [1 + n_cap] * N
-    for t, i in events:
-        target = base + t
+    def poisson(server, sids, seconds, first, seed):
+        """Frames arrive as Poisson processes at 30 fps per stream; returns
+        (offered fps, results, latency from arrival, latency from submit,
+        batch sizes, seconds run)."""
+        rng = np.random.default_rng(seed)
+        events = []
+        for i in range(N):
+            t = rng.exponential(1 / 30.0)
+            while t < seconds:
+                events.append((t, i))
+                t += rng.exponential(1 / 30.0)
+        events.sort()
+        sched, res = {}, []
+        base = server.now() + 0.05
+        nxt = [first] * N
+        for t, i in events:
+            target = base + t
+            while True:
+                d = target - server.now()
+                if d <= 0:
+                    break
+                time.sleep(min(d, 0.002) if d > 0.0005 else 0)
+            k = nxt[i]
+            nxt[i] += 1
+            sched[(sids[i], k)] = target
+            server.submit(sids[i], frames[i][k % cycle], tag=k)
+            if len(res) % 64 == 0:
+                while True:
+                    got = server.poll(1024)
+                    if not got:
+                        break
+                    res.extend(got)
+        server.drain()
+        t_end = server.now()
         while True:
-            d = target - srv.now()
-            if d <= 0:
+            got = server.poll(1024)
+            if not got:
                 break
-            time.sleep(min(d, 0.002) if d > 0.0005 else 0)
-        k = nxt[i]
-        nxt[i] += 1
-        sched[(ids[i], k)] = target
-        srv.submit(ids[i], frames[i][k % cycle], tag=k)
-        if len(res_p) % 64 == 0:
-            collect(res_p)
-    srv.drain()
-    t_end = srv.now()
-    collect(res_p)
-    lat = np.array([r["t_done"] - sched[(r["stream_id"], r["tag"])] for r in res_p])
-    lat_sub = np.array([r["t_done"] - r["t_submit"] for r in res_p])
-    bsz = np.array([r["batch_size"] for r in res_p])
+            res.extend(got)
+        lat = np.array([r["t_done"] - sched[(r["stream_id"], r["tag"])] for r in res])
+        lat_sub = np.array([r["t_done"] - r["t_submit"] for r in res])
+        bsz = np.array([r["batch_size"] for r in res])
+        return len(events) / seconds, res, lat, lat_sub, bsz, t_end - base
+
+    offered, res_p, lat, lat_sub, bsz, p_seconds = poisson(srv, ids, args.seconds, 1 + n_cap, 1)
     st = srv.stats()
     srv.close()
+
+    # The same Poisson load with a batching window: a launch waits up to
+    # the window for more streams' frames (batch sizes against latency).
+    sweep = []
+    for window in (250, 500, 1000, 2000):
+        sw = serve.Server(width=W, height=H, max_streams=N, max_batch=N, queue_depth=4,
+                          host_workers=args.host_workers, max_inflight=2, batch_window_us=window)
+        sids = [sw.admit() for _ in range(N)]
+        for i, sid in enumerate(sids):
+            sw.submit(sid, frames[i][0])
+        sw.drain()
+        while sw.poll(1024):
+            pass
+        off_w, res_w, lat_w, _, bsz_w, _ = poisson(sw, sids, 2.0, 1, 2)
+        sw.close()
+        hist = {int(b): int(c) for b, c in zip(*np.unique(bsz_w, return_counts=True))}
+        sweep.append({"batch_window_us": window, "offered_fps": round(off_w, 1), "frames": len(res_w),
+                      "latency_ms_p50": round(float(np.percentile(lat_w, 50)) * 1e3, 3),
+                      "latency_ms_p99": round(float(np.percentile(lat_w, 99)) * 1e3, 3),
+                      "mean_batch": round(float(bsz_w.mean()), 2), "batch_size_frames": hist})
 
     # The plain per-call path on one stream's frames.
     va.set_sync(True)  # each plain call returns with its output written, as the C ABI does by default
@@ -388,12 +422,14 @@ def bench_serve(args, torch, va):
         "capacity": {"frames": N * n_cap, "seconds": round(cap_seconds, 3), "fps": round(cap_fps, 1),
                      "latency_ms_p50": round(float(np.percentile(cap_lat, 50)) * 1e3, 3),
                      "latency_ms_p99": round(float(np.percentile(cap_lat, 99)) * 1e3, 3)},
-        "poisson": {"offered_fps": round(len(events) / args.seconds, 1), "frames": len(res_p),
-                    "achieved_fps": round(len(res_p) / (t_end - base), 1),
+        "poisson": {"offered_fps": round(offered, 1), "frames": len(res_p),
+                    "achieved_fps": round(len(res_p) / p_seconds, 1),
                     "latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                     "latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
                     "latency_from_submit_ms_p99": round(float(np.percentile(lat_sub, 99)) * 1e3, 3),
-                    "mean_batch": round(float(bsz.mean()), 2), "max_batch": int(bsz.max())},
+                    "mean_batch": round(float(bsz.mean()), 2), "max_batch": int(bsz.max()),
+                    "batch_size_frames": {int(b): int(c) for b, c in zip(*np.unique(bsz, return_counts=True))}},
+        "poisson_batch_window": sweep,
         "admission": {"admitted": st["admitted"], "rejected": st["rejected"],
                       "bytes_per_stream": st0["bytes_per_stream"], "device_bytes": st0["device_bytes"]},
         "plain_per_call": {"fps": round(plain_fps, 1), "speedup_served": round(cap_fps / plain_fps, 2)},

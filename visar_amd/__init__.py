@@ -359,7 +359,8 @@ class Background:
         return mask
 
     def __del__(self):
-        if getattr(self, "_h", None) and self._h.value:
+        # At interpreter exit the module globals may already be gone.
+        if getattr(self, "_h", None) and self._h.value and lib is not None:
             lib.visar_background_free(self._h)
             self._h = None
 

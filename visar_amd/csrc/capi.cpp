@@ -261,6 +261,33 @@ void *Call::upload(const void *src, size_t bytes) {
     return dev;
 }
 
+void *Call::pinned(size_t bytes) {
+    void *p = pinned_pool().get(bytes ? bytes : 1);
+    if (p) pinned_.push_back(p);
+    return p;
+}
+
+// Rows of `rb` bytes from src (stride ss) to dst (stride ds) on the CPU.
+static void copy_rows(void *dst, size_t ds, const void *src, size_t ss, size_t rb, size_t h) {
+    if (ds == rb && ss == rb) {
+        std::memcpy(dst, src, rb * h);
+        return;
+    }
+    for (size_t y = 0; y < h; y++)
+        std::memcpy(static_cast<uint8_t *>(dst) + y * ds, static_cast<const uint8_t *>(src) + y * ss, rb);
+}
+
+// A host image's rows into device buffer `buf` (pitch), through pinned
+// staging rows on the call's stream.
+static int stage_up(Call &call, void *buf, size_t pitch, const visar_image *c, size_t rb) {
+    void *pin = call.pinned(rb * c->height);
+    if (!pin) return fail(OUT_OF_MEMORY, "OutOfMemory");
+    copy_rows(pin, rb, c->data, c->stride, rb, c->height);
+    if (hipMemcpy2DAsync(buf, pitch, pin, rb, rb, c->height, hipMemcpyHostToDevice, call.stream()) != hipSuccess)
+        return fail(INTERNAL, "CopyFailed");
+    return OK;
+}
+
 int Call::input(const visar_image *c, DImg &out) {
     if (is_device_ptr(c->data)) {
         out = view(c, c->data, c->stride);
@@ -271,8 +298,8 @@ int Call::input(const visar_image *c, DImg &out) {
     size_t pitch = (rb + 255) & ~(size_t)255;
     void *buf = scratch(pitch * c->height);
     if (!buf) return fail(OUT_OF_MEMORY, "OutOfMemory");
-    if (hipMemcpy2DAsync(buf, pitch, c->data, c->stride, rb, c->height, hipMemcpyHostToDevice, st_) != hipSuccess)
-        return fail(INTERNAL, "CopyFailed");
+    const int rc = stage_up(*this, buf, pitch, c, rb);
+    if (rc) return rc;
     staged_ = true;
     out = view(c, buf, pitch);
     return OK;
@@ -293,7 +320,7 @@ int Call::output(const visar_image *c, DImg &out) {
         size_t pitch = (rb + 255) & ~(size_t)255;
         void *buf = scratch(pitch * c->height);
         if (!buf) return fail(OUT_OF_MEMORY, "OutOfMemory");
-        pending_.push_back({c, buf, pitch, true});
+        pending_.push_back({c, buf, pitch, true, nullptr});
         out = view(c, buf, pitch);
         return OK;
     }
@@ -302,7 +329,7 @@ int Call::output(const visar_image *c, DImg &out) {
     void *buf = scratch(pitch * c->height);
     if (!buf) return fail(OUT_OF_MEMORY, "OutOfMemory");
     staged_ = true;
-    pending_.push_back({c, buf, pitch, false});
+    pending_.push_back({c, buf, pitch, false, nullptr});
     out = view(c, buf, pitch);
     return OK;
 }
@@ -310,8 +337,9 @@ int Call::output(const visar_image *c, DImg &out) {
 int Call::output_keep(const visar_image *c, DImg &out) {
     int rc = output(c, out);
     if (rc || out.p == c->data) return rc;
-    const hipMemcpyKind kind = is_device_ptr(c->data) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    if (hipMemcpy2DAsync(out.p, (size_t)out.stride, c->data, c->stride, row_bytes(c), c->height, kind, st_) != hipSuccess)
+    if (!is_device_ptr(c->data)) return stage_up(*this, out.p, (size_t)out.stride, c, row_bytes(c));
+    if (hipMemcpy2DAsync(out.p, (size_t)out.stride, c->data, c->stride, row_bytes(c), c->height, hipMemcpyDeviceToDevice,
+                         st_) != hipSuccess)
         return fail(INTERNAL, "CopyFailed");
     return OK;
 }
@@ -335,14 +363,26 @@ int Call::finish(int launch_status) {
     }
     for (auto &pd : pending_) {
         const visar_image *c = pd.desc;
-        hipMemcpyKind kind = pd.device_target ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-        if (hipMemcpy2DAsync(c->data, c->stride, pd.buf, pd.pitch, row_bytes(c), c->height, kind, st_) != hipSuccess)
+        const size_t rb = row_bytes(c);
+        if (pd.device_target) {
+            if (hipMemcpy2DAsync(c->data, c->stride, pd.buf, pd.pitch, rb, c->height, hipMemcpyDeviceToDevice, st_) !=
+                hipSuccess)
+                return fail(INTERNAL, "CopyFailed");
+            continue;
+        }
+        // Host targets: device rows to pinned rows here, to the caller's
+        // memory on the CPU once the stream has drained.
+        pd.pin = pinned(rb * c->height);
+        if (!pd.pin) return fail(OUT_OF_MEMORY, "OutOfMemory");
+        if (hipMemcpy2DAsync(pd.pin, rb, pd.buf, pd.pitch, rb, c->height, hipMemcpyDeviceToHost, st_) != hipSuccess)
             return fail(INTERNAL, "CopyFailed");
     }
     if (staged_ || sync_mode()) {
         if (hipStreamSynchronize(st_) != hipSuccess) return fail(INTERNAL, "KernelFailed");
         waited_ = true;
     }
+    for (auto &pd : pending_)
+        if (pd.pin) copy_rows(pd.desc->data, pd.desc->stride, pd.pin, row_bytes(pd.desc), row_bytes(pd.desc), pd.desc->height);
     return OK;
 }
 
@@ -1011,11 +1051,11 @@ int visar_threshold(const visar_image *src, const visar_image *dst, double thres
     if (autom) {
         if (T != 0 || src->channels != 1) return fail(UNSUPPORTED, "Unsupported");
         uint32_t *hd = (uint32_t *)call.scratch(256 * sizeof(uint32_t));
-        if (!hd) return fail(OUT_OF_MEMORY, "OutOfMemory");
-        uint32_t hist[256];
-        if (hipMemsetAsync(hd, 0, sizeof hist, call.stream()) != hipSuccess ||
+        uint32_t *hist = (uint32_t *)call.pinned(256 * sizeof(uint32_t));
+        if (!hd || !hist) return fail(OUT_OF_MEMORY, "OutOfMemory");
+        if (hipMemsetAsync(hd, 0, 256 * sizeof(uint32_t), call.stream()) != hipSuccess ||
             launch_hist_u8(p.s, hd, call.stream()) != OK ||
-            hipMemcpyAsync(hist, hd, sizeof hist, hipMemcpyDeviceToHost, call.stream()) != hipSuccess ||
+            hipMemcpyAsync(hist, hd, 256 * sizeof(uint32_t), hipMemcpyDeviceToHost, call.stream()) != hipSuccess ||
             hipStreamSynchronize(call.stream()) != hipSuccess)
             return fail(INTERNAL, "KernelFailed");
         thresh = autom == 1 ? threshold_otsu(hist, (double)src->width * src->height) : threshold_triangle(hist);

@@ -205,15 +205,18 @@ extern "C" int visar_connected_components(const visar_image *src, const visar_im
     c.labels = reinterpret_cast<int32_t *>(lv.p);
     c.lstride = (size_t)lv.stride;
     void *scratch = call.scratch(cc_scratch_bytes(w, h));
-    if (!scratch) return fail(OUT_OF_MEMORY, "OutOfMemory");
+    c.host_total = (uint32_t *)call.pinned(sizeof(uint32_t));
+    if (!scratch || !c.host_total) return fail(OUT_OF_MEMORY, "OutOfMemory");
     uint32_t n = 0;
     if (cc_label(c, scratch, call.stream(), &n) != OK) return call.finish(INTERNAL);
-    std::vector<uint64_t> acc(4 * (size_t)n), rows(4 * (size_t)h);
-    std::vector<uint32_t> box(4 * (size_t)n);
+    // The statistics come back through pinned memory.
+    uint64_t *acc = (uint64_t *)call.pinned(4 * (size_t)n * 8), *rows = (uint64_t *)call.pinned(4 * (size_t)h * 8);
+    uint32_t *box = (uint32_t *)call.pinned(4 * (size_t)n * 4);
+    if (!acc || !rows || !box) return fail(OUT_OF_MEMORY, "OutOfMemory");
     hipStream_t st = call.stream();
-    if (hipMemcpyAsync(acc.data(), c.acc, acc.size() * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(box.data(), c.box, box.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(rows.data(), c.rows, rows.size() * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+    if (hipMemcpyAsync(acc, c.acc, 4 * (size_t)n * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(box, c.box, 4 * (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(rows, c.rows, 4 * (size_t)h * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return call.finish(INTERNAL);
     std::vector<CompStats> out(n);

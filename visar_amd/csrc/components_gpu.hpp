@@ -23,6 +23,7 @@ struct CcArgs {
     uint64_t *acc;  // per label: area, twice the x sum, y sum, unused
     uint32_t *box;  // per label: min x, min y, max x, max y
     uint64_t *rows; // per row: zeros, first zero, last zero, twice the nonzero x sum
+    uint32_t *host_total;  // pinned host word for the label count (set by the caller)
 };
 
 // Device scratch one cc_label call needs for a w x h mask.

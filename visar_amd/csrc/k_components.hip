@@ -281,9 +281,10 @@ int cc_label(CcArgs &c, void *scratch, hipStream_t st, uint32_t *count) {
     hipLaunchKernelGGL(cc_mark, dim3(blocks(n)), dim3(256), 0, st, c);
     hipLaunchKernelGGL(cc_scan_blocks, dim3(nb), dim3(256), 0, st, c.mark, nkeys, c.bsum);
     hipLaunchKernelGGL(cc_scan_sums, dim3(1), dim3(256), 0, st, c.bsum, nb);
-    uint32_t total = 0;
-    if (hipMemcpyAsync(&total, c.bsum + nb, 4, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+    if (hipMemcpyAsync(c.host_total, c.bsum + nb, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
         return INTERNAL;
+    const uint32_t total = *c.host_total;
     const uint32_t nlab = total + 1;
     hipLaunchKernelGGL(cc_root_lab, dim3(blocks(n)), dim3(256), 0, st, c);
     hipLaunchKernelGGL(cc_clear_stats, dim3(blocks(nlab)), dim3(256), 0, st, c, nlab);

@@ -44,6 +44,9 @@ class Call {
     // A device copy of `bytes` host bytes, uploaded on the call's stream
     // through pinned memory (the host data may change once this returns).
     void *upload(const void *src, size_t bytes);
+    // Pinned host memory owned by the call (returned with its buffers); the
+    // HIP runtime's own copies to and from pageable memory are slow.
+    void *pinned(size_t bytes);
     int finish(int launch_status);
     hipStream_t stream() const { return st_; }
 
@@ -53,6 +56,7 @@ class Call {
         void *buf;
         size_t pitch;
         bool device_target;
+        void *pin;  // host targets: the pinned staging rows
     };
     DImg view(const visar_image *c, void *p, size_t stride);
 

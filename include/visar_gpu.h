@@ -52,7 +52,13 @@ int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D
  * in the last pass, others run fused chains of passes in one LDS tile, where
  * the element is a union of centred rectangles; 1 the general kernels, one
  * per erode / dilate pass and one per subtraction; 2 the fused LDS chains);
- * "morph_rows" output rows per tile of the fused chains (8-128).
+ * "morph_rows" output rows per tile of the fused chains (8-128);
+ * "stencil_direct" the packed one-channel u8 stencils (GaussianBlur 3x3, 5x5,
+ * 7x7 with sigma 0, Sobel 3x3 to i16): 0 rows streamed through registers
+ * where rows are dword multiples starting on dwords and the border is not
+ * WRAP, 1 the LDS tile kernels; "stencil_band" output rows per wave of the
+ * streamed kernels (1, 2, 4, 8, 16; 0 picks from the image size);
+ * "stencil_wg" their waves per workgroup (1, 2, 4).
  * Unknown keys or values out of range return VISAR_INVALID_ARGUMENT. */
 int visar_gpu_set_tuning(const char *key, int value);
 int visar_gpu_get_tuning(const char *key, int *value);

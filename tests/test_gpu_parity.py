@@ -185,6 +185,60 @@ def test_sobel(va, orc, dx, dy, ks):
     check(va, orc, "sobel", u, np.float32, dx, dy, ksize=ks)
 
 
+def _stencil_cases():
+    """The packed stencils the streamed one-channel kernels take: GaussianBlur
+    3/5/7 with sigma 0 and Sobel 3x3 dx / dy to i16."""
+    return [("gaussian_blur", ((k, k), 0), {}) for k in (3, 5, 7)] + \
+           [("sobel", (np.int16, dx, dy), {"ksize": 3}) for dx, dy in ((1, 0), (0, 1))]
+
+
+def test_stencil_streamed_layouts(va, orc):
+    """The streamed kernels (one-channel images with dword rows) against the
+    oracle at widths on both sides of their 256-byte strips, heights below
+    and above their row bands, every border (WRAP takes the LDS tile kernels)
+    and every band height, on contiguous tensors and on dword-aligned views
+    of wider buffers."""
+    from visar_amd import autotune
+    big = natural(80, 1100, 1, 91)
+    try:
+        for band in (0, 1, 2, 4, 8, 16):
+            autotune.set_knob("stencil_band", band)
+            sizes = [(1, 8), (2, 12), (3, 252), (5, 256), (9, 260), (17, 516), (33, 1028), (70, 64)] if band == 0 else \
+                    [(1, 8), (9, 260), (70, 516)]
+            for h, w in sizes:
+                a = natural(h, w, 1, h * 131 + w)
+                v = big[3:3 + h, 8:8 + w]
+                for name, args, kw in _stencil_cases():
+                    for border in range(5):
+                        want = getattr(orc, name)(a, *args, border=border, **kw)
+                        got = getattr(va, name)(dev(a), *args, border=border, **kw).cpu().numpy()
+                        assert_same(got, want, f"{name} {args} {h}x{w} border {border} band {band}")
+                        if border == 4:
+                            dbig = dev(big)
+                            gv = getattr(va, name)(dbig[3:3 + h, 8:8 + w], *args, border=border, **kw).cpu().numpy()
+                            assert_same(gv, getattr(orc, name)(np.ascontiguousarray(v), *args, border=border, **kw),
+                                        f"{name} {args} view {h}x{w}")
+    finally:
+        autotune.set_knob("stencil_band", 0)
+
+
+def test_stencil_streamed_equals_tiles(va):
+    """The streamed kernels equal the LDS tile kernels (tuning
+    "stencil_direct" 1) on 1080p frames for every packed stencil and border."""
+    from visar_amd import autotune, synth
+    a = dev(synth.uniform_bytes(5, (1080, 1920)))
+    try:
+        for name, args, kw in _stencil_cases():
+            for border in (0, 1, 2, 4):
+                outs = []
+                for knob in (0, 1):
+                    autotune.set_knob("stencil_direct", knob)
+                    outs.append(getattr(va, name)(a, *args, border=border, **kw))
+                assert torch.equal(outs[0], outs[1]), f"{name} {args} border {border}"
+    finally:
+        autotune.set_knob("stencil_direct", 0)
+
+
 def test_sobel_1080p_config(va, orc):
     from visar_amd import synth
     a = synth.uniform_bytes(3, (1080, 1920))

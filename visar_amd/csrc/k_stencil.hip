@@ -4,7 +4,7 @@
 // shared with the persistent batch kernel.
 #include <initializer_list>
 
-#include "tiles.hpp"
+#include "stencil_direct.hpp"
 
 namespace visar {
 
@@ -81,6 +81,65 @@ bool launch_d(const StencilArgs &p, int k, hipStream_t st, int th) {
     }
 }
 
+template <int MODE>
+void launch_direct_tb(const StencilDirectArgs &a, int tb, hipStream_t st) {
+    const dim3 g(tile_grid(a.ntiles)), b(64 * a.wpg);
+    switch (tb) {
+    case 4: hipLaunchKernelGGL((sdirect::sep_direct_u8<MODE, 4>), g, b, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((sdirect::sep_direct_u8<MODE, 2>), g, b, 0, st, a); break;
+    case 8: hipLaunchKernelGGL((sdirect::sep_direct_u8<MODE, 8>), g, b, 0, st, a); break;
+    case 16: hipLaunchKernelGGL((sdirect::sep_direct_u8<MODE, 16>), g, b, 0, st, a); break;
+    default: hipLaunchKernelGGL((sdirect::sep_direct_u8<MODE, 1>), g, b, 0, st, a); break;
+    }
+}
+
+// The streamed one-channel kernels (stencil_direct.hpp) where the layout
+// allows them: rows of a multiple of 4 bytes (at least 8) starting on
+// dwords, images spanning less than 2^31 bytes, a border other than WRAP.
+bool launch_direct(const StencilArgs &p, int mode, hipStream_t st) {
+    if (tuning().stencil_direct == 1 || p.src.cn != 1) return false;
+    if (p.border != BORDER_CONSTANT && p.border != BORDER_REPLICATE && p.border != BORDER_REFLECT &&
+        p.border != BORDER_REFLECT101)
+        return false;
+    const DImg &s = p.src, &d = p.dst;
+    const long n = s.w;
+    const long ob = mode == SWAR_SOBEL_DX || mode == SWAR_SOBEL_DY ? 2 : 1;
+    if (n < 8 || n % 4 != 0 || s.h < 1 || d.w != s.w || d.h != s.h) return false;
+    if ((((uintptr_t)s.p | (uintptr_t)d.p) & 3) != 0 || s.stride % 4 != 0 || d.stride % 4 != 0) return false;
+    const long span_s = (long)(s.h - 1) * s.stride + n, span_d = (long)(d.h - 1) * d.stride + n * ob;
+    if (span_s >= (1L << 31) || span_d >= (1L << 31)) return false;
+    const int wg = tuning().stencil_wg;
+    const int strips = (int)((n + sdirect::kStrip - 1) / sdirect::kStrip);
+    // Band height: the tallest of 4, 2, 1 rows that still gives about 4000
+    // waves (measured at 640x480 to 4096x4096: shorter bands issue every
+    // row's loads sooner and keep more waves in flight; taller ones reread
+    // fewer halo rows).
+    int tb = tuning().stencil_band;
+    if (tb != 1 && tb != 2 && tb != 4 && tb != 8 && tb != 16) {
+        tb = 4;
+        while (tb > 1 && (long)strips * ((s.h + tb - 1) / tb) < 4000) tb /= 2;
+    }
+    StencilDirectArgs a;
+    a.s = p;
+    a.span_s = (uint32_t)span_s;
+    a.span_d = (uint32_t)span_d;
+    a.strips = strips;
+    a.bands = (s.h + tb - 1) / tb;
+    a.wpg = wg == 1 || wg == 2 ? wg : 4;
+    const TileOrder o = tile_order(a.strips, (a.bands + a.wpg - 1) / a.wpg, true);
+    a.ntiles = o.ntiles;
+    a.xcd = o.xcd;
+    switch (mode) {
+    case SWAR_G3: launch_direct_tb<SWAR_G3>(a, tb, st); break;
+    case SWAR_G5: launch_direct_tb<SWAR_G5>(a, tb, st); break;
+    case SWAR_G7: launch_direct_tb<SWAR_G7>(a, tb, st); break;
+    case SWAR_SOBEL_DX: launch_direct_tb<SWAR_SOBEL_DX>(a, tb, st); break;
+    case SWAR_SOBEL_DY: launch_direct_tb<SWAR_SOBEL_DY>(a, tb, st); break;
+    default: return false;
+    }
+    return true;
+}
+
 }  // namespace
 
 // Rows per thread for an image of n samples per row and h rows: enough
@@ -134,8 +193,10 @@ int swar_mode(const StencilArgs &a, int k, int ddepth) {
 bool launch_sep_small_u8(const SepParams &p, hipStream_t st, int th) {
     StencilArgs a;
     if (!stencil_args(p, a)) return false;
+    const int mode = swar_mode(a, p.nkx, p.ddepth);
+    if (mode >= 0 && launch_direct(a, mode, st)) return hipGetLastError() == hipSuccess;
     bool ok;
-    switch (swar_mode(a, p.nkx, p.ddepth)) {
+    switch (mode) {
     case SWAR_G3: ok = launch_swar<SWAR_G3, uint8_t>(a, st, th); break;
     case SWAR_G5: ok = launch_swar<SWAR_G5, uint8_t>(a, st, th); break;
     case SWAR_G7: ok = launch_swar<SWAR_G7, uint8_t>(a, st, th); break;

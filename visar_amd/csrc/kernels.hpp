@@ -15,6 +15,9 @@ struct Tuning {
     int warp_layout = 0;         // u8 bilinear constant-border warps: 0 block layout, loads per pixel; 1 row / LDS-box kernels; 2 block layout, loads after a wave vote
     int morph_chain = 0;         // u8 morphology: 0 automatic, 1 one kernel per pass and subtraction, 2 fused LDS chains
     int morph_rows = 0;          // fused morphology: output rows per tile (8-128)
+    int stencil_direct = 0;      // one-channel packed stencils: 0 rows streamed through registers where the layout allows, 1 LDS tiles
+    int stencil_band = 0;        // streamed stencils: output rows per wave (2, 4, 8, 16)
+    int stencil_wg = 0;          // streamed stencils: waves per workgroup (1, 2, 4)
 };
 Tuning &tuning();
 bool set_tuning(const char *key, int value);

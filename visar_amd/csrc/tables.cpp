@@ -27,6 +27,9 @@ const Knob kKnobs[] = {
     {"warp_layout", &Tuning::warp_layout, 0, 2},
     {"morph_chain", &Tuning::morph_chain, 0, 2},
     {"morph_rows", &Tuning::morph_rows, 0, 128},
+    {"stencil_direct", &Tuning::stencil_direct, 0, 1},
+    {"stencil_band", &Tuning::stencil_band, 0, 16},
+    {"stencil_wg", &Tuning::stencil_wg, 0, 4},
 };
 }  // namespace
 

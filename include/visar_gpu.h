@@ -58,7 +58,11 @@ int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D
  * where rows are dword multiples starting on dwords and the border is not
  * WRAP, 1 the LDS tile kernels; "stencil_band" output rows per wave of the
  * streamed kernels (1, 2, 4, 8, 16; 0 picks from the image size);
- * "stencil_wg" their waves per workgroup (1, 2, 4).
+ * "stencil_wg" their waves per workgroup (1, 2, 4); "gray_layout" u8
+ * cvtColor to gray on contiguous 16-byte aligned images (0 each wave reads
+ * one contiguous 1024-pixel block, 1 each thread its own 16 pixels);
+ * "warp_blocks" 64 x 4 pixel blocks per wave of the default u8 warps (1, or
+ * 2 with the second block's loads issued before the first is blended).
  * Unknown keys or values out of range return VISAR_INVALID_ARGUMENT. */
 int visar_gpu_set_tuning(const char *key, int value);
 int visar_gpu_get_tuning(const char *key, int *value);

@@ -292,6 +292,27 @@ def test_cvt_gray(va, orc):
                 check(va, orc, "cvt_color", a, code)
 
 
+def test_cvt_gray_wave_blocks(va, orc):
+    """The block-per-wave u8 kernel (contiguous device images of 1024 pixels
+    or more): whole and partial last blocks and workgroups, 3 and 4 channels,
+    both channel orders, against the oracle and the per-thread kernel
+    (tuning "gray_layout" 1)."""
+    from visar_amd import autotune
+    try:
+        for code, scn in [(6, 3), (7, 3), (10, 4), (11, 4)]:
+            for (h, w) in [(32, 32), (100, 100), (1, 1024), (1, 1025), (17, 1000), (513, 16), (64, 128), (65, 128)]:
+                a = rnd((h, w, scn), code * 1000 + h + w)
+                want = orc.cvt_color(a, code)
+                outs = []
+                for knob in (0, 1):
+                    autotune.set_knob("gray_layout", knob)
+                    outs.append(va.cvt_color(dev(a), code).cpu().numpy())
+                assert_same(outs[0], want, f"code {code} {h}x{w}")
+                assert_same(outs[1], want, f"code {code} {h}x{w} per-thread")
+    finally:
+        autotune.set_knob("gray_layout", 0)
+
+
 def test_cvt_gray_4k_config(va, orc):
     from visar_amd import synth
     a = synth.uniform_bytes(0, (2160, 3840, 3))
@@ -482,8 +503,9 @@ def test_warp_affine_u8_box_cases(va, orc):
 
 def test_warp_u8_layouts_agree_on_views(va, orc):
     """The one-channel u8 bilinear constant-border warps in every kernel
-    layout (block layout with 8-byte tap loads issued per pixel or after a
-    wave-wide footprint check, and the row / LDS-box kernels): device views at odd offsets and strides, a view that ends at
+    layout (block layout with 8-byte tap loads issued per pixel, one or two
+    blocks per wave, or after a wave-wide footprint check, and the row /
+    LDS-box kernels): device views at odd offsets and strides, a view that ends at
     its buffer's last byte (the 8-byte loads must stop there), and
     perspective maps whose horizon crosses the image (w <= 0, the division's
     slow path)."""
@@ -494,8 +516,9 @@ def test_warp_u8_layouts_agree_on_views(va, orc):
     persp = lambda w, h: ([synth.perspective_matrix(4, w, h)] if w > 1 and h > 1 else []) + [
         np.array([[1, 0.1, 2], [0.05, 1, -3], [0.004, -0.003, 1]]), np.array([[1.2, 0, 0], [0, 1.2, 0], [0, 0.01, 0.4]])]
     try:
-        for layout in (0, 1, 2):
+        for layout, blocks in ((0, 0), (0, 2), (1, 0), (2, 0)):
             autotune.set_knob("warp_layout", layout)
+            autotune.set_knob("warp_blocks", blocks)
             for y0, x0, h, w in cases:
                 a = np.ascontiguousarray(big[y0:y0 + h, x0:x0 + w])
                 dv = dbig[y0:y0 + h, x0:x0 + w]
@@ -519,6 +542,7 @@ def test_warp_u8_layouts_agree_on_views(va, orc):
                 assert_same(got.cpu().numpy(), want, f"layout {layout} packed {m.tolist()}")
     finally:
         autotune.set_knob("warp_layout", 0)
+        autotune.set_knob("warp_blocks", 0)
 
 
 def test_warp_affine_1080p_config(va, orc):

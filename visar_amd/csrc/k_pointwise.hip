@@ -91,6 +91,70 @@ __global__ __launch_bounds__(kThreads) void gray_u8(DImg s, DImg d, long npix) {
     for (long i = 0; i < n; i++) dr[i] = (uint8_t)gray_fix(sr[i * SCN + BI], sr[i * SCN + 1], sr[i * SCN + RI]);
 }
 
+// The same conversion with each wave reading one contiguous block of 1024
+// pixels: load instruction k of the wave covers bytes [1024 k, 1024 k +
+// 1024) of the block's SCN KiB (one nontemporal 16-byte load per lane, so a
+// load instruction touches 8 whole 128-byte lines instead of strided pieces
+// of 24 or 32), the wave regroups the bytes per lane through its own part of
+// LDS, and lane i converts and stores pixels [16 i, 16 i + 16) of the block.
+// On a 3840x2160 BGR frame: 6.6-6.8 us against 7.1-7.4 us for gray_u8 (the
+// strided form with nontemporal loads ran 8.5 us). The last, partial block
+// takes the per-pixel path. Needs contiguous images with 16-byte aligned
+// bases.
+constexpr int kGrayWaveThreads = 512;
+template <int SCN, int SB>
+__global__ __launch_bounds__(kGrayWaveThreads) void gray_u8_wave(DImg s, DImg d, long npix) {
+    __shared__ u32x4 sh[kGrayWaveThreads / 64][64 * SCN];
+    const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+    const long x0 = ((long)blockIdx.x * (kGrayWaveThreads / 64) + wv) * 1024;  // the wave's first pixel
+    if (x0 >= npix) return;
+    const uint8_t *sp = s.p + x0 * SCN;
+    uint8_t *dp = d.p + x0;
+    constexpr int BI = SB, RI = 2 - SB;
+    if (x0 + 1024 > npix) {
+        const long x = 16 * lane, n = min(16L, npix - x0 - x);
+        for (long i = 0; i < n; i++)
+            dp[x + i] = (uint8_t)gray_fix(sp[(x + i) * SCN + BI], sp[(x + i) * SCN + 1], sp[(x + i) * SCN + RI]);
+        return;
+    }
+    u32x4 a[SCN];
+#pragma unroll
+    for (int k = 0; k < SCN; k++)
+        a[k] = __builtin_nontemporal_load((const __attribute__((address_space(1))) u32x4 *)(sp + 1024 * k + 16 * lane));
+#pragma unroll
+    for (int k = 0; k < SCN; k++) sh[wv][64 * k + lane] = a[k];
+    // The wave reads back what its other lanes wrote: LDS operations of one
+    // wave complete in order; the fence keeps the compiler from moving the
+    // reads above the writes.
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t words[4 * SCN];
+#pragma unroll
+    for (int k = 0; k < SCN; k++) {
+        const u32x4 w = sh[wv][SCN * lane + k];
+        words[4 * k + 0] = w.x;
+        words[4 * k + 1] = w.y;
+        words[4 * k + 2] = w.z;
+        words[4 * k + 3] = w.w;
+    }
+    uint32_t o[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int px = q * 4 + k;
+            const int ib = px * SCN + BI, ig = px * SCN + 1, ir = px * SCN + RI;
+            const uint32_t b = (words[ib >> 2] >> ((ib & 3) * 8)) & 255u;
+            const uint32_t gg = (words[ig >> 2] >> ((ig & 3) * 8)) & 255u;
+            const uint32_t r = (words[ir >> 2] >> ((ir & 3) * 8)) & 255u;
+            v |= gray_fix(b, gg, r) << (8 * k);
+        }
+        o[q] = v;
+    }
+    *(__attribute__((address_space(1))) u32x4 *)(dp + 16 * lane) = u32x4{o[0], o[1], o[2], o[3]};
+}
+
 // videoio/frame.zig:167-177, 292-336 grayDownsample: the k x k block sums of
 // B, G and R weighted 1868 / 9617 / 4899; k = 2 rounds (l + 2^15) >> 16,
 // other k take (f32)l * (f32)(1 / (k*k*2^14)) + 0.5 truncated, at most 255.
@@ -440,6 +504,14 @@ int launch_gray(const DImg &s, const DImg &d, int depth, int scn, int sb, hipStr
     if (depth == 0) {
         const bool flat = s.stride == (long)s.w * scn && d.stride == (long)d.w;
         const long npix = (long)s.w * s.h;
+        if (flat && npix >= 1024 && tuning().gray_layout == 0 && (((uintptr_t)s.p | (uintptr_t)d.p) & 15) == 0) {
+            const dim3 grid((unsigned)((npix + 1024L * (kGrayWaveThreads / 64) - 1) / (1024L * (kGrayWaveThreads / 64))));
+            if (scn == 3 && sb == 0) hipLaunchKernelGGL((gray_u8_wave<3, 0>), grid, dim3(kGrayWaveThreads), 0, st, s, d, npix);
+            else if (scn == 3) hipLaunchKernelGGL((gray_u8_wave<3, 2>), grid, dim3(kGrayWaveThreads), 0, st, s, d, npix);
+            else if (sb == 0) hipLaunchKernelGGL((gray_u8_wave<4, 0>), grid, dim3(kGrayWaveThreads), 0, st, s, d, npix);
+            else hipLaunchKernelGGL((gray_u8_wave<4, 2>), grid, dim3(kGrayWaveThreads), 0, st, s, d, npix);
+            return launched();
+        }
         dim3 grid = flat ? dim3(blocks_for((npix + 15) / 16)) : dim3((unsigned)((s.w + 16 * kThreads - 1) / (16 * kThreads)), (unsigned)s.h);
 #define GRAY(SCN, SB)                                                                                       \
     if (flat) hipLaunchKernelGGL((gray_u8<SCN, SB, true>), grid, dim3(kThreads), 0, st, s, d, npix); \

@@ -221,6 +221,8 @@ __device__ __attribute__((always_inline)) inline float blend_persp(float p00, fl
 }
 
 typedef uint64_t u64_a4 __attribute__((aligned(4)));
+typedef unsigned int vu32x2 __attribute__((ext_vector_type(2)));
+constexpr int kWarpBufWord3 = 0x00020000;  // buffer resource word 3 for gfx9 raw (untyped) access
 
 // Blends of the lane's 4 pixels. `lim` = (image base & 3) + (H - 1) * stride
 // + W: the bytes from the dword-aligned address below the image base to the
@@ -292,46 +294,59 @@ __device__ __attribute__((always_inline)) inline void blk_sample(const DImg &s, 
 // computed (the coordinates of a projective map are most of the kernel's
 // arithmetic, and gating the loads on a vote over all four pixels would
 // hold every load until the last coordinate). Each pixel loads its two tap
-// rows at addresses clamped into the image; a pixel whose footprint is not
-// wholly inside (or whose second row's 8-byte load would pass the image's
-// last byte) is redone afterwards by the byte-gather rules, by the lanes
-// that have one. Needs W >= 2 and H >= 2.
-template <bool PERSP, typename CoordFn>
-__device__ __attribute__((always_inline)) inline void blk_sample_eager(const DImg &s, float cv, uint32_t lim,
-                                                                       CoordFn coord, BlkCoord (&c)[4],
-                                                                       float (&out)[4]) {
+// rows with buffer loads over the lim bytes from the dword below the image
+// base: the hardware's range check returns zeros for an offset outside them
+// (coordinates outside the image, or offsets wrapped in 32 bits), so the
+// offsets need no clamping. A pixel whose footprint is not wholly inside
+// (or whose second row's 8-byte load would pass the image's last byte) is
+// redone afterwards by the byte-gather rules, by the lanes that have one.
+// blk_issue computes the coordinates and issues the loads of one block,
+// blk_finish blends them, so a wave can issue its next block's loads before
+// it blends the current one. Needs W >= 2 and H >= 2.
+struct BlkLoads {
+    BlkCoord c[4];
+    vu32x2 r0[4], r1[4];
+    uint32_t o0[4];
+    bool ok[4];
+};
+
+template <typename CoordFn>
+__device__ __attribute__((always_inline)) inline void blk_issue(const DImg &s, uint32_t lim, CoordFn coord, BlkLoads &L) {
     const int W = s.w, H = s.h;
     const uint32_t st = (uint32_t)s.stride;
     const uint32_t b3 = (uint32_t)((uintptr_t)s.p & 3);
     const uint32_t lim8 = lim - 8;
-    const gptr<const uint8_t> base = gp(s.p - b3);
-    uint64_t r0[4], r1[4];
-    uint32_t o0[4];
-    bool ok[4];
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(s.p - b3, 0, (int)lim, kWarpBufWord3);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        c[j] = coord(j);
-        const int cx = min(max(c[j].ix, 0), W - 2), cy = min(max(c[j].iy, 0), H - 2);
-        o0[j] = (uint32_t)cy * st + (uint32_t)cx + b3;
-        const uint32_t o1 = o0[j] + st;
-        ok[j] = cx == c[j].ix && cy == c[j].iy && o1 <= lim8;
-        r0[j] = *(const gptr<const u64_a4>)(base + (min(o0[j], lim8) & ~3u));
-        r1[j] = *(const gptr<const u64_a4>)(base + (min(o1, lim8) & ~3u));
+        L.c[j] = coord(j);
+        L.o0[j] = (uint32_t)L.c[j].iy * st + (uint32_t)L.c[j].ix + b3;
+        const uint32_t o1 = L.o0[j] + st;
+        L.ok[j] = (unsigned)L.c[j].ix < (unsigned)(W - 1) && (unsigned)L.c[j].iy < (unsigned)(H - 1) && o1 <= lim8;
+        L.r0[j] = __builtin_bit_cast(vu32x2, __builtin_amdgcn_raw_buffer_load_b64(rs, L.o0[j] & ~3u, 0, 0));
+        L.r1[j] = __builtin_bit_cast(vu32x2, __builtin_amdgcn_raw_buffer_load_b64(rs, o1 & ~3u, 0, 0));
     }
+}
+
+template <bool PERSP>
+__device__ __attribute__((always_inline)) inline void blk_finish(const DImg &s, float cv, const BlkLoads &L,
+                                                                 float (&out)[4]) {
+    const int W = s.w, H = s.h;
+    const uint32_t st = (uint32_t)s.stride;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        const uint32_t t = __builtin_amdgcn_alignbyte((uint32_t)(r0[j] >> 32), (uint32_t)r0[j], o0[j] & 3);
-        const uint32_t u = __builtin_amdgcn_alignbyte((uint32_t)(r1[j] >> 32), (uint32_t)r1[j], (o0[j] + st) & 3);
+        const uint32_t t = __builtin_amdgcn_alignbyte(L.r0[j].y, L.r0[j].x, L.o0[j] & 3);
+        const uint32_t u = __builtin_amdgcn_alignbyte(L.r1[j].y, L.r1[j].x, (L.o0[j] + st) & 3);
         const float p00 = (float)(t & 0xff), p01 = (float)((t >> 8) & 0xff);
         const float p10 = (float)(u & 0xff), p11 = (float)((u >> 8) & 0xff);
-        out[j] = PERSP ? blend_persp(p00, p01, p10, p11, c[j].ax, c[j].ay)
-                       : blend_affine(p00, p01, p10, p11, c[j].ax, c[j].ay);
+        out[j] = PERSP ? blend_persp(p00, p01, p10, p11, L.c[j].ax, L.c[j].ay)
+                       : blend_affine(p00, p01, p10, p11, L.c[j].ax, L.c[j].ay);
     }
     const gptr<const uint8_t> img = gp(s.p);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        if (!ok[j]) {
-            const int x0 = c[j].ix, y0 = c[j].iy;
+        if (!L.ok[j]) {
+            const int x0 = L.c[j].ix, y0 = L.c[j].iy;
             if (x0 < -1 || y0 < -1 || x0 >= W || y0 >= H) {
                 out[j] = cv;
             } else {
@@ -343,10 +358,31 @@ __device__ __attribute__((always_inline)) inline void blk_sample_eager(const DIm
                 const float p01 = vx1 && vy0 ? (float)img[ya + xb] : cv;
                 const float p10 = vx0 && vy1 ? (float)img[yb + xa] : cv;
                 const float p11 = vx1 && vy1 ? (float)img[yb + xb] : cv;
-                out[j] = PERSP ? blend_persp(p00, p01, p10, p11, c[j].ax, c[j].ay)
-                               : blend_affine(p00, p01, p10, p11, c[j].ax, c[j].ay);
+                out[j] = PERSP ? blend_persp(p00, p01, p10, p11, L.c[j].ax, L.c[j].ay)
+                               : blend_affine(p00, p01, p10, p11, L.c[j].ax, L.c[j].ay);
             }
         }
+    }
+}
+
+// Blocks (64 x 4 pixels) of a wave in the eager kernels: T consecutive
+// block rows, the next block's coordinates and loads issued before the
+// current block is blended, so one block's arithmetic overlaps the other's
+// loads (with one block per wave every wave computes, then waits, then
+// blends at about the same time). make_coord(t) gives block t's coordinate
+// function; store(t, v) writes its results.
+template <bool PERSP, int T, typename MakeCoord, typename Store>
+__device__ __attribute__((always_inline)) inline void blk_pipeline(const DImg &s, float cv, uint32_t lim, int rows_left,
+                                                                   MakeCoord make_coord, Store store) {
+    BlkLoads L[2];
+    blk_issue(s, lim, make_coord(0), L[0]);
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        if (t + 1 < T && t + 1 < rows_left) blk_issue(s, lim, make_coord(t + 1), L[(t + 1) & 1]);
+        float v[4];
+        blk_finish<PERSP>(s, cv, L[t & 1], v);
+        store(t, v);
+        if (t + 1 >= rows_left) break;
     }
 }
 
@@ -364,6 +400,25 @@ __device__ __attribute__((always_inline)) inline void blk_store(const DImg &d, i
     if (y < d.h && x < d.w) store4_u8(d, x, y, word);
 }
 
+// blk_store for a wave that owns its LDS words: no workgroup barrier (LDS
+// operations of one wave complete in order; the fences keep the compiler
+// from moving the read above the writes, or the next block's writes above
+// this read).
+__device__ __attribute__((always_inline)) inline void blk_store_wave(const DImg &d, int bx0, int by0, const float (&v)[4],
+                                                                     uint32_t *lds) {
+    const int l = (int)(threadIdx.x & 63);
+    uint8_t *row = reinterpret_cast<uint8_t *>(lds) + (l / kBlkW) * 64;
+#pragma unroll
+    for (int j = 0; j < 4; j++) row[kBlkW * j + (l % kBlkW)] = (uint8_t)pack_u8(v[j], 0, 0);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t word = lds[l];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int y = by0 + (l >> 4), x = bx0 + 4 * (l & 15);
+    if (y < d.h && x < d.w) store4_u8(d, x, y, word);
+}
+
 // Affine coordinates of pixel (x, y) (geometry.zig:432-494). The host runs
 // this kernel only when |sx| and |sy| stay below 1e9 for every pixel, where
 // toIdx's clamp changes nothing.
@@ -374,7 +429,7 @@ __device__ __attribute__((always_inline)) inline BlkCoord blk_affine_coord(const
     return BlkCoord{(int)fx, (int)fy, sx - fx, sy - fy};
 }
 
-template <int NW, bool EAGER>
+template <int NW>
 __global__ __launch_bounds__(64 * NW) void warp_affine_blk_u8(AffineParams p, TileOrder o, uint32_t lim) {
     __shared__ uint32_t lds[NW][64];
     int bx, by;
@@ -386,15 +441,29 @@ __global__ __launch_bounds__(64 * NW) void warp_affine_blk_u8(AffineParams p, Ti
     const float bxv = p.m[1] * yf + p.m[2], byv = p.m[4] * yf + p.m[5];
     BlkCoord c[4];
     float v[4];
-    if (EAGER) {
-        blk_sample_eager<false>(p.s, p.cv, lim, [&](int j) { return blk_affine_coord(p.m, bx0 + kBlkW * j + l % kBlkW, bxv, byv); },
-                                c, v);
-    } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++) c[j] = blk_affine_coord(p.m, bx0 + kBlkW * j + l % kBlkW, bxv, byv);
-        blk_sample<false>(p.s, p.cv, lim, c, v);
-    }
+    for (int j = 0; j < 4; j++) c[j] = blk_affine_coord(p.m, bx0 + kBlkW * j + l % kBlkW, bxv, byv);
+    blk_sample<false>(p.s, p.cv, lim, c, v);
     blk_store(p.d, bx0, by0, v, lds[wv]);
+}
+
+// The eager form: each wave T block rows of one 64-pixel column.
+template <int NW, int T>
+__global__ __launch_bounds__(64 * NW) void warp_affine_blk_pipe_u8(AffineParams p, TileOrder o, uint32_t lim) {
+    __shared__ uint32_t lds[NW][64];
+    int bx, by;
+    if (!tile_of(o, bx, by)) return;
+    const int wv = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
+    const int bx0 = bx * 4 * kBlkW, by0 = (by * NW + wv) * T * kBlkH;
+    if (by0 >= p.d.h) return;
+    const int rows_left = (p.d.h - by0 + kBlkH - 1) / kBlkH;
+    auto make_coord = [&](int t) {
+        const float yf = (float)(by0 + t * kBlkH + l / kBlkW);
+        const float bxv = p.m[1] * yf + p.m[2], byv = p.m[4] * yf + p.m[5];
+        return [&p, bx0, l, bxv, byv](int j) { return blk_affine_coord(p.m, bx0 + kBlkW * j + l % kBlkW, bxv, byv); };
+    };
+    blk_pipeline<false, T>(p.s, p.cv, lim, rows_left, make_coord,
+                           [&](int t, const float (&v)[4]) { blk_store_wave(p.d, bx0, by0 + t * kBlkH, v, lds[wv]); });
 }
 
 // 1 / w in f64, bit for bit as the compiler's correctly rounded division
@@ -412,11 +481,32 @@ __device__ __attribute__((always_inline)) inline double recip_f64(double w) {
     return fma(q, r, r);
 }
 
-// FAST: the host has found every |w| of the launch's pixels in [2^-60,
-// 2^60] with one sign (w is affine in x and y, so its corners bound it),
-// where recip_f64 equals the division; otherwise 1 / w (0 for w = 0) and
-// the reference's clamp of the coordinates to +-1e9.
-template <int NW, bool FAST, bool EAGER>
+// Coordinates of pixel (x, y) of warpPerspective (geometry.zig:625-645):
+// f64 map, x0 = floor(clamp(sx, +-1e9)), ax = f32(sx - x0). FAST: the host
+// has found every |w| of the launch's pixels in [2^-60, 2^60] with one sign
+// (w is affine in x and y, so its corners bound it), where recip_f64 equals
+// the division and a coordinate beyond 1e9 only ever lands outside the
+// image, so the clamp is left out; otherwise 1 / w (0 for w = 0) and the
+// clamp. c1, c4, c7 are m[1] y, m[4] y, m[7] y.
+template <bool FAST>
+__device__ __attribute__((always_inline)) inline BlkCoord persp_coord(const double *m, int x, double c1, double c4,
+                                                                      double c7) {
+    const double fx = (double)x;
+    double sx = m[0] * fx + c1 + m[2];
+    double sy = m[3] * fx + c4 + m[5];
+    const double w = m[6] * fx + c7 + m[8];
+    const double iw = FAST ? recip_f64(w) : (w != 0 ? 1.0 / w : 0.0);
+    sx *= iw;
+    sy *= iw;
+    if (!FAST) {
+        sx = clampd(sx, -1e9, 1e9);
+        sy = clampd(sy, -1e9, 1e9);
+    }
+    const double x0f = floor(sx), y0f = floor(sy);
+    return BlkCoord{(int)x0f, (int)y0f, (float)(sx - x0f), (float)(sy - y0f)};
+}
+
+template <int NW, bool FAST>
 __global__ __launch_bounds__(64 * NW) void warp_persp_blk_u8(PerspParams p, TileOrder o, uint32_t lim) {
     __shared__ uint32_t lds[NW][64];
     int bx, by;
@@ -425,34 +515,31 @@ __global__ __launch_bounds__(64 * NW) void warp_persp_blk_u8(PerspParams p, Tile
     const int bx0 = bx * 4 * kBlkW, by0 = (by * NW + wv) * kBlkH;
     const double fy = (double)(by0 + l / kBlkW);
     const double c1 = p.m[1] * fy, c4 = p.m[4] * fy, c7 = p.m[7] * fy;
-    // geometry.zig:625-645: f64 map, x0 = floor(clamp(sx, +-1e9)),
-    // ax = f32(sx - x0). Without the clamp where 1 / w is in range: a
-    // coordinate beyond 1e9 then only ever lands outside the image.
-    auto coord = [&](int j) {
-        const double fx = (double)(bx0 + kBlkW * j + l % kBlkW);
-        double sx = p.m[0] * fx + c1 + p.m[2];
-        double sy = p.m[3] * fx + c4 + p.m[5];
-        const double w = p.m[6] * fx + c7 + p.m[8];
-        const double iw = FAST ? recip_f64(w) : (w != 0 ? 1.0 / w : 0.0);
-        sx *= iw;
-        sy *= iw;
-        if (!FAST) {
-            sx = clampd(sx, -1e9, 1e9);
-            sy = clampd(sy, -1e9, 1e9);
-        }
-        const double x0f = floor(sx), y0f = floor(sy);
-        return BlkCoord{(int)x0f, (int)y0f, (float)(sx - x0f), (float)(sy - y0f)};
-    };
     BlkCoord c[4];
     float v[4];
-    if (EAGER) {
-        blk_sample_eager<true>(p.s, (float)p.cvd, lim, coord, c, v);
-    } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++) c[j] = coord(j);
-        blk_sample<true>(p.s, (float)p.cvd, lim, c, v);
-    }
+    for (int j = 0; j < 4; j++) c[j] = persp_coord<FAST>(p.m, bx0 + kBlkW * j + l % kBlkW, c1, c4, c7);
+    blk_sample<true>(p.s, (float)p.cvd, lim, c, v);
     blk_store(p.d, bx0, by0, v, lds[wv]);
+}
+
+// The eager form: each wave T block rows of one 64-pixel column.
+template <int NW, int T, bool FAST>
+__global__ __launch_bounds__(64 * NW) void warp_persp_blk_pipe_u8(PerspParams p, TileOrder o, uint32_t lim) {
+    __shared__ uint32_t lds[NW][64];
+    int bx, by;
+    if (!tile_of(o, bx, by)) return;
+    const int wv = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
+    const int bx0 = bx * 4 * kBlkW, by0 = (by * NW + wv) * T * kBlkH;
+    if (by0 >= p.d.h) return;
+    const int rows_left = (p.d.h - by0 + kBlkH - 1) / kBlkH;
+    auto make_coord = [&](int t) {
+        const double fy = (double)(by0 + t * kBlkH + l / kBlkW);
+        const double c1 = p.m[1] * fy, c4 = p.m[4] * fy, c7 = p.m[7] * fy;
+        return [&p, bx0, l, c1, c4, c7](int j) { return persp_coord<FAST>(p.m, bx0 + kBlkW * j + l % kBlkW, c1, c4, c7); };
+    };
+    blk_pipeline<true, T>(p.s, (float)p.cvd, lim, rows_left, make_coord,
+                          [&](int t, const float (&v)[4]) { blk_store_wave(p.d, bx0, by0 + t * kBlkH, v, lds[wv]); });
 }
 
 inline unsigned blocks_for(long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
@@ -461,6 +548,14 @@ inline int launched() { return hipGetLastError() == hipSuccess ? OK : INTERNAL; 
 inline dim3 grid2d(int w, int h, int px) { return dim3((unsigned)((w + 64 * px - 1) / (64 * px)), (unsigned)((h + 3) / 4)); }
 
 }  // namespace
+
+// Block rows per wave of the eager block kernels (tuning "warp_blocks"):
+// one by default. Two, with the second block's loads issued before the first
+// is blended, measured slower (1080p: affine 5.6 -> 6.3 us, perspective 6.4
+// -> 7.3 us): the kernels are bound by VALU issue with every wave of the
+// launch resident at once, and the pipelined form needs about twice the
+// registers.
+inline int warp_blocks_per_wave() { return tuning().warp_blocks == 2 ? 2 : 1; }
 
 // The block kernels' 8-byte gathers address the source with 32-bit offsets
 // from the dword below its base: 0 if those do not fit.
@@ -492,11 +587,16 @@ int launch_warp_affine(const AffineParams &p, hipStream_t st) {
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1 && p.border == BORDER_CONSTANT && lim && affine_bounded(p) &&
         tuning().warp_layout != 1) {
         constexpr int NW = 2;
-        const TileOrder o = tile_order((p.d.w + 63) / 64, (p.d.h + 4 * NW - 1) / (4 * NW), true);
-        if (p.s.w >= 2 && p.s.h >= 2 && tuning().warp_layout != 2)
-            hipLaunchKernelGGL((warp_affine_blk_u8<NW, true>), dim3(tile_grid(o.ntiles)), dim3(64 * NW), 0, st, p, o, lim);
-        else
-            hipLaunchKernelGGL((warp_affine_blk_u8<NW, false>), dim3(tile_grid(o.ntiles)), dim3(64 * NW), 0, st, p, o, lim);
+        if (p.s.w >= 2 && p.s.h >= 2 && tuning().warp_layout != 2) {
+            const int T = warp_blocks_per_wave();
+            const TileOrder o = tile_order((p.d.w + 63) / 64, (p.d.h + 4 * NW * T - 1) / (4 * NW * T), true);
+            const dim3 g(tile_grid(o.ntiles)), t(64 * NW);
+            if (T == 1) hipLaunchKernelGGL((warp_affine_blk_pipe_u8<NW, 1>), g, t, 0, st, p, o, lim);
+            else hipLaunchKernelGGL((warp_affine_blk_pipe_u8<NW, 2>), g, t, 0, st, p, o, lim);
+        } else {
+            const TileOrder o = tile_order((p.d.w + 63) / 64, (p.d.h + 4 * NW - 1) / (4 * NW), true);
+            hipLaunchKernelGGL((warp_affine_blk_u8<NW>), dim3(tile_grid(o.ntiles)), dim3(64 * NW), 0, st, p, o, lim);
+        }
         return launched();
     }
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1 && p.border == BORDER_CONSTANT) {
@@ -545,11 +645,20 @@ int launch_warp_perspective(const PerspParams &p, hipStream_t st) {
                 sign = sg;
             }
         const bool eager = p.s.w >= 2 && p.s.h >= 2 && tuning().warp_layout != 2;
-        const dim3 g(tile_grid(o.ntiles)), t(64 * NW);
-        if (fast && eager) hipLaunchKernelGGL((warp_persp_blk_u8<NW, true, true>), g, t, 0, st, p, o, lim);
-        else if (fast) hipLaunchKernelGGL((warp_persp_blk_u8<NW, true, false>), g, t, 0, st, p, o, lim);
-        else if (eager) hipLaunchKernelGGL((warp_persp_blk_u8<NW, false, true>), g, t, 0, st, p, o, lim);
-        else hipLaunchKernelGGL((warp_persp_blk_u8<NW, false, false>), g, t, 0, st, p, o, lim);
+        const dim3 t(64 * NW);
+        if (eager) {
+            const int T = warp_blocks_per_wave();
+            const TileOrder ot = tile_order((p.d.w + 63) / 64, (p.d.h + 4 * NW * T - 1) / (4 * NW * T), true);
+            const dim3 g(tile_grid(ot.ntiles));
+#define PERSP_PIPE(TT, F) hipLaunchKernelGGL((warp_persp_blk_pipe_u8<NW, TT, F>), g, t, 0, st, p, ot, lim)
+            if (T == 1) { if (fast) PERSP_PIPE(1, true); else PERSP_PIPE(1, false); }
+            else { if (fast) PERSP_PIPE(2, true); else PERSP_PIPE(2, false); }
+#undef PERSP_PIPE
+        } else {
+            const dim3 g(tile_grid(o.ntiles));
+            if (fast) hipLaunchKernelGGL((warp_persp_blk_u8<NW, true>), g, t, 0, st, p, o, lim);
+            else hipLaunchKernelGGL((warp_persp_blk_u8<NW, false>), g, t, 0, st, p, o, lim);
+        }
         return launched();
     }
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1) {

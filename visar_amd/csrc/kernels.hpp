@@ -18,6 +18,8 @@ struct Tuning {
     int stencil_direct = 0;      // one-channel packed stencils: 0 rows streamed through registers where the layout allows, 1 LDS tiles
     int stencil_band = 0;        // streamed stencils: output rows per wave (2, 4, 8, 16)
     int stencil_wg = 0;          // streamed stencils: waves per workgroup (1, 2, 4)
+    int warp_blocks = 0;         // eager u8 block warps: 64 x 4 pixel block rows per wave (1, 2; 0 = 1)
+    int gray_layout = 0;         // u8 cvtColor to gray on contiguous images: 0 one contiguous 1024-pixel block per wave, 1 16 pixels per thread
 };
 Tuning &tuning();
 bool set_tuning(const char *key, int value);

@@ -30,6 +30,8 @@ const Knob kKnobs[] = {
     {"stencil_direct", &Tuning::stencil_direct, 0, 1},
     {"stencil_band", &Tuning::stencil_band, 0, 16},
     {"stencil_wg", &Tuning::stencil_wg, 0, 4},
+    {"gray_layout", &Tuning::gray_layout, 0, 1},
+    {"warp_blocks", &Tuning::warp_blocks, 0, 2},
 };
 }  // namespace
 

@@ -665,6 +665,51 @@ def test_morphology_fused_long_chains(va, orc):
             check(va, orc, "morphology", a, op, shape, (ks, ks), iterations=it)
 
 
+MORPH_BASES = [(0, 3), (0, 5), (0, 7), (1, 3), (1, 5), (1, 7), (2, 5), (2, 7)]  # (shape, size): rect, cross, ellipse
+
+
+def test_morphology_streamed_powers(va, orc):
+    """The n-fold streamed passes (one launch for n iterations, gradient's
+    two chains in one launch) against the oracle's pass-by-pass result for
+    every base element, op and n = 1..4 (4 runs as 2 + 2), at widths on both
+    sides of the 496- and 480-byte strips, heights below and above the row
+    bands and the largest element's reach, and several band heights."""
+    from visar_amd import autotune
+    sizes = [(1, 8), (3, 12), (7, 500), (23, 496), (41, 1004), (97, 964)]
+    try:
+        for band in (0, 4, 40):
+            autotune.set_knob("morph_band", band)
+            for shape, ks in MORPH_BASES:
+                for it in (1, 2, 3, 4):
+                    for (h, w) in (sizes if band == 0 else sizes[2:4]):
+                        a = (rnd((h, w), h * 7 + w + ks + it) > 160).astype(np.uint8) * 255 if it % 2 else rnd((h, w), h + w)
+                        for op in range(7):
+                            want = orc.morphology(a, op, shape, (ks, ks), it)
+                            got = va.morphology(dev(a), op, shape, (ks, ks), it).cpu().numpy()
+                            assert_same(got, want, f"op{op} shape{shape} k{ks} it{it} {h}x{w} band {band}")
+    finally:
+        autotune.set_knob("morph_band", 0)
+
+
+def test_morphology_streamed_equals_per_pass(va):
+    """The streamed n-fold passes equal the pass-by-pass kernels (tuning
+    "morph_stream" 1) on a 1000x1400 image and a 4096x4096 mask."""
+    from visar_amd import autotune, synth
+    imgs = [dev(natural(1000, 1400, 1, 81)), dev(synth.morph_mask(0, 4096))]
+    try:
+        for x in imgs:
+            for shape, ks in [(0, 3), (2, 5), (1, 7)]:
+                for it in (1, 2, 3):
+                    for op in range(7):
+                        outs = []
+                        for knob in (0, 1):
+                            autotune.set_knob("morph_stream", knob)
+                            outs.append(va.morphology(x, op, shape, (ks, ks), it))
+                        assert torch.equal(outs[0], outs[1]), f"op{op} shape{shape} k{ks} it{it} {tuple(x.shape)}"
+    finally:
+        autotune.set_knob("morph_stream", 0)
+
+
 def test_morphology_fused_equals_per_pass(va):
     """The fused chains equal the one-kernel-per-pass path (tuning
     "morph_chain" 1) on a 1000x1400 image, every op and element of the

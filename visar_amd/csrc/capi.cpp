@@ -1459,6 +1459,94 @@ static int morph_direct_launch(const DImg &in, const DImg &out, uint32_t el, int
     return hipGetLastError() == hipSuccess ? OK : INTERNAL;
 }
 
+// One launch of morph_stream.hpp: base element `el` taken n times (n = 2,
+// 3; n = 1 for gradient), mode 0 erode, 1 dilate, 2 gradient; post as in
+// morph_direct_launch. UNSUPPORTED (nothing launched) when there is no kernel.
+static int morph_stream_launch(const DImg &in, const DImg &out, uint32_t el, int n, int mode, int post, const DImg *c,
+                               hipStream_t st) {
+    MorphStreamArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.s = in;
+    a.d = out;
+    a.c = c ? *c : in;
+    auto span = [](const DImg &m) { return (uint64_t)(m.h - 1) * (uint64_t)m.stride + (uint64_t)m.w * m.cn; };
+    if (in.w < 8 || span(a.s) >= (1ull << 31) || span(a.d) >= (1ull << 31) || span(a.c) >= (1ull << 31))
+        return UNSUPPORTED;
+    a.span_s = (uint32_t)span(a.s);
+    a.span_d = (uint32_t)span(a.d);
+    a.span_c = (uint32_t)span(a.c);
+    a.tb = tuning().morph_band > 0 ? tuning().morph_band : 16;
+    a.xcd = tuning().tile_order != 1;
+    if (!launch_morph_stream(a, el, n, mode, post, st)) return UNSUPPORTED;
+    return hipGetLastError() == hipSuccess ? OK : INTERNAL;
+}
+
+// morphologyEx (morph.zig:532-571) with the streamed n-fold passes of
+// morph_stream.hpp: n iterations of an erode or dilate as one launch (runs
+// of 3 and 2 for n > 3), gradient's dilation and erosion as one launch (n
+// up to 3), the subtraction of tophat and blackhat folded into the last
+// launch. For one-channel images whose rows are a multiple of 4 bytes (at
+// least 8) starting on dwords, base elements with codes of their own where
+// morph_stream_pays, and n >= 2 (1 <= n <= 3 for gradient). UNSUPPORTED
+// (nothing launched) otherwise.
+// Whether one streamed pass of `el` taken n times beats n passes: each input
+// row costs the streamed kernel one min / max per row of every rectangle's
+// height, measured on a 4096x4096 mask to pay up to 9 such rows (rect 3x3
+// taken 3 times 15 against 23 us, gradient 5x5 ellipse 19 against 24 us)
+// and not at 21 (5x5 ellipse twice 27 against 21 us, 7x7 cross three times
+// 61 against 34 us).
+static bool morph_stream_pays(uint32_t el, int n) {
+    const mstream::RectSet r = mstream::power(el, n);
+    int rows = 0;
+    for (int k = 0; k < r.n; k++) rows += 2 * r.b[k] + 1;
+    return rows <= 9;
+}
+
+static int morph_stream_ex(Call &call, const DImg &s, const DImg &d, int op, int n, uint32_t el) {
+    auto aligned = [](const DImg &m) { return ((((uintptr_t)m.p) | (uintptr_t)m.stride) & 3) == 0; };
+    if (tuning().morph_stream == 1 || s.cn != 1 || (s.w & 3) || s.w < 8 || !aligned(s) || !aligned(d))
+        return UNSUPPORTED;
+    if ((el & 15u) == 0 || !morph_stream_pays(el, std::min(n, 3))) return UNSUPPORTED;
+    hipStream_t st = call.stream();
+    if (op == 4 && n <= 3) return morph_stream_launch(s, d, el, n, 2, 0, nullptr, st);
+    if (n < 2) return UNSUPPORTED;
+    // Passes of one kind: n as runs of 3 and 2 from `in` into `out`, the
+    // last run with post against c.
+    DImg tmp[2] = {};
+    int k = 0;
+    bool first = true;
+    auto passes = [&](const DImg &in, const DImg &out, int mode, int post, const DImg *c) -> int {
+        DImg cur = in;
+        for (int left = n; left > 0;) {
+            const int run = left == 4 || left == 2 ? 2 : 3;
+            const bool last = run == left;
+            DImg o = out;
+            if (!last) {
+                DImg &t = tmp[k++ & 1];
+                if (!t.p && call.temp_like(s, 1, t)) return OUT_OF_MEMORY;
+                o = t;
+            }
+            const int rc = morph_stream_launch(cur, o, el, run, mode, last ? post : 0, c, st);
+            if (rc == UNSUPPORTED && !first) return INTERNAL;
+            if (rc) return rc;
+            first = false;
+            cur = o;
+            left -= run;
+        }
+        return OK;
+    };
+    if (op <= 1) return passes(s, d, op, 0, nullptr);
+    if (op == 4) return UNSUPPORTED;  // more than 3 iterations: pass by pass
+    // open / tophat: erode then dilate; close / blackhat: dilate then erode.
+    const int m1 = op == 2 || op == 5 ? 0 : 1;
+    DImg t;
+    if (call.temp_like(s, 1, t)) return OUT_OF_MEMORY;
+    int rc = passes(s, t, m1, 0, nullptr);
+    if (rc) return rc;
+    rc = passes(t, d, 1 - m1, op == 5 ? 1 : op == 6 ? 2 : 0, &s);
+    return rc == UNSUPPORTED ? INTERNAL : rc;
+}
+
 // morphologyEx (morph.zig:532-571) as one-channel launches of
 // morph_direct.hpp, one or two passes each, intermediates in temporaries, the
 // subtraction folded into the last launch: for one-channel images whose
@@ -1469,6 +1557,10 @@ static int morph_direct_ex(Call &call, const DImg &s, const DImg &d, int op, int
     if (s.cn != 1 || (s.w & 3) || !aligned(s) || !aligned(d) || tuning().morph_chain == 2) return UNSUPPORTED;
     uint32_t el = (uint32_t)r.nr;
     for (int k = 0; k < r.nr; k++) el |= (uint32_t)(r.rw[k] | r.rv[k] << 2) << (4 + 4 * k);
+    {
+        const int rc = morph_stream_ex(call, s, d, op, n, el);
+        if (rc != UNSUPPORTED) return rc;
+    }
     std::vector<int> A, B;
     int post = 0;
     switch (op) {

@@ -19,6 +19,8 @@ struct Tuning {
     int stencil_band = 0;        // streamed stencils: output rows per wave (2, 4, 8, 16)
     int stencil_wg = 0;          // streamed stencils: waves per workgroup (1, 2, 4)
     int warp_blocks = 0;         // eager u8 block warps: 64 x 4 pixel block rows per wave (1, 2; 0 = 1)
+    int morph_stream = 0;        // one-channel u8 morphology with iterations or gradient: 0 streamed n-fold passes, 1 pass by pass
+    int morph_band = 0;          // streamed n-fold passes: output rows per band (0 = 16)
     int gray_layout = 0;         // u8 cvtColor to gray on contiguous images: 0 one contiguous 1024-pixel block per wave, 1 16 pixels per thread
 };
 Tuning &tuning();
@@ -260,6 +262,90 @@ struct MorphDirectArgs {
     int strips, bands4, ntiles, xcd;
     int ops;
 };
+namespace mstream {
+// A set of centred rectangles (half-width a, half-height b), none inside
+// another, ordered by b.
+struct RectSet {
+    int n;
+    int a[16], b[16];
+};
+
+constexpr RectSet rects_of(uint32_t el) {
+    RectSet r{};
+    r.n = (int)(el & 15u);
+    for (int k = 0; k < r.n; k++) {
+        r.a[k] = (int)((el >> (4 + 4 * k)) & 3u);
+        r.b[k] = (int)((el >> (6 + 4 * k)) & 3u);
+    }
+    return r;
+}
+
+// The Minkowski sum of two unions of centred rectangles: the pairwise sums,
+// without those inside another.
+constexpr RectSet minkowski(const RectSet &x, const RectSet &y) {
+    RectSet o{};
+    for (int i = 0; i < x.n; i++)
+        for (int j = 0; j < y.n; j++) {
+            const int a = x.a[i] + y.a[j], b = x.b[i] + y.b[j];
+            bool inside = false;
+            for (int k = 0; k < o.n; k++)
+                if (o.a[k] >= a && o.b[k] >= b) inside = true;
+            if (inside) continue;
+            int m = 0;
+            for (int k = 0; k < o.n; k++)
+                if (!(a >= o.a[k] && b >= o.b[k])) {
+                    o.a[m] = o.a[k];
+                    o.b[m] = o.b[k];
+                    m++;
+                }
+            o.a[m] = a;
+            o.b[m] = b;
+            o.n = m + 1;
+        }
+    for (int i = 1; i < o.n; i++)  // order by b
+        for (int k = i; k > 0 && o.b[k] < o.b[k - 1]; k--) {
+            const int ta = o.a[k], tb = o.b[k];
+            o.a[k] = o.a[k - 1];
+            o.b[k] = o.b[k - 1];
+            o.a[k - 1] = ta;
+            o.b[k - 1] = tb;
+        }
+    return o;
+}
+
+constexpr RectSet power(uint32_t el, int n) {
+    RectSet r = rects_of(el);
+    for (int k = 1; k < n; k++) r = minkowski(r, rects_of(el));
+    return r;
+}
+
+constexpr int max_a(const RectSet &r) {
+    int m = 0;
+    for (int k = 0; k < r.n; k++) m = r.a[k] > m ? r.a[k] : m;
+    return m;
+}
+constexpr int max_b(const RectSet &r) {
+    int m = 0;
+    for (int k = 0; k < r.n; k++) m = r.b[k] > m ? r.b[k] : m;
+    return m;
+}
+
+// Half-width and half-height limits of the streamed kernel (12 bytes of halo).
+constexpr int kMaxA = 12, kMaxB = 12;
+
+}  // namespace mstream
+
+// One erode / dilate / gradient pass of a base element (an element code of
+// morph_chain.hpp) taken n times in one launch (morph_stream.hpp), on one
+// channel: mode 0 erode, 1 dilate, 2 gradient; post as above. False (nothing
+// launched) when (element, n, mode, post) has no kernel.
+struct MorphStreamArgs {
+    DImg s, d, c;                     // c: the combine image of post 1 / 2
+    uint32_t span_s, span_d, span_c;  // bytes from each image's first to last byte + 1 (< 2^31)
+    int strips, ntiles, xcd;          // tiles: (strip, group of 4 bands)
+    int tb;                           // output rows per band
+};
+bool launch_morph_stream(const MorphStreamArgs &a, uint32_t el, int n, int mode, int post, hipStream_t st);
 bool launch_morph_direct1(const MorphDirectArgs &a, uint32_t el, int post, hipStream_t st);
 bool launch_morph_direct2(const MorphDirectArgs &a, uint32_t el, int post, hipStream_t st);
 

@@ -32,6 +32,8 @@ const Knob kKnobs[] = {
     {"stencil_wg", &Tuning::stencil_wg, 0, 4},
     {"gray_layout", &Tuning::gray_layout, 0, 1},
     {"warp_blocks", &Tuning::warp_blocks, 0, 2},
+    {"morph_stream", &Tuning::morph_stream, 0, 1},
+    {"morph_band", &Tuning::morph_band, 0, 256},
 };
 }  // namespace
 

@@ -1489,17 +1489,10 @@ static int morph_stream_launch(const DImg &in, const DImg &out, uint32_t el, int
 // least 8) starting on dwords, base elements with codes of their own where
 // morph_stream_pays, and n >= 2 (1 <= n <= 3 for gradient). UNSUPPORTED
 // (nothing launched) otherwise.
-// Whether one streamed pass of `el` taken n times beats n passes: each input
-// row costs the streamed kernel one min / max per row of every rectangle's
-// height, measured on a 4096x4096 mask to pay up to 9 such rows (rect 3x3
-// taken 3 times 15 against 23 us, gradient 5x5 ellipse 19 against 24 us)
-// and not at 21 (5x5 ellipse twice 27 against 21 us, 7x7 cross three times
-// 61 against 34 us).
+// Whether one streamed pass of `el` taken n times beats n passes (and so
+// has a kernel): mstream::kStreamMaxRows.
 static bool morph_stream_pays(uint32_t el, int n) {
-    const mstream::RectSet r = mstream::power(el, n);
-    int rows = 0;
-    for (int k = 0; k < r.n; k++) rows += 2 * r.b[k] + 1;
-    return rows <= 9;
+    return mstream::acc_rows(mstream::power(el, n)) <= mstream::kStreamMaxRows;
 }
 
 static int morph_stream_ex(Call &call, const DImg &s, const DImg &d, int op, int n, uint32_t el) {

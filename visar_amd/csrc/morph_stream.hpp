@@ -233,17 +233,20 @@ void launch_one(MorphStreamArgs a, hipStream_t st) {
 
 // The launches of one base element: n = 1 gradient only; n = 2, 3 erode,
 // dilate, gradient, dilate with post 1 (tophat's second pass) and erode
-// with post 2 (blackhat's). False for other combinations.
+// with post 2 (blackhat's), where the n-th power takes at most
+// kStreamMaxRows accumulator rows. False for other combinations.
 template <uint32_t BASE>
 bool launch_base(const MorphStreamArgs &a, int n, int mode, int post, hipStream_t st) {
     auto pick = [&](auto nc) -> bool {
         constexpr int NN = decltype(nc)::value;
-        if (mode == 2 && post == 0) return launch_one<BASE, NN, 2, 0>(a, st), true;
-        if constexpr (NN >= 2) {
-            if (mode == 0 && post == 0) return launch_one<BASE, NN, 0, 0>(a, st), true;
-            if (mode == 1 && post == 0) return launch_one<BASE, NN, 1, 0>(a, st), true;
-            if (mode == 1 && post == 1) return launch_one<BASE, NN, 1, 1>(a, st), true;
-            if (mode == 0 && post == 2) return launch_one<BASE, NN, 0, 2>(a, st), true;
+        if constexpr (acc_rows(power(BASE, NN)) <= kStreamMaxRows) {
+            if (mode == 2 && post == 0) return launch_one<BASE, NN, 2, 0>(a, st), true;
+            if constexpr (NN >= 2) {
+                if (mode == 0 && post == 0) return launch_one<BASE, NN, 0, 0>(a, st), true;
+                if (mode == 1 && post == 0) return launch_one<BASE, NN, 1, 0>(a, st), true;
+                if (mode == 1 && post == 1) return launch_one<BASE, NN, 1, 1>(a, st), true;
+                if (mode == 0 && post == 2) return launch_one<BASE, NN, 0, 2>(a, st), true;
+            }
         }
         return false;
     };

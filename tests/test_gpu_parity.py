@@ -545,6 +545,32 @@ def test_warp_u8_layouts_agree_on_views(va, orc):
         autotune.set_knob("warp_blocks", 0)
 
 
+def test_warp_affine_u8_c3(va, orc):
+    """Three-channel u8 bilinear BORDER_CONSTANT (12-byte tap-row loads):
+    device views at odd offsets and strides, a packed image filling its whole
+    buffer (the last row's loads must stop at its end), tiny images, and
+    maps that leave the image or shrink and enlarge it."""
+    from visar_amd import synth
+    big = natural(230, 410, 3, 97)
+    dbig = dev(big)
+    for y0, x0, h, w in [(0, 0, 200, 307), (3, 1, 197, 301), (5, 7, 1, 1), (0, 0, 2, 3), (29, 103, 200, 305)]:
+        a = np.ascontiguousarray(big[y0:y0 + h, x0:x0 + w])
+        dv = dbig[y0:y0 + h, x0:x0 + w]
+        ms = [synth.affine_matrix(5, w, h), synth.rotation_matrix(w / 2, h / 2, -25, 0.85),
+              synth.rotation_matrix(w / 2, h / 2, 60, 2.5), np.array([[1, 0, -w * 0.6], [0, 1, h * 0.3]])]
+        for m in ms:
+            for size in [(w, h), (w + 33, h + 9)]:
+                want = orc.warp_affine(a, m, size, 1, 0, 5.0)
+                assert_same(va.warp_affine(dv, m, size, 1, 0, 5.0).cpu().numpy(), want, f"view {(y0, x0, h, w)} {size}")
+    t = dev(natural(37, 61, 3, 98))
+    tn = t.cpu().numpy()
+    for m in [np.array([[1, 0, 0.4], [0, 1, 0.3]]), synth.affine_matrix(6, 61, 37)]:
+        assert_same(va.warp_affine(t, m, (61, 37), 1, 0, 3.0).cpu().numpy(), orc.warp_affine(tn, m, (61, 37), 1, 0, 3.0),
+                    f"packed {m.tolist()}")
+    a = synth.uniform_bytes(9, (1080, 1920, 3))
+    check(va, orc, "warp_affine", a, synth.affine_matrix(0, 1920, 1080), (1920, 1080), interpolation=1, border=0)
+
+
 def test_warp_affine_1080p_config(va, orc):
     from visar_amd import synth
     a = synth.uniform_bytes(7, (1080, 1920))

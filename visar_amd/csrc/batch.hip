@@ -43,6 +43,7 @@ enum JobKind : int {
     JK_AFFINE_BOX = 10,
     JK_LINEAR_BOX = 11,
     JK_AREA_BOX = 12,
+    JK_AFFINE_C3 = 13,
 };
 
 struct ResizeArgs {
@@ -57,6 +58,7 @@ struct ResizeArgs {
 struct BatchJob {
     int kind;
     int variant;  // template selector (kernel size, channels, rows per thread, min/max)
+    uint32_t lim; // JK_AFFINE_C3: the source's tap_limit
     int tiles_x;
     int ntiles;
     union {
@@ -135,6 +137,7 @@ template <int KIND>
 __global__ __launch_bounds__(kTileThreads) void batch_other(BATCH_ARGS) {
     run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) {
         if constexpr (KIND == JK_AFFINE_LIN4) affine_lin4_tile(j.af, bx, by);
+        else if constexpr (KIND == JK_AFFINE_C3) affine_c3_u8_tile(j.af, j.lim, bx, by);
         else if constexpr (KIND == JK_AFFINE_PX) affine_px_tile<uint8_t>(j.af, bx, by);
         else if constexpr (KIND == JK_HALF_C1) half_c1_tile(j.rs.s, j.rs.d, bx, by);
         else if constexpr (KIND == JK_HALF) half_tile(j.rs.s, j.rs.d, bx, by);
@@ -178,6 +181,7 @@ GroupKernel group_kernel(int kind, int variant) {
     case JK_AFFINE_BOX: return batch_affine_box;
     case JK_AFFINE_LIN4: return batch_other<JK_AFFINE_LIN4>;
     case JK_AFFINE_PX: return batch_other<JK_AFFINE_PX>;
+    case JK_AFFINE_C3: return batch_other<JK_AFFINE_C3>;
     case JK_HALF_C1: return batch_other<JK_HALF_C1>;
     case JK_HALF: return batch_other<JK_HALF>;
     case JK_LINEAR: return batch_other<JK_LINEAR>;
@@ -342,6 +346,10 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
         if (it.src.channels == 1 && p.interp == 1) {
             j.kind = JK_AFFINE_LIN4;
             j.tiles_x = (d.w + 255) / 256;
+        } else if (it.src.channels == 3 && p.interp == 1 && p.border == BORDER_CONSTANT && tap_limit(s)) {
+            j.kind = JK_AFFINE_C3;
+            j.lim = tap_limit(s);
+            j.tiles_x = (d.w + 63) / 64;
         } else {
             j.kind = JK_AFFINE_PX;
             j.tiles_x = (d.w + 63) / 64;

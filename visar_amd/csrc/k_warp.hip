@@ -429,6 +429,11 @@ __device__ __attribute__((always_inline)) inline BlkCoord blk_affine_coord(const
     return BlkCoord{(int)fx, (int)fy, sx - fx, sy - fy};
 }
 
+__global__ __launch_bounds__(kTileThreads) void warp_affine_c3_u8(AffineParams p, TileOrder o, uint32_t lim) {
+    int bx, by;
+    if (tile_of(o, bx, by)) affine_c3_u8_tile(p, lim, bx, by);
+}
+
 template <int NW>
 __global__ __launch_bounds__(64 * NW) void warp_affine_blk_u8(AffineParams p, TileOrder o, uint32_t lim) {
     __shared__ uint32_t lds[NW][64];
@@ -597,6 +602,11 @@ int launch_warp_affine(const AffineParams &p, hipStream_t st) {
             const TileOrder o = tile_order((p.d.w + 63) / 64, (p.d.h + 4 * NW - 1) / (4 * NW), true);
             hipLaunchKernelGGL((warp_affine_blk_u8<NW>), dim3(tile_grid(o.ntiles)), dim3(64 * NW), 0, st, p, o, lim);
         }
+        return launched();
+    }
+    if (p.depth == 0 && p.d.cn == 3 && p.interp == 1 && p.border == BORDER_CONSTANT && tap_limit(p.s)) {
+        const TileOrder o = tile_order((p.d.w + 63) / 64, (p.d.h + 3) / 4, true);
+        hipLaunchKernelGGL(warp_affine_c3_u8, dim3(tile_grid(o.ntiles)), b, 0, st, p, o, tap_limit(p.s));
         return launched();
     }
     if (p.depth == 0 && p.d.cn == 1 && p.interp == 1 && p.border == BORDER_CONSTANT) {

@@ -190,6 +190,13 @@ struct AffineParams {
     double cvd;  // border value as stored sample
 };
 int launch_warp_affine(const AffineParams &p, hipStream_t st);
+// Bytes from the dword at or below an image's base to the end of its last
+// row (the range of the warps' buffer loads), or 0 when that does not fit
+// in 32 bits with room for a 16-byte load.
+inline uint32_t tap_limit(const DImg &s) {
+    const uint64_t lim = ((uintptr_t)s.p & 3) + (uint64_t)(s.h - 1) * (uint64_t)s.stride + (uint64_t)s.w * s.cn;
+    return lim >= 16 && lim + 16 < (1ull << 31) ? (uint32_t)lim : 0;
+}
 
 struct PerspParams {
     DImg s, d;

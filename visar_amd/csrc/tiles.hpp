@@ -1018,6 +1018,71 @@ __device__ __attribute__((always_inline)) inline void affine_pixel(const AffineP
     }
 }
 
+// Three-channel u8 bilinear warpAffine with a constant border
+// (geometry.zig:508-533, the same arithmetic as affine_pixel): 64 x 4 pixels
+// per tile, one per thread. A pixel whose footprint lies inside the image
+// loads each tap row's two pixels (6 bytes) with one 12-byte buffer load at
+// the dword at or below them, v_alignbyte picking the bytes, instead of six
+// byte gathers; the range check of the buffer (lim bytes from the dword
+// below the image base) makes every offset safe, and pixels whose footprint
+// leaves the image (or whose second row's load would pass the image's last
+// byte) take the byte-by-byte rules of affine_pixel.
+__device__ __attribute__((always_inline)) inline void affine_c3_u8_tile(const AffineParams &p, uint32_t lim, int bx, int by) {
+    const DImg &s = p.s, &d = p.d;
+    const int y = by * 4 + tile_ty();
+    const int x = bx * 64 + tile_tx();
+    if (y >= d.h || x >= d.w) return;
+    const float yf = (float)y, xf = (float)x;
+    const float sx = fmaf(p.m[0], xf, p.m[1] * yf + p.m[2]);
+    const float sy = fmaf(p.m[3], xf, p.m[4] * yf + p.m[5]);
+    const float fx = floorf(sx), fy = floorf(sy);
+    const float ax = sx - fx, ay = sy - fy;
+    const int ix = (int)to_idx(fx), iy = (int)to_idx(fy);
+    const int W = s.w, H = s.h;
+    const gptr<uint8_t> o = grow<uint8_t>(d, y) + 3L * x;
+    if (ix < -1 || iy < -1 || ix >= W || iy >= H) {
+        const uint8_t c = (uint8_t)p.cvd;
+        o[0] = c;
+        o[1] = c;
+        o[2] = c;
+        return;
+    }
+    const uint32_t st = (uint32_t)s.stride, b3 = (uint32_t)((uintptr_t)s.p & 3);
+    const uint32_t o0 = (uint32_t)iy * st + 3u * (uint32_t)ix + b3, o1 = o0 + st;
+    if ((unsigned)ix < (unsigned)(W - 1) && (unsigned)iy < (unsigned)(H - 1) && (o1 & ~3u) + 12 <= lim) {
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(s.p - b3, 0, (int)lim, 0x00020000);
+        typedef unsigned int vu32x3 __attribute__((ext_vector_type(3)));
+        const vu32x3 r0 = __builtin_bit_cast(vu32x3, __builtin_amdgcn_raw_buffer_load_b96(rs, o0 & ~3u, 0, 0));
+        const vu32x3 r1 = __builtin_bit_cast(vu32x3, __builtin_amdgcn_raw_buffer_load_b96(rs, o1 & ~3u, 0, 0));
+        // bytes 0..5 of each tap row: pixel ix (0-2) and ix + 1 (3-5)
+        const uint32_t a0 = __builtin_amdgcn_alignbyte(r0.y, r0.x, o0 & 3), a1 = __builtin_amdgcn_alignbyte(r0.z, r0.y, o0 & 3);
+        const uint32_t b0 = __builtin_amdgcn_alignbyte(r1.y, r1.x, o1 & 3), b1 = __builtin_amdgcn_alignbyte(r1.z, r1.y, o1 & 3);
+        const uint64_t ra = (uint64_t)a1 << 32 | a0, rb = (uint64_t)b1 << 32 | b0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float p00 = (float)((ra >> (8 * c)) & 255u), p01 = (float)((ra >> (8 * c + 24)) & 255u);
+            const float p10 = (float)((rb >> (8 * c)) & 255u), p11 = (float)((rb >> (8 * c + 24)) & 255u);
+            const float v0 = fmaf(ax, p01 - p00, p00);
+            const float v1 = fmaf(ax, p11 - p10, p10);
+            o[c] = (uint8_t)pack_u8(fmaf(ay, v1 - v0, v0), 0, 0);
+        }
+        return;
+    }
+    const bool vx0 = (unsigned)ix < (unsigned)W, vx1 = (unsigned)(ix + 1) < (unsigned)W;
+    const bool vy0 = (unsigned)iy < (unsigned)H, vy1 = (unsigned)(iy + 1) < (unsigned)H;
+    const gptr<const uint8_t> ra = grow<const uint8_t>(s, vy0 ? iy : 0), rb = grow<const uint8_t>(s, vy1 ? iy + 1 : 0);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float p00 = vx0 && vy0 ? (float)ra[3 * ix + c] : p.cv;
+        const float p01 = vx1 && vy0 ? (float)ra[3 * (ix + 1) + c] : p.cv;
+        const float p10 = vx0 && vy1 ? (float)rb[3 * ix + c] : p.cv;
+        const float p11 = vx1 && vy1 ? (float)rb[3 * (ix + 1) + c] : p.cv;
+        const float v0 = fmaf(ax, p01 - p00, p00);
+        const float v1 = fmaf(ax, p11 - p10, p10);
+        o[c] = (uint8_t)pack_u8(fmaf(ay, v1 - v0, v0), 0, 0);
+    }
+}
+
 // Four output bytes of lane tx at x0 + tx + 64j (j = 0..3) of row y: each
 // store instruction writes 64 consecutive bytes.
 __device__ __attribute__((always_inline)) inline void store4_strided_u8(const DImg &d, int x0, int y, uint32_t v) {

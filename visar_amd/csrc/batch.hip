@@ -21,7 +21,7 @@
 #include "../../include/visar_batch.h"
 #include "runtime.hpp"
 #include "tables.hpp"
-#include "tiles.hpp"
+#include "warp_blk.hpp"
 
 namespace visar {
 
@@ -44,6 +44,7 @@ enum JobKind : int {
     JK_LINEAR_BOX = 11,
     JK_AREA_BOX = 12,
     JK_AFFINE_C3 = 13,
+    JK_AFFINE_BLK = 14,
 };
 
 struct ResizeArgs {
@@ -58,7 +59,7 @@ struct ResizeArgs {
 struct BatchJob {
     int kind;
     int variant;  // template selector (kernel size, channels, rows per thread, min/max)
-    uint32_t lim; // JK_AFFINE_C3: the source's tap_limit
+    uint32_t lim; // JK_AFFINE_C3: the source's tap_limit; JK_AFFINE_BLK: its blk_limit
     int tiles_x;
     int ntiles;
     union {
@@ -112,6 +113,11 @@ __global__ __launch_bounds__(kTileThreads) void batch_stencil_swar(BATCH_ARGS) {
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(5))) void batch_affine_box(BATCH_ARGS) {
     __shared__ uint32_t lds[kWarpLdsWords];
     run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) { affine_box_tile(j.af, bx, by, lds); });
+}
+
+__global__ __launch_bounds__(kTileThreads) void batch_affine_blk(BATCH_ARGS) {
+    __shared__ uint32_t lds[4 * 64];
+    run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) { affine_blk_tile(j.af, j.lim, bx, by, lds); });
 }
 
 template <int K, int CN, bool MAX>
@@ -179,6 +185,7 @@ GroupKernel group_kernel(int kind, int variant) {
     case JK_STENCIL_SWAR:
         return k == SWAR_G3 ? swar_kernel<SWAR_G3>(cn, last) : k == SWAR_G5 ? swar_kernel<SWAR_G5>(cn, last) : swar_kernel<SWAR_G7>(cn, last);
     case JK_AFFINE_BOX: return batch_affine_box;
+    case JK_AFFINE_BLK: return batch_affine_blk;
     case JK_AFFINE_LIN4: return batch_other<JK_AFFINE_LIN4>;
     case JK_AFFINE_PX: return batch_other<JK_AFFINE_PX>;
     case JK_AFFINE_C3: return batch_other<JK_AFFINE_C3>;
@@ -337,6 +344,14 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
         for (int i = 0; i < 6; i++) p.m[i] = (float)inv[i];
         p.cvd = sat_cast<uint8_t>(it.value);
         p.cv = (float)p.cvd;
+        if (it.src.channels == 1 && p.interp == 1 && p.border == BORDER_CONSTANT && s.w >= 2 && s.h >= 2 &&
+            blk_limit(s) && affine_bounded(p) && tuning().warp_layout != 1) {
+            j.kind = JK_AFFINE_BLK;
+            j.lim = blk_limit(s);
+            j.tiles_x = (d.w + 63) / 64;
+            j.ntiles = j.tiles_x * ((d.h + 15) / 16);
+            return OK;
+        }
         if (it.src.channels == 1 && p.interp == 1 && p.border == BORDER_CONSTANT) {
             j.kind = JK_AFFINE_BOX;
             j.tiles_x = (d.w + kWarpTW - 1) / kWarpTW;

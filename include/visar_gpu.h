@@ -39,7 +39,7 @@ int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D
 
 /* Launch-shape knobs read by the kernels (the autotuner's interface), 0 =
  * automatic: "stencil_rows" output rows per thread of the small separable
- * stencils (1, 2, 4); "batch_stencil_rows" the same inside batches (1, 2);
+ * stencils (1, 2, 4); "batch_stencil_rows" the same inside batches (1, 2, 4);
  * "batch_unit_kb" KiB of input + output per batch workgroup (1-1024);
  * "tile_order" the order of a plain call's tiles over the XCDs (0 each
  * kernel's default, 1 row-major, 2 one band of neighbouring tiles per XCD);

@@ -43,8 +43,15 @@ def _compare_batch(va, orc, n, lo, hi, seed, oracle_every, use_graph):
             assert np.array_equal(got, pipelines.batch_item(orc, j)), f"item {i} {j['spec']} differs from the oracle"
 
 
-def test_batch_small_mixed_graph(va, orc):
-    _compare_batch(va, orc, 96, 16, 300, 11, 1, True)
+@pytest.mark.parametrize("rows", [0, 2, 4])
+def test_batch_small_mixed_graph(va, orc, rows):
+    """Every stencil launch shape (tuning "batch_stencil_rows" 1, 2, 4)."""
+    from visar_amd import autotune
+    autotune.set_knob("batch_stencil_rows", rows)
+    try:
+        _compare_batch(va, orc, 96, 16, 300, 11 + rows, 1, True)
+    finally:
+        autotune.set_knob("batch_stencil_rows", 0)
 
 
 def test_batch_small_mixed_eager(va, orc):

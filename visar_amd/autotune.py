@@ -123,7 +123,7 @@ def batch_signature(specs):
     return ",".join(f"{k}:{counts[k]}" for k in sorted(counts))
 
 
-BATCH_SPACE = {"batch_stencil_rows": [1, 2], "batch_unit_kb": [4, 8, 16, 32, 64]}
+BATCH_SPACE = {"batch_stencil_rows": [1, 2, 4], "batch_unit_kb": [4, 8, 16, 32, 64]}
 STENCIL_SPACE = {"stencil_rows": [1, 2, 4]}
 
 

@@ -157,13 +157,14 @@ using GroupKernel = void (*)(BATCH_ARGS);
 
 template <int K, int CN>
 GroupKernel stencil_kernel(int th) {
-    return th == 2 ? batch_stencil<K, CN, 2> : batch_stencil<K, CN, 1>;
+    return th == 4 ? batch_stencil<K, CN, 4> : th == 2 ? batch_stencil<K, CN, 2> : batch_stencil<K, CN, 1>;
 }
 
 template <int MODE>
 GroupKernel swar_kernel(int cn, int th) {
-    if (cn == 1) return th == 2 ? batch_stencil_swar<MODE, 1, 2> : batch_stencil_swar<MODE, 1, 1>;
-    return th == 2 ? batch_stencil_swar<MODE, 3, 2> : batch_stencil_swar<MODE, 3, 1>;
+    if (cn == 1)
+        return th == 4 ? batch_stencil_swar<MODE, 1, 4> : th == 2 ? batch_stencil_swar<MODE, 1, 2> : batch_stencil_swar<MODE, 1, 1>;
+    return th == 4 ? batch_stencil_swar<MODE, 3, 4> : th == 2 ? batch_stencil_swar<MODE, 3, 2> : batch_stencil_swar<MODE, 3, 1>;
 }
 
 template <int K, int CN>
@@ -285,7 +286,7 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
         j.st.scale = 1.0f;
         for (int i = 0; i < 7; i++) j.st.kx[i] = j.st.ky[i] = i < k ? (int)taps[i] : 0;
         int th = tuning().batch_stencil_rows ? tuning().batch_stencil_rows : 1;
-        th = th >= 2 ? 2 : 1;
+        th = th >= 4 ? 4 : th >= 2 ? 2 : 1;
         j.variant = k * 100 + (int)it.src.channels * 10 + th;
         const int mode = swar_mode(j.st, k, 0);
         if (mode == SWAR_G3 || mode == SWAR_G5 || mode == SWAR_G7) {

@@ -8,7 +8,7 @@ namespace visar {
 // Launch-shape knobs (see visar_gpu_set_tuning); 0 means automatic.
 struct Tuning {
     int stencil_rows = 0;        // plain stencils: output rows per thread (1, 2, 4)
-    int batch_stencil_rows = 0;  // batched stencils: output rows per thread (1, 2)
+    int batch_stencil_rows = 0;  // batched stencils: output rows per thread (1, 2, 4)
     int batch_unit_kb = 0;       // batched calls: KiB of input + output per workgroup
     int tile_order = 0;          // plain tile kernels: 0 each kernel's default, 1 row-major, 2 one band of tiles per XCD
     int sep_rows = 0;            // separable engine: output rows per tile (1-16)

@@ -20,7 +20,7 @@ struct Knob {
 };
 const Knob kKnobs[] = {
     {"stencil_rows", &Tuning::stencil_rows, 0, 4},
-    {"batch_stencil_rows", &Tuning::batch_stencil_rows, 0, 2},
+    {"batch_stencil_rows", &Tuning::batch_stencil_rows, 0, 4},
     {"batch_unit_kb", &Tuning::batch_unit_kb, 0, 1024},
     {"tile_order", &Tuning::tile_order, 0, 2},
     {"sep_rows", &Tuning::sep_rows, 0, 16},

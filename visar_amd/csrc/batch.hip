@@ -21,6 +21,7 @@
 #include "../../include/visar_batch.h"
 #include "runtime.hpp"
 #include "tables.hpp"
+#include "morph_chain.hpp"
 #include "warp_blk.hpp"
 
 namespace visar {
@@ -45,6 +46,7 @@ enum JobKind : int {
     JK_AREA_BOX = 12,
     JK_AFFINE_C3 = 13,
     JK_AFFINE_BLK = 14,
+    JK_MORPH_CHAIN = 15,
 };
 
 struct ResizeArgs {
@@ -67,6 +69,7 @@ struct BatchJob {
         MorphSmallArgs mo;
         AffineParams af;
         ResizeArgs rs;
+        MorphChainArgs mc;
     };
 };
 
@@ -120,6 +123,39 @@ __global__ __launch_bounds__(kTileThreads) void batch_affine_blk(BATCH_ARGS) {
     run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) { affine_blk_tile(j.af, j.lim, bx, by, lds); });
 }
 
+using GroupKernel = void (*)(BATCH_ARGS);
+
+// One erode / dilate pass of the fused chain kernel (morph_chain.hpp) on
+// tiles of kBatchChainRows rows.
+constexpr int kBatchChainRows = 16;
+template <int CN, int K, uint32_t EL>
+__global__ __launch_bounds__(kTileThreads) void batch_morph_chain(BATCH_ARGS) {
+    __shared__ __align__(16) uint32_t lds[(kBatchChainRows + 2 * K) * mchain::ChainGeo<CN, K>::PITCH];
+    run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) { mchain::chain_tile<CN, K, EL>(j.mc, bx, by, lds); });
+}
+
+template <int CN>
+GroupKernel chain_kernel(uint32_t el) {
+    using namespace mchain;
+    switch (el) {
+    case kElRect3: return batch_morph_chain<CN, 1, kElRect3>;
+    case kElRect5: return batch_morph_chain<CN, 2, kElRect5>;
+    case kElRect7: return batch_morph_chain<CN, 3, kElRect7>;
+    case kElCross3: return batch_morph_chain<CN, 1, kElCross3>;
+    case kElCross5: return batch_morph_chain<CN, 2, kElCross5>;
+    case kElCross7: return batch_morph_chain<CN, 3, kElCross7>;
+    case kElEllipse5: return batch_morph_chain<CN, 2, kElEllipse5>;
+    default: return batch_morph_chain<CN, 3, kElEllipse7>;
+    }
+}
+
+// Whether element code el has a chain kernel in the batch.
+bool batch_chain_el(uint32_t el) {
+    using namespace mchain;
+    return el == kElRect3 || el == kElRect5 || el == kElRect7 || el == kElCross3 || el == kElCross5 || el == kElCross7 ||
+           el == kElEllipse5 || el == kElEllipse7;
+}
+
 template <int K, int CN, bool MAX>
 __global__ __launch_bounds__(kTileThreads) void batch_morph(BATCH_ARGS) {
     __shared__ uint32_t lds[MorphShape<K, CN, 2>::LDS_WORDS];
@@ -153,7 +189,6 @@ __global__ __launch_bounds__(kTileThreads) void batch_other(BATCH_ARGS) {
     });
 }
 
-using GroupKernel = void (*)(BATCH_ARGS);
 
 template <int K, int CN>
 GroupKernel stencil_kernel(int th) {
@@ -187,6 +222,7 @@ GroupKernel group_kernel(int kind, int variant) {
         return k == SWAR_G3 ? swar_kernel<SWAR_G3>(cn, last) : k == SWAR_G5 ? swar_kernel<SWAR_G5>(cn, last) : swar_kernel<SWAR_G7>(cn, last);
     case JK_AFFINE_BOX: return batch_affine_box;
     case JK_AFFINE_BLK: return batch_affine_blk;
+    case JK_MORPH_CHAIN: return variant % 10 == 1 ? chain_kernel<1>((uint32_t)(variant / 10)) : chain_kernel<3>((uint32_t)(variant / 10));
     case JK_AFFINE_LIN4: return batch_other<JK_AFFINE_LIN4>;
     case JK_AFFINE_PX: return batch_other<JK_AFFINE_PX>;
     case JK_AFFINE_C3: return batch_other<JK_AFFINE_C3>;
@@ -323,6 +359,40 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
                 while (x < it.ksize && mask[(size_t)ky * it.ksize + x]) x++;
                 mp.runs[mp.nruns++] = MorphRun{(int16_t)ky, (int16_t)x0, (int16_t)(x - x0), 0};
             }
+        }
+        MorphRects mr;
+        if (tuning().morph_chain != 1 && morph_rects(mp, mr) && batch_chain_el(morph_el_code(mr))) {
+            // One pass of the fused chain kernel, as the plain call's
+            // morph_chain_launch sets it up for one chain of one pass.
+            MorphChainArgs &a = j.mc;
+            std::memset(&a, 0, sizeof a);
+            const int cn = (int)it.src.channels;
+            const int K = std::max(1, std::max(mr.eh, mr.ev));
+            a.s = s;
+            a.d = d;
+            a.c = s;
+            a.ev = mr.ev;
+            a.vb = mr.ev;
+            a.hb = (mr.eh * cn + 7) & ~7;
+            a.tw = 512 - 2 * a.hb;
+            a.nr = mr.nr;
+            for (int k = 0; k < mr.nr; k++) {
+                a.rw[k] = mr.rw[k];
+                a.rv[k] = mr.rv[k];
+                for (int e = -mr.rv[k]; e <= mr.rv[k]; e++) a.cmask[mr.rw[k]] |= 1u << (e + K);
+            }
+            a.pass[0] = MorphChainPass{0, -1, (int8_t)it.morph_op, 1};
+            a.npass = 1;
+            a.nbuf = 1;
+            a.th = kBatchChainRows;
+            a.rows = a.th + 2 * a.vb;
+            a.tiles_x = (s.w * cn + a.tw - 1) / a.tw;
+            a.ntiles = a.tiles_x * ((s.h + a.th - 1) / a.th);
+            j.kind = JK_MORPH_CHAIN;
+            j.variant = (int)morph_el_code(mr) * 10 + cn;
+            j.tiles_x = a.tiles_x;
+            j.ntiles = a.ntiles;
+            return OK;
         }
         const int kf = morph_small_args(mp, j.mo);
         if (!kf) return fail(UNSUPPORTED, "Unsupported");

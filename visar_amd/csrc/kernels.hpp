@@ -225,6 +225,48 @@ struct MorphParams {
     MorphRun runs[kMaxRuns];
 };
 int launch_morph_pass(const MorphParams &p, hipStream_t st);
+
+// The element as a union of centred rectangles (k_morph_chain.hip), or false:
+// odd sides up to 7, the anchor at the centre, at most one run per row,
+// centred on the anchor, with half-widths symmetric about the centre row
+// and non-increasing away from it. Rectangle k: half-width rw[k], half-height
+// rv[k]; eh / ev: the largest of each.
+struct MorphRects {
+    int nr, rw[4], rv[4], eh, ev;
+};
+inline bool morph_rects(const MorphParams &p, MorphRects &r) {
+    if (p.kw > 7 || p.kh > 7 || !(p.kw & 1) || !(p.kh & 1) || p.ax != p.kw / 2 || p.ay != p.kh / 2) return false;
+    int hw[7] = {-1, -1, -1, -1, -1, -1, -1};
+    for (int k = 0; k < p.nruns; k++) {
+        const MorphRun &run = p.runs[k];
+        if (hw[run.ky] >= 0 || !(run.len & 1) || run.x0 + run.len / 2 != p.ax) return false;
+        hw[run.ky] = run.len / 2;
+    }
+    const int ay = p.ay;
+    if (hw[ay] < 0) return false;
+    for (int d = 1; d <= ay; d++)
+        if (hw[ay - d] != hw[ay + d] || hw[ay + d] > hw[ay + d - 1]) return false;
+    r.nr = 0;
+    r.ev = 0;
+    for (int d = 0; d <= ay; d++) {
+        const int next = d < ay ? hw[ay + d + 1] : -1;
+        if (hw[ay + d] >= 0) r.ev = d;
+        if (hw[ay + d] >= 0 && next < hw[ay + d]) {
+            r.rw[r.nr] = hw[ay + d];
+            r.rv[r.nr] = d;
+            r.nr++;
+        }
+    }
+    r.eh = hw[ay];
+    return true;
+}
+// The element code of a union of centred rectangles (RectOf in morph_chain.hpp).
+inline uint32_t morph_el_code(const MorphRects &r) {
+    uint32_t el = (uint32_t)r.nr;
+    for (int k = 0; k < r.nr; k++) el |= (uint32_t)(r.rw[k] | r.rv[k] << 2) << (4 + 4 * k);
+    return el;
+}
+
 struct MorphSmallArgs;
 int morph_small_args(const MorphParams &p, MorphSmallArgs &a);
 // d = saturate(a - b) (morph.zig:574-593)

@@ -381,12 +381,12 @@ __device__ __attribute__((always_inline)) inline void chain_pass(const MorphChai
     }
 }
 
+// Tile (bx, by) of one launch (also run by the batch kernels, with the
+// arguments in device memory). lds: nbuf buffers of a.rows x PITCH dwords
+// (plus the combine image's for comb 3).
 template <int CN, int K, uint32_t EL>
-__global__ __launch_bounds__(kThreads) void morph_chain_u8(MorphChainArgs a) {
+__device__ __attribute__((always_inline)) inline void chain_tile(const MorphChainArgs &a, int bx, int by, uint32_t *lds) {
     using G = ChainGeo<CN, K>;
-    extern __shared__ __align__(16) uint32_t lds[];
-    int bx, by;
-    if (!tile_of(TileOrder{a.tiles_x, a.ntiles, a.xcd}, bx, by)) return;
     const int buf_dw = a.rows * G::PITCH;
     PassCtx cx;
     cx.lane = (int)threadIdx.x & 63;
@@ -424,6 +424,14 @@ __global__ __launch_bounds__(kThreads) void morph_chain_u8(MorphChainArgs a) {
             else chain_pass<CN, K, EL, false, true>(a, cx, src, nullptr, cb, ps.q, 0u);
         }
     }
+}
+
+template <int CN, int K, uint32_t EL>
+__global__ __launch_bounds__(kThreads) void morph_chain_u8(MorphChainArgs a) {
+    extern __shared__ __align__(16) uint32_t lds[];
+    int bx, by;
+    if (!tile_of(TileOrder{a.tiles_x, a.ntiles, a.xcd}, bx, by)) return;
+    chain_tile<CN, K, EL>(a, bx, by, lds);
 }
 
 template <int CN, int K, uint32_t EL>

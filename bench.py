@@ -52,6 +52,9 @@ def parse():
                          "0: the measured best per workload (chain 3, pyramid/box/adaptive step 4, others 2)")
     ap.add_argument("--seconds", type=float, default=6.0, help="served workload: Poisson run length")
     ap.add_argument("--host-workers", type=int, default=8, help="served workload: connected-components threads")
+    ap.add_argument("--morph-cases", type=int, default=63,
+                    help="morphology workload: time the first N of the 63 cases (0: only the roofline kernel, "
+                         "for counter passes)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (scales the HBM byte counters of a profile)")
@@ -251,7 +254,11 @@ def bench_morph(args, torch, va):
     for op in range(7):
         for shape, ks, en in MORPH_ELEMENTS:
             for it in (1, 2, 3):
-                cases[f"{MORPH_OPS[op]}/{en}/it{it}"] = min(per_call_ms(op, shape, ks, it, 5) for _ in range(3))
+                if len(cases) < args.morph_cases:
+                    cases[f"{MORPH_OPS[op]}/{en}/it{it}"] = min(per_call_ms(op, shape, ks, it, 5) for _ in range(3))
+    if not cases:  # roofline kernel only
+        print(json.dumps({"roofline_only": True, "ms_per_launch": round(single, 5)}))
+        return
     total_ms = sum(cases.values())
     value = len(cases) * S * S / (total_ms * 1e-3) / 1e9
     gbs = nbytes / (single * 1e-3) / 1e9

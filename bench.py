@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--streams", type=int, default=64)
     ap.add_argument("--lanes", type=int, default=0, choices=[0, 1, 2, 3, 4],
                     help="frames in flight in the graph-replayed workloads (step j runs on stream j %% lanes); "
-                         "0: the measured best per workload (chain 3, pyramid/box/adaptive step 4, others 2)")
+                         "0: the measured best per workload (GaussianBlur 1, chain 3, pyramid/box/adaptive step 4, others 2)")
     ap.add_argument("--seconds", type=float, default=6.0, help="served workload: Poisson run length")
     ap.add_argument("--host-workers", type=int, default=8, help="served workload: connected-components threads")
     ap.add_argument("--morph-cases", type=int, default=63,
@@ -526,7 +526,7 @@ def main():
 
     R = 24
     if args.lanes == 0:
-        args.lanes = {2: 3, 8: 4}.get(args.config, 2)
+        args.lanes = {1: 1, 2: 3, 8: 4}.get(args.config, 2)
     NB = max(2, args.lanes)  # output buffer sets: step i writes set i % NB (one per frame in flight)
     if args.config == 2:
         W, H = 3840, 2160

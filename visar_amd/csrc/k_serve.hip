@@ -32,10 +32,19 @@ __global__ __launch_bounds__(kTileThreads) void serve_k(const ServeLaunch a) {
     __shared__ uint32_t fr[kFR * kFW + kFR];
     __shared__ uint32_t tm[kMR][kMC];
     __shared__ uint32_t em[kER][kMC];
+    // Workgroup b runs on XCD b % 8 and takes tile (b % 8) ceil(N / 8) + b / 8
+    // of the launch's N = frames x tiles, frame-major: each XCD walks one
+    // contiguous run of tiles, so neighbouring tiles' halo rows come from
+    // the L2 that read them for the tile above.
     const int tiles_x = (a.w + kSW - 1) / kSW;
-    const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - ty * tiles_x;
+    const int tiles = tiles_x * ((a.h + kSH - 1) / kSH);
+    const int total = tiles * a.n;
+    const int t = (int)(blockIdx.x & 7) * ((total + 7) >> 3) + (int)(blockIdx.x >> 3);
+    if (t >= total) return;
+    const int frame = t / tiles, tile = t - frame * tiles;
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
     const int x0 = tx * kSW, y0 = ty * kSH;
-    const uint32_t job = a.job[blockIdx.y];
+    const uint32_t job = a.job[frame];
     const ServeStream &ss = a.streams[job & 0xffffu];
     const int par = (job >> 16) & 1, primed = (job >> 17) & 1;
     const int W = a.w, H = a.h;
@@ -177,8 +186,8 @@ __global__ __launch_bounds__(kTileThreads) void serve_k(const ServeLaunch a) {
 
 int launch_serve(const ServeLaunch &a, hipStream_t st) {
     if (a.n <= 0) return OK;
-    const int tiles = ((a.w + kSW - 1) / kSW) * ((a.h + kSH - 1) / kSH);
-    hipLaunchKernelGGL(serve_k, dim3((unsigned)tiles, (unsigned)a.n), dim3(kTileThreads), 0, st, a);
+    const long tiles = (long)((a.w + kSW - 1) / kSW) * ((a.h + kSH - 1) / kSH) * a.n;
+    hipLaunchKernelGGL(serve_k, dim3(tile_grid((int)tiles)), dim3(kTileThreads), 0, st, a);
     return hipGetLastError() == hipSuccess ? OK : INTERNAL;
 }
 

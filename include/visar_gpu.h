@@ -62,7 +62,13 @@ int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D
  * cvtColor to gray on contiguous 16-byte aligned images (0 each wave reads
  * one contiguous 1024-pixel block, 1 each thread its own 16 pixels);
  * "warp_blocks" 64 x 4 pixel blocks per wave of the default u8 warps (1, or
- * 2 with the second block's loads issued before the first is blended).
+ * 2 with the second block's loads issued before the first is blended);
+ * "morph_stream" one-channel u8 morphology with iterations or gradient (0 n
+ * iterations as one pass with the n-fold element where that is faster, 1
+ * pass by pass), "morph_band" its output rows per band (0 = 16);
+ * "morph_pass" the other one-channel u8 single passes (0 column window
+ * first, 1 the two-pass-capable kernel), "morph_pass_rows" their output rows
+ * per wave (4, 8, 16; 0 = 4).
  * Unknown keys or values out of range return VISAR_INVALID_ARGUMENT. */
 int visar_gpu_set_tuning(const char *key, int value);
 int visar_gpu_get_tuning(const char *key, int *value);

@@ -717,6 +717,38 @@ def test_morphology_streamed_powers(va, orc):
         autotune.set_knob("morph_band", 0)
 
 
+def test_morphology_band_passes(va, orc):
+    """The single passes with the column window first (morph_band.hpp) against
+    the oracle for every base element and op at widths on both sides of the
+    248-byte strips and heights below and above the row bands, for each band
+    height, and equal to morph_direct.hpp's passes (tuning "morph_pass" 1)
+    on a 4096x4096 mask."""
+    from visar_amd import autotune, synth
+    sizes = [(1, 8), (3, 12), (9, 252), (17, 496), (40, 1000)]
+    try:
+        for rows in (0, 8, 16):
+            autotune.set_knob("morph_pass_rows", rows)
+            for shape, ks in MORPH_BASES:
+                for (h, w) in (sizes if rows == 0 else sizes[2:]):
+                    a = (rnd((h, w), h * 5 + w + ks) > 150).astype(np.uint8) * 255 if ks == 5 else rnd((h, w), h + w + ks)
+                    for op in (0, 1, 2, 3, 5, 6):
+                        want = orc.morphology(a, op, shape, (ks, ks), 1)
+                        got = va.morphology(dev(a), op, shape, (ks, ks), 1).cpu().numpy()
+                        assert_same(got, want, f"op{op} shape{shape} k{ks} {h}x{w} rows {rows}")
+        autotune.set_knob("morph_pass_rows", 0)
+        m = dev(synth.morph_mask(1, 4096))
+        for shape, ks in MORPH_BASES:
+            for op in (0, 1, 5, 6):
+                outs = []
+                for knob in (0, 1):
+                    autotune.set_knob("morph_pass", knob)
+                    outs.append(va.morphology(m, op, shape, (ks, ks), 1))
+                assert torch.equal(outs[0], outs[1]), f"op{op} shape{shape} k{ks}"
+    finally:
+        autotune.set_knob("morph_pass_rows", 0)
+        autotune.set_knob("morph_pass", 0)
+
+
 def test_morphology_streamed_equals_per_pass(va):
     """The streamed n-fold passes equal the pass-by-pass kernels (tuning
     "morph_stream" 1) on a 1000x1400 image and a 4096x4096 mask."""

@@ -1505,6 +1505,26 @@ static int morph_stream_ex(Call &call, const DImg &s, const DImg &d, int op, int
     return rc == UNSUPPORTED ? INTERNAL : rc;
 }
 
+// One erode / dilate pass of morph_band.hpp from `in` into `out` (post as in
+// morph_direct_launch). UNSUPPORTED (nothing launched) when it has no kernel.
+static int morph_band_launch(const DImg &in, const DImg &out, uint32_t el, int op_max, int post, const DImg *c,
+                             hipStream_t st) {
+    MorphDirectArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.s = in;
+    a.d = out;
+    a.c = c ? *c : in;
+    auto span = [](const DImg &m) { return (uint64_t)(m.h - 1) * (uint64_t)m.stride + (uint64_t)m.w * m.cn; };
+    if (in.w < 8 || span(a.s) >= (1ull << 31) || span(a.d) >= (1ull << 31) || span(a.c) >= (1ull << 31))
+        return UNSUPPORTED;
+    a.span_s = (uint32_t)span(a.s);
+    a.span_d = (uint32_t)span(a.d);
+    a.span_c = (uint32_t)span(a.c);
+    a.xcd = tuning().tile_order != 1;
+    if (!launch_morph_band(a, el, op_max, post, st)) return UNSUPPORTED;
+    return hipGetLastError() == hipSuccess ? OK : INTERNAL;
+}
+
 // morphologyEx (morph.zig:532-571) as one-channel launches of
 // morph_direct.hpp, one or two passes each, intermediates in temporaries, the
 // subtraction folded into the last launch: for one-channel images whose
@@ -1550,7 +1570,9 @@ static int morph_direct_ex(Call &call, const DImg &s, const DImg &d, int op, int
                 o = t;
             }
             const int bits = ops[i] | (np == 2 ? ops[i + 1] << 1 : 0);
-            const int rc = morph_direct_launch(cur, o, el, np, bits, last ? pst : 0, comb, st);
+            int rc = UNSUPPORTED;
+            if (np == 1 && tuning().morph_pass != 1) rc = morph_band_launch(cur, o, el, ops[i], last ? pst : 0, comb, st);
+            if (rc == UNSUPPORTED) rc = morph_direct_launch(cur, o, el, np, bits, last ? pst : 0, comb, st);
             if (rc == UNSUPPORTED && !first) return INTERNAL;
             if (rc) return rc;
             first = false;

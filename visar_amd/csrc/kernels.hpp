@@ -21,6 +21,8 @@ struct Tuning {
     int warp_blocks = 0;         // eager u8 block warps: 64 x 4 pixel block rows per wave (1, 2; 0 = 1)
     int morph_stream = 0;        // one-channel u8 morphology with iterations or gradient: 0 streamed n-fold passes, 1 pass by pass
     int morph_band = 0;          // streamed n-fold passes: output rows per band (0 = 16)
+    int morph_pass = 0;          // one-channel u8 single passes: 0 column window first (morph_band.hpp), 1 morph_direct.hpp
+    int morph_pass_rows = 0;     // column-window-first passes: output rows per wave (4, 8, 16; 0 = 4)
     int gray_layout = 0;         // u8 cvtColor to gray on contiguous images: 0 one contiguous 1024-pixel block per wave, 1 16 pixels per thread
 };
 Tuning &tuning();
@@ -409,6 +411,10 @@ struct MorphStreamArgs {
 };
 bool launch_morph_stream(const MorphStreamArgs &a, uint32_t el, int n, int mode, int post, hipStream_t st);
 bool launch_morph_direct1(const MorphDirectArgs &a, uint32_t el, int post, hipStream_t st);
+// One erode (op_max 0) or dilate (1) pass of morph_band.hpp (one channel,
+// the column window first); post 1 with dilate, 2 with erode. Strips and
+// bands are set here. False (nothing launched) for other elements and posts.
+bool launch_morph_band(const MorphDirectArgs &a, uint32_t el, int op_max, int post, hipStream_t st);
 bool launch_morph_direct2(const MorphDirectArgs &a, uint32_t el, int post, hipStream_t st);
 
 // --------------------------------------------------------------- histogram

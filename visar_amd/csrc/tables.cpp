@@ -34,6 +34,8 @@ const Knob kKnobs[] = {
     {"warp_blocks", &Tuning::warp_blocks, 0, 2},
     {"morph_stream", &Tuning::morph_stream, 0, 1},
     {"morph_band", &Tuning::morph_band, 0, 256},
+    {"morph_pass", &Tuning::morph_pass, 0, 1},
+    {"morph_pass_rows", &Tuning::morph_pass_rows, 0, 16},
 };
 }  // namespace
 

@@ -1207,22 +1207,36 @@ template <int PITCH>
 __device__ __attribute__((always_inline)) inline void stage_box(const DImg &s, int ylo, int nr, int c_lo, int nd, uint32_t *lds) {
     const int ns = s.w * s.cn;
     const float rcp = 1.0f / (float)nd;
-    for (int i = threadIdx.x; i < nr * nd; i += kTileThreads) {
-        int r = (int)(((float)i + 0.5f) * rcp);
-        int k = i - r * nd;
-        if (k >= nd) r++, k -= nd;
-        if (k < 0) r--, k += nd;
-        const gptr<const uint8_t> row = grow<const uint8_t>(s, ylo + r);
-        const int c = c_lo - box_shift(s, ylo + r, c_lo) + 4 * k;
-        uint32_t v = 0;
-        if (c >= 0 && c + 4 <= ns) {
-            v = *(gptr<const uint32_t>)(row + c);
-        } else {
+    const int total = nr * nd;
+    // Four slots per thread per round, their loads issued together.
+    for (int base = threadIdx.x; base < total; base += 4 * kTileThreads) {
+        uint32_t v[4];
+        int slot[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++)
-                if (c + t >= 0 && c + t < ns) v |= (uint32_t)row[c + t] << (8 * t);
+        for (int u = 0; u < 4; u++) {
+            const int i = base + u * kTileThreads;
+            slot[u] = -1;
+            v[u] = 0;
+            if (i < total) {
+                int r = (int)(((float)i + 0.5f) * rcp);
+                int k = i - r * nd;
+                if (k >= nd) r++, k -= nd;
+                if (k < 0) r--, k += nd;
+                const gptr<const uint8_t> row = grow<const uint8_t>(s, ylo + r);
+                const int c = c_lo - box_shift(s, ylo + r, c_lo) + 4 * k;
+                if (c >= 0 && c + 4 <= ns) {
+                    v[u] = *(gptr<const uint32_t>)(row + c);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < 4; t++)
+                        if (c + t >= 0 && c + t < ns) v[u] |= (uint32_t)row[c + t] << (8 * t);
+                }
+                slot[u] = r * PITCH + k;
+            }
         }
-        lds[r * PITCH + k] = v;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (slot[u] >= 0) lds[slot[u]] = v[u];
     }
 }
 
@@ -1255,6 +1269,18 @@ __device__ __attribute__((always_inline)) inline LinTap load_tap(gptr<const LinT
     return t;
 }
 
+// e / cn for the channel counts 1-4 without an integer division (cn 3:
+// the high half of e * ceil(2^33 / 3), exact for every 32-bit e).
+__device__ __attribute__((always_inline)) inline int div_cn(int e, int cn) {
+    switch (cn) {
+    case 1: return e;
+    case 2: return e >> 1;
+    case 3: return (int)(__umulhi((uint32_t)e, 0xAAAAAAABu) >> 1);
+    case 4: return e >> 2;
+    default: return e / cn;
+    }
+}
+
 // Through LDS: 256 output samples x kLinTH rows, for tap tables that
 // linear_box_fits() accepted (taps that never decrease, every tile's source
 // box within kLinRows x kLinWords). The box of a tile, rows yt[y0].o0 ..
@@ -1268,23 +1294,27 @@ __device__ __attribute__((always_inline)) inline void linear_box_tile(const DImg
     const int e_first = bx * 256, y0 = by * kLinTH;
     const int e_last = min(n, e_first + 256) - 1, y_last = min(d.h, y0 + kLinTH) - 1;
     const gptr<const LinTap> xg = gp(xt), yg = gp(yt);
-    const int c_lo = load_tap(xg, e_first / cn).o0;
+    const int g = tile_tx(), wv = __builtin_amdgcn_readfirstlane(tile_ty());
+    // Every tap this thread needs, loaded before the box so the loads fly
+    // together: its 4 samples' column taps and its wave's 4 row taps.
+    LinTap tx[4], ty[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) tx[j] = load_tap(xg, div_cn(min(e_first + 4 * g + j, e_last), cn));
+#pragma unroll
+    for (int q = 0; q < 4; q++) ty[q] = load_tap(yg, min(y0 + wv * 4 + q, y_last));
+    const int c_lo = load_tap(xg, div_cn(e_first, cn)).o0;
     const int ylo = load_tap(yg, y0).o0, yhi = load_tap(yg, y_last).o1;
-    const int nr = yhi - ylo + 1, nd = (load_tap(xg, e_last / cn).o1 + cn - 1 - c_lo + 7) >> 2;
+    const int nr = yhi - ylo + 1, nd = (load_tap(xg, div_cn(e_last, cn)).o1 + cn - 1 - c_lo + 7) >> 2;
     uint8_t *box = reinterpret_cast<uint8_t *>(lds);
     uint32_t *hb = lds + kLinRows * kLinWords;
-    const int g = tile_tx(), wv = __builtin_amdgcn_readfirstlane(tile_ty());
     stage_box<kLinWords>(s, ylo, nr, c_lo, nd, lds);
-    int32_t b0[4], b1[4], w0[4], w1[4];
+    int32_t b0[4], b1[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int e = min(e_first + 4 * g + j, e_last);
-        const int x = e / cn, ch = e - x * cn;
-        const LinTap t = load_tap(xg, x);
-        b0[j] = t.o0 + ch - c_lo;
-        b1[j] = t.o1 + ch - c_lo;
-        w0[j] = t.w0;
-        w1[j] = t.w1;
+        const int ch = e - div_cn(e, cn) * cn;
+        b0[j] = tx[j].o0 + ch - c_lo;
+        b1[j] = tx[j].o1 + ch - c_lo;
     }
     __syncthreads();
     // Pass 1: H rows.
@@ -1292,7 +1322,8 @@ __device__ __attribute__((always_inline)) inline void linear_box_tile(const DImg
         const uint8_t *rb = box + r * (kLinWords * 4) + box_shift(s, ylo + r, c_lo);
         uint32_t p[2] = {0u, 0u};
 #pragma unroll
-        for (int j = 0; j < 4; j++) p[j >> 1] |= ((uint32_t)lin_h(rb[b0[j]], rb[b1[j]], w0[j], w1[j]) & 0xffffu) << (16 * (j & 1));
+        for (int j = 0; j < 4; j++)
+            p[j >> 1] |= ((uint32_t)lin_h(rb[b0[j]], rb[b1[j]], tx[j].w0, tx[j].w1) & 0xffffu) << (16 * (j & 1));
         *reinterpret_cast<u32x2 *>(hb + r * 128 + 2 * g) = u32x2{p[0], p[1]};
     }
     __syncthreads();
@@ -1302,15 +1333,14 @@ __device__ __attribute__((always_inline)) inline void linear_box_tile(const DImg
     for (int q = 0; q < 4; q++) {
         const int y = y0 + wv * 4 + q;
         if (y > y_last) break;
-        const LinTap ty = load_tap(yg, y);
-        const u32x2 h0 = *reinterpret_cast<const u32x2 *>(hb + (ty.o0 - ylo) * 128 + 2 * g);
-        const u32x2 h1 = *reinterpret_cast<const u32x2 *>(hb + (ty.o1 - ylo) * 128 + 2 * g);
+        const u32x2 h0 = *reinterpret_cast<const u32x2 *>(hb + (ty[q].o0 - ylo) * 128 + 2 * g);
+        const u32x2 h1 = *reinterpret_cast<const u32x2 *>(hb + (ty[q].o1 - ylo) * 128 + 2 * g);
         uint32_t out = 0;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const uint32_t a = j < 2 ? h0.x : h0.y, b = j < 2 ? h1.x : h1.y;
             const int32_t v0 = (int16_t)(a >> (16 * (j & 1))), v1 = (int16_t)(b >> (16 * (j & 1)));
-            out |= lin_v(v0, v1, ty.w0, ty.w1) << (8 * j);
+            out |= lin_v(v0, v1, ty[q].w0, ty[q].w1) << (8 * j);
         }
         if (e > e_last) continue;
         const gptr<uint8_t> o = grow<uint8_t>(d, y) + e;

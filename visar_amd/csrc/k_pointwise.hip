@@ -7,6 +7,7 @@
 // when the row segment is 16-byte aligned.
 #include <utility>
 #include "tiles.hpp"
+#include "resize_stream.hpp"
 
 namespace visar {
 
@@ -627,7 +628,43 @@ int launch_nearest(const DImg &s, const DImg &d, size_t elem, hipStream_t st) {
     return launched();
 }
 
-int launch_linear_u8(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, hipStream_t st, bool box) {
+// The streamed kernel on rounded-down bases; false when the byte spans do
+// not fit its 32-bit offsets.
+static bool launch_linear_stream(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, int tb, hipStream_t st) {
+    if (s.stride < 0 || d.stride < 0) return false;
+    LinStreamArgs a;
+    a.off_s = (uint32_t)((uintptr_t)s.p & 3);
+    a.off_d = (uint32_t)((uintptr_t)d.p & 3);
+    const long lim_s = a.off_s + (long)(s.h - 1) * s.stride + (long)s.w * s.cn;
+    const long lim_d = a.off_d + (long)(d.h - 1) * d.stride + (long)d.w * d.cn;
+    // Source rows of at least 12 bytes, so only the last row's 12-byte
+    // windows can reach the end of the range; the dword holding the image's
+    // last byte is readable whole.
+    if (lim_s >= (1L << 31) - 4 || lim_d >= (1L << 31) || (long)s.w * s.cn < 12) return false;
+    a.s = s;
+    a.s.p = s.p - a.off_s;
+    a.d = d;
+    a.d.p = d.p - a.off_d;
+    a.xt = xt;
+    a.yt = yt;
+    a.lim_s = (uint32_t)((lim_s + 3) & ~3L);
+    a.lim_d = (uint32_t)lim_d;
+    const long n = (long)d.w * d.cn;
+    a.strips = (int)((n + kLinStreamStrip) / kLinStreamStrip);
+    a.bands = (d.h + tb - 1) / tb;
+    a.wpg = 4;
+    const TileOrder o = tile_order(a.strips, (a.bands + a.wpg - 1) / a.wpg, true);
+    a.ntiles = o.ntiles;
+    a.xcd = o.xcd;
+    const dim3 grid(tile_grid(o.ntiles)), block(64 * a.wpg);
+    if (tb == 8) hipLaunchKernelGGL(lstream::linear_stream_u8<8>, grid, block, 0, st, a);
+    else hipLaunchKernelGGL(lstream::linear_stream_u8<4>, grid, block, 0, st, a);
+    return true;
+}
+
+int launch_linear_u8(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, hipStream_t st, bool box,
+                     int stream_rows) {
+    if ((stream_rows == 4 || stream_rows == 8) && launch_linear_stream(s, d, xt, yt, stream_rows, st)) return launched();
     if (box) {
         const dim3 grid((unsigned)(((long)d.w * d.cn + 255) / 256), (unsigned)((d.h + kLinTH - 1) / kLinTH));
         hipLaunchKernelGGL(linear_box_u8, grid, dim3(kThreads), 0, st, s, d, xt, yt);

@@ -31,6 +31,8 @@ const Knob kKnobs[] = {
     {"stencil_band", &Tuning::stencil_band, 0, 16},
     {"stencil_wg", &Tuning::stencil_wg, 0, 4},
     {"gray_layout", &Tuning::gray_layout, 0, 1},
+    {"lin_stream", &Tuning::lin_stream, 0, 1},
+    {"lin_rows", &Tuning::lin_rows, 0, 8},
     {"warp_blocks", &Tuning::warp_blocks, 0, 2},
     {"morph_stream", &Tuning::morph_stream, 0, 1},
     {"morph_band", &Tuning::morph_band, 0, 256},
@@ -275,6 +277,39 @@ bool linear_box_fits(const LinTap *xt, int dw, int cn, const LinTap *yt, int dh)
         if (yt[last].o1 - yt[y].o0 + 1 > kLinRows) return false;
     }
     return true;
+}
+
+int linear_stream_rows(const LinTap *xt, int dw, int cn, const LinTap *yt, int dh) {
+    if (tuning().lin_stream == 1 || dw <= 0 || dh <= 0 || cn < 1 || cn > 4) return 0;
+    auto weights = [](const LinTap &t) { return t.w0 >= 0 && t.w1 >= 0 && t.w0 + t.w1 <= 2050; };
+    for (int x = 0; x < dw; x++)
+        if (!weights(xt[x]) || xt[x].o0 < 0 || xt[x].o1 < xt[x].o0) return 0;
+    for (int y = 0; y < dh; y++) {
+        if (!weights(yt[y]) || yt[y].o0 < 0 || yt[y].o1 - yt[y].o0 < 0 || yt[y].o1 - yt[y].o0 > 1) return 0;
+        if (y && (yt[y].o0 < yt[y - 1].o0 || yt[y].o1 < yt[y - 1].o1)) return 0;
+    }
+    const long n = (long)dw * cn;
+    for (long g = 0; g < n; g += 4) {
+        long lo = 0, hi = 0;
+        for (int j = 0; j < 4; j++) {
+            const long e = std::min(g + j, n - 1), x = e / cn, ch = e - x * cn;
+            const long a = xt[x].o0 + ch, b = xt[x].o1 + ch;
+            lo = j ? std::min(lo, a) : a;
+            hi = j ? std::max(hi, b) : b;
+        }
+        if (hi - lo > 7) return 0;
+    }
+    auto fits = [&](int tb) {
+        for (int y = 0; y < dh; y += tb)
+            if (yt[std::min(dh, y + tb) - 1].o1 - yt[y].o0 + 1 > kLinStreamRows) return false;
+        return true;
+    };
+    const int k = tuning().lin_rows;
+    if (k == 4 || k == 8) return fits(k) ? k : 0;
+    // 8-row bands where they fit and still give about 4000 waves.
+    const long strips = (n + kLinStreamStrip) / kLinStreamStrip;
+    if (fits(8) && strips * ((dh + 7) / 8) >= 4000) return 8;
+    return fits(4) ? 4 : 0;
 }
 
 bool area_box_fits(const AreaEntry *xt, const int32_t *xs, int dw, int cn, const AreaEntry *yt, const int32_t *ys, int dh) {

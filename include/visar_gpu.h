@@ -70,8 +70,8 @@ int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D
  * first, 1 the two-pass-capable kernel), "morph_pass_rows" their output rows
  * per wave (4, 8, 16; 0 = 4); "lin_stream" u8 LINEAR resize (0 source rows
  * streamed through registers where the tap tables allow, 1 the LDS tiles),
- * "lin_rows" the streamed kernel's output rows per wave (4, 8; 0 picks from
- * the image size).
+ * "lin_rows" the streamed kernel's output rows per band (4, 8) and
+ * "lin_run" its bands per wave (1-16), 0 picking from the image size.
  * Unknown keys or values out of range return VISAR_INVALID_ARGUMENT. */
 int visar_gpu_set_tuning(const char *key, int value);
 int visar_gpu_get_tuning(const char *key, int *value);

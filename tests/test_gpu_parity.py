@@ -425,8 +425,9 @@ def test_resize_misaligned_rows(va, orc):
 
 
 def test_resize_linear_streamed_layouts(va, orc):
-    """The register-streamed LINEAR u8 kernel at each band height (4, 8 and
-    the automatic choice) and the LDS tiles it replaces, against the oracle:
+    """The register-streamed LINEAR u8 kernel at several band heights and
+    bands per wave (and the automatic choice) and the LDS tiles it replaces,
+    against the oracle:
     1-4 channels, upscales (1.5x, 2x, 3.7x) and downscales down to about
     1/2, widths on both sides of its 252-sample strips, ragged row bands,
     and misaligned views of wider buffers."""
@@ -434,25 +435,27 @@ def test_resize_linear_streamed_layouts(va, orc):
     cases = [((61, 97), (146, 92)), ((37, 253), (380, 56)), ((40, 64), (128, 80)), ((33, 70), (259, 122)),
              ((90, 300), (180, 55)), ((81, 501), (255, 41)), ((20, 1000), (1500, 30)), ((7, 5), (1, 1))]
     try:
-        for stream, rows in [(0, 0), (0, 4), (0, 8), (1, 0)]:
+        for stream, rows, run in [(0, 0, 0), (0, 4, 1), (0, 8, 1), (0, 4, 3), (0, 8, 16), (1, 0, 0)]:
             autotune.set_knob("lin_stream", stream)
             autotune.set_knob("lin_rows", rows)
+            autotune.set_knob("lin_run", run)
             for cn in (1, 2, 3, 4):
                 for (h, w), (dw, dh) in cases:
                     a = natural(h, w, cn, 17 * h + w + cn)
                     want = orc.resize(a, (dw, dh), 1)
                     assert_same(va.resize(dev(a), (dw, dh), 1).cpu().numpy(), want,
-                                f"{w}x{h}->{dw}x{dh} cn{cn} stream {stream} rows {rows}")
+                                f"{w}x{h}->{dw}x{dh} cn{cn} stream {stream} rows {rows} run {run}")
                     pitch = w * cn + 3
                     big = torch.zeros((h + 1) * pitch, dtype=torch.uint8, device="cuda")
                     sv = big[1:1 + h * pitch].view(h, pitch)[:, :w * cn]
                     sv.copy_(dev(a.reshape(h, w * cn)))
                     sv = sv.view(h, w, cn) if cn > 1 else sv
                     assert_same(va.resize(sv, (dw, dh), 1).cpu().numpy(), want,
-                                f"view {w}x{h}->{dw}x{dh} cn{cn} stream {stream} rows {rows}")
+                                f"view {w}x{h}->{dw}x{dh} cn{cn} stream {stream} rows {rows} run {run}")
     finally:
         autotune.set_knob("lin_stream", 0)
         autotune.set_knob("lin_rows", 0)
+        autotune.set_knob("lin_run", 0)
 
 
 def test_queued_calls_keep_their_tables_and_temporaries(va, orc):

@@ -23,6 +23,7 @@
 #include "tables.hpp"
 #include "morph_chain.hpp"
 #include "warp_blk.hpp"
+#include "resize_stream.hpp"
 
 namespace visar {
 
@@ -47,6 +48,7 @@ enum JobKind : int {
     JK_AFFINE_C3 = 13,
     JK_AFFINE_BLK = 14,
     JK_MORPH_CHAIN = 15,
+    JK_LINEAR_STREAM = 16,
 };
 
 struct ResizeArgs {
@@ -70,6 +72,7 @@ struct BatchJob {
         AffineParams af;
         ResizeArgs rs;
         MorphChainArgs mc;
+        LinStreamArgs ls;
     };
 };
 
@@ -168,6 +171,17 @@ __global__ __launch_bounds__(kTileThreads) void batch_linear(BATCH_ARGS) {
              [&](const BatchJob &j, int bx, int by) { linear_box_tile(j.rs.s, j.rs.d, j.rs.xt, j.rs.yt, bx, by, lds); });
 }
 
+// LINEAR through the register-streamed waves (resize_stream.hpp): tile
+// (strip, 4 runs of bands), one run per wave: bands of 4 rows in runs of
+// j.ls.run, or single bands of 8 rows.
+template <int TB, bool AD, bool AS>
+__global__ __launch_bounds__(kTileThreads) void batch_linear_stream(BATCH_ARGS) {
+    run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) {
+        const int w = 4 * by + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+        lstream::lin_stream_wave<TB, AD, AS, TB == 4>(j.ls, bx, w * j.ls.run, (w + 1) * j.ls.run);
+    });
+}
+
 __global__ __launch_bounds__(kTileThreads) void batch_area(BATCH_ARGS) {
     __shared__ uint32_t lds[kAreaLdsWords];
     run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) {
@@ -230,6 +244,17 @@ GroupKernel group_kernel(int kind, int variant) {
     case JK_HALF: return batch_other<JK_HALF>;
     case JK_LINEAR: return batch_other<JK_LINEAR>;
     case JK_LINEAR_BOX: return batch_linear;
+    case JK_LINEAR_STREAM:
+        switch (variant) {
+        case 7: return batch_linear_stream<8, true, true>;
+        case 6: return batch_linear_stream<8, true, false>;
+        case 5: return batch_linear_stream<8, false, true>;
+        case 4: return batch_linear_stream<8, false, false>;
+        case 3: return batch_linear_stream<4, true, true>;
+        case 2: return batch_linear_stream<4, true, false>;
+        case 1: return batch_linear_stream<4, false, true>;
+        default: return batch_linear_stream<4, false, false>;
+        }
     case JK_AREA_BOX: return batch_area;
     case JK_AREA_INT: return batch_other<JK_AREA_INT>;
     default: return batch_other<JK_AREA_FRAC>;
@@ -475,7 +500,20 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
             }
             for (uint32_t y = 0; y < it.dst.height; y++)
                 t[it.dst.width + y] = linear_tap_u8(y, it.src.height, it.dst.height, area_up, false);
-            if (linear_box_fits(t.data(), (int)it.dst.width, (int)it.src.channels, t.data() + it.dst.width, (int)it.dst.height)) {
+            LinStreamArgs ls;
+            // Bands of 4 rows in runs of 4 unless the tuning asks for others
+            // (8-row bands run singly).
+            const int tb = tuning().lin_rows == 8 ? 8 : 4;
+            const int run = tb == 8 ? 1 : tuning().lin_run ? tuning().lin_run : 4;
+            if (linear_stream_fits(t.data(), (int)it.dst.width, (int)it.src.channels, t.data() + it.dst.width, (int)it.dst.height) &&
+                linear_stream_args(s, d, nullptr, nullptr, tb, run, ls)) {
+                j.kind = JK_LINEAR_STREAM;
+                j.variant = (tb == 8 ? 4 : 0) + (ls.off_d == 0 && d.stride % 4 == 0 ? 2 : 0) + (ls.off_s == 0 && s.stride % 4 == 0 ? 1 : 0);
+                j.ls = ls;
+                j.tiles_x = ls.strips;
+                j.ntiles = ls.strips * ((ls.bands + 4 * ls.run - 1) / (4 * ls.run));
+            } else if (linear_box_fits(t.data(), (int)it.dst.width, (int)it.src.channels, t.data() + it.dst.width,
+                                       (int)it.dst.height)) {
                 j.kind = JK_LINEAR_BOX;
                 j.ntiles = j.tiles_x * ((d.h + kLinTH - 1) / kLinTH);
             }
@@ -610,7 +648,11 @@ int visar_batch_plan(visar_batch *b, const visar_batch_item *items, size_t n, si
     for (size_t k = 0; k < n; k++) {
         size_t i = order[k];
         BatchJob j = pl[i].job;
-        if (pl[i].table_kind == 1) {
+        if (pl[i].table_kind == 1 && j.kind == JK_LINEAR_STREAM) {
+            const LinTap *t = reinterpret_cast<const LinTap *>(base + tab_at[i]);
+            j.ls.xt = t;
+            j.ls.yt = t + pl[i].nx;
+        } else if (pl[i].table_kind == 1) {
             const LinTap *t = reinterpret_cast<const LinTap *>(base + tab_at[i]);
             j.rs.xt = t;
             j.rs.yt = t + pl[i].nx;

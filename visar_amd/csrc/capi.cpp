@@ -1154,8 +1154,8 @@ int visar_resize(const visar_image *src, const visar_image *dst, int interp) {
         LinTap *dev = (LinTap *)call.upload(tabs.data(), tabs.size() * sizeof(LinTap));
         if (!dev) return fail(OUT_OF_MEMORY, "OutOfMemory");
         const bool box = linear_box_fits(tabs.data(), (int)dst->width, (int)src->channels, tabs.data() + dst->width, (int)dst->height);
-        const int rows = linear_stream_rows(tabs.data(), (int)dst->width, (int)src->channels, tabs.data() + dst->width, (int)dst->height);
-        return call.finish(launch_linear_u8(s, d, dev, dev + dst->width, st, box, rows));
+        const bool stream = linear_stream_fits(tabs.data(), (int)dst->width, (int)src->channels, tabs.data() + dst->width, (int)dst->height);
+        return call.finish(launch_linear_u8(s, d, dev, dev + dst->width, st, box, stream));
     }
     // INTER_AREA shrinking
     if (exact_half) return call.finish(launch_half_u8(s, d, st));

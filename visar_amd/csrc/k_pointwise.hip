@@ -628,11 +628,8 @@ int launch_nearest(const DImg &s, const DImg &d, size_t elem, hipStream_t st) {
     return launched();
 }
 
-// The streamed kernel on rounded-down bases; false when the byte spans do
-// not fit its 32-bit offsets.
-static bool launch_linear_stream(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, int tb, hipStream_t st) {
+bool linear_stream_args(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, int tb, int run, LinStreamArgs &a) {
     if (s.stride < 0 || d.stride < 0) return false;
-    LinStreamArgs a;
     a.off_s = (uint32_t)((uintptr_t)s.p & 3);
     a.off_d = (uint32_t)((uintptr_t)d.p & 3);
     const long lim_s = a.off_s + (long)(s.h - 1) * s.stride + (long)s.w * s.cn;
@@ -651,20 +648,43 @@ static bool launch_linear_stream(const DImg &s, const DImg &d, const LinTap *xt,
     a.lim_d = (uint32_t)lim_d;
     const long n = (long)d.w * d.cn;
     a.strips = (int)((n + kLinStreamStrip) / kLinStreamStrip);
+    // 8-row bands where they still give about 4000 waves, one band per
+    // wave (measured from 1024^2 to 8192^2 sources: runs of 8-row bands need
+    // more scalar registers than a wave has, and runs of 4-row bands issue
+    // fewer loads at a time).
+    if (tb != 4 && tb != 8) tb = a.strips * (long)((d.h + 7) / 8) >= 4000 ? 8 : 4;
+    a.tb = tb;
     a.bands = (d.h + tb - 1) / tb;
-    a.wpg = 4;
-    const TileOrder o = tile_order(a.strips, (a.bands + a.wpg - 1) / a.wpg, true);
+    a.run = run > 0 ? run : 1;
+    return true;
+}
+
+template <int TB, bool RUN>
+static void launch_stream_tb(const LinStreamArgs &a, bool ad, bool as, dim3 grid, hipStream_t st) {
+    if (ad && as) hipLaunchKernelGGL((lstream::linear_stream_u8<TB, true, true, RUN>), grid, dim3(256), 0, st, a);
+    else if (ad) hipLaunchKernelGGL((lstream::linear_stream_u8<TB, true, false, RUN>), grid, dim3(256), 0, st, a);
+    else if (as) hipLaunchKernelGGL((lstream::linear_stream_u8<TB, false, true, RUN>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((lstream::linear_stream_u8<TB, false, false, RUN>), grid, dim3(256), 0, st, a);
+}
+
+static bool launch_linear_stream(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, hipStream_t st) {
+    LinStreamArgs a;
+    if (!linear_stream_args(s, d, xt, yt, tuning().lin_rows, tuning().lin_run, a)) return false;
+    const TileOrder o = tile_order(a.strips, (a.bands + 4 * a.run - 1) / (4 * a.run), true);
     a.ntiles = o.ntiles;
     a.xcd = o.xcd;
-    const dim3 grid(tile_grid(o.ntiles)), block(64 * a.wpg);
-    if (tb == 8) hipLaunchKernelGGL(lstream::linear_stream_u8<8>, grid, block, 0, st, a);
-    else hipLaunchKernelGGL(lstream::linear_stream_u8<4>, grid, block, 0, st, a);
+    const dim3 grid(tile_grid(o.ntiles));
+    const bool ad = a.off_d == 0 && d.stride % 4 == 0, as = a.off_s == 0 && s.stride % 4 == 0;
+    if (a.tb == 8 && a.run == 1) launch_stream_tb<8, false>(a, ad, as, grid, st);
+    else if (a.tb == 8) launch_stream_tb<8, true>(a, ad, as, grid, st);
+    else if (a.run == 1) launch_stream_tb<4, false>(a, ad, as, grid, st);
+    else launch_stream_tb<4, true>(a, ad, as, grid, st);
     return true;
 }
 
 int launch_linear_u8(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, hipStream_t st, bool box,
-                     int stream_rows) {
-    if ((stream_rows == 4 || stream_rows == 8) && launch_linear_stream(s, d, xt, yt, stream_rows, st)) return launched();
+                     bool stream) {
+    if (stream && launch_linear_stream(s, d, xt, yt, st)) return launched();
     if (box) {
         const dim3 grid((unsigned)(((long)d.w * d.cn + 255) / 256), (unsigned)((d.h + kLinTH - 1) / kLinTH));
         hipLaunchKernelGGL(linear_box_u8, grid, dim3(kThreads), 0, st, s, d, xt, yt);

@@ -25,7 +25,8 @@ struct Tuning {
     int morph_pass_rows = 0;     // column-window-first passes: output rows per wave (4, 8, 16; 0 = 4)
     int gray_layout = 0;         // u8 cvtColor to gray on contiguous images: 0 one contiguous 1024-pixel block per wave, 1 16 pixels per thread
     int lin_stream = 0;          // u8 LINEAR resize: 0 rows streamed through registers where the tables allow, 1 LDS tiles
-    int lin_rows = 0;            // streamed LINEAR resize: output rows per wave (4, 8; 0 automatic)
+    int lin_rows = 0;            // streamed LINEAR resize: output rows per band (4, 8; 0 automatic)
+    int lin_run = 0;             // streamed LINEAR resize: bands per wave (1-16; 0 automatic)
 };
 Tuning &tuning();
 bool set_tuning(const char *key, int value);
@@ -137,7 +138,7 @@ struct LinTap {
     int32_t w0, w1;
 };
 int launch_linear_u8(const DImg &s, const DImg &d, const LinTap *xt_dev, const LinTap *yt_dev, hipStream_t st, bool box,
-                     int stream_rows = 0);
+                     bool stream = false);
 
 // Tiles of the LDS-staged linear resize: 256 output samples x kLinTH rows,
 // whose source box is at most kLinRows rows of kLinWords dwords.
@@ -151,24 +152,26 @@ constexpr int kLinLdsWords = kLinRows * kLinWords + kLinRows * 128;
 bool linear_box_fits(const LinTap *xt, int dw, int cn, const LinTap *yt, int dh);
 
 // The register-streamed linear resize (resize_stream.hpp): strips of
-// kLinStreamStrip output samples, bands of 4 or 8 output rows whose source
-// rows number at most kLinStreamRows.
+// kLinStreamStrip output samples, bands of tb (4 or 8) output rows, each
+// wave on a run of `run` bands of one strip.
 constexpr int kLinStreamStrip = 252;
-constexpr int kLinStreamRows = 10;
 struct LinStreamArgs {
     DImg s, d;  // p of each rounded down to a dword; the bytes skipped in off_s / off_d
     const LinTap *xt, *yt;
     uint32_t off_s, off_d;
     uint32_t lim_s, lim_d;    // bytes from each rounded-down base to the image's last byte + 1 (< 2^31), lim_s rounded up to 4
-    int strips, ntiles, xcd;  // the tile order: tile (strip, group of wpg bands)
-    int bands, wpg;
+    int strips, ntiles, xcd;  // the tile order: tile (strip, 4 runs of bands)
+    int tb, bands, run;
 };
-// Output rows per band (4 or 8) of the streamed kernel for these tables (x
-// taps in samples, premultiplied by cn), or 0 where it does not apply:
-// weights outside [0, 2050] or summing past 2050, y taps that decrease or
-// whose o1 - o0 is not 0 or 1, four consecutive samples whose taps span
-// more than 8 bytes, bands needing more than kLinStreamRows source rows.
-int linear_stream_rows(const LinTap *xt, int dw, int cn, const LinTap *yt, int dh);
+// True where the streamed kernel applies to these tables (x taps in
+// samples, premultiplied by cn): weights in [0, 2050] summing to at most
+// 2050, y taps that never decrease with o1 - o0 of 0 or 1, and every four
+// consecutive samples' taps within 8 bytes.
+bool linear_stream_fits(const LinTap *xt, int dw, int cn, const LinTap *yt, int dh);
+// The streamed kernel's arguments with bands of tb rows and runs of `run`
+// bands (all but the tile order; tb 0 picks from the image size, run 0 is
+// 1); false where its 32-bit offsets or 12-byte windows do not fit.
+bool linear_stream_args(const DImg &s, const DImg &d, const LinTap *xt, const LinTap *yt, int tb, int run, LinStreamArgs &a);
 
 // resize.zig:470-524 area tables: entries grouped per destination index,
 // start offsets in `xstart` / `ystart` (length dsize + 1).

@@ -33,6 +33,7 @@ const Knob kKnobs[] = {
     {"gray_layout", &Tuning::gray_layout, 0, 1},
     {"lin_stream", &Tuning::lin_stream, 0, 1},
     {"lin_rows", &Tuning::lin_rows, 0, 8},
+    {"lin_run", &Tuning::lin_run, 0, 16},
     {"warp_blocks", &Tuning::warp_blocks, 0, 2},
     {"morph_stream", &Tuning::morph_stream, 0, 1},
     {"morph_band", &Tuning::morph_band, 0, 256},
@@ -279,14 +280,14 @@ bool linear_box_fits(const LinTap *xt, int dw, int cn, const LinTap *yt, int dh)
     return true;
 }
 
-int linear_stream_rows(const LinTap *xt, int dw, int cn, const LinTap *yt, int dh) {
-    if (tuning().lin_stream == 1 || dw <= 0 || dh <= 0 || cn < 1 || cn > 4) return 0;
+bool linear_stream_fits(const LinTap *xt, int dw, int cn, const LinTap *yt, int dh) {
+    if (tuning().lin_stream == 1 || dw <= 0 || dh <= 0 || cn < 1 || cn > 4) return false;
     auto weights = [](const LinTap &t) { return t.w0 >= 0 && t.w1 >= 0 && t.w0 + t.w1 <= 2050; };
     for (int x = 0; x < dw; x++)
-        if (!weights(xt[x]) || xt[x].o0 < 0 || xt[x].o1 < xt[x].o0) return 0;
+        if (!weights(xt[x]) || xt[x].o0 < 0 || xt[x].o1 < xt[x].o0) return false;
     for (int y = 0; y < dh; y++) {
-        if (!weights(yt[y]) || yt[y].o0 < 0 || yt[y].o1 - yt[y].o0 < 0 || yt[y].o1 - yt[y].o0 > 1) return 0;
-        if (y && (yt[y].o0 < yt[y - 1].o0 || yt[y].o1 < yt[y - 1].o1)) return 0;
+        if (!weights(yt[y]) || yt[y].o0 < 0 || yt[y].o1 - yt[y].o0 < 0 || yt[y].o1 - yt[y].o0 > 1) return false;
+        if (y && (yt[y].o0 < yt[y - 1].o0 || yt[y].o1 < yt[y - 1].o1)) return false;
     }
     const long n = (long)dw * cn;
     for (long g = 0; g < n; g += 4) {
@@ -297,19 +298,9 @@ int linear_stream_rows(const LinTap *xt, int dw, int cn, const LinTap *yt, int d
             lo = j ? std::min(lo, a) : a;
             hi = j ? std::max(hi, b) : b;
         }
-        if (hi - lo > 7) return 0;
+        if (hi - lo > 7) return false;
     }
-    auto fits = [&](int tb) {
-        for (int y = 0; y < dh; y += tb)
-            if (yt[std::min(dh, y + tb) - 1].o1 - yt[y].o0 + 1 > kLinStreamRows) return false;
-        return true;
-    };
-    const int k = tuning().lin_rows;
-    if (k == 4 || k == 8) return fits(k) ? k : 0;
-    // 8-row bands where they fit and still give about 4000 waves.
-    const long strips = (n + kLinStreamStrip) / kLinStreamStrip;
-    if (fits(8) && strips * ((dh + 7) / 8) >= 4000) return 8;
-    return fits(4) ? 4 : 0;
+    return true;
 }
 
 bool area_box_fits(const AreaEntry *xt, const int32_t *xs, int dw, int cn, const AreaEntry *yt, const int32_t *ys, int dh) {

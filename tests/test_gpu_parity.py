@@ -239,6 +239,47 @@ def test_stencil_streamed_equals_tiles(va):
         autotune.set_knob("stencil_direct", 0)
 
 
+def test_stencil_general_streamed(va, orc):
+    """The general streamed GaussianBlur (one and three channels, any row
+    alignment and width) against the oracle: widths from 1 pixel to both
+    sides of its 240-pixel strips, heights below and above its bands, every
+    border, both band heights, contiguous tensors (rows off dwords when the
+    row length is not a multiple of 4) and views at odd offsets; then equal
+    to the LDS tile kernels on larger frames."""
+    from visar_amd import autotune
+    sizes = [(1, 1), (2, 3), (5, 7), (9, 13), (3, 236), (16, 240), (7, 241), (11, 246), (4, 483), (21, 97)]
+    try:
+        for band in (0, 2, 4):
+            autotune.set_knob("stencil_band", band)
+            for cn in (1, 3):
+                for h, w in (sizes if band == 0 else sizes[3::2]):
+                    a = natural(h, w, cn, 7 * h + w + cn)
+                    big = torch.zeros(h * (w * cn + 5) + 8, dtype=torch.uint8, device="cuda")
+                    sv = big[3:3 + h * (w * cn + 5)].view(h, w * cn + 5)[:, :w * cn]
+                    sv.copy_(dev(a.reshape(h, w * cn)))
+                    sv = sv.view(h, w, cn) if cn == 3 else sv
+                    for k in (3, 5, 7):
+                        for border in range(5):
+                            want = orc.gaussian_blur(a, (k, k), 0, border=border)
+                            got = va.gaussian_blur(dev(a), (k, k), 0, border=border).cpu().numpy()
+                            assert_same(got, want, f"k{k} cn{cn} {h}x{w} border {border} band {band}")
+                        gv = va.gaussian_blur(sv, (k, k), 0, border=4).cpu().numpy()
+                        assert_same(gv, orc.gaussian_blur(a, (k, k), 0, border=4), f"view k{k} cn{cn} {h}x{w}")
+        autotune.set_knob("stencil_band", 0)
+        for cn in (1, 3):
+            a = dev(natural(270, 1001, cn, 40 + cn))
+            for k in (3, 5, 7):
+                for border in (0, 1, 2, 3, 4):
+                    outs = []
+                    for knob in (0, 1):
+                        autotune.set_knob("stencil_direct", knob)
+                        outs.append(va.gaussian_blur(a, (k, k), 0, border=border))
+                    assert torch.equal(outs[0], outs[1]), f"k{k} cn{cn} border {border}"
+    finally:
+        autotune.set_knob("stencil_band", 0)
+        autotune.set_knob("stencil_direct", 0)
+
+
 def test_sobel_1080p_config(va, orc):
     from visar_amd import synth
     a = synth.uniform_bytes(3, (1080, 1920))

@@ -24,6 +24,7 @@
 #include "morph_chain.hpp"
 #include "warp_blk.hpp"
 #include "resize_stream.hpp"
+#include "stencil_stream.hpp"
 
 namespace visar {
 
@@ -49,6 +50,7 @@ enum JobKind : int {
     JK_AFFINE_BLK = 14,
     JK_MORPH_CHAIN = 15,
     JK_LINEAR_STREAM = 16,
+    JK_STENCIL_STREAM = 17,
 };
 
 struct ResizeArgs {
@@ -73,6 +75,7 @@ struct BatchJob {
         ResizeArgs rs;
         MorphChainArgs mc;
         LinStreamArgs ls;
+        SStreamArgs ss;
     };
 };
 
@@ -171,6 +174,21 @@ __global__ __launch_bounds__(kTileThreads) void batch_linear(BATCH_ARGS) {
              [&](const BatchJob &j, int bx, int by) { linear_box_tile(j.rs.s, j.rs.d, j.rs.xt, j.rs.yt, bx, by, lds); });
 }
 
+// GaussianBlur 5x5 and 7x7 through the general streamed waves
+// (stencil_stream.hpp): tile (strip, group of 4 bands of 4 rows), one band
+// per wave, destinations of any alignment in one group (3x3 stays on the
+// LDS tiles, measured faster in batches).
+template <int MODE, int CN>
+__global__ __launch_bounds__(kTileThreads) void batch_stencil_stream(BATCH_ARGS) {
+    run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) {
+        sstream::stream_wave<MODE, CN, 4, false>(j.ss, bx, 4 * by + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6));
+    });
+}
+template <int MODE>
+GroupKernel stencil_stream_kernel(int cn) {
+    return cn == 1 ? batch_stencil_stream<MODE, 1> : batch_stencil_stream<MODE, 3>;
+}
+
 // LINEAR through the register-streamed waves (resize_stream.hpp): tile
 // (strip, 4 runs of bands), one run per wave: bands of 4 rows in runs of
 // j.ls.run, or single bands of 8 rows.
@@ -244,6 +262,9 @@ GroupKernel group_kernel(int kind, int variant) {
     case JK_HALF: return batch_other<JK_HALF>;
     case JK_LINEAR: return batch_other<JK_LINEAR>;
     case JK_LINEAR_BOX: return batch_linear;
+    case JK_STENCIL_STREAM:
+        return variant / 100 == SWAR_G5 ? stencil_stream_kernel<SWAR_G5>((variant / 10) % 10)
+                                        : stencil_stream_kernel<SWAR_G7>((variant / 10) % 10);
     case JK_LINEAR_STREAM:
         switch (variant) {
         case 7: return batch_linear_stream<8, true, true>;
@@ -350,6 +371,15 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
         th = th >= 4 ? 4 : th >= 2 ? 2 : 1;
         j.variant = k * 100 + (int)it.src.channels * 10 + th;
         const int mode = swar_mode(j.st, k, 0);
+        SStreamArgs ss;
+        if ((mode == SWAR_G5 || mode == SWAR_G7) && tuning().stencil_direct != 1 && stencil_stream_args(j.st, 4, ss)) {
+            j.kind = JK_STENCIL_STREAM;
+            j.variant = mode * 100 + (int)it.src.channels * 10;
+            j.ss = ss;
+            j.tiles_x = ss.strips;
+            j.ntiles = ss.strips * ((ss.bands + 3) / 4);
+            return OK;
+        }
         if (mode == SWAR_G3 || mode == SWAR_G5 || mode == SWAR_G7) {
             j.kind = JK_STENCIL_SWAR;
             j.variant = mode * 100 + (int)it.src.channels * 10 + th;

@@ -4,7 +4,7 @@
 // shared with the persistent batch kernel.
 #include <initializer_list>
 
-#include "stencil_direct.hpp"
+#include "stencil_stream.hpp"
 
 namespace visar {
 
@@ -190,11 +190,71 @@ int swar_mode(const StencilArgs &a, int k, int ddepth) {
 }
 
 // Returns false when the call is not a shape this kernel covers.
+template <int MODE, int CN, int TB>
+static void launch_stream_ad(const SStreamArgs &a, bool ad, hipStream_t st) {
+    const dim3 g(tile_grid(a.ntiles)), b(256);
+    if (ad) hipLaunchKernelGGL((sstream::sep_stream_u8<MODE, CN, TB, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((sstream::sep_stream_u8<MODE, CN, TB, false>), g, b, 0, st, a);
+}
+
+template <int MODE>
+static void launch_stream_mode(const SStreamArgs &a, int cn, int tb, bool ad, hipStream_t st) {
+    if (cn == 1) {
+        if (tb == 2) launch_stream_ad<MODE, 1, 2>(a, ad, st);
+        else launch_stream_ad<MODE, 1, 4>(a, ad, st);
+    } else {
+        if (tb == 4) launch_stream_ad<MODE, 3, 4>(a, ad, st);
+        else launch_stream_ad<MODE, 3, 2>(a, ad, st);
+    }
+}
+
+bool stencil_stream_args(const StencilArgs &p, int tb, SStreamArgs &a) {
+    const DImg &s = p.src, &d = p.dst;
+    if (s.cn != 1 && s.cn != 3) return false;
+    if (s.w < 1 || s.h < 1 || d.w != s.w || d.h != s.h || d.cn != s.cn || s.stride < 0 || d.stride < 0) return false;
+    a.s = p;
+    a.off_s = (uint32_t)((uintptr_t)s.p & 3);
+    a.off_d = (uint32_t)((uintptr_t)d.p & 3);
+    const long lim_s = a.off_s + (long)(s.h - 1) * s.stride + (long)s.w * s.cn;
+    const long lim_d = a.off_d + (long)(d.h - 1) * d.stride + (long)d.w * d.cn;
+    if (lim_s >= (1L << 31) || lim_d >= (1L << 31)) return false;
+    a.s.src.p = s.p - a.off_s;
+    a.s.dst.p = d.p - a.off_d;
+    a.lim_s = (uint32_t)lim_s;
+    a.lim_d = (uint32_t)lim_d;
+    a.strips = (s.w + 6 + sstream::kStripPx - 1) / sstream::kStripPx;
+    a.bands = (s.h + tb - 1) / tb;
+    return true;
+}
+
+// The general streamed kernels (stencil_stream.hpp): GaussianBlur 3x3, 5x5,
+// 7x7 on one or three channels, any alignment, width and border. Bands of
+// 4 rows on one channel, 2 on three (the tuning's stencil_band 2 or 4
+// overrides).
+static bool launch_stream(const StencilArgs &p, int mode, hipStream_t st) {
+    if (tuning().stencil_direct == 1 || (mode != SWAR_G3 && mode != SWAR_G5 && mode != SWAR_G7)) return false;
+    const int k = tuning().stencil_band;
+    const int tb = k == 2 || k == 4 ? k : p.src.cn == 1 ? 4 : 2;
+    SStreamArgs a;
+    if (!stencil_stream_args(p, tb, a)) return false;
+    const TileOrder o = tile_order(a.strips, (a.bands + 3) / 4, true);
+    a.ntiles = o.ntiles;
+    a.xcd = o.xcd;
+    const bool ad = a.off_d == 0 && p.dst.stride % 4 == 0;
+    switch (mode) {
+    case SWAR_G3: launch_stream_mode<SWAR_G3>(a, p.src.cn, tb, ad, st); break;
+    case SWAR_G5: launch_stream_mode<SWAR_G5>(a, p.src.cn, tb, ad, st); break;
+    default: launch_stream_mode<SWAR_G7>(a, p.src.cn, tb, ad, st); break;
+    }
+    return true;
+}
+
 bool launch_sep_small_u8(const SepParams &p, hipStream_t st, int th) {
     StencilArgs a;
     if (!stencil_args(p, a)) return false;
     const int mode = swar_mode(a, p.nkx, p.ddepth);
     if (mode >= 0 && launch_direct(a, mode, st)) return hipGetLastError() == hipSuccess;
+    if (mode >= 0 && launch_stream(a, mode, st)) return hipGetLastError() == hipSuccess;
     bool ok;
     switch (mode) {
     case SWAR_G3: ok = launch_swar<SWAR_G3, uint8_t>(a, st, th); break;

@@ -70,3 +70,41 @@ def test_batch_rejects_bad_items(va):
         batch.Batch().plan([batch.item(batch.GAUSSIAN_BLUR, a, o, ksize=9)])
     with pytest.raises(RuntimeError):
         batch.Batch().plan([batch.item(batch.GAUSSIAN_BLUR, a.cpu().numpy(), o, ksize=3)])
+
+
+def test_batch_morphology_elements(va, orc):
+    """Erode and dilate in a batch with every element kind (rectangles run in
+    the streamed waves, crosses and ellipses in the fused chain tiles) at
+    3/5/7, one and three channels, sizes from 1 pixel to several 240-pixel
+    strips, views at odd offsets: each image equals its plain call and the
+    oracle."""
+    from visar_amd import batch
+    rng = np.random.default_rng(5)
+    jobs = []
+    sizes = [(1, 1), (2, 5), (7, 3), (13, 241), (37, 500), (64, 64)]
+    i = 0
+    for shape in (0, 1, 2):
+        for k in (3, 5, 7):
+            for mop in (0, 1):
+                for cn in (1, 3):
+                    h, w = sizes[i % len(sizes)]
+                    i += 1
+                    a = torch.from_numpy(rng.integers(0, 256, (h, w, cn) if cn == 3 else (h, w), dtype=np.uint8)).cuda()
+                    if i % 3 == 0:
+                        big = torch.zeros(h * (w * cn + 3) + 4, dtype=torch.uint8, device="cuda")
+                        v = big[1:1 + h * (w * cn + 3)].view(h, w * cn + 3)[:, :w * cn]
+                        v.copy_(a.reshape(h, w * cn))
+                        a = v.view(h, w, cn) if cn == 3 else v
+                    dst = torch.empty((h, w, cn) if cn == 3 else (h, w), dtype=torch.uint8, device="cuda")
+                    jobs.append({"spec": (w, h, cn, f"morph{mop}/{shape}/{k}"), "src": a, "dst": dst,
+                                 "kw": dict(op=batch.MORPHOLOGY, ksize=k, morph_op=mop, shape=shape, iterations=1)})
+    b = batch.Batch().plan(batch.items(jobs))
+    b.run()
+    torch.cuda.synchronize()
+    for j in jobs:
+        got = j["dst"].cpu().numpy()
+        plain = batch.run_plain(va, j, dst=torch.empty_like(j["dst"])).cpu().numpy()
+        assert np.array_equal(got, plain), f"{j['spec']} differs from the plain call"
+        kw = j["kw"]
+        want = orc.morphology(np.ascontiguousarray(j["src"].cpu().numpy()), kw["morph_op"], kw["shape"], (kw["ksize"], kw["ksize"]))
+        assert np.array_equal(got, want), f"{j['spec']} differs from the oracle"

@@ -51,6 +51,7 @@ enum JobKind : int {
     JK_MORPH_CHAIN = 15,
     JK_LINEAR_STREAM = 16,
     JK_STENCIL_STREAM = 17,
+    JK_MORPH_STREAM = 18,
 };
 
 struct ResizeArgs {
@@ -181,12 +182,26 @@ __global__ __launch_bounds__(kTileThreads) void batch_linear(BATCH_ARGS) {
 template <int MODE, int CN>
 __global__ __launch_bounds__(kTileThreads) void batch_stencil_stream(BATCH_ARGS) {
     run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) {
-        sstream::stream_wave<MODE, CN, 4, false>(j.ss, bx, 4 * by + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6));
+        sstream::stream_wave<sstream::GaussOp<MODE>, CN, 4, false>(j.ss, bx, 4 * by + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6));
     });
 }
 template <int MODE>
 GroupKernel stencil_stream_kernel(int cn) {
     return cn == 1 ? batch_stencil_stream<MODE, 1> : batch_stencil_stream<MODE, 3>;
+}
+
+// erode / dilate with a K x K rectangle through the same streamed waves
+// (tile as above).
+template <int K, bool MAX, int CN>
+__global__ __launch_bounds__(kTileThreads) void batch_morph_stream(BATCH_ARGS) {
+    run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) {
+        sstream::stream_wave<sstream::RectOp<K, MAX>, CN, 4, false>(j.ss, bx, 4 * by + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6));
+    });
+}
+template <int K>
+GroupKernel morph_stream_kernel(bool mx, int cn) {
+    if (cn == 1) return mx ? batch_morph_stream<K, true, 1> : batch_morph_stream<K, false, 1>;
+    return mx ? batch_morph_stream<K, true, 3> : batch_morph_stream<K, false, 3>;
 }
 
 // LINEAR through the register-streamed waves (resize_stream.hpp): tile
@@ -262,6 +277,11 @@ GroupKernel group_kernel(int kind, int variant) {
     case JK_HALF: return batch_other<JK_HALF>;
     case JK_LINEAR: return batch_other<JK_LINEAR>;
     case JK_LINEAR_BOX: return batch_linear;
+    case JK_MORPH_STREAM: {
+        const int k = variant / 100, cn = variant % 10;
+        const bool mx = (variant / 10) % 10 != 0;
+        return k == 3 ? morph_stream_kernel<3>(mx, cn) : k == 5 ? morph_stream_kernel<5>(mx, cn) : morph_stream_kernel<7>(mx, cn);
+    }
     case JK_STENCIL_STREAM:
         return variant / 100 == SWAR_G5 ? stencil_stream_kernel<SWAR_G5>((variant / 10) % 10)
                                         : stencil_stream_kernel<SWAR_G7>((variant / 10) % 10);
@@ -416,7 +436,25 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
             }
         }
         MorphRects mr;
-        if (tuning().morph_chain != 1 && morph_rects(mp, mr) && batch_chain_el(morph_el_code(mr))) {
+        const bool rects = morph_rects(mp, mr);
+        const uint32_t el = rects ? morph_el_code(mr) : 0u;
+        SStreamArgs ss;
+        StencilArgs sa;
+        std::memset(&sa, 0, sizeof sa);
+        sa.src = s;
+        sa.dst = d;
+        sa.border = BORDER_REPLICATE;
+        if (tuning().morph_chain == 0 && (el == mchain::kElRect3 || el == mchain::kElRect5 || el == mchain::kElRect7) &&
+            stencil_stream_args(sa, 4, ss)) {
+            // A full rectangle: the streamed waves of stencil_stream.hpp.
+            j.kind = JK_MORPH_STREAM;
+            j.variant = it.ksize * 100 + it.morph_op * 10 + (int)it.src.channels;
+            j.ss = ss;
+            j.tiles_x = ss.strips;
+            j.ntiles = ss.strips * ((ss.bands + 3) / 4);
+            return OK;
+        }
+        if (tuning().morph_chain != 1 && rects && batch_chain_el(el)) {
             // One pass of the fused chain kernel, as the plain call's
             // morph_chain_launch sets it up for one chain of one pass.
             MorphChainArgs &a = j.mc;

@@ -193,8 +193,8 @@ int swar_mode(const StencilArgs &a, int k, int ddepth) {
 template <int MODE, int CN, int TB>
 static void launch_stream_ad(const SStreamArgs &a, bool ad, hipStream_t st) {
     const dim3 g(tile_grid(a.ntiles)), b(256);
-    if (ad) hipLaunchKernelGGL((sstream::sep_stream_u8<MODE, CN, TB, true>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((sstream::sep_stream_u8<MODE, CN, TB, false>), g, b, 0, st, a);
+    if (ad) hipLaunchKernelGGL((sstream::sep_stream_u8<sstream::GaussOp<MODE>, CN, TB, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((sstream::sep_stream_u8<sstream::GaussOp<MODE>, CN, TB, false>), g, b, 0, st, a);
 }
 
 template <int MODE>

@@ -31,6 +31,7 @@
 // ends are written byte by byte.
 #pragma once
 
+#include "morph_band.hpp"
 #include "stencil_direct.hpp"
 
 namespace visar {
@@ -116,14 +117,44 @@ __device__ __attribute__((always_inline)) inline uint32_t h_pass(uint32_t cl, ui
     }
 }
 
+// The reductions a streamed wave runs: the vertical one on K rows of 16-bit
+// pairs, the horizontal one on a plane's column results (cl, ch) giving the
+// lane's four output bytes, and the border the source is read with.
+template <int MODE>
+struct GaussOp {
+    static constexpr int K = SwarTaps<MODE>::K;
+    __device__ static int border(int b) { return b; }
+    __device__ static uint32_t vert(const uint32_t (&xs)[K]) { return pdot<VTap<MODE>, K>(xs, 0u, true); }
+    __device__ static uint32_t horiz(uint32_t cl, uint32_t ch) { return h_pass<MODE>(cl, ch); }
+};
+// erode (MAX false) / dilate with a K x K rectangle: morph.zig:489-519 with
+// cells outside the image ignored, which for a rectangle is the min / max
+// over the edge-replicated image (every rectangle holding an outside cell
+// holds the in-image cell it replicates).
+template <int KK, bool MAX>
+struct RectOp {
+    static constexpr int K = KK;
+    __device__ static int border(int) { return BORDER_REPLICATE; }
+    __device__ static uint32_t vert(const uint32_t (&xs)[K]) {
+        uint32_t v = xs[0];
+#pragma unroll
+        for (int k = 1; k < K; k++) v = mband::pop2<MAX>(v, xs[k]);
+        return v;
+    }
+    __device__ static uint32_t horiz(uint32_t cl, uint32_t ch) {
+        uint32_t o01, o23;
+        mband::h_window<MAX, K / 2>(cl, ch, false, false, false, o01, o23);
+        return perm(o23, o01, 0x06040200u);
+    }
+};
+
 // AD: every destination row starts on a dword (stores need no shift).
-template <int MODE, int CN, int TB, bool AD>
+template <class OP, int CN, int TB, bool AD>
 __device__ __attribute__((always_inline)) inline void stream_wave(const SStreamArgs &a, int strip, int band) {
-    using T = SwarTaps<MODE>;
-    constexpr int K = T::K, R = K / 2, ROWS = TB + 2 * R, NB = 4 * CN;
+    constexpr int K = OP::K, R = K / 2, ROWS = TB + 2 * R, NB = 4 * CN;
     const StencilArgs &p = a.s;
     const int lane = (int)threadIdx.x & 63;
-    const int w = p.src.w, H = p.src.h, border = p.border;
+    const int w = p.src.w, H = p.src.h, border = OP::border(p.border);
     const int Y0 = band * TB;
     if (Y0 >= H) return;  // uniform
     const int P0 = strip * kStripPx;
@@ -226,18 +257,18 @@ __device__ __attribute__((always_inline)) inline void stream_wave(const SStreamA
             uint32_t xs[K];
 #pragma unroll
             for (int k = 0; k < K; k++) xs[k] = pr[t + k][j];
-            cs[j] = pdot<VTap<MODE>, K>(xs, 0u, true);
+            cs[j] = OP::vert(xs);
         }
         uint32_t out[CN];
         if constexpr (CN == 1) {
-            out[0] = h_pass<MODE>(cs[0], cs[1]);
+            out[0] = OP::horiz(cs[0], cs[1]);
         } else {
             // Per channel c: (S[c], S[c + 3]) and (S[c + 6], S[c + 9]), S[i]
             // the 16-bit sum of byte i (pair i / 2, half i % 2).
             const uint32_t c0l = perm(cs[1], cs[0], 0x07060100u), c0h = perm(cs[4], cs[3], 0x07060100u);
             const uint32_t c1l = perm(cs[2], cs[0], 0x05040302u), c1h = perm(cs[5], cs[3], 0x05040302u);
             const uint32_t c2l = perm(cs[2], cs[1], 0x07060100u), c2h = perm(cs[5], cs[4], 0x07060100u);
-            const uint32_t o0 = h_pass<MODE>(c0l, c0h), o1 = h_pass<MODE>(c1l, c1h), o2 = h_pass<MODE>(c2l, c2h);
+            const uint32_t o0 = OP::horiz(c0l, c0h), o1 = OP::horiz(c1l, c1h), o2 = OP::horiz(c2l, c2h);
             // Pixels 0-3 of channels 0-2 (oc.k) back to byte order:
             // o0.0 o1.0 o2.0 o0.1 | o1.1 o2.1 o0.2 o1.2 | o2.2 o0.3 o1.3 o2.3
             out[0] = perm(o2, perm(o1, o0, 0x01000400u), 0x03040100u);
@@ -274,11 +305,11 @@ __device__ __attribute__((always_inline)) inline void stream_wave(const SStreamA
 
 // One workgroup: 4 waves on 4 consecutive bands of one strip, tiles in XCD
 // bands.
-template <int MODE, int CN, int TB, bool AD>
+template <class OP, int CN, int TB, bool AD>
 __global__ __launch_bounds__(256) void sep_stream_u8(SStreamArgs a) {
     int strip, bg;
     if (!tile_of(TileOrder{a.strips, a.ntiles, a.xcd}, strip, bg)) return;
-    stream_wave<MODE, CN, TB, AD>(a, strip, 4 * bg + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6));
+    stream_wave<OP, CN, TB, AD>(a, strip, 4 * bg + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6));
 }
 
 }  // namespace sstream

@@ -205,13 +205,13 @@ GroupKernel morph_stream_kernel(bool mx, int cn) {
 }
 
 // LINEAR through the register-streamed waves (resize_stream.hpp): tile
-// (strip, 4 runs of bands), one run per wave: bands of 4 rows in runs of
-// j.ls.run, or single bands of 8 rows.
-template <int TB, bool AD, bool AS>
+// (strip, 4 runs of j.ls.run bands of 4 rows), one run per wave. One kernel
+// for every alignment (one group measured faster than a group per aligned
+// case: 1.570 -> 1.543 ms for the 512-image batch).
 __global__ __launch_bounds__(kTileThreads) void batch_linear_stream(BATCH_ARGS) {
     run_unit(jobs, units, count, [&](const BatchJob &j, int bx, int by) {
         const int w = 4 * by + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-        lstream::lin_stream_wave<TB, AD, AS, TB == 4>(j.ls, bx, w * j.ls.run, (w + 1) * j.ls.run);
+        lstream::lin_stream_wave<4, false, false, true>(j.ls, bx, w * j.ls.run, (w + 1) * j.ls.run);
     });
 }
 
@@ -285,17 +285,7 @@ GroupKernel group_kernel(int kind, int variant) {
     case JK_STENCIL_STREAM:
         return variant / 100 == SWAR_G5 ? stencil_stream_kernel<SWAR_G5>((variant / 10) % 10)
                                         : stencil_stream_kernel<SWAR_G7>((variant / 10) % 10);
-    case JK_LINEAR_STREAM:
-        switch (variant) {
-        case 7: return batch_linear_stream<8, true, true>;
-        case 6: return batch_linear_stream<8, true, false>;
-        case 5: return batch_linear_stream<8, false, true>;
-        case 4: return batch_linear_stream<8, false, false>;
-        case 3: return batch_linear_stream<4, true, true>;
-        case 2: return batch_linear_stream<4, true, false>;
-        case 1: return batch_linear_stream<4, false, true>;
-        default: return batch_linear_stream<4, false, false>;
-        }
+    case JK_LINEAR_STREAM: return batch_linear_stream;
     case JK_AREA_BOX: return batch_area;
     case JK_AREA_INT: return batch_other<JK_AREA_INT>;
     default: return batch_other<JK_AREA_FRAC>;
@@ -569,14 +559,11 @@ int plan_item(const visar_batch_item &it, Planned &pl) {
             for (uint32_t y = 0; y < it.dst.height; y++)
                 t[it.dst.width + y] = linear_tap_u8(y, it.src.height, it.dst.height, area_up, false);
             LinStreamArgs ls;
-            // Bands of 4 rows in runs of 4 unless the tuning asks for others
-            // (8-row bands run singly).
-            const int tb = tuning().lin_rows == 8 ? 8 : 4;
-            const int run = tb == 8 ? 1 : tuning().lin_run ? tuning().lin_run : 4;
+            // Bands of 4 rows in runs of 4 (tuning "lin_run" overrides).
+            const int run = tuning().lin_run ? tuning().lin_run : 4;
             if (linear_stream_fits(t.data(), (int)it.dst.width, (int)it.src.channels, t.data() + it.dst.width, (int)it.dst.height) &&
-                linear_stream_args(s, d, nullptr, nullptr, tb, run, ls)) {
+                linear_stream_args(s, d, nullptr, nullptr, 4, run, ls)) {
                 j.kind = JK_LINEAR_STREAM;
-                j.variant = (tb == 8 ? 4 : 0) + (ls.off_d == 0 && d.stride % 4 == 0 ? 2 : 0) + (ls.off_s == 0 && s.stride % 4 == 0 ? 1 : 0);
                 j.ls = ls;
                 j.tiles_x = ls.strips;
                 j.ntiles = ls.strips * ((ls.bands + 4 * ls.run - 1) / (4 * ls.run));

@@ -50,14 +50,18 @@ int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D
  * check of the footprints); "morph_chain" u8 morphology (0 automatic:
  * one-channel images stream each pass through registers with the subtraction
  * in the last pass, others run fused chains of passes in one LDS tile, where
- * the element is a union of centred rectangles; 1 the general kernels, one
+ * the element is a union of centred rectangles, and batched erode / dilate
+ * with a full rectangle runs in streamed waves; 1 the general kernels, one
  * per erode / dilate pass and one per subtraction; 2 the fused LDS chains);
  * "morph_rows" output rows per tile of the fused chains (8-128);
- * "stencil_direct" the packed one-channel u8 stencils (GaussianBlur 3x3, 5x5,
- * 7x7 with sigma 0, Sobel 3x3 to i16): 0 rows streamed through registers
- * where rows are dword multiples starting on dwords and the border is not
- * WRAP, 1 the LDS tile kernels; "stencil_band" output rows per wave of the
- * streamed kernels (1, 2, 4, 8, 16; 0 picks from the image size);
+ * "stencil_direct" the packed u8 stencils (GaussianBlur 3x3, 5x5, 7x7 with
+ * sigma 0, Sobel 3x3 to i16): 0 rows streamed through registers (one-channel
+ * rows of dword multiples starting on dwords with a border other than WRAP
+ * in the one-channel kernel, other one- and three-channel GaussianBlur in the
+ * general one; batched 5x5 / 7x7 GaussianBlur likewise), 1 the LDS tile
+ * kernels; "stencil_band" output rows per wave of the streamed kernels (1, 2,
+ * 4, 8, 16 for the one-channel kernel, 2 or 4 for the general one; 0 picks
+ * from the image size);
  * "stencil_wg" their waves per workgroup (1, 2, 4); "gray_layout" u8
  * cvtColor to gray on contiguous 16-byte aligned images (0 each wave reads
  * one contiguous 1024-pixel block, 1 each thread its own 16 pixels);

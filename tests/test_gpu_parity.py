@@ -770,18 +770,20 @@ MORPH_BASES = [(0, 3), (0, 5), (0, 7), (1, 3), (1, 5), (1, 7), (2, 5), (2, 7)]  
 
 
 def test_morphology_streamed_powers(va, orc):
-    """The n-fold streamed passes (one launch for n iterations, gradient's
-    two chains in one launch) against the oracle's pass-by-pass result for
-    every base element, op and n = 1..4 (4 runs as 2 + 2), at widths on both
-    sides of the 496- and 480-byte strips, heights below and above the row
-    bands and the largest element's reach, and several band heights."""
+    """The n-fold passes in one launch (the accumulator kernel, gradient's
+    two chains in one launch, and the column-window kernel of morph_pow.hpp
+    for the sums with too many rectangles) against the oracle's pass-by-pass
+    result for every base element, op and n = 1..5 (4 runs as 2 + 2, 5 as
+    3 + 2, a 7x7 cross's 3 as 2 + 1), at widths on both sides of the 496-,
+    480-, 248- and 240-byte strips, heights below and above the row bands and
+    the largest element's reach, and several band heights."""
     from visar_amd import autotune
-    sizes = [(1, 8), (3, 12), (7, 500), (23, 496), (41, 1004), (97, 964)]
+    sizes = [(1, 8), (3, 12), (7, 500), (23, 496), (41, 1004), (97, 964), (13, 248), (6, 244)]
     try:
         for band in (0, 4, 40):
             autotune.set_knob("morph_band", band)
             for shape, ks in MORPH_BASES:
-                for it in (1, 2, 3, 4):
+                for it in (1, 2, 3, 4, 5):
                     for (h, w) in (sizes if band == 0 else sizes[2:4]):
                         a = (rnd((h, w), h * 7 + w + ks + it) > 160).astype(np.uint8) * 255 if it % 2 else rnd((h, w), h + w)
                         for op in range(7):
@@ -790,6 +792,26 @@ def test_morphology_streamed_powers(va, orc):
                             assert_same(got, want, f"op{op} shape{shape} k{ks} it{it} {h}x{w} band {band}")
     finally:
         autotune.set_knob("morph_band", 0)
+
+
+def test_morphology_powers_views(va, orc):
+    """The n-fold launches on dword-aligned device views (pitch and offset
+    multiples of 4) with both images inside larger buffers: results equal
+    the oracle's and no byte outside the destination view changes."""
+    for shape, ks in MORPH_BASES:
+        for it in (2, 3):
+            for op in range(7):
+                h, w = 37, 252
+                big = dev(rnd((h + 4, w + 24), ks * 10 + it))
+                src = big[2:2 + h, 8:8 + w]
+                outer = torch.full((h + 6, w + 16), 77, dtype=torch.uint8, device="cuda")
+                dst = outer[3:3 + h, 4:4 + w]
+                va.morphology(src, op, shape, (ks, ks), it, dst=dst)
+                want = orc.morphology(src.cpu().numpy(), op, shape, (ks, ks), it)
+                assert_same(dst.cpu().numpy(), want, f"op{op} shape{shape} k{ks} it{it}")
+                rest = outer.clone()
+                rest[3:3 + h, 4:4 + w] = 77
+                assert bool((rest == 77).all()), f"write outside the view op{op} shape{shape} k{ks} it{it}"
 
 
 def test_morphology_band_passes(va, orc):

@@ -1461,45 +1461,99 @@ static bool morph_stream_pays(uint32_t el, int n) {
     return mstream::acc_rows(mstream::power(el, n)) <= mstream::kStreamMaxRows;
 }
 
+static int morph_band_launch(const DImg &in, const DImg &out, uint32_t el, int op_max, int post, const DImg *c,
+                             hipStream_t st);
+
+// The n-fold sums morph_pow.hpp has launches for: half-width up to 7,
+// half-height up to 6, where the accumulator kernel does not pay.
+static bool pow_fits(uint32_t el, int n) { return morph_pow_has(el, n); }
+
+// n passes of a base element as one launch of morph_pow.hpp (post as in
+// morph_direct_launch). UNSUPPORTED (nothing launched) when it has none.
+static int morph_pow_launch(const DImg &in, const DImg &out, uint32_t el, int n, int op_max, int post, const DImg *c,
+                            hipStream_t st) {
+    MorphDirectArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.s = in;
+    a.d = out;
+    a.c = c ? *c : in;
+    auto span = [](const DImg &m) { return (uint64_t)(m.h - 1) * (uint64_t)m.stride + (uint64_t)m.w * m.cn; };
+    if (in.w < 8 || span(a.s) >= (1ull << 31) || span(a.d) >= (1ull << 31) || span(a.c) >= (1ull << 31))
+        return UNSUPPORTED;
+    a.span_s = (uint32_t)span(a.s);
+    a.span_d = (uint32_t)span(a.d);
+    a.span_c = (uint32_t)span(a.c);
+    if (!launch_morph_pow(a, el, n, op_max, post, st)) return UNSUPPORTED;
+    return hipGetLastError() == hipSuccess ? OK : INTERNAL;
+}
+
 static int morph_stream_ex(Call &call, const DImg &s, const DImg &d, int op, int n, uint32_t el) {
     auto aligned = [](const DImg &m) { return ((((uintptr_t)m.p) | (uintptr_t)m.stride) & 3) == 0; };
-    if (tuning().morph_stream == 1 || s.cn != 1 || (s.w & 3) || s.w < 8 || !aligned(s) || !aligned(d))
+    auto span = [](const DImg &m) { return (uint64_t)(m.h - 1) * (uint64_t)m.stride + (uint64_t)m.w * m.cn; };
+    if (tuning().morph_stream == 1 || s.cn != 1 || (s.w & 3) || s.w < 8 || !aligned(s) || !aligned(d) ||
+        span(s) >= (1ull << 31) || span(d) >= (1ull << 31))
         return UNSUPPORTED;
-    if ((el & 15u) == 0 || !morph_stream_pays(el, std::min(n, 3))) return UNSUPPORTED;
+    if ((el & 15u) == 0) return UNSUPPORTED;
     hipStream_t st = call.stream();
-    if (op == 4 && n <= 3) return morph_stream_launch(s, d, el, n, 2, 0, nullptr, st);
+    if (op == 4 && n <= 3 && morph_stream_pays(el, n)) return morph_stream_launch(s, d, el, n, 2, 0, nullptr, st);
     if (n < 2) return UNSUPPORTED;
-    // Passes of one kind: n as runs of 3 and 2 from `in` into `out`, the
-    // last run with post against c.
+    // Runs of r passes in one launch each: the accumulator kernel where it
+    // pays, else the n-fold column-window kernel, else single passes. Runs
+    // of 3 and 2 (4 as 2 + 2), single passes last; no run of 2 or more:
+    // UNSUPPORTED (pass by pass). The first launch is a multi-pass run, so
+    // an element without kernels fails before anything is launched.
+    auto run_ok = [&](int r) { return morph_stream_pays(el, r) || pow_fits(el, r); };
+    std::vector<int> runs;
+    int ones = 0;
+    for (int left = n; left > 0;) {
+        int r = left == 4 ? 2 : std::min(3, left);
+        while (r > 1 && !run_ok(r)) r--;
+        if (r == 1) ones++;
+        else runs.push_back(r);
+        left -= r;
+    }
+    if (runs.empty()) return UNSUPPORTED;
+    runs.insert(runs.end(), ones, 1);
+    auto run_launch = [&](const DImg &in, const DImg &out, int r, int mode, int post, const DImg *c) {
+        if (r >= 2 && morph_stream_pays(el, r)) return morph_stream_launch(in, out, el, r, mode, post, c, st);
+        if (r >= 2) return morph_pow_launch(in, out, el, r, mode, post, c, st);
+        return morph_band_launch(in, out, el, mode, post, c, st);
+    };
+    // Passes of one kind from `in` into `out`, the last run with post
+    // against c.
     DImg tmp[2] = {};
     int k = 0;
     bool first = true;
     auto passes = [&](const DImg &in, const DImg &out, int mode, int post, const DImg *c) -> int {
         DImg cur = in;
-        for (int left = n; left > 0;) {
-            const int run = left == 4 || left == 2 ? 2 : 3;
-            const bool last = run == left;
+        for (size_t q = 0; q < runs.size(); q++) {
+            const bool last = q + 1 == runs.size();
             DImg o = out;
             if (!last) {
                 DImg &t = tmp[k++ & 1];
                 if (!t.p && call.temp_like(s, 1, t)) return OUT_OF_MEMORY;
                 o = t;
             }
-            const int rc = morph_stream_launch(cur, o, el, run, mode, last ? post : 0, c, st);
+            const int rc = run_launch(cur, o, runs[q], mode, last ? post : 0, c);
             if (rc == UNSUPPORTED && !first) return INTERNAL;
             if (rc) return rc;
             first = false;
             cur = o;
-            left -= run;
         }
         return OK;
     };
     if (op <= 1) return passes(s, d, op, 0, nullptr);
-    if (op == 4) return UNSUPPORTED;  // more than 3 iterations: pass by pass
-    // open / tophat: erode then dilate; close / blackhat: dilate then erode.
-    const int m1 = op == 2 || op == 5 ? 0 : 1;
     DImg t;
     if (call.temp_like(s, 1, t)) return OUT_OF_MEMORY;
+    if (op == 4) {
+        // gradient: the n-fold dilation, then the erosion subtracted from it
+        int rc = passes(s, t, 1, 0, nullptr);
+        if (rc) return rc;
+        rc = passes(s, d, 0, 1, &t);
+        return rc == UNSUPPORTED ? INTERNAL : rc;
+    }
+    // open / tophat: erode then dilate; close / blackhat: dilate then erode.
+    const int m1 = op == 2 || op == 5 ? 0 : 1;
     int rc = passes(s, t, m1, 0, nullptr);
     if (rc) return rc;
     rc = passes(t, d, 1 - m1, op == 5 ? 1 : op == 6 ? 2 : 0, &s);

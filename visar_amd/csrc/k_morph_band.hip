@@ -23,6 +23,7 @@ bool launch_el(const MorphDirectArgs &a, int op_max, int post, int tb, hipStream
     if (op_max && post == 0) return launch_tb<EL, true, 0>(a, tb, st), true;
     if (!op_max && post == 0) return launch_tb<EL, false, 0>(a, tb, st), true;
     if (op_max && post == 1) return launch_tb<EL, true, 1>(a, tb, st), true;
+    if (!op_max && post == 1) return launch_tb<EL, false, 1>(a, tb, st), true;
     if (!op_max && post == 2) return launch_tb<EL, false, 2>(a, tb, st), true;
     return false;
 }

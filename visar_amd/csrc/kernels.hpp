@@ -425,6 +425,13 @@ constexpr int kStreamMaxRows = 9;
 
 }  // namespace mstream
 
+// n = 2 or 3 iterated erode / dilate passes of a base element as one pass
+// with its n-fold sum (morph_pow.hpp), for the sums up to half-width 7 and
+// half-height 6 that morph_stream.hpp does not take (post as in
+// launch_morph_band). False (nothing launched) for other combinations.
+bool launch_morph_pow(const MorphDirectArgs &a, uint32_t el, int n, int op_max, int post, hipStream_t st);
+bool morph_pow_has(uint32_t el, int n);
+
 // One erode / dilate / gradient pass of a base element (an element code of
 // morph_chain.hpp) taken n times in one launch (morph_stream.hpp), on one
 // channel: mode 0 erode, 1 dilate, 2 gradient; post as above. False (nothing

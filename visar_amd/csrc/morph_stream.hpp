@@ -232,8 +232,9 @@ void launch_one(MorphStreamArgs a, hipStream_t st) {
 }
 
 // The launches of one base element: n = 1 gradient only; n = 2, 3 erode,
-// dilate, gradient, dilate with post 1 (tophat's second pass) and erode
-// with post 2 (blackhat's), where the n-th power takes at most
+// dilate, gradient, dilate with post 1 (tophat's second pass), erode with
+// post 1 (the last run of a gradient split over launches) and erode with
+// post 2 (blackhat's), where the n-th power takes at most
 // kStreamMaxRows accumulator rows. False for other combinations.
 template <uint32_t BASE>
 bool launch_base(const MorphStreamArgs &a, int n, int mode, int post, hipStream_t st) {
@@ -245,6 +246,7 @@ bool launch_base(const MorphStreamArgs &a, int n, int mode, int post, hipStream_
                 if (mode == 0 && post == 0) return launch_one<BASE, NN, 0, 0>(a, st), true;
                 if (mode == 1 && post == 0) return launch_one<BASE, NN, 1, 0>(a, st), true;
                 if (mode == 1 && post == 1) return launch_one<BASE, NN, 1, 1>(a, st), true;
+                if (mode == 0 && post == 1) return launch_one<BASE, NN, 0, 1>(a, st), true;
                 if (mode == 0 && post == 2) return launch_one<BASE, NN, 0, 2>(a, st), true;
             }
         }

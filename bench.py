@@ -273,7 +273,7 @@ def bench_morph(args, torch, va):
         "algorithmic_gbs_over_cases": round(len(cases) * nbytes / (total_ms * 1e-3) / 1e9, 1),
         "cases_ms": {k2: round(v, 4) for k2, v in cases.items()},
         "slowest": {slow: round(cases[slow], 4)},
-        "roofline": {"kernel": "morph_direct_u8 (dilate 3x3 rect, one pass)", "bound": "hbm", "achieved": round(gbs, 1),
+        "roofline": {"kernel": "morph_band_u8 (dilate 3x3 rect, one pass)", "bound": "hbm", "achieved": round(gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic("morph_dilate_rect3"), "ms_per_launch": round(single, 5)},
     }

@@ -69,7 +69,9 @@ int visar_gpu_memcpy(void *dst, const void *src, size_t bytes, int kind /* 1 H2D
  * 2 with the second block's loads issued before the first is blended);
  * "morph_stream" one-channel u8 morphology with iterations or gradient (0 n
  * iterations as one pass with the n-fold element where that is faster, 1
- * pass by pass), "morph_band" its output rows per band (0 = 16);
+ * pass by pass), "morph_band" its output rows per band (0 = 16),
+ * "morph_pow_rows" the output rows per wave of the n-fold passes whose sums
+ * have too many rectangles for the accumulator kernel (4, 8; 0 = 8);
  * "morph_pass" the other one-channel u8 single passes (0 column window
  * first, 1 the two-pass-capable kernel), "morph_pass_rows" their output rows
  * per wave (4, 8, 16; 0 = 4); "lin_stream" u8 LINEAR resize (0 source rows

@@ -775,13 +775,14 @@ def test_morphology_streamed_powers(va, orc):
     for the sums with too many rectangles) against the oracle's pass-by-pass
     result for every base element, op and n = 1..5 (4 runs as 2 + 2, 5 as
     3 + 2, a 7x7 cross's 3 as 2 + 1), at widths on both sides of the 496-,
-    480-, 248- and 240-byte strips, heights below and above the row bands and
-    the largest element's reach, and several band heights."""
+    480-, 248-, 240- and 232-byte strips, heights below and above the row
+    bands and the largest element's reach, and several band heights."""
     from visar_amd import autotune
-    sizes = [(1, 8), (3, 12), (7, 500), (23, 496), (41, 1004), (97, 964), (13, 248), (6, 244)]
+    sizes = [(1, 8), (3, 12), (7, 500), (23, 496), (41, 1004), (97, 964), (13, 248), (6, 244), (30, 468)]
     try:
-        for band in (0, 4, 40):
+        for band, pow_rows in ((0, 0), (4, 8), (40, 4)):
             autotune.set_knob("morph_band", band)
+            autotune.set_knob("morph_pow_rows", pow_rows)
             for shape, ks in MORPH_BASES:
                 for it in (1, 2, 3, 4, 5):
                     for (h, w) in (sizes if band == 0 else sizes[2:4]):
@@ -789,9 +790,10 @@ def test_morphology_streamed_powers(va, orc):
                         for op in range(7):
                             want = orc.morphology(a, op, shape, (ks, ks), it)
                             got = va.morphology(dev(a), op, shape, (ks, ks), it).cpu().numpy()
-                            assert_same(got, want, f"op{op} shape{shape} k{ks} it{it} {h}x{w} band {band}")
+                            assert_same(got, want, f"op{op} shape{shape} k{ks} it{it} {h}x{w} band {band} {pow_rows}")
     finally:
         autotune.set_knob("morph_band", 0)
+        autotune.set_knob("morph_pow_rows", 0)
 
 
 def test_morphology_powers_views(va, orc):

@@ -1464,8 +1464,8 @@ static bool morph_stream_pays(uint32_t el, int n) {
 static int morph_band_launch(const DImg &in, const DImg &out, uint32_t el, int op_max, int post, const DImg *c,
                              hipStream_t st);
 
-// The n-fold sums morph_pow.hpp has launches for: half-width up to 7,
-// half-height up to 6, where the accumulator kernel does not pay.
+// The n-fold sums morph_pow.hpp has launches for: half-width up to 11,
+// half-height up to 9, where the accumulator kernel does not pay.
 static bool pow_fits(uint32_t el, int n) { return morph_pow_has(el, n); }
 
 // n passes of a base element as one launch of morph_pow.hpp (post as in

@@ -23,6 +23,7 @@ struct Tuning {
     int morph_band = 0;          // streamed n-fold passes: output rows per band (0 = 16)
     int morph_pass = 0;          // one-channel u8 single passes: 0 column window first (morph_band.hpp), 1 morph_direct.hpp
     int morph_pass_rows = 0;     // column-window-first passes: output rows per wave (4, 8, 16; 0 = 4)
+    int morph_pow_rows = 0;      // n-fold column-window passes: output rows per wave (4, 8; 0 = 8)
     int gray_layout = 0;         // u8 cvtColor to gray on contiguous images: 0 one contiguous 1024-pixel block per wave, 1 16 pixels per thread
     int lin_stream = 0;          // u8 LINEAR resize: 0 rows streamed through registers where the tables allow, 1 LDS tiles
     int lin_rows = 0;            // streamed LINEAR resize: output rows per band (4, 8; 0 automatic)
@@ -426,8 +427,8 @@ constexpr int kStreamMaxRows = 9;
 }  // namespace mstream
 
 // n = 2 or 3 iterated erode / dilate passes of a base element as one pass
-// with its n-fold sum (morph_pow.hpp), for the sums up to half-width 7 and
-// half-height 6 that morph_stream.hpp does not take (post as in
+// with its n-fold sum (morph_pow.hpp), for the sums up to half-width 11 and
+// half-height 9 that morph_stream.hpp does not take (post as in
 // launch_morph_band). False (nothing launched) for other combinations.
 bool launch_morph_pow(const MorphDirectArgs &a, uint32_t el, int n, int op_max, int post, hipStream_t st);
 bool morph_pow_has(uint32_t el, int n);

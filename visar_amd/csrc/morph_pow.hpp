@@ -1,7 +1,7 @@
 // morph_pow.hpp -- n iterated erode or dilate passes (n = 2, 3) of a union of
 // centred rectangles on one-channel u8 images as one pass with the n-fold
 // Minkowski sum of the element, column window first, for gfx950: the form
-// of morph_band.hpp for sums up to half-width 7 and half-height 6 (the
+// of morph_band.hpp for sums up to half-width 11 and half-height 9 (the
 // powers whose rectangles are too many for morph_stream.hpp's accumulators).
 //
 // Semantics: morph.zig:63-96 (iterations repeat the pass, each ignoring
@@ -11,7 +11,7 @@
 // edge-replicated image (each rectangle holding an outside cell holds the
 // in-image cell it replicates).
 //
-// Layout as in morph_band.hpp, with HL = 1 or 2 lanes of halo on each side:
+// Layout as in morph_band.hpp, with HL = 1, 2 or 3 lanes of halo on each side:
 // lane i holds bytes X0 + 4 (i - HL) .. + 3 of every row (one dword load per
 // row, all issued before the first use); lanes HL..63-HL produce output.
 // Lanes wholly left or right of the row hold its first or last byte in all
@@ -38,12 +38,14 @@ struct Pow {
     static constexpr RectSet R = mstream::power(BASE, N);
     static constexpr int A = mstream::max_a(R), B = mstream::max_b(R);
 };
-// Launches exist for the sums with A <= 7, B <= 6 that the accumulator
+// Launches exist for the sums with A <= 11, B <= 9 that the accumulator
 // kernel does not take.
 template <uint32_t BASE, int N>
 constexpr bool fits() {
-    return Pow<BASE, N>::A <= 7 && Pow<BASE, N>::B <= 6 && mstream::acc_rows(Pow<BASE, N>::R) > mstream::kStreamMaxRows;
+    return Pow<BASE, N>::A <= 11 && Pow<BASE, N>::B <= 9 && mstream::acc_rows(Pow<BASE, N>::R) > mstream::kStreamMaxRows;
 }
+// Lanes of halo on each side: 4 HL bytes cover the half-width.
+constexpr int halo_lanes(int a) { return (a + 3) / 4; }
 
 template <int... I, typename F>
 __device__ __attribute__((always_inline)) inline void for_seq(std::integer_sequence<int, I...>, F &&f) {
@@ -82,7 +84,8 @@ template <uint32_t BASE, int N, bool MAX, int POST, int TB>
 __device__ __attribute__((always_inline)) inline void pow_wave(const MorphDirectArgs &a, int strip, int band) {
     using PW = Pow<BASE, N>;
     constexpr int A = PW::A, B = PW::B, ROWS = TB + 2 * B;
-    constexpr int HL = A <= 3 ? 1 : 2;
+    constexpr int HL = halo_lanes(A);
+    static_assert(HL >= 1 && HL <= 3, "half-width outside the kernel's limits");
     constexpr int SW = 4 * (64 - 2 * HL);  // output bytes per strip
     const int lane = (int)threadIdx.x & 63;
     const int n = a.s.w, H = a.s.h;
@@ -130,11 +133,12 @@ __device__ __attribute__((always_inline)) inline void pow_wave(const MorphDirect
             lh[0] = mband::dpp_shr(vh);
             rl[0] = mband::dpp_shl(vl);
             rh[0] = mband::dpp_shl(vh);
-            if constexpr (HL == 2) {
-                ll[1] = mband::dpp_shr(ll[0]);
-                lh[1] = mband::dpp_shr(lh[0]);
-                rl[1] = mband::dpp_shl(rl[0]);
-                rh[1] = mband::dpp_shl(rh[0]);
+#pragma unroll
+            for (int m = 1; m < HL; m++) {
+                ll[m] = mband::dpp_shr(ll[m - 1]);
+                lh[m] = mband::dpp_shr(lh[m - 1]);
+                rl[m] = mband::dpp_shl(rl[m - 1]);
+                rh[m] = mband::dpp_shl(rh[m - 1]);
             }
             uint32_t o01, o23;
             h_win<MAX, ak, HL>(vl, vh, ll, lh, rl, rh, o01, o23);

@@ -39,6 +39,7 @@ const Knob kKnobs[] = {
     {"morph_band", &Tuning::morph_band, 0, 256},
     {"morph_pass", &Tuning::morph_pass, 0, 1},
     {"morph_pass_rows", &Tuning::morph_pass_rows, 0, 16},
+    {"morph_pow_rows", &Tuning::morph_pow_rows, 0, 8},
 };
 }  // namespace
 

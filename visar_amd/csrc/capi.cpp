@@ -213,7 +213,9 @@ int check_image(const visar_image *c, uint32_t depth) {
 }
 
 Call::~Call() {
-    if (waited_ || !st_) {
+    // Buffers of a call that queued work on a stream (the default stream
+    // included) return behind that work.
+    if (waited_ || !begun_) {
         for (void *p : owned_) pool_put(p);
         for (void *p : pinned_) pinned_pool().put(p);
     } else {
@@ -226,6 +228,7 @@ int Call::begin() {
     int rc = ensure_runtime();
     if (rc) return rc;
     st_ = current_stream();
+    begun_ = true;
     return OK;
 }
 

@@ -50,11 +50,12 @@ constexpr int kDppShr1 = 0x138, kDppShl1 = 0x130;  // lane i takes lane i - 1's 
 constexpr int kBufWord3 = 0x00020000;              // buffer resource word 3 for gfx9 raw (untyped) access
 typedef unsigned int vu32x2 __attribute__((ext_vector_type(2)));
 
+// The lane without a source reads 0 (bound_ctrl): no old value to set up.
 __device__ __attribute__((always_inline)) inline uint32_t shr1(uint32_t v) {
-    return __builtin_amdgcn_update_dpp(0u, v, kDppShr1, 0xf, 0xf, false);
+    return __builtin_amdgcn_mov_dpp(v, kDppShr1, 0xf, 0xf, true);
 }
 __device__ __attribute__((always_inline)) inline uint32_t shl1(uint32_t v) {
-    return __builtin_amdgcn_update_dpp(0u, v, kDppShl1, 0xf, 0xf, false);
+    return __builtin_amdgcn_mov_dpp(v, kDppShl1, 0xf, 0xf, true);
 }
 
 template <int... I, typename F>

@@ -60,7 +60,8 @@ class Call {
     };
     DImg view(const visar_image *c, void *p, size_t stride);
 
-    hipStream_t st_ = nullptr;
+    hipStream_t st_ = nullptr;  // nullptr is also HIP's default stream
+    bool begun_ = false;
     bool staged_ = false;
     bool waited_ = false;
     std::vector<void *> owned_;

@@ -40,16 +40,17 @@ namespace sdirect {
 
 constexpr int kStrip = 248;  // output bytes per strip (lanes 1..62)
 // DPP controls: lane i takes lane i - 1's value (shr) or lane i + 1's (shl);
-// the lane without a source keeps the old value (0 here).
+// the lane without a source reads 0 (bound_ctrl), so no old value has to be
+// set up in the destination first.
 constexpr int kDppWaveShr1 = 0x138, kDppWaveShl1 = 0x130;
 constexpr int kBufWord3 = 0x00020000;  // buffer resource word 3 for gfx9 raw (untyped) access
 typedef unsigned int vu32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __attribute__((always_inline)) inline uint32_t dpp_shr(uint32_t v) {
-    return __builtin_amdgcn_update_dpp(0u, v, kDppWaveShr1, 0xf, 0xf, false);
+    return __builtin_amdgcn_mov_dpp(v, kDppWaveShr1, 0xf, 0xf, true);
 }
 __device__ __attribute__((always_inline)) inline uint32_t dpp_shl(uint32_t v) {
-    return __builtin_amdgcn_update_dpp(0u, v, kDppWaveShl1, 0xf, 0xf, false);
+    return __builtin_amdgcn_mov_dpp(v, kDppWaveShl1, 0xf, 0xf, true);
 }
 __device__ __attribute__((always_inline)) inline uint32_t perm(uint32_t a, uint32_t b, uint32_t sel) {
     return __builtin_amdgcn_perm(a, b, sel);

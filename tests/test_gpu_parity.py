@@ -8,6 +8,8 @@ Case lists follow the reference's difftests (tests/py/test_core_vs_opencv.py,
 test_calib3d_vs_opencv.py, test_motion_vs_opencv.py): every border, 1/3/4
 channels, strided views and degenerate sizes.
 """
+import contextlib
+
 import numpy as np
 import pytest
 
@@ -499,10 +501,12 @@ def test_resize_linear_streamed_layouts(va, orc):
         autotune.set_knob("lin_run", 0)
 
 
-def test_queued_calls_keep_their_tables_and_temporaries(va, orc):
+@pytest.mark.parametrize("own_stream", [False, True])
+def test_queued_calls_keep_their_tables_and_temporaries(va, orc, own_stream):
     """Queued calls (no per-call wait) whose tap tables and temporaries come
     from the shared pool: each buffer is reused only after the work that
-    reads it, so every output equals the oracle after one final wait."""
+    reads it, so every output equals the oracle after one final wait. On
+    HIP's default stream (a null handle) and on a stream of its own."""
     cases = []
     for i in range(24):
         cn = 1 + 2 * (i % 2)
@@ -516,16 +520,23 @@ def test_queued_calls_keep_their_tables_and_temporaries(va, orc):
             cases.append((a, ("morphology", 2, 0, (3, 3), 2)))
     srcs = [dev(a) for a, _ in cases]
     outs = []
+    side = torch.cuda.Stream() if own_stream else None
+    if side is not None:
+        side.wait_stream(torch.cuda.current_stream())
+        va.set_stream(side)
     va.set_sync(False)
     try:
-        for sv, (_, op) in zip(srcs, cases):
-            if op[0] == "resize":
-                outs.append(va.resize(sv, op[1], op[2]))
-            else:
-                outs.append(va.morphology(sv, op[1], op[2], op[3], iterations=op[4]))
+        with torch.cuda.stream(side) if side is not None else contextlib.nullcontext():
+            for sv, (_, op) in zip(srcs, cases):
+                if op[0] == "resize":
+                    outs.append(va.resize(sv, op[1], op[2]))
+                else:
+                    outs.append(va.morphology(sv, op[1], op[2], op[3], iterations=op[4]))
         va.synchronize()
     finally:
         va.set_sync(True)
+        if side is not None:
+            va.set_stream(torch.cuda.current_stream())
     for i, ((a, op), got) in enumerate(zip(cases, outs)):
         want = orc.resize(a, op[1], op[2]) if op[0] == "resize" else orc.morphology(a, op[1], op[2], op[3], iterations=op[4])
         assert_same(got.cpu().numpy(), want, f"queued call {i} {op}")

@@ -52,14 +52,14 @@ __device__ __attribute__((always_inline)) inline void for_seq(std::integer_seque
     (f(std::integral_constant<int, I>{}), ...);
 }
 
-// Output pairs (C0, C1), (C2, C3) of the min / max over [-A, A] columns:
-// P(d) = (C_d, C_d+1) from the lane's pairs (lo = (C0, C1), hi = (C2, C3))
-// and its neighbours' (lanes i -+ m in lm / rm), odd d from two even ones.
+// Output pairs (C0, C1), (C2, C3) of the min / max over [-A, A] columns,
+// from the even pairs P(e) = (C_e, C_e+1): the lane's own (lo = (C0, C1),
+// hi = (C2, C3)) and its neighbours' (lanes i -+ m in ll, lh / rl, rh).
 template <bool MAX, int A, int HL>
 __device__ __attribute__((always_inline)) inline void h_win(uint32_t lo, uint32_t hi, const uint32_t (&ll)[HL], const uint32_t (&lh)[HL],
                                                             const uint32_t (&rl)[HL], const uint32_t (&rh)[HL], uint32_t &o01,
                                                             uint32_t &o23) {
-    auto even = [&](int d) -> uint32_t {  // d even, -4 HL <= d <= 4 HL + 2
+    auto even = [&](int d) -> uint32_t {  // d even, -4 HL <= d <= 4 HL + 2 (A + 3 <= 4 HL + 2)
         if (d >= 0 && d <= 2) return d == 0 ? lo : hi;
         if (d < 0) {
             const int m = (-d + 3) / 4, r = d + 4 * m;  // lane i - m, its pair at r (0 or 2)
@@ -68,15 +68,28 @@ __device__ __attribute__((always_inline)) inline void h_win(uint32_t lo, uint32_
         const int m = d / 4, r = d - 4 * m;
         return r == 0 ? rl[m - 1] : rh[m - 1];
     };
-    auto pair = [&](int d) -> uint32_t { return (d & 1) ? __builtin_amdgcn_alignbyte(even(d + 1), even(d - 1), 2) : even(d); };
-    uint32_t a = pair(-A), b = pair(2 - A);
+    if constexpr (A == 0) {
+        o01 = lo;
+        o23 = hi;
+    } else {
+        // W(x): the pairwise min / max of the even pairs P(e), e in [x, x + 2A]:
+        // its low halves cover the even columns of [x, x + 2A], its high
+        // halves the odd columns of [x + 1, x + 2A + 1]. Each output half is
+        // one window's even and odd columns, so two W's and the halves of a
+        // third give it, with no odd pair assembled.
+        auto W = [&](int x) -> uint32_t {
+            const int e0 = x + (x & 1), e1 = x + 2 * A - ((x + 2 * A) & 1);
+            uint32_t w = even(e0);
 #pragma unroll
-    for (int d = -A + 1; d <= A; d++) {
-        a = pop2<MAX>(a, pair(d));
-        b = pop2<MAX>(b, pair(d + 2));
+            for (int e = e0 + 2; e <= e1; e += 2) w = pop2<MAX>(w, even(e));
+            return w;
+        };
+        const uint32_t w1 = W(-A), w2 = W(-A - 1), w3 = W(1 - A), w5 = W(2 - A), w7 = W(3 - A);
+        // o01 = (C[-A..A], C[1-A..A+1]) = W(-A) with (W(-A-1)'s high half, W(1-A)'s low half).
+        o01 = pop2<MAX>(w1, __builtin_amdgcn_alignbyte(w3, w2, 2));
+        // o23 = (C[2-A..A+2], C[3-A..A+3]) = W(2-A) with (W(1-A)'s high half, W(3-A)'s low half).
+        o23 = pop2<MAX>(w5, __builtin_amdgcn_alignbyte(w7, w3, 2));
     }
-    o01 = a;
-    o23 = b;
 }
 
 // POST 1: out = saturate(C - v), 2: out = saturate(v - C).
